@@ -1,0 +1,186 @@
+/*
+ * soapfilm.h - C ABI of the MI355X soap-film step (libsoapfilm.so).
+ *
+ * Drop-in for the time step of RiverIntheSky/SoapBubble: the Solver surface
+ * (include/solver.cuh:107-150) and the AmgX solve behind it (bubble.cu:
+ * 1479-1508). The reference has no FFI of its own; these entry points are
+ * what its C++ Solver would bind to in place of its private CUDA stages and
+ * the AmgX C API. Every function returns SF_OK (0) or an sf_status code;
+ * nothing in the library exits the process (the reference's checkCudaErrors
+ * exits, helper_cuda.h:981-995).
+ *
+ * Fields cross the boundary as FP64 host arrays in the reference's index
+ * convention: row-major by theta, phi contiguous, nPhi = 2 nTheta,
+ *   SF_FIELD_UTHETA : (nTheta-1) x nPhi   at theta = (j+1)h,   phi = i h
+ *   SF_FIELD_UPHI   :  nTheta    x nPhi   at theta = (j+1/2)h, phi = (i-1/2)h
+ *   SF_FIELD_ETA, SF_FIELD_GAMMA : nTheta x nPhi at theta = (j+1/2)h, phi = i h
+ * (header.cuh:24-35, 72-74; solver.cu:96-103). The caller owns the host
+ * buffers; the library copies.
+ */
+#ifndef SOAPFILM_H
+#define SOAPFILM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct sf_ctx sf_ctx;
+
+typedef enum {
+    SF_OK = 0,
+    SF_ERR_INVALID = 1,       /* bad argument                                 */
+    SF_ERR_HIP = 2,           /* HIP runtime error (message in sf_last_error)  */
+    SF_ERR_UNSUPPORTED = 3,   /* configuration outside what the build supports */
+    SF_ERR_NOMEM = 4,         /* device allocation failed                      */
+    SF_ERR_NOT_CONVERGED = 5, /* a solve missed its tolerance (bubble.cu:1500) */
+    SF_ERR_ORDER = 6          /* stage called out of order                     */
+} sf_status;
+
+typedef enum {
+    SF_PREC_FP64 = 0,     /* FP64 fields, FP64 PCG: the plain path             */
+    SF_PREC_MIXED = 1,    /* FP32 fields, FP32 PCG inside FP64 refinement      */
+    SF_PREC_FP32_ADV = 2  /* FP32 fields, FP64 PCG inside FP64 refinement      */
+} sf_precision;
+
+typedef enum {
+    SF_FIELD_UTHETA = 0,
+    SF_FIELD_UPHI = 1,
+    SF_FIELD_ETA = 2,
+    SF_FIELD_GAMMA = 3,
+    SF_FIELD_UAIR0 = 4,       /* uair_init on u_phi faces (solver.cu:82)          */
+    SF_FIELD_VAIR0 = 5,       /* vair_init on u_theta faces (solver.cu:83)        */
+    /* intermediates of the current step (read-only): */
+    SF_FIELD_UTHETA_STAR = 6, /* after advection                                 */
+    SF_FIELD_UPHI_STAR = 7,
+    SF_FIELD_ETA_STAR = 8,
+    SF_FIELD_GAMMA_STAR = 9,  /* after advection and the broken-cell override    */
+    SF_FIELD_RHS = 10,        /* assembled right-hand side                       */
+    SF_FIELD_DIAG = 11,       /* diagonal of the assembled operator              */
+    SF_FIELD_WNORTH = 12,     /* -A(j,i; j-1,i): north-face weight               */
+    SF_FIELD_WWEST = 13,      /* -A(j,i; j,i-1): west-face weight                */
+    SF_FIELD_SOLUTION = 14    /* solution of the implicit solve (Gamma n+1)      */
+} sf_field;
+
+/* One bubble: the config.txt inputs of Bubble::Bubble (bubble.cu:2568-2580,
+ * main.cu:20-36) that reach the step, plus the switches of header.cuh:86-98
+ * as run-time flags. */
+typedef struct {
+    int32_t n_theta;      /* numberOfGridsInTheta; nPhi = 2 n_theta (power of two) */
+    int32_t gravity;      /* header.cuh:94 `gravity`                            */
+    int32_t evaporation;  /* header.cuh:95 `evaporation`                        */
+    int32_t airflow;      /* header.cuh:96 `airflow`                            */
+    double radius;        /* m                                                  */
+    double H;             /* mean half film thickness, m                        */
+    double U;             /* characteristic velocity, m/s (the reference assumes 1) */
+    double Gamma_m;       /* surface mean concentration, mol/m^2               */
+    double T;             /* temperature, K                                     */
+    double dt;            /* timePerStep, s                                     */
+    double M_scale;       /* multiplies the derived Marangoni number M (1 = reference) */
+} sf_bubble_desc;
+
+/* Derived non-dimensional parameters (bubble.cu:2573-2580, 2627-2638). */
+typedef struct {
+    double M, Cr, gs, gamma_max, eva, dt, radius, U, h;
+} sf_derived;
+
+typedef struct {
+    int32_t precision;    /* sf_precision                                        */
+    int32_t device;       /* HIP device ordinal (Solver(..., device), solver.cu:72) */
+    int32_t use_graph;    /* capture sf_step in a hipGraph (mixed paths)          */
+    int32_t tol_absolute; /* 1: stop on |r| <= tol (PCGF_F.json "ABSOLUTE")       */
+    double tol;           /* solver tolerance; 0 -> 1e-10 relative                */
+    int32_t max_outer;    /* refinement iterations; 0 -> automatic                */
+    int32_t max_inner;    /* inner PCG iterations per refinement; 0 -> automatic  */
+    double inner_rho;     /* inner residual reduction; 0 -> automatic             */
+    int32_t advance_time; /* 1: air-flow time advances by dt every step           */
+    int32_t pad_;
+} sf_options;
+
+typedef struct {
+    int32_t iterations;   /* inner PCG iterations of the last solve              */
+    int32_t outer;        /* FP64 residual evaluations of the last solve         */
+    int32_t converged;    /* 1 if the last solve met the tolerance               */
+    int32_t failed_solves;/* solves that missed the tolerance since creation     */
+    double residual;      /* |b - A x| / |b| (or |b - A x|) in FP64, last solve   */
+    double max_residual;  /* largest final residual since creation               */
+    double volume;        /* film volume sum eta sin(theta) h^2                  */
+    int32_t broken;       /* 1 if no cell has eta > 0, or a solve failed         */
+    int32_t steps;        /* steps taken                                         */
+    double time;          /* air-flow time, s                                    */
+} sf_bubble_stats;
+
+typedef struct {
+    double advect_ms, assemble_ms, solve_ms, update_ms, step_ms;
+    int32_t steps;        /* steps the timings accumulate over                   */
+    int32_t pad_;
+} sf_timing;
+
+/* Replaces Solver::Solver (solver.cu:63-160): allocates the packed state of
+ * nbubbles bubbles on the device. Fields start as the reference's
+ * initWithConst values (u = 0, eta = 0.5, Gamma = 1, air = 0). */
+int sf_create(const sf_options* opt, int32_t nbubbles, const sf_bubble_desc* bubbles,
+              sf_ctx** out);
+/* Replaces Solver::~Solver (solver.cu:163-216). */
+int sf_destroy(sf_ctx* ctx);
+
+/* Host <-> device copies of one bubble's field; replace Quantity::copyToGPU /
+ * copyBackToCPU and Solver::copyToCPU (quantity.cu:3-17, solver.cu:243-249). */
+int sf_upload_field(sf_ctx* ctx, int32_t bubble, int32_t field, const double* host);
+int sf_download_field(sf_ctx* ctx, int32_t bubble, int32_t field, double* host);
+
+/* currentTimeGlobal (bubble.cu:12, 2662): the air-flow time in seconds;
+ * bubble = -1 sets every bubble. */
+int sf_set_time(sf_ctx* ctx, int32_t bubble, double t_seconds);
+/* Overrides the step of one bubble (or -1: all) like Solver::stepForward(dt)
+ * with the timeStepGlobal upload (solver.cu:252-257, bubble.cu:2673-2677);
+ * dt in seconds. */
+int sf_set_dt(sf_ctx* ctx, int32_t bubble, double dt_seconds);
+
+/* Solver::stepForward (solver.cu:260-292), plain branch: advection then
+ * bodyforce, nsteps times, on the context's stream. */
+int sf_step(sf_ctx* ctx, int32_t nsteps);
+
+/* Stage-level entry points, in this order, for one step:
+ * Solver::advection (bubble.cu:1171-1214); the air flow and assembly of
+ * Solver::bodyforce (bubble.cu:2389-2432); the AmgX solve
+ * (AlgebraicMultiGridCG, bubble.cu:1479-1508); the post-solve updates
+ * (bubble.cu:2444-2506). */
+int sf_advect(sf_ctx* ctx);
+int sf_assemble(sf_ctx* ctx);
+int sf_solve(sf_ctx* ctx);
+int sf_update(sf_ctx* ctx);
+
+int sf_synchronize(sf_ctx* ctx);
+/* Solver statistics and Solver::isBroken (solver.cu:296-298, 568-574). */
+int sf_get_stats(sf_ctx* ctx, int32_t bubble, sf_bubble_stats* out);
+/* Device time per stage, accumulated since creation or the last reset. */
+int sf_get_timing(sf_ctx* ctx, sf_timing* out);
+int sf_reset_timing(sf_ctx* ctx);
+/* Message of the last failure on ctx (or of the last sf_create when ctx is
+ * NULL). */
+const char* sf_last_error(const sf_ctx* ctx);
+
+/* Parameter derivation of Bubble::Bubble (host only, no device needed). */
+int sf_derive_params(const sf_bubble_desc* desc, sf_derived* out);
+
+/* Deterministic synthetic initial fields (host only): eta = 1 + 0.1 f_s,
+ * Gamma = 1 + 0.05 f_{s+1}, f = sum of 16 seeded real spherical harmonics of
+ * degree <= l_max normalised to max |f| = 1, plus noise * U[-1,1) on eta;
+ * air velocity from the stream function f_{s+2} (degree <= l_air), scaled to
+ * max speed air_amp. Replaces the texture init of initThicknessfromPic /
+ * initAirflowfromPic (initializer.cu:50-135). Arrays sized as the fields. */
+int sf_synth_fields(int32_t n_theta, uint64_t seed, int32_t l_max, int32_t l_air, double noise,
+                    double air_amp, double* eta, double* gamma, double* uair0, double* vair0);
+/* Real spherical harmonic Y_lm(theta, phi), orthonormal on the unit sphere. */
+double sf_real_ylm(int32_t l, int32_t m, double theta, double phi);
+
+/* write_thickness_img (solver.cu:541-593): eta * H * 2e5 as a HALF RGB EXR
+ * (channels B, G, R, uncompressed). */
+int sf_write_thickness_exr(sf_ctx* ctx, int32_t bubble, const char* path);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,843 @@
+// Host side of libsoapfilm.so: context, packed device layout, stage
+// orchestration and the C ABI declared in include/soapfilm.h.
+//
+// One context = one HIP stream on one device with the packed state of any
+// number of bubbles (the reference holds one bubble per process in
+// __constant__ symbols, bubble.cu:6-25; here every parameter lives in a
+// per-bubble descriptor array in device memory, so bubbles of different
+// resolution and parameters advance together in one launch per stage).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/soapfilm.h"
+#include "sf_launch.h"
+#include "sf_host.h"
+
+using namespace sfk;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct HipError {
+    std::string msg;
+};
+
+#define SF_HIP(call)                                                                   \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            throw HipError{std::string(#call) + ": " + hipGetErrorString(e_)};          \
+    } while (0)
+
+struct Status {
+    int code;
+    std::string msg;
+};
+
+constexpr double kPi = 3.14159265358979323846;
+
+template <typename X>
+X* dalloc(size_t n, std::vector<void*>& owned) {
+    void* p = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc(&p, n * sizeof(X));
+    if (e != hipSuccess) throw Status{SF_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e)};
+    owned.push_back(p);
+    SF_HIP(hipMemset(p, 0, n * sizeof(X)));
+    return static_cast<X*>(p);
+}
+
+// Everything that does not depend on the precision.
+struct Base {
+    sf_options opt{};
+    int device = 0;
+    int nb = 0;
+    std::vector<sf_bubble_desc> desc;
+    std::vector<sf_derived> der;
+    std::vector<BubbleDev> bub_h;
+    std::vector<BubbleTime> time_h;
+    long long ncells = 0;
+    int nrow_total = 0;
+    long stride = 0;          // partial-sum stride (rows)
+    hipStream_t stream = nullptr;
+    std::vector<void*> owned;
+    BubbleDev* bub_d = nullptr;
+    BubbleTime* time_d = nullptr;
+    SolveCtl* ctl_d = nullptr;
+    SolveHist* hist_d = nullptr;
+    Tables tab{};
+    LaunchGeom geom{};
+    double* partR = nullptr;
+    double* partA = nullptr;
+    double* partB = nullptr;
+    double* partS = nullptr;
+    double* x64[2] = {nullptr, nullptr};
+    double *wN64 = nullptr, *wW64 = nullptr, *diag64 = nullptr, *b64 = nullptr;
+    SolveParams sp{};
+    int max_outer = 3, max_inner = 500;
+    bool calibrated = false;
+    int calib_inner = 0, calib_outer = 0;
+    int steps_done = 0;
+    int stage = 0;            // 0 idle, 1 advected, 2 assembled, 3 solved
+    hipGraphExec_t graph = nullptr;
+    hipEvent_t ev[6] = {};
+    sf_timing timing{};
+    std::string err;
+    virtual ~Base() {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (void* p : owned) (void)hipFree(p);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    virtual void advect() = 0;
+    virtual void assemble() = 0;
+    virtual void solve(bool host_driven) = 0;
+    virtual void update() = 0;
+    virtual void upload(int b, int field, const double* h) = 0;
+    virtual void download(int b, int field, double* h) = 0;
+    virtual void row_stats() = 0;
+    virtual bool fp64_inner() const = 0;
+};
+
+template <typename T, typename Ti>
+struct Impl : Base {
+    StatePtrs<T> cur{}, nxt{};
+    T *ua0 = nullptr, *va0 = nullptr, *au = nullptr, *av = nullptr;
+    Ti *wN = nullptr, *wW = nullptr, *dg = nullptr, *e = nullptr, *r = nullptr, *z = nullptr,
+       *Ap = nullptr;
+    Ti* p[2] = {nullptr, nullptr};
+
+    AirPtrs<T> air() const { return AirPtrs<T>{ua0, va0, au, av}; }
+    SysPtrs<Ti> sys(double* x_init) const {
+        return SysPtrs<Ti>{wN64, wW64, diag64, b64, x_init, wN, wW, dg, e, r, z, Ap};
+    }
+    bool fp64_inner() const override { return sizeof(Ti) == 8; }
+
+    void alloc_fields() {
+        const size_t n = (size_t)ncells;
+        for (StatePtrs<T>* s : {&cur, &nxt}) {
+            s->uth = dalloc<T>(n, owned);
+            s->uph = dalloc<T>(n, owned);
+            s->eta = dalloc<T>(n, owned);
+            s->gam = dalloc<T>(n, owned);
+        }
+        ua0 = dalloc<T>(n, owned);
+        va0 = dalloc<T>(n, owned);
+        au = dalloc<T>(n, owned);
+        av = dalloc<T>(n, owned);
+        wN = dalloc<Ti>(n, owned);
+        wW = dalloc<Ti>(n, owned);
+        dg = dalloc<Ti>(n, owned);
+        e = dalloc<Ti>(n, owned);
+        r = dalloc<Ti>(n, owned);
+        z = dalloc<Ti>(n, owned);
+        Ap = dalloc<Ti>(n, owned);
+        p[0] = dalloc<Ti>(n, owned);
+        p[1] = dalloc<Ti>(n, owned);
+        // reference initWithConst (solver.cu:106-110): u = 0, eta = 0.5, Gamma = 1
+        std::vector<double> h;
+        for (int b = 0; b < nb; ++b) {
+            const size_t N = (size_t)bub_h[b].nth * bub_h[b].nph;
+            h.assign(N, 0.5);
+            upload(b, SF_FIELD_ETA, h.data());
+            h.assign(N, 1.0);
+            upload(b, SF_FIELD_GAMMA, h.data());
+        }
+    }
+
+    T state_gmax(double gmax) const {
+        T v = (T)gmax;
+        while ((double)v <= gmax - 1e-7) v = std::nextafter(v, (T)INFINITY);
+        return v;
+    }
+
+    void advect() override { launch_advect<T>(stream, geom, bub_d, tab, cur, nxt); }
+
+    void assemble() override {
+        // all bubbles share one launch; the broken-cell value of Gamma* per
+        // bubble is the state-precision gamma_max, identical for a context
+        // whose bubbles share Gamma_m and T (checked at creation).
+        launch_assemble<T, Ti>(stream, geom, bub_d, time_d, tab, nxt, air(), sys(x64[1]),
+                               (double)state_gmax(bub_h[0].gamma_max));
+    }
+
+    void resid(int k) {
+        const double* xo = x64[(k + 1) & 1];
+        double* xn = x64[k & 1];
+        launch_resid<Ti>(stream, geom, bub_d, ctl_d, sys(x64[1]), xo, xn, k > 0 ? 1 : 0, k, partR,
+                         stride);
+    }
+    void inner(int it) {
+        launch_spmv<Ti>(stream, geom, bub_d, ctl_d, sys(x64[1]), p[(it + 1) & 1], p[it & 1], it, sp,
+                        partR, partB, partA, stride);
+        launch_cgstep<Ti>(stream, geom, bub_d, ctl_d, sys(x64[1]), p[it & 1], it, partA, partB,
+                          stride);
+    }
+
+    bool all_flags(std::vector<SolveCtl>& c, bool inner_flag) {
+        SF_HIP(hipMemcpyAsync(c.data(), ctl_d, sizeof(SolveCtl) * nb, hipMemcpyDeviceToHost, stream));
+        SF_HIP(hipStreamSynchronize(stream));
+        for (int b = 0; b < nb; ++b) {
+            if (c[b].outer_done) continue;
+            if (!inner_flag || !c[b].inner_done) return false;
+        }
+        return true;
+    }
+
+    void solve(bool host_driven) override {
+        launch_reset_ctl(stream, ctl_d, nb);
+        resid(0);
+        if (host_driven) {
+            std::vector<SolveCtl> c(nb);
+            std::vector<int> before(nb);
+            int max_in_used = 0, outers_used = 0;
+            for (int k = 0; k < max_outer; ++k) {
+                launch_spmv<Ti>(stream, geom, bub_d, ctl_d, sys(x64[1]), p[1], p[0], 0, sp, partR,
+                                partB, partA, stride);
+                if (all_flags(c, false)) break;
+                for (int b = 0; b < nb; ++b) before[b] = c[b].iters;
+                launch_cgstep<Ti>(stream, geom, bub_d, ctl_d, sys(x64[1]), p[0], 0, partA, partB,
+                                  stride);
+                for (int it = 1; it < max_inner; ++it) {
+                    inner(it);
+                    if ((it & 3) == 3 && all_flags(c, true)) break;
+                }
+                all_flags(c, true);
+                for (int b = 0; b < nb; ++b) max_in_used = std::max(max_in_used, c[b].iters - before[b]);
+                outers_used = k + 1;
+                resid(k + 1);
+            }
+            if (!calibrated && !fp64_inner() && opt.max_inner == 0) {
+                calib_inner = std::max(calib_inner, max_in_used);
+                calib_outer = std::max(calib_outer, outers_used);
+            }
+        } else {
+            for (int k = 0; k < max_outer; ++k) {
+                for (int it = 0; it < max_inner; ++it) inner(it);
+                resid(k + 1);
+            }
+        }
+        launch_finalize(stream, nb, bub_d, ctl_d, hist_d, sp, partR, stride);
+    }
+
+    void update() override {
+        launch_update<T>(stream, geom, bub_d, time_d, ctl_d, tab, nxt, air(), x64[0], x64[1], cur);
+    }
+
+    // ---- host <-> device copies
+    void upload(int b, int field, const double* h) override {
+        const BubbleDev& B = bub_h[b];
+        const size_t N = (size_t)B.nth * B.nph;
+        size_t n = N;
+        T* dst = nullptr;
+        switch (field) {
+            case SF_FIELD_UTHETA: dst = cur.uth; n = N - B.nph; break;
+            case SF_FIELD_UPHI: dst = cur.uph; break;
+            case SF_FIELD_ETA: dst = cur.eta; break;
+            case SF_FIELD_GAMMA: dst = cur.gam; break;
+            case SF_FIELD_UAIR0: dst = ua0; break;
+            case SF_FIELD_VAIR0: dst = va0; n = N - B.nph; break;
+            default: throw Status{SF_ERR_INVALID, "field cannot be uploaded"};
+        }
+        std::vector<T> tmp(N, T(0));
+        for (size_t k = 0; k < n; ++k) tmp[k] = (T)h[k];
+        SF_HIP(hipMemcpyAsync(dst + B.off, tmp.data(), N * sizeof(T), hipMemcpyHostToDevice, stream));
+        SF_HIP(hipStreamSynchronize(stream));
+    }
+
+    template <typename X>
+    void fetch(const X* src, size_t off, size_t n, double* h) {
+        std::vector<X> tmp(n);
+        SF_HIP(hipMemcpyAsync(tmp.data(), src + off, n * sizeof(X), hipMemcpyDeviceToHost, stream));
+        SF_HIP(hipStreamSynchronize(stream));
+        for (size_t k = 0; k < n; ++k) h[k] = (double)tmp[k];
+    }
+
+    void download(int b, int field, double* h) override {
+        const BubbleDev& B = bub_h[b];
+        const size_t N = (size_t)B.nth * B.nph, Nt = N - B.nph, o = (size_t)B.off;
+        switch (field) {
+            case SF_FIELD_UTHETA: fetch(cur.uth, o, Nt, h); break;
+            case SF_FIELD_UPHI: fetch(cur.uph, o, N, h); break;
+            case SF_FIELD_ETA: fetch(cur.eta, o, N, h); break;
+            case SF_FIELD_GAMMA: fetch(cur.gam, o, N, h); break;
+            case SF_FIELD_UAIR0: fetch(ua0, o, N, h); break;
+            case SF_FIELD_VAIR0: fetch(va0, o, Nt, h); break;
+            case SF_FIELD_UTHETA_STAR: fetch(nxt.uth, o, Nt, h); break;
+            case SF_FIELD_UPHI_STAR: fetch(nxt.uph, o, N, h); break;
+            case SF_FIELD_ETA_STAR: fetch(nxt.eta, o, N, h); break;
+            case SF_FIELD_GAMMA_STAR: fetch(nxt.gam, o, N, h); break;
+            case SF_FIELD_RHS: fetch(b64, o, N, h); break;
+            case SF_FIELD_DIAG: fetch(diag64, o, N, h); break;
+            case SF_FIELD_WNORTH: fetch(wN64, o, N, h); break;
+            case SF_FIELD_WWEST: fetch(wW64, o, N, h); break;
+            case SF_FIELD_SOLUTION: {
+                SolveCtl c;
+                SF_HIP(hipMemcpyAsync(&c, ctl_d + b, sizeof(c), hipMemcpyDeviceToHost, stream));
+                SF_HIP(hipStreamSynchronize(stream));
+                fetch(x64[c.xbuf & 1], o, N, h);
+                break;
+            }
+            default: throw Status{SF_ERR_INVALID, "unknown field"};
+        }
+    }
+
+    void row_stats() override { launch_row_stats<T>(stream, geom, bub_d, tab, cur.eta, partS, stride); }
+};
+
+// ------------------------------------------------------------------ setup --
+bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
+
+void derive(const sf_bubble_desc& d, sf_derived& o) { sfh::derive_params(d, o); }
+
+void build_tables(Base& c, std::vector<void*>& owned) {
+    std::vector<double> sTc, cTc, sTf, cTf, sF, cscTc, sPc, cPc, sPh, cPh;
+    for (int b = 0; b < c.nb; ++b) {
+        const BubbleDev& B = c.bub_h[b];
+        const double h = B.h;
+        for (int j = 0; j <= B.nth; ++j) {
+            // centres (j + 1/2) h, as gTheta = (thetaId + 0.5) * gridLen
+            sTc.push_back(std::sin(((double)j + 0.5) * h));
+            cTc.push_back(std::cos(((double)j + 0.5) * h));
+            cscTc.push_back(1.0 / std::sin(((double)j + 0.5) * h));
+            // u_theta positions (thetaId + 1.0) * gridLen, indexed by face j
+            sTf.push_back(std::sin((double)j * h));
+            cTf.push_back(std::cos((double)j * h));
+            // assembly face sines sin(gTheta - halfStep); both pole faces
+            // carry no flux
+            sF.push_back((j == 0 || j == B.nth) ? 0.0 : std::sin(((double)j + 0.5) * h - 0.5 * h));
+        }
+        for (int i = 0; i < B.nph; ++i) {
+            sPc.push_back(std::sin((double)i * h));
+            cPc.push_back(std::cos((double)i * h));
+            sPh.push_back(std::sin(((double)i - 0.5) * h));
+            cPh.push_back(std::cos(((double)i - 0.5) * h));
+        }
+    }
+    auto up64 = [&](const std::vector<double>& v) {
+        double* d = dalloc<double>(v.size(), owned);
+        SF_HIP(hipMemcpy(d, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice));
+        return d;
+    };
+    auto up32 = [&](const std::vector<double>& v) {
+        std::vector<float> f(v.begin(), v.end());
+        float* d = dalloc<float>(f.size(), owned);
+        SF_HIP(hipMemcpy(d, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice));
+        return d;
+    };
+    Tables& t = c.tab;
+    t.sinTc = up64(sTc); t.cosTc = up64(cTc); t.sinTf = up64(sTf); t.cosTf = up64(cTf);
+    t.sinFace = up64(sF); t.cscTc = up64(cscTc);
+    t.sinTc32 = up32(sTc); t.cosTc32 = up32(cTc); t.sinTf32 = up32(sTf); t.cosTf32 = up32(cTf);
+    t.sinPc = up64(sPc); t.cosPc = up64(cPc); t.sinPh = up64(sPh); t.cosPh = up64(cPh);
+    t.sinPc32 = up32(sPc); t.cosPc32 = up32(cPc); t.sinPh32 = up32(sPh); t.cosPh32 = up32(cPh);
+}
+
+template <typename T, typename Ti>
+void upload_air_shapes(Impl<T, Ti>& c) {
+    for (int b = 0; b < c.nb; ++b) {
+        const BubbleDev& B = c.bub_h[b];
+        const size_t N = (size_t)B.nth * B.nph;
+        std::vector<double> su(N, 0.0), sv(N, 0.0);
+        sfh::air_shapes(B.nth, c.der[b].radius, c.der[b].U, su.data(), sv.data());
+        std::vector<T> tu(su.begin(), su.end()), tv(sv.begin(), sv.end());
+        SF_HIP(hipMemcpy(c.au + B.off, tu.data(), N * sizeof(T), hipMemcpyHostToDevice));
+        SF_HIP(hipMemcpy(c.av + B.off, tv.data(), N * sizeof(T), hipMemcpyHostToDevice));
+    }
+}
+
+std::unique_ptr<Base> make_impl(int prec) {
+    switch (prec) {
+        case SF_PREC_FP64: return std::unique_ptr<Base>(new Impl<double, double>());
+        case SF_PREC_MIXED: return std::unique_ptr<Base>(new Impl<float, float>());
+        case SF_PREC_FP32_ADV: return std::unique_ptr<Base>(new Impl<float, double>());
+        default: return nullptr;
+    }
+}
+
+template <typename T, typename Ti>
+void finish_setup(Impl<T, Ti>& c) {
+    c.alloc_fields();
+    upload_air_shapes(c);
+}
+
+void setup(Base& c, int prec) {
+    const int nb = c.nb;
+    SF_HIP(hipSetDevice(c.device));
+    SF_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+    for (auto& e : c.ev) SF_HIP(hipEventCreate(&e));
+    hipDeviceProp_t prop;
+    SF_HIP(hipGetDeviceProperties(&prop, c.device));
+
+    // bubble descriptors, offsets, row/column table bases
+    long long off = 0;
+    int rt = 0, ct = 0, pr = 0, nph_max = 0;
+    c.bub_h.resize(nb);
+    c.der.resize(nb);
+    c.time_h.resize(nb);
+    for (int b = 0; b < nb; ++b) {
+        const sf_bubble_desc& d = c.desc[b];
+        derive(d, c.der[b]);
+        BubbleDev& B = c.bub_h[b];
+        B.nth = d.n_theta;
+        B.nph = 2 * d.n_theta;
+        B.mask = B.nph - 1;
+        B.rt0 = rt;
+        B.ct0 = ct;
+        B.pr0 = pr;
+        B.off = off;
+        B.h = kPi / d.n_theta;
+        B.invh = d.n_theta / kPi;
+        B.dt = c.der[b].dt;
+        B.M = c.der[b].M;
+        B.Cr = c.der[b].Cr;
+        B.gs = c.der[b].gs;
+        B.gamma_max = c.der[b].gamma_max;
+        B.eva = c.der[b].eva;
+        B.gravity = d.gravity;
+        B.evaporation = d.evaporation;
+        B.airflow = d.airflow;
+        off += (long long)B.nth * B.nph;
+        rt += B.nth + 1;
+        ct += B.nph;
+        pr += B.nth;
+        nph_max = std::max(nph_max, B.nph);
+        c.time_h[b] = BubbleTime{0.0, d.dt * 1.0, c.opt.advance_time, 0};
+    }
+    c.ncells = off;
+    c.nrow_total = pr;
+    c.stride = pr;
+
+    c.bub_d = dalloc<BubbleDev>(nb, c.owned);
+    SF_HIP(hipMemcpy(c.bub_d, c.bub_h.data(), nb * sizeof(BubbleDev), hipMemcpyHostToDevice));
+    c.time_d = dalloc<BubbleTime>(nb, c.owned);
+    SF_HIP(hipMemcpy(c.time_d, c.time_h.data(), nb * sizeof(BubbleTime), hipMemcpyHostToDevice));
+    c.ctl_d = dalloc<SolveCtl>(nb, c.owned);
+    c.hist_d = dalloc<SolveHist>(nb, c.owned);
+    build_tables(c, c.owned);
+
+    // cell tiles of 256 threads: one row chunk of 256, or several short rows
+    std::vector<int4> tiles;
+    for (int b = 0; b < nb; ++b) {
+        const BubbleDev& B = c.bub_h[b];
+        const int w = std::min(256, B.nph), rpt = 256 / w;
+        for (int j = 0; j < B.nth; j += rpt)
+            for (int i = 0; i < B.nph; i += w) tiles.push_back(make_int4(b, j, i, w));
+    }
+    std::vector<int2> rows;
+    for (int b = 0; b < nb; ++b)
+        for (int j = 0; j < c.bub_h[b].nth; ++j) rows.push_back(make_int2(b, j));
+    int4* tiles_d = dalloc<int4>(tiles.size(), c.owned);
+    SF_HIP(hipMemcpy(tiles_d, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice));
+    int2* rows_d = dalloc<int2>(rows.size(), c.owned);
+    SF_HIP(hipMemcpy(rows_d, rows.data(), rows.size() * sizeof(int2), hipMemcpyHostToDevice));
+    c.geom.tiles = tiles_d;
+    c.geom.ntiles = (int)tiles.size();
+    c.geom.rows = rows_d;
+    c.geom.nrows = (int)rows.size();
+    int nt = std::max(64, std::min(1024, nph_max / 4));
+    nt = (nt + 63) / 64 * 64;
+    int cpt = (nph_max + nt - 1) / nt;
+    int cp2 = 1;
+    while (cp2 < cpt) cp2 <<= 1;
+    if (cp2 > 8) throw Status{SF_ERR_UNSUPPORTED, "nPhi above 8192 is not supported"};
+    c.geom.row_threads = nt;
+    c.geom.cpt = cp2;
+    const size_t esz = c.fp64_inner() ? 8 : 4;
+    c.geom.lds_bytes = 3 * (size_t)nph_max * esz;
+    const size_t lds_limit = (size_t)prop.sharedMemPerBlock - 1024;
+    if (c.geom.lds_bytes > lds_limit) {
+        throw Status{SF_ERR_UNSUPPORTED,
+                     "line solve of nPhi = " + std::to_string(nph_max) + " needs " +
+                         std::to_string(c.geom.lds_bytes) + " B of LDS per block, device allows " +
+                         std::to_string(lds_limit) + " (use SF_PREC_MIXED for this size)"};
+    }
+
+    c.partR = dalloc<double>(4 * (size_t)c.stride, c.owned);
+    c.partA = dalloc<double>((size_t)c.stride, c.owned);
+    c.partB = dalloc<double>(2 * (size_t)c.stride, c.owned);
+    c.partS = dalloc<double>(2 * (size_t)c.stride, c.owned);
+    const size_t n = (size_t)c.ncells;
+    c.x64[0] = dalloc<double>(n, c.owned);
+    c.x64[1] = dalloc<double>(n, c.owned);
+    c.wN64 = dalloc<double>(n, c.owned);
+    c.wW64 = dalloc<double>(n, c.owned);
+    c.diag64 = dalloc<double>(n, c.owned);
+    c.b64 = dalloc<double>(n, c.owned);
+
+    // solver schedule
+    c.sp.tol = c.opt.tol > 0 ? c.opt.tol : 1e-10;
+    c.sp.absolute = c.opt.tol_absolute;
+    c.sp.inner_frac = 0.5;
+    if (c.fp64_inner()) {
+        c.sp.inner_rho2 = 0.0;
+        c.max_outer = c.opt.max_outer > 0 ? c.opt.max_outer : 3;
+        c.max_inner = c.opt.max_inner > 0 ? c.opt.max_inner : 1000;
+    } else {
+        const double rho = c.opt.inner_rho > 0 ? c.opt.inner_rho : 1e-4;
+        c.sp.inner_rho2 = rho * rho;
+        c.max_outer = c.opt.max_outer > 0 ? c.opt.max_outer : 4;
+        c.max_inner = c.opt.max_inner > 0 ? c.opt.max_inner : 200;
+    }
+
+    switch (prec) {
+        case SF_PREC_FP64: finish_setup(static_cast<Impl<double, double>&>(c)); break;
+        case SF_PREC_MIXED: finish_setup(static_cast<Impl<float, float>&>(c)); break;
+        default: finish_setup(static_cast<Impl<float, double>&>(c)); break;
+    }
+    SF_HIP(hipStreamSynchronize(c.stream));
+}
+
+}  // namespace
+
+struct sf_ctx {
+    std::unique_ptr<Base> impl;
+    int prec = 0;
+};
+
+namespace {
+
+int fail(sf_ctx* ctx, int code, const std::string& msg) {
+    if (ctx && ctx->impl) ctx->impl->err = msg;
+    g_last_error = msg;
+    return code;
+}
+
+template <typename F>
+int guarded(sf_ctx* ctx, F&& f) {
+    try {
+        return f();
+    } catch (const HipError& e) {
+        return fail(ctx, SF_ERR_HIP, e.msg);
+    } catch (const Status& s) {
+        return fail(ctx, s.code, s.msg);
+    } catch (const std::exception& e) {
+        return fail(ctx, SF_ERR_INVALID, e.what());
+    }
+}
+
+void time_stage(Base& c, int e0, int e1, double& acc) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, c.ev[e0], c.ev[e1]) == hipSuccess) acc += ms;
+}
+
+// One eager step with per-stage events.
+void eager_step(Base& c, bool host_driven) {
+    SF_HIP(hipEventRecord(c.ev[0], c.stream));
+    c.advect();
+    SF_HIP(hipEventRecord(c.ev[1], c.stream));
+    c.assemble();
+    SF_HIP(hipEventRecord(c.ev[2], c.stream));
+    c.solve(host_driven);
+    SF_HIP(hipEventRecord(c.ev[3], c.stream));
+    c.update();
+    SF_HIP(hipEventRecord(c.ev[4], c.stream));
+    launch_advance_time(c.stream, c.time_d, c.nb);
+    SF_HIP(hipEventSynchronize(c.ev[4]));
+    time_stage(c, 0, 1, c.timing.advect_ms);
+    time_stage(c, 1, 2, c.timing.assemble_ms);
+    time_stage(c, 2, 3, c.timing.solve_ms);
+    time_stage(c, 3, 4, c.timing.update_ms);
+    time_stage(c, 0, 4, c.timing.step_ms);
+    c.timing.steps += 1;
+}
+
+void build_graph(Base& c) {
+    hipGraph_t g;
+    SF_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+    c.advect();
+    c.assemble();
+    c.solve(false);
+    c.update();
+    launch_advance_time(c.stream, c.time_d, c.nb);
+    SF_HIP(hipStreamEndCapture(c.stream, &g));
+    hipError_t e = hipGraphInstantiate(&c.graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    SF_HIP(e);
+}
+
+void check_index(sf_ctx* ctx, int b) {
+    if (!ctx || !ctx->impl) throw Status{SF_ERR_INVALID, "null context"};
+    if (b < 0 || b >= ctx->impl->nb) throw Status{SF_ERR_INVALID, "bubble index out of range"};
+}
+
+}  // namespace
+
+extern "C" {
+
+int sf_derive_params(const sf_bubble_desc* desc, sf_derived* out) {
+    if (!desc || !out) return fail(nullptr, SF_ERR_INVALID, "null argument");
+    sfh::derive_params(*desc, *out);
+    return SF_OK;
+}
+
+int sf_create(const sf_options* opt, int32_t nbubbles, const sf_bubble_desc* bubbles,
+              sf_ctx** out) {
+    if (!out) return fail(nullptr, SF_ERR_INVALID, "null output pointer");
+    *out = nullptr;
+    if (!opt || !bubbles || nbubbles <= 0) return fail(nullptr, SF_ERR_INVALID, "no bubbles");
+    for (int b = 0; b < nbubbles; ++b) {
+        const sf_bubble_desc& d = bubbles[b];
+        if (d.n_theta < 4 || !is_pow2(d.n_theta))
+            return fail(nullptr, SF_ERR_UNSUPPORTED,
+                        "n_theta must be a power of two >= 4 (bubble " + std::to_string(b) + ")");
+        if (!(d.radius > 0) || !(d.H > 0) || !(d.U > 0) || !(d.Gamma_m > 0) || !(d.T > 0) ||
+            !(d.dt > 0) || !(d.M_scale > 0))
+            return fail(nullptr, SF_ERR_INVALID, "non-positive physical parameter");
+        if (d.Gamma_m != bubbles[0].Gamma_m || d.T != bubbles[0].T)
+            return fail(nullptr, SF_ERR_UNSUPPORTED, "bubbles of one context share Gamma_m and T");
+    }
+    if (opt->precision < 0 || opt->precision > 2) return fail(nullptr, SF_ERR_INVALID, "bad precision");
+    auto ctx = std::unique_ptr<sf_ctx>(new sf_ctx());
+    ctx->prec = opt->precision;
+    ctx->impl = make_impl(opt->precision);
+    Base& c = *ctx->impl;
+    c.opt = *opt;
+    c.device = opt->device;
+    c.nb = nbubbles;
+    c.desc.assign(bubbles, bubbles + nbubbles);
+    int rc = guarded(nullptr, [&] {
+        setup(c, opt->precision);
+        return (int)SF_OK;
+    });
+    if (rc != SF_OK) return rc;
+    *out = ctx.release();
+    return SF_OK;
+}
+
+int sf_destroy(sf_ctx* ctx) {
+    if (!ctx) return SF_OK;
+    delete ctx;
+    return SF_OK;
+}
+
+int sf_upload_field(sf_ctx* ctx, int32_t b, int32_t field, const double* host) {
+    return guarded(ctx, [&] {
+        check_index(ctx, b);
+        if (!host) throw Status{SF_ERR_INVALID, "null host buffer"};
+        ctx->impl->upload(b, field, host);
+        return (int)SF_OK;
+    });
+}
+
+int sf_download_field(sf_ctx* ctx, int32_t b, int32_t field, double* host) {
+    return guarded(ctx, [&] {
+        check_index(ctx, b);
+        if (!host) throw Status{SF_ERR_INVALID, "null host buffer"};
+        ctx->impl->download(b, field, host);
+        return (int)SF_OK;
+    });
+}
+
+int sf_set_time(sf_ctx* ctx, int32_t b, double t) {
+    return guarded(ctx, [&] {
+        if (!ctx || !ctx->impl) throw Status{SF_ERR_INVALID, "null context"};
+        Base& c = *ctx->impl;
+        SF_HIP(hipStreamSynchronize(c.stream));
+        SF_HIP(hipMemcpy(c.time_h.data(), c.time_d, c.nb * sizeof(BubbleTime), hipMemcpyDeviceToHost));
+        for (int k = 0; k < c.nb; ++k)
+            if (b < 0 || b == k) c.time_h[k].t = t;
+        if (b >= c.nb) throw Status{SF_ERR_INVALID, "bubble index out of range"};
+        SF_HIP(hipMemcpy(c.time_d, c.time_h.data(), c.nb * sizeof(BubbleTime), hipMemcpyHostToDevice));
+        return (int)SF_OK;
+    });
+}
+
+int sf_set_dt(sf_ctx* ctx, int32_t b, double dt_seconds) {
+    return guarded(ctx, [&] {
+        if (!ctx || !ctx->impl) throw Status{SF_ERR_INVALID, "null context"};
+        if (!(dt_seconds > 0)) throw Status{SF_ERR_INVALID, "dt must be positive"};
+        Base& c = *ctx->impl;
+        if (b >= c.nb) throw Status{SF_ERR_INVALID, "bubble index out of range"};
+        SF_HIP(hipStreamSynchronize(c.stream));
+        SF_HIP(hipMemcpy(c.time_h.data(), c.time_d, c.nb * sizeof(BubbleTime), hipMemcpyDeviceToHost));
+        for (int k = 0; k < c.nb; ++k) {
+            if (b >= 0 && b != k) continue;
+            // timeStepGlobal = dt U / radius with the member U = 1 (bubble.cu:2675)
+            c.bub_h[k].dt = dt_seconds * 1.0 / c.der[k].radius;
+            c.time_h[k].dt_s = dt_seconds;
+        }
+        SF_HIP(hipMemcpy(c.bub_d, c.bub_h.data(), c.nb * sizeof(BubbleDev), hipMemcpyHostToDevice));
+        SF_HIP(hipMemcpy(c.time_d, c.time_h.data(), c.nb * sizeof(BubbleTime), hipMemcpyHostToDevice));
+        return (int)SF_OK;
+    });
+}
+
+int sf_step(sf_ctx* ctx, int32_t nsteps) {
+    return guarded(ctx, [&] {
+        if (!ctx || !ctx->impl) throw Status{SF_ERR_INVALID, "null context"};
+        if (nsteps < 0) throw Status{SF_ERR_INVALID, "negative step count"};
+        Base& c = *ctx->impl;
+        if (c.stage != 0) throw Status{SF_ERR_ORDER, "a stage-level step is in progress"};
+        const bool want_graph = c.opt.use_graph && !c.fp64_inner();
+        int k = 0;
+        if (want_graph && !c.calibrated) {
+            // calibration: two host-driven steps fix the refinement schedule
+            for (; k < nsteps && k < 2; ++k) eager_step(c, true);
+            if (k == 2 || k == nsteps) {
+                if (c.opt.max_inner == 0 && c.calib_inner > 0) {
+                    c.max_inner = c.calib_inner + 2;
+                    c.max_outer = std::max(c.calib_outer, 1) + 1;
+                }
+                if (k == 2) {
+                    c.calibrated = true;
+                    build_graph(c);
+                }
+            }
+        }
+        if (want_graph && c.graph) {
+            if (k < nsteps) {
+                SF_HIP(hipEventRecord(c.ev[0], c.stream));
+                for (; k < nsteps; ++k) SF_HIP(hipGraphLaunch(c.graph, c.stream));
+                SF_HIP(hipEventRecord(c.ev[5], c.stream));
+                SF_HIP(hipEventSynchronize(c.ev[5]));
+                float ms = 0.f;
+                SF_HIP(hipEventElapsedTime(&ms, c.ev[0], c.ev[5]));
+                c.timing.step_ms += ms;
+            }
+        }
+        for (; k < nsteps; ++k) eager_step(c, true);
+        c.steps_done += nsteps;
+        return (int)SF_OK;
+    });
+}
+
+int sf_advect(sf_ctx* ctx) {
+    return guarded(ctx, [&] {
+        if (!ctx || !ctx->impl) throw Status{SF_ERR_INVALID, "null context"};
+        Base& c = *ctx->impl;
+        if (c.stage != 0) throw Status{SF_ERR_ORDER, "sf_advect must start a step"};
+        c.advect();
+        c.stage = 1;
+        return (int)SF_OK;
+    });
+}
+
+int sf_assemble(sf_ctx* ctx) {
+    return guarded(ctx, [&] {
+        if (!ctx || !ctx->impl) throw Status{SF_ERR_INVALID, "null context"};
+        Base& c = *ctx->impl;
+        if (c.stage != 1) throw Status{SF_ERR_ORDER, "sf_assemble must follow sf_advect"};
+        c.assemble();
+        c.stage = 2;
+        return (int)SF_OK;
+    });
+}
+
+int sf_solve(sf_ctx* ctx) {
+    return guarded(ctx, [&] {
+        if (!ctx || !ctx->impl) throw Status{SF_ERR_INVALID, "null context"};
+        Base& c = *ctx->impl;
+        if (c.stage != 2) throw Status{SF_ERR_ORDER, "sf_solve must follow sf_assemble"};
+        c.solve(true);
+        c.stage = 3;
+        SF_HIP(hipStreamSynchronize(c.stream));
+        std::vector<SolveCtl> h(c.nb);
+        SF_HIP(hipMemcpy(h.data(), c.ctl_d, c.nb * sizeof(SolveCtl), hipMemcpyDeviceToHost));
+        for (auto& s : h)
+            if (!s.converged) throw Status{SF_ERR_NOT_CONVERGED, "solve missed its tolerance"};
+        return (int)SF_OK;
+    });
+}
+
+int sf_update(sf_ctx* ctx) {
+    return guarded(ctx, [&] {
+        if (!ctx || !ctx->impl) throw Status{SF_ERR_INVALID, "null context"};
+        Base& c = *ctx->impl;
+        if (c.stage != 3) throw Status{SF_ERR_ORDER, "sf_update must follow sf_solve"};
+        c.update();
+        launch_advance_time(c.stream, c.time_d, c.nb);
+        c.stage = 0;
+        c.steps_done += 1;
+        SF_HIP(hipStreamSynchronize(c.stream));
+        return (int)SF_OK;
+    });
+}
+
+int sf_synchronize(sf_ctx* ctx) {
+    return guarded(ctx, [&] {
+        if (!ctx || !ctx->impl) throw Status{SF_ERR_INVALID, "null context"};
+        SF_HIP(hipStreamSynchronize(ctx->impl->stream));
+        return (int)SF_OK;
+    });
+}
+
+int sf_get_stats(sf_ctx* ctx, int32_t b, sf_bubble_stats* out) {
+    return guarded(ctx, [&] {
+        check_index(ctx, b);
+        if (!out) throw Status{SF_ERR_INVALID, "null output"};
+        Base& c = *ctx->impl;
+        c.row_stats();
+        const BubbleDev& B = c.bub_h[b];
+        std::vector<double> ps(2 * (size_t)c.stride);
+        SF_HIP(hipMemcpyAsync(ps.data(), c.partS, ps.size() * sizeof(double), hipMemcpyDeviceToHost,
+                              c.stream));
+        SolveCtl sc;
+        SF_HIP(hipMemcpyAsync(&sc, c.ctl_d + b, sizeof(sc), hipMemcpyDeviceToHost, c.stream));
+        SolveHist hs;
+        SF_HIP(hipMemcpyAsync(&hs, c.hist_d + b, sizeof(hs), hipMemcpyDeviceToHost, c.stream));
+        BubbleTime bt;
+        SF_HIP(hipMemcpyAsync(&bt, c.time_d + b, sizeof(bt), hipMemcpyDeviceToHost, c.stream));
+        SF_HIP(hipStreamSynchronize(c.stream));
+        double vol = 0.0, pos = 0.0;
+        for (int j = 0; j < B.nth; ++j) {
+            vol += ps[B.pr0 + j];
+            pos += ps[c.stride + B.pr0 + j];
+        }
+        std::memset(out, 0, sizeof(*out));
+        out->iterations = sc.iters;
+        out->outer = sc.outers;
+        out->converged = sc.converged;
+        out->residual = sc.rel_res;
+        out->volume = vol * B.h * B.h;
+        out->broken = (pos == 0.0) || (c.steps_done > 0 && !sc.converged);
+        out->steps = c.steps_done;
+        out->time = bt.t;
+        out->failed_solves = hs.fails;
+        out->max_residual = hs.max_res;
+        return (int)SF_OK;
+    });
+}
+
+int sf_get_timing(sf_ctx* ctx, sf_timing* out) {
+    if (!ctx || !ctx->impl || !out) return fail(ctx, SF_ERR_INVALID, "null argument");
+    *out = ctx->impl->timing;
+    return SF_OK;
+}
+
+int sf_reset_timing(sf_ctx* ctx) {
+    if (!ctx || !ctx->impl) return fail(ctx, SF_ERR_INVALID, "null argument");
+    std::memset(&ctx->impl->timing, 0, sizeof(sf_timing));
+    return SF_OK;
+}
+
+const char* sf_last_error(const sf_ctx* ctx) {
+    if (ctx && ctx->impl) return ctx->impl->err.c_str();
+    return g_last_error.c_str();
+}
+
+int sf_write_thickness_exr(sf_ctx* ctx, int32_t b, const char* path) {
+    return guarded(ctx, [&] {
+        check_index(ctx, b);
+        if (!path) throw Status{SF_ERR_INVALID, "null path"};
+        Base& c = *ctx->impl;
+        const BubbleDev& B = c.bub_h[b];
+        std::vector<double> eta((size_t)B.nth * B.nph);
+        c.download(b, SF_FIELD_ETA, eta.data());
+        if (!sfh::write_thickness_exr(path, B.nth, B.nph, eta.data(), c.desc[b].H))
+            throw Status{SF_ERR_INVALID, std::string("cannot write ") + path};
+        return (int)SF_OK;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,453 @@
+// Device-side data structures and helpers of the soap-film step for gfx950.
+//
+// Layout in HBM (packed over all bubbles of a context):
+//   every cell field is one array; bubble b owns cells [off, off + nth*nph)
+//   row-major by theta with phi contiguous, as the reference stores them
+//   (quantity.cu:20-29). u_theta uses rows 0..nth-2 of its block; row nth-1
+//   is padding that stays zero.
+//   Per-row tables (nth+1 slots per bubble, base rt0) and per-column tables
+//   (nph slots, base ct0) hold sin/cos of the cell coordinates in FP64 and
+//   FP32, so no kernel evaluates the trigonometry of its own grid point.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sfk {
+
+struct BubbleDev {
+    int nth, nph, mask;          // nph = 2 nth, mask = nph - 1 (nph power of two)
+    int rt0, ct0, pr0;           // row-table, column-table and partial-sum bases
+    long long off;               // first cell of this bubble in every cell field
+    double h, invh, dt;          // grid spacing, 1/h, non-dimensional step
+    double M, Cr, gs, gamma_max, eva;
+    int gravity, evaporation, airflow, pad_;
+};
+
+// Air-flow time of one bubble (seconds) and its step in seconds; kernels
+// derive the corridor factor 3 - |t - 3| and the mirror blend t/12.
+struct BubbleTime {
+    double t;
+    double dt_s;
+    int advance;
+    int pad_;
+};
+
+// Per-bubble control block of the iterative solve. Every field is written by
+// one kernel and read only by later kernels of the same stream.
+struct SolveCtl {
+    double rz[2];        // r.z of inner iterations, slot = iteration parity
+    double rr_in0;       // |r_in|^2 at the start of the current inner solve
+    double rel_res;      // last relative (or absolute) residual, FP64
+    double bb;           // |b|^2
+    int outer_done;      // 1 once the FP64 residual met the tolerance
+    int inner_done;      // 1 once the inner solve met its reduction
+    int iters;           // inner iterations executed
+    int outers;          // FP64 residual evaluations
+    int xbuf;            // which x buffer holds the current iterate
+    int converged;       // final verdict written by the finalize kernel
+};
+
+// Per-bubble history of the solves since creation (written by k_finalize).
+struct SolveHist {
+    int fails;
+    int solves;
+    double max_res;
+};
+
+struct Tables {
+    // per-row, FP64: centres (j+1/2)h, faces j h (advection positions),
+    // face sines of the assembly (pole faces exactly 0), csc of centres
+    const double *sinTc, *cosTc, *sinTf, *cosTf, *sinFace, *cscTc;
+    // per-row, FP32 copies
+    const float *sinTc32, *cosTc32, *sinTf32, *cosTf32;
+    // per-column: phi = i h and phi = (i - 1/2) h
+    const double *sinPc, *cosPc, *sinPh, *cosPh;
+    const float *sinPc32, *cosPc32, *sinPh32, *cosPh32;
+};
+
+template <typename T> struct TabSel;
+template <> struct TabSel<double> {
+    static __device__ __forceinline__ const double* sTc(const Tables& t) { return t.sinTc; }
+    static __device__ __forceinline__ const double* cTc(const Tables& t) { return t.cosTc; }
+    static __device__ __forceinline__ const double* sTf(const Tables& t) { return t.sinTf; }
+    static __device__ __forceinline__ const double* cTf(const Tables& t) { return t.cosTf; }
+    static __device__ __forceinline__ const double* sPc(const Tables& t) { return t.sinPc; }
+    static __device__ __forceinline__ const double* cPc(const Tables& t) { return t.cosPc; }
+    static __device__ __forceinline__ const double* sPh(const Tables& t) { return t.sinPh; }
+    static __device__ __forceinline__ const double* cPh(const Tables& t) { return t.cosPh; }
+};
+template <> struct TabSel<float> {
+    static __device__ __forceinline__ const float* sTc(const Tables& t) { return t.sinTc32; }
+    static __device__ __forceinline__ const float* cTc(const Tables& t) { return t.cosTc32; }
+    static __device__ __forceinline__ const float* sTf(const Tables& t) { return t.sinTf32; }
+    static __device__ __forceinline__ const float* cTf(const Tables& t) { return t.cosTf32; }
+    static __device__ __forceinline__ const float* sPc(const Tables& t) { return t.sinPc32; }
+    static __device__ __forceinline__ const float* cPc(const Tables& t) { return t.cosPc32; }
+    static __device__ __forceinline__ const float* sPh(const Tables& t) { return t.sinPh32; }
+    static __device__ __forceinline__ const float* cPh(const Tables& t) { return t.cosPh32; }
+};
+
+// ---------------------------------------------------------------- math ----
+__device__ __forceinline__ float tfloor(float x) { return floorf(x); }
+__device__ __forceinline__ double tfloor(double x) { return floor(x); }
+__device__ __forceinline__ float tsqrt(float x) { return sqrtf(x); }
+__device__ __forceinline__ double tsqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ float tatan2(float y, float x) { return atan2f(y, x); }
+__device__ __forceinline__ double tatan2(double y, double x) { return atan2(y, x); }
+__device__ __forceinline__ bool tisnan(float x) { return __builtin_isnan(x); }
+__device__ __forceinline__ bool tisnan(double x) { return __builtin_isnan(x); }
+
+// sin/cos of the great-circle step. The step is a small angle (|u| dt), so
+// FP32 uses the Taylor series to x^7 / x^6 below 1/16 rad (truncation below
+// 2e-16 / 6e-15, under FP32 rounding); FP64 and large steps use sincos.
+__device__ __forceinline__ void sincos_step(float x, float* s, float* c) {
+    if (fabsf(x) < 0.0625f) {
+        float x2 = x * x;
+        *s = x * (1.0f + x2 * (-1.0f / 6.0f + x2 * (1.0f / 120.0f + x2 * (-1.0f / 5040.0f))));
+        *c = 1.0f + x2 * (-0.5f + x2 * (1.0f / 24.0f + x2 * (-1.0f / 720.0f)));
+    } else {
+        sincosf(x, s, c);
+    }
+}
+__device__ __forceinline__ void sincos_step(double x, double* s, double* c) { sincos(x, s, c); }
+
+// bubbleLerp (bubble.cu:153-160): NaN operands are skipped.
+template <typename T>
+__device__ __forceinline__ T lerp_nan(T from, T to, T alpha) {
+    if (tisnan(from)) return to;
+    if (tisnan(to)) return from;
+    return (T(1) - alpha) * from + alpha * to;
+}
+
+// validateCoord (bubble.cu:116-137). The phi wrap fmod(y + nph, nph) is done
+// by exact conditional subtraction (y + nph lies in [0, 3 nph) for every
+// point the trace produces), which gives the same value as fmod.
+template <typename T>
+__device__ __forceinline__ bool validate_coord(T& x, T& y, int nth, int nph) {
+    bool ret = false;
+    const T tn = (T)nth, tp = (T)nph;
+    if (x >= tn) { x = tp - x; y += tn; ret = !ret; }
+    if (x < T(0)) { x = -x; y += tn; ret = !ret; }
+    T a = y + tp;
+    if (a >= T(2) * tp) a -= T(2) * tp;
+    else if (a >= tp) a -= tp;
+    y = a;
+    return ret;
+}
+
+template <typename T>
+struct Field {
+    const T* __restrict__ p;   // bubble base
+    int nph, mask, maxrow;
+    __device__ __forceinline__ T at(int row, int col) const {
+        row = row < 0 ? 0 : (row > maxrow ? maxrow : row);
+        return p[(size_t)row * nph + (col & mask)];
+    }
+    __device__ __forceinline__ T belt(int row, int lo, T a) const {
+        return lerp_nan(at(row, lo), at(row, lo + 1), a);
+    }
+};
+
+// Split a sample position into indices and weights; the phi index is
+// clamped into range so a NaN position cannot address memory outside the
+// field.
+template <typename T>
+__device__ __forceinline__ void split_index(T x, T y, int nph, int& tI, int& pI, T& aT, T& aP) {
+    T fy = tfloor(y), fx = tfloor(x);
+    aP = y - fy;
+    aT = x - fx;
+    pI = (int)fy;
+    tI = (int)fx;
+    if ((unsigned)pI >= (unsigned)nph) pI = pI < 0 ? 0 : nph - 1;
+}
+
+// sampleCentered (bubble.cu:320-376)
+template <typename T>
+__device__ T sample_centered(const Field<T>& f, int nth, T rx, T ry) {
+    T x = rx - T(0.5), y = ry;
+    const int nph = f.nph;
+    bool fl = validate_coord(x, y, nth, nph);
+    int tI, pI; T aT, aP;
+    split_index(x, y, nph, tI, pI, aT, aP);
+    if (tI == 0 && fl) {
+        T higher = f.belt(0, pI, aP);
+        T lower = f.belt(0, pI + nph / 2, aP);
+        return lerp_nan(lower, higher, aT);
+    }
+    if (fl) tI -= 1;
+    if (tI == nth - 1) {
+        T lower = f.belt(tI, pI, aP);
+        T higher = f.belt(tI, pI + nph / 2, aP);
+        return lerp_nan(lower, higher, aT);
+    }
+    T lower = f.belt(tI, pI, aP);
+    T higher = f.belt(tI + 1, pI, aP);
+    return lerp_nan(lower, higher, aT);
+}
+
+// sampleVPhi (bubble.cu:178-240): values taken across the pole flip sign.
+template <typename T>
+__device__ T sample_vphi(const Field<T>& f, int nth, T rx, T ry) {
+    T x = rx - T(0.5), y = ry + T(0.5);
+    const int nph = f.nph;
+    bool fl = validate_coord(x, y, nth, nph);
+    int tI, pI; T aT, aP;
+    split_index(x, y, nph, tI, pI, aT, aP);
+    if (tI == 0 && fl) {
+        T higher = -f.belt(0, pI, aP);
+        T lower = f.belt(0, pI + nph / 2, aP);
+        return lerp_nan(lower, higher, aT);
+    }
+    if (fl) tI -= 1;
+    if (tI == nth - 1) {
+        T lower = f.belt(tI, pI, aP);
+        T higher = -f.belt(tI, pI + nph / 2, aP);
+        T v = lerp_nan(lower, higher, aT);
+        return fl ? -v : v;
+    }
+    T lower = f.belt(tI, pI, aP);
+    T higher = f.belt(tI + 1, pI, aP);
+    T v = lerp_nan(lower, higher, aT);
+    return fl ? -v : v;
+}
+
+// sampleVTheta (bubble.cu:247-313): u_theta has nth-1 rows; the polar caps
+// interpolate between the first/last row and its phi+pi mirror with half
+// weight. A sample exactly at theta = pi (which the reference reads past
+// the array for) takes the south-cap branch.
+template <typename T>
+__device__ T sample_vtheta(const Field<T>& f, int nth, T rx, T ry) {
+    T x = rx - T(1), y = ry;
+    const int nph = f.nph;
+    bool fl = validate_coord(x, y, nth, nph);
+    int tI, pI; T aT, aP;
+    split_index(x, y, nph, tI, pI, aT, aP);
+    const T tn = (T)nth;
+    if ((rx < T(0) && rx > T(-1)) || (rx > tn && rx < tn + T(1))) {
+        tI -= 1;
+        aT += T(1);
+    } else if (rx >= tn + T(1) || rx <= T(-1)) {
+        tI -= 2;
+    }
+    if (tI == 0 && fl && rx > T(-1)) {
+        T higher = -f.belt(0, pI, aP);
+        T lower = f.belt(0, pI + nph / 2, aP);
+        return lerp_nan(lower, higher, T(0.5) * aT);
+    }
+    if (tI >= nth - 1 && !fl) { tI = nth - 2; aT = T(1); }
+    if (tI == nth - 2) {
+        T lower = f.belt(tI, pI, aP);
+        T higher = -f.belt(tI, pI + nph / 2, aP);
+        T v = lerp_nan(lower, higher, T(0.5) * aT);
+        return fl ? -v : v;
+    }
+    T lower = f.belt(tI, pI, aP);
+    T higher = f.belt(tI + 1, pI, aP);
+    T v = lerp_nan(lower, higher, aT);
+    return fl ? -v : v;
+}
+
+// ------------------------------------------------------ great circle ----
+// The start frame (w, u_, v_) and speed after the transported midpoint
+// velocity, i.e. the first two stages shared by traceGreatCircle
+// (bubble.cu:416-496) and the vector advection kernels (bubble.cu:651-731,
+// 787-867). Positions on the sphere are kept Cartesian: the sin/cos of a
+// traced point's theta and phi are its z, rho, x/rho and y/rho, which is
+// the reference's cos(acos(z)), sin(acos(z)), cos(atan2(y,x)), sin(atan2(y,x))
+// without the round trip, and theta = atan2(rho, z) stays accurate at the
+// poles in FP32 where acos(z) does not.
+template <typename T>
+struct Circle {
+    T w0, w1, w2, u0, u1, u2, v0, v1, v2, unorm;
+};
+
+template <typename T>
+struct VelCtx {
+    Field<T> uth, uph;
+    int nth;
+    T invh, dt;
+};
+
+template <typename T>
+__device__ __forceinline__ void cart_to_index(T x, T y, T z, T invh, T& ti, T& pi_, T& rho) {
+    rho = tsqrt(x * x + y * y);
+    ti = tatan2(rho, z) * invh;
+    pi_ = tatan2(y, x) * invh;
+}
+
+template <typename T>
+__device__ __forceinline__ void tangent_vec(T x, T y, T z, T rho, T vT, T vP, T& o0, T& o1, T& o2) {
+    T cp = T(1), sp = T(0);
+    if (rho > T(0)) { T ir = T(1) / rho; cp = x * ir; sp = y * ir; }
+    o0 = vT * z * cp - vP * sp;
+    o1 = vT * z * sp + vP * cp;
+    o2 = -vT * rho;
+}
+
+template <typename T>
+__device__ __forceinline__ bool circle_mid(const VelCtx<T>& c, T sinT, T cosT, T sinP, T cosP,
+                                           T guT, T guP, Circle<T>& o) {
+    T u0 = guT * (cosT * cosP) - guP * sinP;
+    T u1 = guT * (cosT * sinP) + guP * cosP;
+    T u2 = -guT * sinT;
+    o.w0 = cosP * sinT; o.w1 = sinP * sinT; o.w2 = cosT;
+    o.unorm = tsqrt(u0 * u0 + u1 * u1 + u2 * u2);
+    if (!(o.unorm > T(0))) return false;
+    T inv = T(1) / o.unorm;
+    o.u0 = u0 * inv; o.u1 = u1 * inv; o.u2 = u2 * inv;
+    T s, co;
+    sincos_step(T(-0.5) * o.unorm * c.dt, &s, &co);
+    T mx = o.u0 * s + o.w0 * co, my = o.u1 * s + o.w1 * co, mz = o.u2 * s + o.w2 * co;
+    T mti, mpi, rho;
+    cart_to_index(mx, my, mz, c.invh, mti, mpi, rho);
+    T muT = sample_vtheta(c.uth, c.nth, mti, mpi);
+    T muP = sample_vphi(c.uph, c.nth, mti, mpi);
+    T m0, m1, m2;
+    tangent_vec(mx, my, mz, rho, muT, muP, m0, m1, m2);
+    T uc0 = o.u0 * co - o.w0 * s, uc1 = o.u1 * co - o.w1 * s, uc2 = o.u2 * co - o.w2 * s;
+    T vc0 = my * uc2 - mz * uc1, vc1 = mz * uc0 - mx * uc2, vc2 = mx * uc1 - my * uc0;
+    T mguT = m0 * vc0 + m1 * vc1 + m2 * vc2;
+    T mguP = m0 * uc0 + m1 * uc1 + m2 * uc2;
+    // v_ of the start frame = w x u_
+    T v0 = o.w1 * o.u2 - o.w2 * o.u1, v1 = o.w2 * o.u0 - o.w0 * o.u2, v2 = o.w0 * o.u1 - o.w1 * o.u0;
+    u0 = mguP * o.u0 + mguT * v0;
+    u1 = mguP * o.u1 + mguT * v1;
+    u2 = mguP * o.u2 + mguT * v2;
+    o.unorm = tsqrt(u0 * u0 + u1 * u1 + u2 * u2);
+    if (!(o.unorm > T(0))) return false;
+    inv = T(1) / o.unorm;
+    o.u0 = u0 * inv; o.u1 = u1 * inv; o.u2 = u2 * inv;
+    o.v0 = o.w1 * o.u2 - o.w2 * o.u1;
+    o.v1 = o.w2 * o.u0 - o.w0 * o.u2;
+    o.v2 = o.w0 * o.u1 - o.w1 * o.u0;
+    return true;
+}
+
+// Departure point of the full step: returns its index-space coordinates and
+// the Cartesian point (bubble.cu:498-505, 734-743).
+template <typename T>
+__device__ __forceinline__ void circle_depart(const VelCtx<T>& c, const Circle<T>& o, T& s, T& co,
+                                              T& px, T& py, T& pz, T& ti, T& pi_, T& rho) {
+    sincos_step(-o.unorm * c.dt, &s, &co);
+    px = o.u0 * s + o.w0 * co;
+    py = o.u1 * s + o.w1 * co;
+    pz = o.u2 * s + o.w2 * co;
+    cart_to_index(px, py, pz, c.invh, ti, pi_, rho);
+}
+
+// Velocity at the departure point transported back along the circle into
+// the start frame (bubble.cu:745-766, 881-902).
+template <typename T>
+__device__ __forceinline__ void transport_back(const VelCtx<T>& c, const Circle<T>& o, T& r0, T& r1,
+                                               T& r2) {
+    T s, co, px, py, pz, ti, pi_, rho;
+    circle_depart(c, o, s, co, px, py, pz, ti, pi_, rho);
+    T puT = sample_vtheta(c.uth, c.nth, ti, pi_);
+    T puP = sample_vphi(c.uph, c.nth, ti, pi_);
+    T p0, p1, p2;
+    tangent_vec(px, py, pz, rho, puT, puP, p0, p1, p2);
+    T uc0 = o.u0 * co - o.w0 * s, uc1 = o.u1 * co - o.w1 * s, uc2 = o.u2 * co - o.w2 * s;
+    T vc0 = py * uc2 - pz * uc1, vc1 = pz * uc0 - px * uc2, vc2 = px * uc1 - py * uc0;
+    T a = p0 * vc0 + p1 * vc1 + p2 * vc2;
+    T b = p0 * uc0 + p1 * uc1 + p2 * uc2;
+    r0 = b * o.u0 + a * o.v0;
+    r1 = b * o.u1 + a * o.v1;
+    r2 = b * o.u2 + a * o.v2;
+}
+
+// -------------------------------------------------------- reductions ----
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;
+}
+
+// Deterministic block sum of NV values; every thread receives the sums.
+// `red` must hold 16 * NV doubles of LDS.
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double* red) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
+    __syncthreads();
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) red[k * 16 + wid] = v[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        double s = 0.0;
+        for (int w = 0; w < nw; ++w) s += red[k * 16 + w];
+        v[k] = s;
+    }
+    __syncthreads();
+}
+
+// Sum the per-row partials [base, base+n) of NV interleaved slots
+// (layout part[slot * stride + row]).
+template <int NV>
+__device__ __forceinline__ void sum_partials(const double* __restrict__ part, long stride, int base,
+                                             int n, double (&out)[NV], double* red) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) out[k] = 0.0;
+    for (int r = threadIdx.x; r < n; r += blockDim.x) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) out[k] += part[k * stride + base + r];
+    }
+    block_sum<NV>(out, red);
+}
+
+// --------------------------------------------- periodic line solve ----
+// Parallel cyclic reduction of one periodic tridiagonal row held in LDS,
+// normalised to unit diagonal: x_i + a_i x_{i-1} + c_i x_{i+1} = d_i.
+// n is a power of two >= 4. After log2(n)-1 reductions the stride reaches
+// n/2, where x_{i-n/2} and x_{i+n/2} are one unknown and the rows pair up
+// into 2x2 systems. On exit sd holds x. CPT bounds the cells per thread.
+template <typename Ti, int CPT>
+__device__ __forceinline__ void pcr_periodic(Ti* sa, Ti* sc, Ti* sd, int n) {
+    const int tid = threadIdx.x, nt = blockDim.x, m = n - 1;
+    for (int s = 1; s < n / 2; s <<= 1) {
+        Ti na[CPT], nc[CPT], nd[CPT];
+#pragma unroll
+        for (int k = 0; k < CPT; ++k) {
+            int i = tid + k * nt;
+            if (i < n) {
+                int im = (i - s) & m, ip = (i + s) & m;
+                Ti ai = sa[i], ci = sc[i], di = sd[i];
+                Ti am = sa[im], cm = sc[im], dm = sd[im];
+                Ti ap = sa[ip], cp = sc[ip], dp = sd[ip];
+                Ti iD = Ti(1) / (Ti(1) - ai * cm - ci * ap);
+                na[k] = -ai * am * iD;
+                nc[k] = -ci * cp * iD;
+                nd[k] = (di - ai * dm - ci * dp) * iD;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < CPT; ++k) {
+            int i = tid + k * nt;
+            if (i < n) { sa[i] = na[k]; sc[i] = nc[k]; sd[i] = nd[k]; }
+        }
+        __syncthreads();
+    }
+    Ti xs[CPT];
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+        int i = tid + k * nt;
+        if (i < n) {
+            int j = (i + n / 2) & m;
+            Ti ei = sa[i] + sc[i], ej = sa[j] + sc[j];
+            xs[k] = (sd[i] - ei * sd[j]) / (Ti(1) - ei * ej);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+        int i = tid + k * nt;
+        if (i < n) sd[i] = xs[k];
+    }
+    __syncthreads();
+}
+
+}  // namespace sfk

@@ -1,0 +1,484 @@
+// Host-only helpers: parameter derivation, air-flow corridor shape, seeded
+// synthetic initial fields and the OpenEXR thickness writer.
+#include "sf_host.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+
+namespace sfh {
+
+namespace {
+// header.cuh:76-83 physical constants
+const double kR = 8.3144598;
+const double kSigmaA = 0.07275;
+const double kRho = 1e3;
+const double kG = 9.8;
+const double kRhoA = 1.184;
+const double kNuA = 1.562e-5;
+const double kEvaporation = -0.00000003;  // header.cuh:95
+const double kPi = 3.14159265358979323846;
+const double kHalfPi = 1.57079632679489661923;
+
+struct V3 { double x, y, z; };
+V3 operator-(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+V3 operator*(double s, V3 a) { return {a.x * s, a.y * s, a.z * s}; }
+V3 operator*(V3 a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+double dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+V3 cross(V3 a, V3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+V3 normalize(V3 v) { return v * (1.0 / std::sqrt(dot(v, v))); }
+
+// smoothstep of helper_math.h (float arithmetic)
+float smoothstepf(float a, float b, float x) {
+    float y = (x - a) / (b - a);
+    y = std::min(std::max(y, 0.0f), 1.0f);
+    return y * y * (3.0f - 2.0f * y);
+}
+
+// Corridor vector without its time factor: a great circle through the x
+// axis normal to the wind (0, cos(pi/2), sin(pi/2)), speed 0.5 sin(beta) r/U
+// faded out by smoothstep(0.2, 0.4, |beta - pi/2|).
+V3 corridor(double sinT, double cosT, double sinP, double cosP, double U, double radius) {
+    const double angle = kPi / 2.;
+    V3 wind = normalize(V3{0., std::cos(angle), std::sin(angle)});
+    V3 position{1., 0., 0.};
+    V3 normal = normalize(cross(position, wind));
+    V3 cur{sinT * cosP, sinT * sinP, cosT};
+    double c = std::max(-1.0, std::min(1.0, dot(normal, cur)));
+    double beta = std::acos(c);
+    V3 projected = normalize(cur - dot(normal, cur) * normal);
+    V3 pw = cross(normal, projected);
+    double ss = (double)smoothstepf(0.2f, 0.4f, (float)std::fabs(beta - kHalfPi));
+    V3 a = 0.5 * std::sin(beta) / U * radius * pw;
+    return a * (1. - ss);
+}
+
+// SplitMix64 (Steele, Lea, Flood 2014) seeding xoshiro256** (Blackman and
+// Vigna 2018).
+struct Rng {
+    uint64_t s[4];
+    explicit Rng(uint64_t seed) {
+        uint64_t z = seed;
+        for (auto& w : s) {
+            z += 0x9E3779B97F4A7C15ull;
+            uint64_t x = z;
+            x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+            x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+            w = x ^ (x >> 31);
+        }
+    }
+    static uint64_t rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+    uint64_t next() {
+        uint64_t r = rotl(s[1] * 5, 7) * 9;
+        uint64_t t = s[1] << 17;
+        s[2] ^= s[0];
+        s[3] ^= s[1];
+        s[1] ^= s[2];
+        s[0] ^= s[3];
+        s[2] ^= t;
+        s[3] = rotl(s[3], 45);
+        return r;
+    }
+    double uniform() { return (double)(next() >> 11) * 0x1.0p-53; }  // [0, 1)
+};
+
+struct Mode { double a; int l, m; };
+
+std::vector<Mode> draw_modes(uint64_t seed, int lmax) {
+    Rng r(seed);
+    std::vector<Mode> modes(16);
+    for (auto& md : modes) {
+        md.a = 2.0 * r.uniform() - 1.0;
+        md.l = 1 + (int)std::floor(r.uniform() * lmax);
+        md.m = -md.l + (int)std::floor(r.uniform() * (2 * md.l + 1));
+    }
+    return modes;
+}
+
+// Fully normalised associated Legendre function Pbar_l^m(cos theta) (no
+// Condon-Shortley phase) by the standard three-term recurrence in l.
+double pbar(int l, int m, double theta) {
+    const double x = std::cos(theta), s = std::sin(theta);
+    double pmm = (2 * m + 1) / (4.0 * kPi);
+    for (int k = 1; k <= m; ++k) pmm *= (2.0 * k - 1.0) / (2.0 * k);
+    pmm = std::sqrt(pmm) * std::pow(std::fabs(s), m);
+    if (l == m) return pmm;
+    double p1 = x * std::sqrt(2.0 * m + 3.0) * pmm;
+    if (l == m + 1) return p1;
+    double p0 = pmm;
+    double a_prev = std::sqrt(2.0 * m + 3.0);  // a_{m+1,m}
+    for (int ll = m + 2; ll <= l; ++ll) {
+        double a = std::sqrt((4.0 * ll * ll - 1.0) / ((double)ll * ll - (double)m * m));
+        double p2 = a * (x * p1 - p0 / a_prev);
+        p0 = p1;
+        p1 = p2;
+        a_prev = a;
+    }
+    return p1;
+}
+
+// f = sum of the 16 modes of `seed` at points (theta_j, phi_i), returned on
+// an (nrow x ncol) grid given by the coordinate lists.
+std::vector<double> mode_sum(uint64_t seed, int lmax, const std::vector<double>& th,
+                             const std::vector<double>& ph) {
+    const size_t nr = th.size(), nc = ph.size();
+    std::vector<double> f(nr * nc, 0.0), leg(nr), trig(nc);
+    for (const Mode& md : draw_modes(seed, lmax)) {
+        const int am = std::abs(md.m);
+        for (size_t j = 0; j < nr; ++j) leg[j] = pbar(md.l, am, th[j]);
+        const double w = md.m == 0 ? 1.0 : std::sqrt(2.0);
+        for (size_t i = 0; i < nc; ++i)
+            trig[i] = md.m > 0 ? std::cos(am * ph[i]) : (md.m < 0 ? std::sin(am * ph[i]) : 1.0);
+        for (size_t j = 0; j < nr; ++j) {
+            const double cj = md.a * w * leg[j];
+            double* row = &f[j * nc];
+            for (size_t i = 0; i < nc; ++i) row[i] += cj * trig[i];
+        }
+    }
+    return f;
+}
+
+}  // namespace
+
+void derive_params(const sf_bubble_desc& d, sf_derived& o) {
+    // Inside Bubble's initialiser list the parameter U shadows the member
+    // (forced to 1.0); run() uses the member, so UGlobal = 1 and the
+    // evaporation rate is evaporation * radius / (H * 1).
+    const double sigma_r = kR * d.T;
+    o.M = d.Gamma_m * sigma_r / (3 * kRho * d.H * d.U * d.U) * d.M_scale;
+    o.Cr = kRhoA * std::sqrt(kNuA * d.radius / d.U) / (kRho * d.H);
+    o.gs = kG * d.radius / (d.U * d.U);
+    o.gamma_max = kSigmaA / sigma_r / d.Gamma_m;
+    o.dt = d.dt * d.U / d.radius;
+    o.radius = d.radius;
+    o.U = 1.0;
+    o.eva = d.evaporation ? kEvaporation * d.radius / (d.H * 1.0) : 0.0;
+    o.h = kPi / d.n_theta;
+}
+
+void air_shapes(int nth, double radius, double U, double* su, double* sv) {
+    const int nph = 2 * nth;
+    const double h = kPi / nth;
+    for (int j = 0; j < nth; ++j) {
+        const double th = ((double)j + 0.5) * h;
+        const double sT = std::sin(th), cT = std::cos(th);
+        for (int i = 0; i < nph; ++i) {
+            const double ph = ((double)i - 0.5) * h;
+            const double sP = std::sin(ph), cP = std::cos(ph);
+            V3 a = corridor(sT, cT, sP, cP, U, radius);
+            su[(size_t)j * nph + i] = dot(a, V3{-sP, cP, 0.});
+        }
+    }
+    for (int j = 0; j < nth - 1; ++j) {
+        const double th = ((double)j + 1.0) * h;
+        const double sT = std::sin(th), cT = std::cos(th);
+        for (int i = 0; i < nph; ++i) {
+            const double ph = (double)i * h;
+            const double sP = std::sin(ph), cP = std::cos(ph);
+            V3 a = corridor(sT, cT, sP, cP, U, radius);
+            sv[(size_t)j * nph + i] = dot(a, V3{cT * cP, cT * sP, -sT});
+        }
+    }
+}
+
+double real_ylm(int l, int m, double theta, double phi) {
+    const int am = std::abs(m);
+    if (am > l || l < 0) return 0.0;
+    const double p = pbar(l, am, theta);
+    if (m == 0) return p;
+    return std::sqrt(2.0) * p * (m > 0 ? std::cos(am * phi) : std::sin(am * phi));
+}
+
+void synth_fields(int nth, uint64_t seed, int lmax, int lair, double noise, double air_amp,
+                  double* eta, double* gam, double* uair0, double* vair0) {
+    const int nph = 2 * nth;
+    const double h = kPi / nth;
+    const size_t N = (size_t)nth * nph;
+    std::vector<double> thc(nth), phc(nph), thn(nth + 1), phw(nph);
+    for (int j = 0; j < nth; ++j) thc[j] = ((double)j + 0.5) * h;
+    for (int j = 0; j <= nth; ++j) thn[j] = (double)j * h;
+    for (int i = 0; i < nph; ++i) {
+        phc[i] = (double)i * h;
+        phw[i] = ((double)i - 0.5) * h;
+    }
+    auto normalised = [&](uint64_t s) {
+        std::vector<double> f = mode_sum(s, lmax, thc, phc);
+        double mx = 0.0;
+        for (double v : f) mx = std::max(mx, std::fabs(v));
+        if (mx > 0)
+            for (double& v : f) v /= mx;
+        return f;
+    };
+    if (eta) {
+        std::vector<double> f = normalised(seed);
+        Rng r(seed + 3);
+        for (size_t k = 0; k < N; ++k) {
+            double v = 1.0 + 0.1 * f[k];
+            if (noise != 0.0) v += noise * (2.0 * r.uniform() - 1.0);
+            eta[k] = v;
+        }
+    }
+    if (gam) {
+        std::vector<double> f = normalised(seed + 1);
+        for (size_t k = 0; k < N; ++k) gam[k] = 1.0 + 0.05 * f[k];
+    }
+    if (uair0 || vair0) {
+        // stream function at cell corners (theta = j h, phi = (i - 1/2) h);
+        // u_phi = d(psi)/d(theta), u_theta = -d(psi)/d(phi) / sin(theta) as
+        // differences across the face, so the discrete MAC divergence of
+        // (u_theta, u_phi) vanishes identically.
+        std::vector<double> psi = mode_sum(seed + 2, lair, thn, phw);
+        std::vector<double> u(N, 0.0), v(N - nph, 0.0);
+        for (int j = 0; j < nth; ++j)
+            for (int i = 0; i < nph; ++i)
+                u[(size_t)j * nph + i] = (psi[(size_t)(j + 1) * nph + i] - psi[(size_t)j * nph + i]) / h;
+        for (int j = 0; j < nth - 1; ++j) {
+            const double s = std::sin((double)(j + 1) * h);
+            for (int i = 0; i < nph; ++i) {
+                const int ie = (i + 1) % nph;
+                v[(size_t)j * nph + i] =
+                    -(psi[(size_t)(j + 1) * nph + ie] - psi[(size_t)(j + 1) * nph + i]) / (h * s);
+            }
+        }
+        double mx = 0.0;
+        for (int j = 0; j < nth; ++j)
+            for (int i = 0; i < nph; ++i) {
+                const double up = 0.5 * (u[(size_t)j * nph + i] + u[(size_t)j * nph + (i + 1) % nph]);
+                const double vn = j > 0 ? v[(size_t)(j - 1) * nph + i] : 0.0;
+                const double vs = j < nth - 1 ? v[(size_t)j * nph + i] : 0.0;
+                const double vt = 0.5 * (vn + vs);
+                mx = std::max(mx, std::sqrt(up * up + vt * vt));
+            }
+        const double sc = mx > 0 ? air_amp / mx : 0.0;
+        if (uair0)
+            for (size_t k = 0; k < N; ++k) uair0[k] = u[k] * sc;
+        if (vair0)
+            for (size_t k = 0; k < N - nph; ++k) vair0[k] = v[k] * sc;
+    }
+}
+
+// ------------------------------------------------------------------ EXR ----
+namespace {
+
+// IEEE binary32 -> binary16, round to nearest even.
+uint16_t float_to_half(float f) {
+    uint32_t x;
+    std::memcpy(&x, &f, 4);
+    const uint32_t sign = (x >> 16) & 0x8000u;
+    const uint32_t absx = x & 0x7FFFFFFFu;
+    if (absx >= 0x7F800000u) return (uint16_t)(sign | 0x7C00u | (absx > 0x7F800000u ? 0x200u : 0u));
+    if (absx >= 0x477FF000u) return (uint16_t)(sign | 0x7C00u);  // overflow -> inf
+    if (absx < 0x38800000u) {                                    // subnormal half or zero
+        if (absx < 0x33000000u) return (uint16_t)sign;
+        const uint32_t mant = (absx & 0x007FFFFFu) | 0x00800000u;
+        const int shift = 126 - (int)(absx >> 23) + 14;
+        uint32_t r = mant >> shift;
+        const uint32_t rem = mant & ((1u << shift) - 1u), half = 1u << (shift - 1);
+        if (rem > half || (rem == half && (r & 1u))) ++r;
+        return (uint16_t)(sign | r);
+    }
+    uint32_t r = ((absx - 0x38000000u) >> 13);
+    const uint32_t rem = absx & 0x1FFFu;
+    if (rem > 0x1000u || (rem == 0x1000u && (r & 1u))) ++r;
+    return (uint16_t)(sign | r);
+}
+
+float half_to_float(uint16_t h) {
+    const uint32_t sign = (uint32_t)(h & 0x8000u) << 16;
+    const uint32_t e = (h >> 10) & 0x1Fu, m = h & 0x3FFu;
+    uint32_t x;
+    if (e == 0) {
+        if (m == 0) {
+            x = sign;
+        } else {
+            int k = -1;
+            uint32_t mm = m;
+            do { ++k; mm <<= 1; } while (!(mm & 0x400u));
+            x = sign | ((uint32_t)(127 - 15 - k) << 23) | ((mm & 0x3FFu) << 13);
+        }
+    } else if (e == 31) {
+        x = sign | 0x7F800000u | (m << 13);
+    } else {
+        x = sign | ((e + 112u) << 23) | (m << 13);
+    }
+    float f;
+    std::memcpy(&f, &x, 4);
+    return f;
+}
+
+void put32(std::string& s, int32_t v) { s.append(reinterpret_cast<const char*>(&v), 4); }
+void putf(std::string& s, float v) { s.append(reinterpret_cast<const char*>(&v), 4); }
+void attr(std::string& s, const char* name, const char* type, const std::string& val) {
+    s += name;
+    s += '\0';
+    s += type;
+    s += '\0';
+    put32(s, (int32_t)val.size());
+    s += val;
+}
+
+}  // namespace
+
+bool write_thickness_exr(const std::string& path, int nth, int nph, const double* eta, double H) {
+    // write_thickness_img (solver.cu:560-582): gray = eta * H * 2e5 in B, G, R
+    const int w = nph, hgt = nth;
+    std::string hdr;
+    const uint32_t magic = 20000630u;
+    hdr.append(reinterpret_cast<const char*>(&magic), 4);
+    put32(hdr, 2);
+    std::string ch;
+    for (const char* name : {"B", "G", "R"}) {
+        ch += name;
+        ch += '\0';
+        put32(ch, 1);  // HALF
+        ch += std::string(4, '\0');
+        put32(ch, 1);
+        put32(ch, 1);
+    }
+    ch += '\0';
+    attr(hdr, "channels", "chlist", ch);
+    attr(hdr, "compression", "compression", std::string(1, '\0'));
+    std::string box;
+    put32(box, 0); put32(box, 0); put32(box, w - 1); put32(box, hgt - 1);
+    attr(hdr, "dataWindow", "box2i", box);
+    attr(hdr, "displayWindow", "box2i", box);
+    attr(hdr, "lineOrder", "lineOrder", std::string(1, '\0'));
+    std::string f1;
+    putf(f1, 1.0f);
+    attr(hdr, "pixelAspectRatio", "float", f1);
+    std::string v2;
+    putf(v2, 0.0f); putf(v2, 0.0f);
+    attr(hdr, "screenWindowCenter", "v2f", v2);
+    attr(hdr, "screenWindowWidth", "float", f1);
+    hdr += '\0';
+    const uint64_t block = 8 + (uint64_t)w * 3 * 2;
+    std::string out = hdr;
+    uint64_t pos = hdr.size() + 8ull * hgt;
+    for (int y = 0; y < hgt; ++y) {
+        out.append(reinterpret_cast<const char*>(&pos), 8);
+        pos += block;
+    }
+    std::vector<uint16_t> line(w);
+    for (int y = 0; y < hgt; ++y) {
+        put32(out, y);
+        put32(out, w * 3 * 2);
+        for (int i = 0; i < w; ++i) line[i] = float_to_half((float)(eta[(size_t)y * nph + i] * H * 2e5));
+        for (int c = 0; c < 3; ++c) out.append(reinterpret_cast<const char*>(line.data()), 2 * (size_t)w);
+    }
+    std::ofstream f(path, std::ios::binary);
+    if (!f) return false;
+    f.write(out.data(), (std::streamsize)out.size());
+    return (bool)f;
+}
+
+bool read_exr_channel(const std::string& path, const std::string& channel, int* width,
+                      int* height, std::vector<float>* data) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) return false;
+    std::string buf((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+    size_t p = 0;
+    auto rd32 = [&](int32_t& v) {
+        if (p + 4 > buf.size()) return false;
+        std::memcpy(&v, buf.data() + p, 4);
+        p += 4;
+        return true;
+    };
+    auto rdstr = [&](std::string& s) {
+        size_t e = buf.find('\0', p);
+        if (e == std::string::npos) return false;
+        s = buf.substr(p, e - p);
+        p = e + 1;
+        return true;
+    };
+    int32_t magic, version;
+    if (!rd32(magic) || magic != 20000630 || !rd32(version) || (version & 0xFF) != 2) return false;
+    if (version & 0x1E00) return false;  // tiled / long names / deep / multipart unsupported
+    struct Chan { std::string name; int type; };
+    std::vector<Chan> chans;
+    int comp = -1, x0 = 0, y0 = 0, x1 = -1, y1 = -1;
+    for (;;) {
+        std::string name, type;
+        if (!rdstr(name)) return false;
+        if (name.empty()) break;
+        int32_t size;
+        if (!rdstr(type) || !rd32(size) || p + size > buf.size()) return false;
+        const size_t end = p + size;
+        if (name == "channels") {
+            while (p < end && buf[p] != '\0') {
+                Chan c;
+                if (!rdstr(c.name)) return false;
+                int32_t t;
+                std::memcpy(&t, buf.data() + p, 4);
+                c.type = t;
+                p += 16;
+                chans.push_back(c);
+            }
+        } else if (name == "compression") {
+            comp = (unsigned char)buf[p];
+        } else if (name == "dataWindow") {
+            std::memcpy(&x0, buf.data() + p, 4);
+            std::memcpy(&y0, buf.data() + p + 4, 4);
+            std::memcpy(&x1, buf.data() + p + 8, 4);
+            std::memcpy(&y1, buf.data() + p + 12, 4);
+        }
+        p = end;
+    }
+    if (comp != 0 || chans.empty()) return false;
+    const int w = x1 - x0 + 1, h = y1 - y0 + 1;
+    if (w <= 0 || h <= 0) return false;
+    std::sort(chans.begin(), chans.end(), [](const Chan& a, const Chan& b) { return a.name < b.name; });
+    int ci = -1;
+    size_t chan_off = 0, line_bytes = 0;
+    for (size_t k = 0; k < chans.size(); ++k) {
+        const size_t bpp = chans[k].type == 1 ? 2 : 4;
+        if (chans[k].name == channel) { ci = (int)k; chan_off = line_bytes; }
+        line_bytes += bpp * (size_t)w;
+    }
+    if (ci < 0) return false;
+    std::vector<uint64_t> offs(h);
+    if (p + 8ull * h > buf.size()) return false;
+    std::memcpy(offs.data(), buf.data() + p, 8ull * h);
+    data->assign((size_t)w * h, 0.0f);
+    for (int k = 0; k < h; ++k) {
+        size_t q = offs[k];
+        int32_t y, sz;
+        if (q + 8 > buf.size()) return false;
+        std::memcpy(&y, buf.data() + q, 4);
+        std::memcpy(&sz, buf.data() + q + 4, 4);
+        q += 8;
+        if ((size_t)sz != line_bytes || q + sz > buf.size() || y < y0 || y > y1) return false;
+        const char* src = buf.data() + q + chan_off;
+        float* dst = data->data() + (size_t)(y - y0) * w;
+        for (int i = 0; i < w; ++i) {
+            if (chans[ci].type == 1) {
+                uint16_t hv;
+                std::memcpy(&hv, src + 2 * i, 2);
+                dst[i] = half_to_float(hv);
+            } else {
+                std::memcpy(&dst[i], src + 4 * i, 4);
+            }
+        }
+    }
+    *width = w;
+    *height = h;
+    return true;
+}
+
+}  // namespace sfh
+
+extern "C" {
+
+int sf_synth_fields(int32_t n_theta, uint64_t seed, int32_t l_max, int32_t l_air, double noise,
+                    double air_amp, double* eta, double* gamma, double* uair0, double* vair0) {
+    if (n_theta < 2 || l_max < 1 || l_air < 1) return SF_ERR_INVALID;
+    sfh::synth_fields(n_theta, seed, l_max, l_air, noise, air_amp, eta, gamma, uair0, vair0);
+    return SF_OK;
+}
+
+double sf_real_ylm(int32_t l, int32_t m, double theta, double phi) {
+    return sfh::real_ylm(l, m, theta, phi);
+}
+
+}  // extern "C"

@@ -1,0 +1,685 @@
+// HIP kernels of the soap-film step for gfx950 (MI355X).
+//
+// Stage kernels (one launch each per step, packed over all bubbles):
+//   k_advect    great-circle semi-Lagrangian advection of u_theta, u_phi,
+//               eta and Gamma (advectionVSphere{Theta,Phi}Kernel and
+//               advectionAllCentered, bubble.cu:639-961) fused per cell
+//   k_assemble  the sinTheta-scaled implicit Gamma system
+//               (concentrationLinearSystemKernel, bubble.cu:1301-1473) as a
+//               matrix-free 5-point operator: face weights, diagonal, FP64 rhs
+//   k_resid     FP64 residual of the refinement loop + phi-line solve
+//   k_spmv      inner PCG: p update, A p, p.Ap
+//   k_cgstep    inner PCG: x, r update, phi-line solve, r.z
+//   k_finalize  convergence verdict of the solve
+//   k_update    divergenceKernel_fromGamma, applyforcevel{theta,phi} and
+//               applyforceThickness (bubble.cu:1276-1295, 1819-1935,
+//               2041-2063) fused per cell
+//
+// Template parameter T is the state precision (u, eta, Gamma, air), Ti the
+// precision of the inner solve. The refinement vectors (x, b, residual) and
+// the assembled coefficients are FP64.
+#include "sf_device.h"
+#include "sf_launch.h"
+
+namespace sfk {
+
+constexpr double kEps = 1e-7;   // bubble.cu:28
+constexpr double kThresh = 0.0; // bubble.cu:27
+
+// ------------------------------------------------------------ advection ----
+template <typename T>
+__global__ __launch_bounds__(256) void k_advect(const int4* __restrict__ tiles,
+                                                const BubbleDev* __restrict__ bub, Tables tab,
+                                                StatePtrs<T> in, StatePtrs<T> out) {
+    const int4 tl = tiles[blockIdx.x];
+    const BubbleDev B = bub[tl.x];
+    const int j = tl.y + (int)threadIdx.x / tl.w;
+    const int i = tl.z + (int)threadIdx.x % tl.w;
+    if (j >= B.nth || i >= B.nph) return;
+    const size_t c = (size_t)j * B.nph + i;
+
+    VelCtx<T> v;
+    v.uth = Field<T>{in.uth + B.off, B.nph, B.mask, B.nth - 2};
+    v.uph = Field<T>{in.uph + B.off, B.nph, B.mask, B.nth - 1};
+    v.nth = B.nth;
+    v.invh = (T)B.invh;
+    v.dt = (T)B.dt;
+    const Field<T> feta{in.eta + B.off, B.nph, B.mask, B.nth - 1};
+    const Field<T> fgam{in.gam + B.off, B.nph, B.mask, B.nth - 1};
+
+    const T sinTc = TabSel<T>::sTc(tab)[B.rt0 + j], cosTc = TabSel<T>::cTc(tab)[B.rt0 + j];
+    const T sinPc = TabSel<T>::sPc(tab)[B.ct0 + i], cosPc = TabSel<T>::cPc(tab)[B.ct0 + i];
+
+    // eta, Gamma at the cell centre (j+1/2, i): traceGreatCircle + two samples
+    {
+        const T gx = (T)j + T(0.5), gy = (T)i;
+        T guT = sample_vtheta(v.uth, B.nth, gx, gy);
+        T guP = sample_vphi(v.uph, B.nth, gx, gy);
+        Circle<T> o;
+        T tx = gx, ty = gy;
+        if (circle_mid(v, sinTc, cosTc, sinPc, cosPc, guT, guP, o)) {
+            T s, co, px, py, pz, rho;
+            circle_depart(v, o, s, co, px, py, pz, tx, ty, rho);
+        }
+        out.eta[B.off + c] = sample_centered(feta, B.nth, tx, ty);
+        out.gam[B.off + c] = sample_centered(fgam, B.nth, tx, ty);
+    }
+    // u_phi at the west face (j+1/2, i-1/2)
+    {
+        const T gx = (T)j + T(0.5), gy = (T)i - T(0.5);
+        const T sinP = TabSel<T>::sPh(tab)[B.ct0 + i], cosP = TabSel<T>::cPh(tab)[B.ct0 + i];
+        T guT = sample_vtheta(v.uth, B.nth, gx, gy);
+        T guP = in.uph[B.off + c];
+        Circle<T> o;
+        T res;
+        if (!circle_mid(v, sinTc, cosTc, sinP, cosP, guT, guP, o)) {
+            res = o.unorm;
+        } else {
+            T r0, r1, r2;
+            transport_back(v, o, r0, r1, r2);
+            res = -sinP * r0 + cosP * r1;
+        }
+        out.uph[B.off + c] = res;
+    }
+    // u_theta at the south face (j+1, i)
+    if (j < B.nth - 1) {
+        const T gx = (T)(j + 1), gy = (T)i;
+        const T sinT = TabSel<T>::sTf(tab)[B.rt0 + j + 1], cosT = TabSel<T>::cTf(tab)[B.rt0 + j + 1];
+        T guT = in.uth[B.off + c];
+        T guP = sample_vphi(v.uph, B.nth, gx, gy);
+        Circle<T> o;
+        T res;
+        if (!circle_mid(v, sinT, cosT, sinPc, cosPc, guT, guP, o)) {
+            res = o.unorm;
+        } else {
+            T r0, r1, r2;
+            transport_back(v, o, r0, r1, r2);
+            res = r0 * (cosT * cosPc) + r1 * (cosT * sinPc) - r2 * sinT;
+        }
+        out.uth[B.off + c] = res;
+    }
+}
+
+// ---------------------------------------------------------- air flow ----
+// u_air(t) = u_air_init + (3 - |t - 3|) * corridor, NaN -> 0 (airFlowU/V,
+// bubble.cu:2301-2380). The corridor shape is time-independent and is
+// precomputed on the host.
+template <typename T>
+__device__ __forceinline__ double air_at(const T* __restrict__ a0, const T* __restrict__ shape,
+                                         size_t k, double kt) {
+    double v = (double)a0[k] + (double)shape[k] * kt;
+    return __builtin_isnan(v) ? 0.0 : v;
+}
+
+__device__ __forceinline__ bool is_broken(double gamma, double eta, double gmax) {
+    return gamma > gmax - kEps || eta == kThresh;
+}
+
+// ---------------------------------------------------------- assembly ----
+// Face weights are the negated off-diagonal entries of the reference matrix:
+// w_N = s2 sin(theta_N) M dt / (eta_N,f + Cr dt), w_W = s2 csc(theta) M dt /
+// (eta_W,f + Cr dt). A broken cell keeps the reference's identity row with
+// rhs gamma_max; its coupling to unbroken neighbours is moved to their
+// right-hand sides (x_broken = gamma_max is known), which leaves the
+// solution unchanged and keeps the operator symmetric.
+template <typename T, typename Ti>
+__global__ __launch_bounds__(256) void k_assemble(const int4* __restrict__ tiles,
+                                                  const BubbleDev* __restrict__ bub,
+                                                  const BubbleTime* __restrict__ btime, Tables tab,
+                                                  StatePtrs<T> s, AirPtrs<T> air, SysPtrs<Ti> sys,
+                                                  T gmax_state) {
+    const int4 tl = tiles[blockIdx.x];
+    const BubbleDev B = bub[tl.x];
+    const int j = tl.y + (int)threadIdx.x / tl.w;
+    const int i = tl.z + (int)threadIdx.x % tl.w;
+    if (j >= B.nth || i >= B.nph) return;
+    const int nph = B.nph, m = B.mask, nth = B.nth;
+    const size_t o = (size_t)B.off;
+    const size_t c = o + (size_t)j * nph + i;
+    const int iE = (i + 1) & m, iW = (i - 1) & m;
+    const double gmax = B.gamma_max;
+
+    const T* eta = s.eta;
+    T* gam = s.gam;
+    const double e = (double)eta[c];
+    const double gamma = (double)gam[c];
+    const bool brk = is_broken(gamma, e, gmax);
+
+    const size_t cW = o + (size_t)j * nph + iW, cE = o + (size_t)j * nph + iE;
+    const size_t cN = j > 0 ? c - nph : o + (size_t)((i + nth) & m);
+    const size_t cS = j < nth - 1 ? c + nph : o + (size_t)j * nph + ((i + nth) & m);
+    const double etaW = (double)eta[cW], etaE = (double)eta[cE];
+    const double etaN = (double)eta[cN], etaS = (double)eta[cS];
+    const bool bW = is_broken((double)gam[cW], etaW, gmax);
+    const bool bE = is_broken((double)gam[cE], etaE, gmax);
+    const bool bN = j > 0 && is_broken((double)gam[cN], etaN, gmax);
+    const bool bS = j < nth - 1 && is_broken((double)gam[cS], etaS, gmax);
+
+    const double dt = B.dt, invDt = 1.0 / dt, Cr = B.Cr;
+    const double CrDt = Cr * dt, MDt = B.M * dt, s2 = B.invh * B.invh;
+    const double sinN = tab.sinFace[B.rt0 + j], sinS = tab.sinFace[B.rt0 + j + 1];
+    const double sinTheta = tab.sinTc[B.rt0 + j], csc = tab.cscTc[B.rt0 + j];
+    const double etaUp = (etaN + e) / 2.0, etaLeft = (etaW + e) / 2.0;
+    const double etaRight = (etaE + e) / 2.0, etaDown = (etaS + e) / 2.0;
+    const double wN = s2 * sinN * MDt / (etaUp + CrDt);
+    const double wW = s2 * csc * MDt / (etaLeft + CrDt);
+    const double wE = s2 * csc * MDt / (etaRight + CrDt);
+    const double wS = s2 * sinS * MDt / (etaDown + CrDt);
+
+    double diag, rhs;
+    if (brk) {
+        diag = 1.0;
+        rhs = gmax;
+        gam[c] = gmax_state;   // the reference overwrites Gamma* (bubble.cu:1325)
+    } else {
+        diag = sinTheta * invDt / gamma - (((-wN) + (-wW)) + (-wE) + (-wS));
+        double uW = (double)s.uph[c], uE = (double)s.uph[cE];
+        double vN = j > 0 ? (double)s.uth[c - nph] : 0.0;
+        double vS = j < nth - 1 ? (double)s.uth[c] : 0.0;
+        if (__builtin_isnan(vN)) vN = 0.0;
+        if (__builtin_isnan(vS)) vS = 0.0;
+        if (__builtin_isnan(uW)) uW = 0.0;
+        if (__builtin_isnan(uE)) uE = 0.0;
+        double div = B.invh * (uE * etaRight / (etaRight + Cr * dt) -
+                               uW * etaLeft / (etaLeft + Cr * dt) +
+                               (vS * sinS * etaDown / (etaDown + Cr * dt) -
+                                vN * sinN * etaUp / (etaUp + Cr * dt)));
+        if (B.airflow) {
+            const double t = btime[tl.x].t, kt = 3.0 - fabs(t - 3.0);
+            const double bl = t / 12.0;
+            const size_t row = o + (size_t)j * nph;
+            const int opp = (nph - i) & m, oppE = (nph - iE) & m;
+            const double uAirW = 0.5 * (air_at(air.ua0, air.au, row + i, kt) * bl +
+                                        air_at(air.ua0, air.au, row + opp, kt) * (1 - bl));
+            const double uAirE = 0.5 * (air_at(air.ua0, air.au, row + iE, kt) * bl +
+                                        air_at(air.ua0, air.au, row + oppE, kt) * (1 - bl));
+            double vAirN = 0.0, vAirS = 0.0;
+            if (j > 0) {
+                const size_t rn = row - nph;
+                vAirN = 0.5 * (air_at(air.va0, air.av, rn + i, kt) * bl -
+                               air_at(air.va0, air.av, rn + opp, kt) * (1 - bl));
+            }
+            if (j < nth - 1) {
+                vAirS = 0.5 * (air_at(air.va0, air.av, row + i, kt) * bl -
+                               air_at(air.va0, air.av, row + opp, kt) * (1 - bl));
+            }
+            div += B.invh * (uAirE / (etaRight / Cr * invDt + 1.0) -
+                             uAirW / (etaLeft / Cr * invDt + 1.0) +
+                             vAirS * sinS / (etaDown / Cr * invDt + 1.0) -
+                             vAirN * sinN / (etaUp / Cr * invDt + 1.0));
+        }
+        if (B.gravity) {
+            div += B.gs * B.invh *
+                   (sinS * sinS * etaDown / (invDt * etaDown + Cr) -
+                    sinN * sinN * etaUp / (invDt * etaUp + Cr));
+        }
+        rhs = sinTheta * invDt - div;
+        if (bN) rhs += wN * gmax;
+        if (bS) rhs += wS * gmax;
+        if (bW) rhs += wW * gmax;
+        if (bE) rhs += wE * gmax;
+    }
+    const double wNe = (brk || bN) ? 0.0 : wN;
+    const double wWe = (brk || bW) ? 0.0 : wW;
+    sys.wN64[c] = wNe;
+    sys.wW64[c] = wWe;
+    sys.diag64[c] = diag;
+    sys.b64[c] = rhs;
+    sys.x64[c] = brk ? gmax : gamma;      // x0 = Gamma* (bubble.cu:1480-1486)
+    sys.wN[c] = (Ti)wNe;
+    sys.wW[c] = (Ti)wWe;
+    sys.diag[c] = (Ti)diag;
+}
+
+// -------------------------------------------------------- line solve ----
+// Loads the phi-line block of row j into LDS (normalised by the diagonal)
+// with right-hand side r, solves it and returns z in sd.
+template <typename Ti>
+__device__ __forceinline__ void line_setup(Ti* sa, Ti* sc, Ti* sd, const Ti* __restrict__ wW,
+                                           const Ti* __restrict__ dg, size_t row, int i, int m,
+                                           Ti r) {
+    const Ti d = dg[row + i];
+    const Ti inv = Ti(1) / d;
+    sa[i] = -wW[row + i] * inv;
+    sc[i] = -wW[row + ((i + 1) & m)] * inv;
+    sd[i] = r * inv;
+}
+
+// ----------------------------------------------------- FP64 residual ----
+// x_new = x_old + e (the inner correction, when apply_e), r = b - A x_new in
+// FP64, r_in = (Ti) r, z = M^-1 r_in. Partials: |r|^2, |b|^2, r_in.z, |r_in|^2.
+template <typename Ti, int CPT>
+__global__ void k_resid(const int2* __restrict__ rows, const BubbleDev* __restrict__ bub,
+                        SolveCtl* __restrict__ ctl, SysPtrs<Ti> sys, const double* __restrict__ xo,
+                        double* __restrict__ xn, int apply_e, int k, double* __restrict__ part,
+                        long stride) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    __shared__ double red[16 * 4];
+    const int2 rw = rows[blockIdx.x];
+    const BubbleDev B = bub[rw.x];
+    if (k > 0 && ctl[rw.x].outer_done) return;
+    const int j = rw.y, nph = B.nph, m = B.mask, nth = B.nth;
+    Ti* sa = reinterpret_cast<Ti*>(lds_raw);
+    Ti* sc = sa + nph;
+    Ti* sd = sc + nph;
+    const size_t row = (size_t)B.off + (size_t)j * nph;
+    const Ti* __restrict__ e = sys.e;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = threadIdx.x; i < nph; i += blockDim.x) {
+        const size_t c = row + i;
+        const size_t cW = row + ((i - 1) & m), cE = row + ((i + 1) & m);
+        double xc = xo[c], xW = xo[cW], xE = xo[cE];
+        if (apply_e) { xc += (double)e[c]; xW += (double)e[cW]; xE += (double)e[cE]; }
+        double ax = sys.diag64[c] * xc - sys.wW64[c] * xW - sys.wW64[cE] * xE;
+        if (j > 0) {
+            double xN = xo[c - nph];
+            if (apply_e) xN += (double)e[c - nph];
+            ax -= sys.wN64[c] * xN;
+        }
+        if (j < nth - 1) {
+            double xS = xo[c + nph];
+            if (apply_e) xS += (double)e[c + nph];
+            ax -= sys.wN64[c + nph] * xS;
+        }
+        xn[c] = xc;
+        const double b = sys.b64[c];
+        const double r = b - ax;
+        const Ti ri = (Ti)r;
+        sys.r[c] = ri;
+        line_setup(sa, sc, sd, sys.wW, sys.diag, row, i, m, ri);
+        acc[0] += r * r;
+        acc[1] += b * b;
+        acc[3] += (double)ri * (double)ri;
+    }
+    __syncthreads();
+    pcr_periodic<Ti, CPT>(sa, sc, sd, nph);
+    for (int i = threadIdx.x; i < nph; i += blockDim.x) {
+        const Ti z = sd[i];
+        sys.z[row + i] = z;
+        acc[2] += (double)sys.r[row + i] * (double)z;
+    }
+    block_sum<4>(acc, red);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) part[q * stride + B.pr0 + j] = acc[q];
+        if (j == 0) {
+            ctl[rw.x].xbuf = k & 1;
+            ctl[rw.x].outers = k + 1;
+        }
+    }
+}
+
+// ------------------------------------------------------ inner PCG: A p ----
+template <typename Ti, int CPT>
+__global__ void k_spmv(const int2* __restrict__ rows, const BubbleDev* __restrict__ bub,
+                       SolveCtl* __restrict__ ctl, SysPtrs<Ti> sys, const Ti* __restrict__ pold,
+                       Ti* __restrict__ pnew, int it, SolveParams sp, const double* __restrict__ partR,
+                       const double* __restrict__ partB, double* __restrict__ partA, long stride) {
+    __shared__ double red[16 * 4];
+    const int2 rw = rows[blockIdx.x];
+    const BubbleDev B = bub[rw.x];
+    SolveCtl* C = ctl + rw.x;
+    if (C->outer_done) return;
+    const int j = rw.y, nph = B.nph, m = B.mask, nth = B.nth;
+    const bool leader = (j == 0 && threadIdx.x == 0);
+    double beta = 0.0;
+    if (it == 0) {
+        double s[4];
+        sum_partials<4>(partR, stride, B.pr0, nth, s, red);
+        const double rr = s[0], bb = s[1], rz = s[2], rrin = s[3];
+        const double res = sp.absolute ? sqrt(rr) : sqrt(rr) / (bb > 0.0 ? sqrt(bb) : 1.0);
+        if (res <= sp.tol) {
+            if (leader) { C->outer_done = 1; C->rel_res = res; }
+            return;
+        }
+        if (leader) {
+            C->rr_in0 = rrin;
+            C->inner_done = 0;
+            C->rz[0] = rz;
+            C->rel_res = res;
+            C->bb = bb;
+        }
+    } else {
+        if (C->inner_done) return;
+        double s[2];
+        sum_partials<2>(partB, stride, B.pr0, nth, s, red);
+        const double rz = s[0], rr = s[1];
+        const double tgt = sp.inner_frac * sp.tol;
+        const double bbe = sp.absolute ? 1.0 : C->bb;
+        if (rr <= sp.inner_rho2 * C->rr_in0 || rr <= tgt * tgt * bbe) {
+            if (leader) C->inner_done = 1;
+            return;
+        }
+        beta = rz / C->rz[(it - 1) & 1];
+        if (leader) C->rz[it & 1] = rz;
+    }
+    const size_t row = (size_t)B.off + (size_t)j * nph;
+    const Ti bt = (Ti)beta;
+    const Ti* __restrict__ z = sys.z;
+    auto pv = [&](size_t q) -> Ti { return it == 0 ? z[q] : z[q] + bt * pold[q]; };
+    double acc[1] = {0.0};
+    for (int i = threadIdx.x; i < nph; i += blockDim.x) {
+        const size_t c = row + i, cW = row + ((i - 1) & m), cE = row + ((i + 1) & m);
+        const Ti pc = pv(c);
+        Ti ap = sys.diag[c] * pc - sys.wW[c] * pv(cW) - sys.wW[cE] * pv(cE);
+        if (j > 0) ap -= sys.wN[c] * pv(c - nph);
+        if (j < nth - 1) ap -= sys.wN[c + nph] * pv(c + nph);
+        pnew[c] = pc;
+        sys.Ap[c] = ap;
+        acc[0] += (double)pc * (double)ap;
+    }
+    block_sum<1>(acc, red);
+    if (threadIdx.x == 0) partA[B.pr0 + j] = acc[0];
+}
+
+// -------------------------------------------- inner PCG: update + M^-1 ----
+template <typename Ti, int CPT>
+__global__ void k_cgstep(const int2* __restrict__ rows, const BubbleDev* __restrict__ bub,
+                         SolveCtl* __restrict__ ctl, SysPtrs<Ti> sys, const Ti* __restrict__ p,
+                         int it, const double* __restrict__ partA, double* __restrict__ partB,
+                         long stride) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    __shared__ double red[16 * 4];
+    const int2 rw = rows[blockIdx.x];
+    const BubbleDev B = bub[rw.x];
+    SolveCtl* C = ctl + rw.x;
+    if (C->outer_done || C->inner_done) return;
+    const int j = rw.y, nph = B.nph, m = B.mask, nth = B.nth;
+    double s[1];
+    sum_partials<1>(partA, stride, B.pr0, nth, s, red);
+    const double alpha = C->rz[it & 1] / s[0];
+    const Ti al = (Ti)alpha;
+    Ti* sa = reinterpret_cast<Ti*>(lds_raw);
+    Ti* sc = sa + nph;
+    Ti* sd = sc + nph;
+    const size_t row = (size_t)B.off + (size_t)j * nph;
+    double acc[2] = {0.0, 0.0};
+    for (int i = threadIdx.x; i < nph; i += blockDim.x) {
+        const size_t c = row + i;
+        const Ti pc = p[c];
+        sys.e[c] = it == 0 ? al * pc : sys.e[c] + al * pc;
+        const Ti r = sys.r[c] - al * sys.Ap[c];
+        sys.r[c] = r;
+        line_setup(sa, sc, sd, sys.wW, sys.diag, row, i, m, r);
+        acc[1] += (double)r * (double)r;
+    }
+    __syncthreads();
+    pcr_periodic<Ti, CPT>(sa, sc, sd, nph);
+    for (int i = threadIdx.x; i < nph; i += blockDim.x) {
+        const Ti z = sd[i];
+        sys.z[row + i] = z;
+        acc[0] += (double)sys.r[row + i] * (double)z;
+    }
+    block_sum<2>(acc, red);
+    if (threadIdx.x == 0) {
+        partB[B.pr0 + j] = acc[0];
+        partB[stride + B.pr0 + j] = acc[1];
+        if (j == 0) C->iters += 1;
+    }
+}
+
+// ----------------------------------------------------------- finalize ----
+// One block per bubble: the verdict from the last FP64 residual evaluated.
+__global__ void k_finalize(const BubbleDev* __restrict__ bub, SolveCtl* __restrict__ ctl,
+                           SolveHist* __restrict__ hist, SolveParams sp,
+                           const double* __restrict__ partR, long stride) {
+    __shared__ double red[16 * 4];
+    const BubbleDev B = bub[blockIdx.x];
+    double s[2];
+    sum_partials<2>(partR, stride, B.pr0, B.nth, s, red);
+    if (threadIdx.x == 0) {
+        SolveCtl* C = ctl + blockIdx.x;
+        const double res = sp.absolute ? sqrt(s[0]) : sqrt(s[0]) / (s[1] > 0.0 ? sqrt(s[1]) : 1.0);
+        C->rel_res = res;
+        C->bb = s[1];
+        C->converged = res <= sp.tol ? 1 : 0;
+        SolveHist* Hh = hist + blockIdx.x;
+        Hh->solves += 1;
+        if (!(res <= sp.tol)) Hh->fails += 1;
+        if (!(res <= Hh->max_res)) Hh->max_res = res;
+    }
+}
+
+__global__ void k_reset_ctl(SolveCtl* __restrict__ ctl, int nb) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    SolveCtl z = {};
+    ctl[b] = z;
+}
+
+// ------------------------------------------------------------- update ----
+__device__ __forceinline__ double clamp_gamma(double g, double gmax) {
+    return (g <= 0.0 || g > gmax - kEps) ? gmax : g;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_update(const int4* __restrict__ tiles,
+                                                const BubbleDev* __restrict__ bub,
+                                                const BubbleTime* __restrict__ btime,
+                                                const SolveCtl* __restrict__ ctl, Tables tab,
+                                                StatePtrs<T> st, AirPtrs<T> air,
+                                                const double* __restrict__ x0,
+                                                const double* __restrict__ x1, StatePtrs<T> out) {
+    const int4 tl = tiles[blockIdx.x];
+    const BubbleDev B = bub[tl.x];
+    const int j = tl.y + (int)threadIdx.x / tl.w;
+    const int i = tl.z + (int)threadIdx.x % tl.w;
+    if (j >= B.nth || i >= B.nph) return;
+    const int nph = B.nph, m = B.mask, nth = B.nth;
+    const double* __restrict__ x = ctl[tl.x].xbuf ? x1 : x0;
+    const size_t c = (size_t)B.off + (size_t)j * nph + i;
+    const size_t cW = (size_t)B.off + (size_t)j * nph + ((i - 1) & m);
+    const double gmax = B.gamma_max, dt = B.dt, invh = B.invh, Cr = B.Cr, M = B.M;
+    double kt = 0.0;
+    if (B.airflow) { const double t = btime[tl.x].t; kt = 3.0 - fabs(t - 3.0); }
+
+    // divergenceKernel_fromGamma (bubble.cu:1276-1295)
+    const double ga = (double)st.gam[c];
+    const double g = x[c];
+    double gnew, div;
+    if (g <= 0.0 || g > gmax - kEps) {
+        gnew = gmax;
+        div = 1.0 / dt;
+    } else {
+        gnew = g;
+        div = (1.0 - fmin(g, 2 * ga) / ga) / dt;
+    }
+    const double eta = (double)st.eta[c];
+    // applyforceThickness (bubble.cu:2041-2063)
+    {
+        double v = eta * (1 - dt * div);
+        if (B.evaporation) v += B.eva * dt;
+        out.eta[c] = (T)fmax(kThresh, v);
+        out.gam[c] = (T)gnew;
+    }
+    // applyforcevelphiKernel (bubble.cu:1879-1935)
+    {
+        const double etaW = (double)st.eta[cW];
+        const double u1 = (double)st.uph[c];
+        const double invEta = 2. / (etaW + eta);
+        double res;
+        if (__builtin_isinf(invEta)) {
+            res = __builtin_nan("");
+        } else {
+            const double gW = clamp_gamma(x[cW], gmax);
+            const double pGpx = invh * (gnew - gW) * tab.cscTc[B.rt0 + j];
+            const double f1 = -M * invEta * pGpx;
+            const double uAir = B.airflow ? air_at(air.ua0, air.au, c, kt) : 0.0;
+            const double f2 = Cr * invEta * uAir;
+            res = (u1 / dt + (f1 + f2 + 0.0 + 0.0)) / (1. / dt + Cr * invEta);
+        }
+        out.uph[c] = (T)res;
+    }
+    // applyforcevelthetaKernel (bubble.cu:1819-1876)
+    if (j < nth - 1) {
+        const double etaS = (double)st.eta[c + nph];
+        const double v1 = (double)st.uth[c];
+        const double invEta = 2. / (eta + etaS);
+        double res;
+        if (__builtin_isinf(invEta)) {
+            res = __builtin_nan("");
+        } else {
+            const double gS = clamp_gamma(x[c + nph], gmax);
+            const double pGpy = invh * (gS - gnew);
+            const double f1 = -M * invEta * pGpy;
+            const double vAir = B.airflow ? air_at(air.va0, air.av, c, kt) : 0.0;
+            const double f2 = Cr * invEta * vAir;
+            const double f3 = B.gravity ? B.gs * tab.sinTf[B.rt0 + j + 1] : 0.0;
+            res = (v1 / dt + f1 + f2 + f3 + 0.0) / (1. / dt + Cr * invEta);
+        }
+        out.uth[c] = (T)res;
+    }
+}
+
+__global__ void k_advance_time(BubbleTime* __restrict__ bt, int nb) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < nb && bt[b].advance) bt[b].t += bt[b].dt_s;
+}
+
+// Per-row film volume (sum eta sin(theta)) and count of cells with eta > 0.
+template <typename T>
+__global__ void k_row_stats(const int2* __restrict__ rows, const BubbleDev* __restrict__ bub,
+                            Tables tab, const T* __restrict__ eta, double* __restrict__ out,
+                            long stride) {
+    __shared__ double red[16 * 4];
+    const int2 rw = rows[blockIdx.x];
+    const BubbleDev B = bub[rw.x];
+    const size_t row = (size_t)B.off + (size_t)rw.y * B.nph;
+    double acc[2] = {0.0, 0.0};
+    for (int i = threadIdx.x; i < B.nph; i += blockDim.x) {
+        const double e = (double)eta[row + i];
+        acc[0] += e;
+        acc[1] += e > 0.0 ? 1.0 : 0.0;
+    }
+    block_sum<2>(acc, red);
+    if (threadIdx.x == 0) {
+        out[B.pr0 + rw.y] = acc[0] * tab.sinTc[B.rt0 + rw.y];
+        out[stride + B.pr0 + rw.y] = acc[1];
+    }
+}
+
+// ------------------------------------------------------------ launchers ----
+template <typename T>
+void launch_advect(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, const Tables& tab,
+                   StatePtrs<T> in, StatePtrs<T> out) {
+    hipLaunchKernelGGL(k_advect<T>, dim3(g.ntiles), dim3(256), 0, s, g.tiles, bub, tab, in, out);
+}
+
+template <typename T, typename Ti>
+void launch_assemble(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub,
+                     const BubbleTime* bt, const Tables& tab, StatePtrs<T> st, AirPtrs<T> air,
+                     SysPtrs<Ti> sys, double gmax_state) {
+    hipLaunchKernelGGL((k_assemble<T, Ti>), dim3(g.ntiles), dim3(256), 0, s, g.tiles, bub, bt, tab,
+                       st, air, sys, (T)gmax_state);
+}
+
+template <typename Ti, int CPT>
+static void resid_cpt(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, SolveCtl* ctl,
+                      SysPtrs<Ti> sys, const double* xo, double* xn, int apply_e, int k,
+                      double* part, long stride) {
+    hipLaunchKernelGGL((k_resid<Ti, CPT>), dim3(g.nrows), dim3(g.row_threads), g.lds_bytes, s,
+                       g.rows, bub, ctl, sys, xo, xn, apply_e, k, part, stride);
+}
+template <typename Ti>
+void launch_resid(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, SolveCtl* ctl,
+                  SysPtrs<Ti> sys, const double* xo, double* xn, int apply_e, int k, double* part,
+                  long stride) {
+    switch (g.cpt) {
+        case 1: resid_cpt<Ti, 1>(s, g, bub, ctl, sys, xo, xn, apply_e, k, part, stride); break;
+        case 2: resid_cpt<Ti, 2>(s, g, bub, ctl, sys, xo, xn, apply_e, k, part, stride); break;
+        case 4: resid_cpt<Ti, 4>(s, g, bub, ctl, sys, xo, xn, apply_e, k, part, stride); break;
+        default: resid_cpt<Ti, 8>(s, g, bub, ctl, sys, xo, xn, apply_e, k, part, stride); break;
+    }
+}
+
+template <typename Ti>
+void launch_spmv(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, SolveCtl* ctl,
+                 SysPtrs<Ti> sys, const Ti* pold, Ti* pnew, int it, const SolveParams& sp,
+                 const double* partR, const double* partB, double* partA, long stride) {
+    hipLaunchKernelGGL((k_spmv<Ti, 1>), dim3(g.nrows), dim3(g.row_threads), 0, s, g.rows, bub, ctl,
+                       sys, pold, pnew, it, sp, partR, partB, partA, stride);
+}
+
+template <typename Ti, int CPT>
+static void cgstep_cpt(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, SolveCtl* ctl,
+                       SysPtrs<Ti> sys, const Ti* p, int it, const double* partA, double* partB,
+                       long stride) {
+    hipLaunchKernelGGL((k_cgstep<Ti, CPT>), dim3(g.nrows), dim3(g.row_threads), g.lds_bytes, s,
+                       g.rows, bub, ctl, sys, p, it, partA, partB, stride);
+}
+template <typename Ti>
+void launch_cgstep(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, SolveCtl* ctl,
+                   SysPtrs<Ti> sys, const Ti* p, int it, const double* partA, double* partB,
+                   long stride) {
+    switch (g.cpt) {
+        case 1: cgstep_cpt<Ti, 1>(s, g, bub, ctl, sys, p, it, partA, partB, stride); break;
+        case 2: cgstep_cpt<Ti, 2>(s, g, bub, ctl, sys, p, it, partA, partB, stride); break;
+        case 4: cgstep_cpt<Ti, 4>(s, g, bub, ctl, sys, p, it, partA, partB, stride); break;
+        default: cgstep_cpt<Ti, 8>(s, g, bub, ctl, sys, p, it, partA, partB, stride); break;
+    }
+}
+
+void launch_finalize(hipStream_t s, int nb, const BubbleDev* bub, SolveCtl* ctl, SolveHist* hist,
+                     const SolveParams& sp, const double* partR, long stride) {
+    hipLaunchKernelGGL(k_finalize, dim3(nb), dim3(256), 0, s, bub, ctl, hist, sp, partR, stride);
+}
+
+void launch_reset_ctl(hipStream_t s, SolveCtl* ctl, int nb) {
+    hipLaunchKernelGGL(k_reset_ctl, dim3((nb + 63) / 64), dim3(64), 0, s, ctl, nb);
+}
+
+template <typename T>
+void launch_update(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, const BubbleTime* bt,
+                   const SolveCtl* ctl, const Tables& tab, StatePtrs<T> st, AirPtrs<T> air,
+                   const double* x0, const double* x1, StatePtrs<T> out) {
+    hipLaunchKernelGGL(k_update<T>, dim3(g.ntiles), dim3(256), 0, s, g.tiles, bub, bt, ctl, tab, st,
+                       air, x0, x1, out);
+}
+
+void launch_advance_time(hipStream_t s, BubbleTime* bt, int nb) {
+    hipLaunchKernelGGL(k_advance_time, dim3((nb + 63) / 64), dim3(64), 0, s, bt, nb);
+}
+
+template <typename T>
+void launch_row_stats(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, const Tables& tab,
+                      const T* eta, double* out, long stride) {
+    hipLaunchKernelGGL(k_row_stats<T>, dim3(g.nrows), dim3(256), 0, s, g.rows, bub, tab, eta, out,
+                       stride);
+}
+
+// explicit instantiations
+template void launch_advect<float>(hipStream_t, const LaunchGeom&, const BubbleDev*, const Tables&,
+                                   StatePtrs<float>, StatePtrs<float>);
+template void launch_advect<double>(hipStream_t, const LaunchGeom&, const BubbleDev*, const Tables&,
+                                    StatePtrs<double>, StatePtrs<double>);
+#define SF_INST_SYS(T, Ti)                                                                        \
+    template void launch_assemble<T, Ti>(hipStream_t, const LaunchGeom&, const BubbleDev*,        \
+                                         const BubbleTime*, const Tables&, StatePtrs<T>,          \
+                                         AirPtrs<T>, SysPtrs<Ti>, double);
+SF_INST_SYS(float, float)
+SF_INST_SYS(float, double)
+SF_INST_SYS(double, double)
+#define SF_INST_TI(Ti)                                                                            \
+    template void launch_resid<Ti>(hipStream_t, const LaunchGeom&, const BubbleDev*, SolveCtl*,   \
+                                   SysPtrs<Ti>, const double*, double*, int, int, double*, long); \
+    template void launch_spmv<Ti>(hipStream_t, const LaunchGeom&, const BubbleDev*, SolveCtl*,    \
+                                  SysPtrs<Ti>, const Ti*, Ti*, int, const SolveParams&,           \
+                                  const double*, const double*, double*, long);                   \
+    template void launch_cgstep<Ti>(hipStream_t, const LaunchGeom&, const BubbleDev*, SolveCtl*,  \
+                                    SysPtrs<Ti>, const Ti*, int, const double*, double*, long);
+SF_INST_TI(float)
+SF_INST_TI(double)
+template void launch_update<float>(hipStream_t, const LaunchGeom&, const BubbleDev*,
+                                   const BubbleTime*, const SolveCtl*, const Tables&,
+                                   StatePtrs<float>, AirPtrs<float>, const double*, const double*,
+                                   StatePtrs<float>);
+template void launch_update<double>(hipStream_t, const LaunchGeom&, const BubbleDev*,
+                                    const BubbleTime*, const SolveCtl*, const Tables&,
+                                    StatePtrs<double>, AirPtrs<double>, const double*,
+                                    const double*, StatePtrs<double>);
+template void launch_row_stats<float>(hipStream_t, const LaunchGeom&, const BubbleDev*,
+                                      const Tables&, const float*, double*, long);
+template void launch_row_stats<double>(hipStream_t, const LaunchGeom&, const BubbleDev*,
+                                       const Tables&, const double*, double*, long);
+
+}  // namespace sfk

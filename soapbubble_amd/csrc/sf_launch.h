@@ -1,0 +1,89 @@
+// Host-side launch interface of the soap-film kernels (sf_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include "sf_device.h"
+
+namespace sfk {
+
+template <typename T>
+struct StatePtrs {
+    T* uth;
+    T* uph;
+    T* eta;
+    T* gam;
+};
+
+template <typename T>
+struct AirPtrs {
+    const T* ua0;  // u_phi-face air velocity at t = 0 (uair_init)
+    const T* va0;  // u_theta-face air velocity at t = 0 (vair_init)
+    const T* au;   // corridor shape projected on e_phi (per unit time factor)
+    const T* av;   // corridor shape projected on e_theta
+};
+
+template <typename Ti>
+struct SysPtrs {
+    double* wN64;    // north-face weight of each cell (row 0: 0)
+    double* wW64;    // west-face weight of each cell
+    double* diag64;  // diagonal
+    double* b64;     // right-hand side
+    double* x64;     // initial guess written by the assembly
+    Ti* wN;          // inner-precision copies of the operator
+    Ti* wW;
+    Ti* diag;
+    Ti* e;           // inner correction
+    Ti* r;           // inner residual
+    Ti* z;           // preconditioned residual
+    Ti* Ap;
+};
+
+struct SolveParams {
+    double tol;         // outer tolerance on |r| / |b| (or |r|)
+    double inner_rho2;  // inner stop: |r|^2 <= inner_rho2 * |r_0|^2
+    double inner_frac;  // ... or |r| <= inner_frac * tol * |b|
+    int absolute;       // 1: tolerance on |r| (AmgX ABSOLUTE)
+    int pad_;
+};
+
+struct LaunchGeom {
+    const int4* tiles;   // cell tiles: (bubble, row, first column, width)
+    int ntiles;
+    const int2* rows;    // row blocks: (bubble, row)
+    int nrows;
+    int row_threads;     // threads of a row block
+    int cpt;             // cells per thread bound of the line solve
+    size_t lds_bytes;    // dynamic LDS of a row block
+};
+
+template <typename T>
+void launch_advect(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, const Tables& tab,
+                   StatePtrs<T> in, StatePtrs<T> out);
+template <typename T, typename Ti>
+void launch_assemble(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub,
+                     const BubbleTime* bt, const Tables& tab, StatePtrs<T> st, AirPtrs<T> air,
+                     SysPtrs<Ti> sys, double gmax_state);
+template <typename Ti>
+void launch_resid(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, SolveCtl* ctl,
+                  SysPtrs<Ti> sys, const double* xo, double* xn, int apply_e, int k, double* part,
+                  long stride);
+template <typename Ti>
+void launch_spmv(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, SolveCtl* ctl,
+                 SysPtrs<Ti> sys, const Ti* pold, Ti* pnew, int it, const SolveParams& sp,
+                 const double* partR, const double* partB, double* partA, long stride);
+template <typename Ti>
+void launch_cgstep(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, SolveCtl* ctl,
+                   SysPtrs<Ti> sys, const Ti* p, int it, const double* partA, double* partB,
+                   long stride);
+void launch_finalize(hipStream_t s, int nb, const BubbleDev* bub, SolveCtl* ctl, SolveHist* hist,
+                     const SolveParams& sp, const double* partR, long stride);
+void launch_reset_ctl(hipStream_t s, SolveCtl* ctl, int nb);
+template <typename T>
+void launch_update(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, const BubbleTime* bt,
+                   const SolveCtl* ctl, const Tables& tab, StatePtrs<T> st, AirPtrs<T> air,
+                   const double* x0, const double* x1, StatePtrs<T> out);
+void launch_advance_time(hipStream_t s, BubbleTime* bt, int nb);
+template <typename T>
+void launch_row_stats(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, const Tables& tab,
+                      const T* eta, double* out, long stride);
+
+}  // namespace sfk

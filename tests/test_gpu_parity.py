@@ -1,0 +1,81 @@
+"""GPU parity of the HIP step against the FP64 oracle (needs an MI355X)."""
+import numpy as np
+import pytest
+
+import soapbubble_amd as sb
+from cases import copy_state, csr_parts, load, make_case
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32_adv", 3e-6), ("mixed", 3e-6)])
+def test_advect_stage(oracle, prec, tol):
+    _say(f"advect stage {prec} n=32")
+    n, dt = 32, 0.005
+    st, ua, va = make_case(n, 11, u_scale=0.8)
+    p = oracle.params(n, dt)
+    ref = oracle.advect(p, st)
+    with sb.SoapFilm(sb.bubble(n, dt), prec) as ctx:
+        load(ctx, 0, st, ua, va)
+        ctx.advect()
+        for k in ("u_theta", "u_phi", "eta", "gamma"):
+            got = ctx.download(0, k + "_star")
+            assert rel_l2(got, ref[k]) < tol, (k, rel_l2(got, ref[k]))
+
+
+def test_stages_fp64(oracle):
+    _say("stages fp64 n=32")
+    n, dt, t = 32, 0.005, 0.7
+    st, ua, va = make_case(n, 12, u_scale=0.8)
+    p = oracle.params(n, dt, time_s=t)
+    adv = oracle.advect(p, st)
+    uair, vair = oracle.airflow(p, ua, va)
+    val, rhs, gam_s = oracle.assemble(p, adv, uair, vair)
+    x, s, rc = oracle.solve(p, val, rhs, gam_s)
+    assert rc == 0 and s.rel_residual < 1e-12
+    adv_s = dict(adv, gamma=gam_s)
+    new = oracle.update(p, adv_s, x, uair, vair)
+    wN, wW, dg, _, _ = csr_parts(val)
+    with sb.SoapFilm(sb.bubble(n, dt), "fp64", advance_time=False) as ctx:
+        load(ctx, 0, st, ua, va)
+        ctx.set_time(t)
+        ctx.advect()
+        ctx.assemble()
+        assert rel_l2(ctx.download(0, "w_north"), wN) < 1e-13
+        assert rel_l2(ctx.download(0, "w_west"), wW) < 1e-13
+        assert rel_l2(ctx.download(0, "diag"), dg) < 1e-13
+        assert rel_l2(ctx.download(0, "rhs"), rhs) < 1e-13
+        ctx.solve()
+        assert rel_l2(ctx.download(0, "solution"), x) < 1e-10
+        st_gpu = ctx.stats(0)
+        assert st_gpu["converged"] == 1 and st_gpu["residual"] <= 1e-10
+        ctx.update()
+        got = ctx.state(0)
+        for k in ("u_theta", "u_phi", "eta", "gamma"):
+            assert rel_l2(got[k], new[k]) < 1e-10, (k, rel_l2(got[k], new[k]))
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-9), ("fp32_adv", 5e-5), ("mixed", 5e-5)])
+def test_steps_c0(oracle, prec, tol):
+    """64 x 128, dt = 0.005, 20 steps (the CPU-runnable workload)."""
+    _say(f"20 steps {prec} 64x128")
+    n, dt, steps = 64, 0.005, 20
+    st, ua, va = make_case(n, 0)
+    p = oracle.params(n, dt)
+    ref = copy_state(st)
+    oracle.step(p, ref, ua, va, steps)
+    with sb.SoapFilm(sb.bubble(n, dt), prec) as ctx:
+        load(ctx, 0, st, ua, va)
+        ctx.step(steps)
+        got = ctx.state(0)
+        s = ctx.stats(0)
+        assert s["failed_solves"] == 0 and s["max_residual"] <= 1e-10
+    for k in ("u_theta", "u_phi", "eta", "gamma"):
+        e = rel_l2(got[k], ref[k])
+        _say(f"  {k}: rel L2 {e:.3e}")
+        assert e < tol, (k, e)
