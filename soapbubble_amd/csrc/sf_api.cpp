@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -84,7 +85,9 @@ struct Base {
     SolveParams sp{};
     int max_outer = 3, max_inner = 500;
     bool calibrated = false;
-    int calib_inner = 0, calib_outer = 0;
+    std::vector<int> calib;   // max inner iterations per refinement seen while calibrating
+    std::vector<int> sched;   // inner iterations per refinement of the fixed schedule
+    bool trace_solve = std::getenv("SF_SOLVE_TRACE") != nullptr;
     int steps_done = 0;
     int stage = 0;            // 0 idle, 1 advected, 2 assembled, 3 solved
     hipGraphExec_t graph = nullptr;
@@ -101,8 +104,8 @@ struct Base {
     }
     virtual void advect() = 0;
     virtual void assemble() = 0;
-    virtual void solve(bool host_driven) = 0;
-    virtual void update() = 0;
+    virtual void solve(bool host_driven, bool finalize) = 0;
+    virtual void update(bool finalize) = 0;
     virtual void upload(int b, int field, const double* h) = 0;
     virtual void download(int b, int field, double* h) = 0;
     virtual void row_stats() = 0;
@@ -161,26 +164,27 @@ struct Impl : Base {
         return v;
     }
 
-    void advect() override { launch_advect<T>(stream, geom, bub_d, tab, cur, nxt); }
+    void advect() override { launch_advect<T>(stream, geom, bub_d, time_d, tab, cur, nxt); }
 
+    // Assembly fused with the first FP64 residual and line solve; x0 lands
+    // in x64[0].
     void assemble() override {
-        // all bubbles share one launch; the broken-cell value of Gamma* per
-        // bubble is the state-precision gamma_max, identical for a context
-        // whose bubbles share Gamma_m and T (checked at creation).
-        launch_assemble<T, Ti>(stream, geom, bub_d, time_d, tab, nxt, air(), sys(x64[1]),
-                               (double)state_gmax(bub_h[0].gamma_max));
+        // the broken-cell value of Gamma* is the state-precision gamma_max,
+        // one value for a context whose bubbles share Gamma_m and T
+        launch_assemble<T, Ti>(stream, geom, bub_d, time_d, tab, nxt, air(), sys(x64[0]),
+                               (double)state_gmax(bub_h[0].gamma_max), ctl_d, partR, stride);
     }
 
+    // FP64 residual evaluation k >= 1: x64[k&1] = x64[(k-1)&1] + e.
     void resid(int k) {
         const double* xo = x64[(k + 1) & 1];
         double* xn = x64[k & 1];
-        launch_resid<Ti>(stream, geom, bub_d, ctl_d, sys(x64[1]), xo, xn, k > 0 ? 1 : 0, k, partR,
-                         stride);
+        launch_resid<Ti>(stream, geom, bub_d, ctl_d, sys(x64[0]), xo, xn, 1, k, partR, stride);
     }
     void inner(int it) {
-        launch_spmv<Ti>(stream, geom, bub_d, ctl_d, sys(x64[1]), p[(it + 1) & 1], p[it & 1], it, sp,
+        launch_spmv<Ti>(stream, geom, bub_d, ctl_d, sys(x64[0]), p[(it + 1) & 1], p[it & 1], it, sp,
                         partR, partB, partA, stride);
-        launch_cgstep<Ti>(stream, geom, bub_d, ctl_d, sys(x64[1]), p[it & 1], it, partA, partB,
+        launch_cgstep<Ti>(stream, geom, bub_d, ctl_d, sys(x64[0]), p[it & 1], it, partA, partB,
                           stride);
     }
 
@@ -194,44 +198,55 @@ struct Impl : Base {
         return true;
     }
 
-    void solve(bool host_driven) override {
-        launch_reset_ctl(stream, ctl_d, nb);
-        resid(0);
+    // Refinement loop after the assembly. Host-driven: convergence flags are
+    // read back after each residual and every four inner iterations (the
+    // plain path, and the calibration steps of the graph schedule). Fixed:
+    // the calibrated schedule, every kernel launched, converged bubbles
+    // skip the work (graph capture).
+    void solve(bool host_driven, bool finalize) override {
         if (host_driven) {
             std::vector<SolveCtl> c(nb);
             std::vector<int> before(nb);
-            int max_in_used = 0, outers_used = 0;
+            std::vector<int> used;
             for (int k = 0; k < max_outer; ++k) {
-                launch_spmv<Ti>(stream, geom, bub_d, ctl_d, sys(x64[1]), p[1], p[0], 0, sp, partR,
+                launch_spmv<Ti>(stream, geom, bub_d, ctl_d, sys(x64[0]), p[1], p[0], 0, sp, partR,
                                 partB, partA, stride);
-                if (all_flags(c, false)) break;
+                const bool done = all_flags(c, false);
+                if (trace_solve)
+                    std::fprintf(stderr, "[sf solve] outer %d residual %.3e (|b|^2 %.3e)%s\n", k,
+                                 c[0].rel_res, c[0].bb, done ? " converged" : "");
+                if (done) break;
                 for (int b = 0; b < nb; ++b) before[b] = c[b].iters;
-                launch_cgstep<Ti>(stream, geom, bub_d, ctl_d, sys(x64[1]), p[0], 0, partA, partB,
+                launch_cgstep<Ti>(stream, geom, bub_d, ctl_d, sys(x64[0]), p[0], 0, partA, partB,
                                   stride);
                 for (int it = 1; it < max_inner; ++it) {
                     inner(it);
                     if ((it & 3) == 3 && all_flags(c, true)) break;
                 }
                 all_flags(c, true);
-                for (int b = 0; b < nb; ++b) max_in_used = std::max(max_in_used, c[b].iters - before[b]);
-                outers_used = k + 1;
+                int mx = 0;
+                for (int b = 0; b < nb; ++b) mx = std::max(mx, c[b].iters - before[b]);
+                used.push_back(mx);
+                if (trace_solve)
+                    std::fprintf(stderr, "[sf solve]   inner iterations %d\n", c[0].iters - before[0]);
                 resid(k + 1);
             }
-            if (!calibrated && !fp64_inner() && opt.max_inner == 0) {
-                calib_inner = std::max(calib_inner, max_in_used);
-                calib_outer = std::max(calib_outer, outers_used);
+            if (!calibrated) {
+                if (calib.size() < used.size()) calib.resize(used.size(), 0);
+                for (size_t k = 0; k < used.size(); ++k) calib[k] = std::max(calib[k], used[k]);
             }
         } else {
-            for (int k = 0; k < max_outer; ++k) {
-                for (int it = 0; it < max_inner; ++it) inner(it);
-                resid(k + 1);
+            for (size_t k = 0; k < sched.size(); ++k) {
+                for (int it = 0; it < sched[k]; ++it) inner(it);
+                resid((int)k + 1);
             }
         }
-        launch_finalize(stream, nb, bub_d, ctl_d, hist_d, sp, partR, stride);
+        if (finalize) launch_finalize(stream, nb, bub_d, ctl_d, hist_d, sp, partR, stride);
     }
 
-    void update() override {
-        launch_update<T>(stream, geom, bub_d, time_d, ctl_d, tab, nxt, air(), x64[0], x64[1], cur);
+    void update(bool finalize) override {
+        launch_update<T>(stream, geom, bub_d, time_d, ctl_d, tab, nxt, air(), x64[0], x64[1], cur,
+                         finalize ? 1 : 0, hist_d, sp, partR, stride);
     }
 
     // ---- host <-> device copies
@@ -484,7 +499,7 @@ void setup(Base& c, int prec) {
         c.max_outer = c.opt.max_outer > 0 ? c.opt.max_outer : 3;
         c.max_inner = c.opt.max_inner > 0 ? c.opt.max_inner : 1000;
     } else {
-        const double rho = c.opt.inner_rho > 0 ? c.opt.inner_rho : 1e-4;
+        const double rho = c.opt.inner_rho > 0 ? c.opt.inner_rho : 1e-7;
         c.sp.inner_rho2 = rho * rho;
         c.max_outer = c.opt.max_outer > 0 ? c.opt.max_outer : 4;
         c.max_inner = c.opt.max_inner > 0 ? c.opt.max_inner : 200;
@@ -538,11 +553,10 @@ void eager_step(Base& c, bool host_driven) {
     SF_HIP(hipEventRecord(c.ev[1], c.stream));
     c.assemble();
     SF_HIP(hipEventRecord(c.ev[2], c.stream));
-    c.solve(host_driven);
+    c.solve(host_driven, false);
     SF_HIP(hipEventRecord(c.ev[3], c.stream));
-    c.update();
+    c.update(true);
     SF_HIP(hipEventRecord(c.ev[4], c.stream));
-    launch_advance_time(c.stream, c.time_d, c.nb);
     SF_HIP(hipEventSynchronize(c.ev[4]));
     time_stage(c, 0, 1, c.timing.advect_ms);
     time_stage(c, 1, 2, c.timing.assemble_ms);
@@ -557,9 +571,8 @@ void build_graph(Base& c) {
     SF_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
     c.advect();
     c.assemble();
-    c.solve(false);
-    c.update();
-    launch_advance_time(c.stream, c.time_d, c.nb);
+    c.solve(false, false);
+    c.update(true);
     SF_HIP(hipStreamEndCapture(c.stream, &g));
     hipError_t e = hipGraphInstantiate(&c.graph, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
@@ -646,7 +659,10 @@ int sf_set_time(sf_ctx* ctx, int32_t b, double t) {
         SF_HIP(hipStreamSynchronize(c.stream));
         SF_HIP(hipMemcpy(c.time_h.data(), c.time_d, c.nb * sizeof(BubbleTime), hipMemcpyDeviceToHost));
         for (int k = 0; k < c.nb; ++k)
-            if (b < 0 || b == k) c.time_h[k].t = t;
+            if (b < 0 || b == k) {
+                c.time_h[k].t = t;
+                c.time_h[k].pending = 0;
+            }
         if (b >= c.nb) throw Status{SF_ERR_INVALID, "bubble index out of range"};
         SF_HIP(hipMemcpy(c.time_d, c.time_h.data(), c.nb * sizeof(BubbleTime), hipMemcpyHostToDevice));
         return (int)SF_OK;
@@ -684,15 +700,18 @@ int sf_step(sf_ctx* ctx, int32_t nsteps) {
         if (want_graph && !c.calibrated) {
             // calibration: two host-driven steps fix the refinement schedule
             for (; k < nsteps && k < 2; ++k) eager_step(c, true);
-            if (k == 2 || k == nsteps) {
-                if (c.opt.max_inner == 0 && c.calib_inner > 0) {
-                    c.max_inner = c.calib_inner + 2;
-                    c.max_outer = std::max(c.calib_outer, 1) + 1;
-                }
-                if (k == 2) {
-                    c.calibrated = true;
-                    build_graph(c);
-                }
+            if (k == 2) {
+                // fixed schedule: the refinements seen, each with one inner
+                // iteration of margin, plus one spare refinement; a bubble
+                // that converged skips the remaining kernels
+                c.sched.clear();
+                for (int n : c.calib) c.sched.push_back(c.opt.max_inner > 0 ? c.opt.max_inner : n + 1);
+                if (c.sched.empty()) c.sched.push_back(1);
+                const int last = c.sched.back();
+                c.sched.push_back(std::max(3, last));
+                if (c.opt.max_outer > 0) c.sched.resize((size_t)c.opt.max_outer, std::max(3, last));
+                c.calibrated = true;
+                build_graph(c);
             }
         }
         if (want_graph && c.graph) {
@@ -739,7 +758,7 @@ int sf_solve(sf_ctx* ctx) {
         if (!ctx || !ctx->impl) throw Status{SF_ERR_INVALID, "null context"};
         Base& c = *ctx->impl;
         if (c.stage != 2) throw Status{SF_ERR_ORDER, "sf_solve must follow sf_assemble"};
-        c.solve(true);
+        c.solve(true, true);
         c.stage = 3;
         SF_HIP(hipStreamSynchronize(c.stream));
         std::vector<SolveCtl> h(c.nb);
@@ -755,8 +774,7 @@ int sf_update(sf_ctx* ctx) {
         if (!ctx || !ctx->impl) throw Status{SF_ERR_INVALID, "null context"};
         Base& c = *ctx->impl;
         if (c.stage != 3) throw Status{SF_ERR_ORDER, "sf_update must follow sf_solve"};
-        c.update();
-        launch_advance_time(c.stream, c.time_d, c.nb);
+        c.update(false);
         c.stage = 0;
         c.steps_done += 1;
         SF_HIP(hipStreamSynchronize(c.stream));
@@ -802,7 +820,7 @@ int sf_get_stats(sf_ctx* ctx, int32_t b, sf_bubble_stats* out) {
         out->volume = vol * B.h * B.h;
         out->broken = (pos == 0.0) || (c.steps_done > 0 && !sc.converged);
         out->steps = c.steps_done;
-        out->time = bt.t;
+        out->time = bt.t + (bt.pending ? bt.dt_s : 0.0);
         out->failed_solves = hs.fails;
         out->max_residual = hs.max_res;
         return (int)SF_OK;

@@ -24,12 +24,14 @@ struct BubbleDev {
 };
 
 // Air-flow time of one bubble (seconds) and its step in seconds; kernels
-// derive the corridor factor 3 - |t - 3| and the mirror blend t/12.
+// derive the corridor factor 3 - |t - 3| and the mirror blend t/12. The
+// update kernel marks a pending advance (when `advance`), which the next
+// step's advection kernel applies before the assembly reads the time.
 struct BubbleTime {
     double t;
     double dt_s;
     int advance;
-    int pad_;
+    int pending;
 };
 
 // Per-bubble control block of the iterative solve. Every field is written by

@@ -29,10 +29,17 @@ constexpr double kThresh = 0.0; // bubble.cu:27
 // ------------------------------------------------------------ advection ----
 template <typename T>
 __global__ __launch_bounds__(256) void k_advect(const int4* __restrict__ tiles,
-                                                const BubbleDev* __restrict__ bub, Tables tab,
+                                                const BubbleDev* __restrict__ bub,
+                                                BubbleTime* __restrict__ btime, Tables tab,
                                                 StatePtrs<T> in, StatePtrs<T> out) {
     const int4 tl = tiles[blockIdx.x];
     const BubbleDev B = bub[tl.x];
+    // the previous step's time advance lands here, before any kernel of this
+    // step reads the time
+    if (tl.y == 0 && tl.z == 0 && threadIdx.x == 0 && btime[tl.x].pending) {
+        btime[tl.x].t += btime[tl.x].dt_s;
+        btime[tl.x].pending = 0;
+    }
     const int j = tl.y + (int)threadIdx.x / tl.w;
     const int i = tl.z + (int)threadIdx.x % tl.w;
     if (j >= B.nth || i >= B.nph) return;
@@ -122,38 +129,40 @@ __device__ __forceinline__ bool is_broken(double gamma, double eta, double gmax)
 // rhs gamma_max; its coupling to unbroken neighbours is moved to their
 // right-hand sides (x_broken = gamma_max is known), which leaves the
 // solution unchanged and keeps the operator symmetric.
-template <typename T, typename Ti>
-__global__ __launch_bounds__(256) void k_assemble(const int4* __restrict__ tiles,
-                                                  const BubbleDev* __restrict__ bub,
-                                                  const BubbleTime* __restrict__ btime, Tables tab,
-                                                  StatePtrs<T> s, AirPtrs<T> air, SysPtrs<Ti> sys,
-                                                  T gmax_state) {
-    const int4 tl = tiles[blockIdx.x];
-    const BubbleDev B = bub[tl.x];
-    const int j = tl.y + (int)threadIdx.x / tl.w;
-    const int i = tl.z + (int)threadIdx.x % tl.w;
-    if (j >= B.nth || i >= B.nph) return;
+struct CellSys {
+    double wN, wS, wW, wE;   // effective face weights (0 across broken cells)
+    double diag, rhs;
+    double x0, x0N, x0S, x0W, x0E;  // initial guess Gamma* at the cell and neighbours
+    bool brk;
+};
+
+template <typename T>
+__device__ __forceinline__ CellSys assemble_cell(const BubbleDev& B, double t, const Tables& tab,
+                                                 const StatePtrs<T>& s, const AirPtrs<T>& air,
+                                                 int j, int i) {
     const int nph = B.nph, m = B.mask, nth = B.nth;
     const size_t o = (size_t)B.off;
     const size_t c = o + (size_t)j * nph + i;
     const int iE = (i + 1) & m, iW = (i - 1) & m;
     const double gmax = B.gamma_max;
-
     const T* eta = s.eta;
-    T* gam = s.gam;
+    const T* gam = s.gam;
     const double e = (double)eta[c];
     const double gamma = (double)gam[c];
-    const bool brk = is_broken(gamma, e, gmax);
+    CellSys r;
+    r.brk = is_broken(gamma, e, gmax);
 
     const size_t cW = o + (size_t)j * nph + iW, cE = o + (size_t)j * nph + iE;
     const size_t cN = j > 0 ? c - nph : o + (size_t)((i + nth) & m);
     const size_t cS = j < nth - 1 ? c + nph : o + (size_t)j * nph + ((i + nth) & m);
     const double etaW = (double)eta[cW], etaE = (double)eta[cE];
     const double etaN = (double)eta[cN], etaS = (double)eta[cS];
-    const bool bW = is_broken((double)gam[cW], etaW, gmax);
-    const bool bE = is_broken((double)gam[cE], etaE, gmax);
-    const bool bN = j > 0 && is_broken((double)gam[cN], etaN, gmax);
-    const bool bS = j < nth - 1 && is_broken((double)gam[cS], etaS, gmax);
+    const double gW = (double)gam[cW], gE = (double)gam[cE];
+    const double gN = (double)gam[cN], gS = (double)gam[cS];
+    const bool bW = is_broken(gW, etaW, gmax);
+    const bool bE = is_broken(gE, etaE, gmax);
+    const bool bN = j > 0 && is_broken(gN, etaN, gmax);
+    const bool bS = j < nth - 1 && is_broken(gS, etaS, gmax);
 
     const double dt = B.dt, invDt = 1.0 / dt, Cr = B.Cr;
     const double CrDt = Cr * dt, MDt = B.M * dt, s2 = B.invh * B.invh;
@@ -166,13 +175,11 @@ __global__ __launch_bounds__(256) void k_assemble(const int4* __restrict__ tiles
     const double wE = s2 * csc * MDt / (etaRight + CrDt);
     const double wS = s2 * sinS * MDt / (etaDown + CrDt);
 
-    double diag, rhs;
-    if (brk) {
-        diag = 1.0;
-        rhs = gmax;
-        gam[c] = gmax_state;   // the reference overwrites Gamma* (bubble.cu:1325)
+    if (r.brk) {
+        r.diag = 1.0;
+        r.rhs = gmax;
     } else {
-        diag = sinTheta * invDt / gamma - (((-wN) + (-wW)) + (-wE) + (-wS));
+        r.diag = sinTheta * invDt / gamma - (((-wN) + (-wW)) + (-wE) + (-wS));
         double uW = (double)s.uph[c], uE = (double)s.uph[cE];
         double vN = j > 0 ? (double)s.uth[c - nph] : 0.0;
         double vS = j < nth - 1 ? (double)s.uth[c] : 0.0;
@@ -185,7 +192,7 @@ __global__ __launch_bounds__(256) void k_assemble(const int4* __restrict__ tiles
                                (vS * sinS * etaDown / (etaDown + Cr * dt) -
                                 vN * sinN * etaUp / (etaUp + Cr * dt)));
         if (B.airflow) {
-            const double t = btime[tl.x].t, kt = 3.0 - fabs(t - 3.0);
+            const double kt = 3.0 - fabs(t - 3.0);
             const double bl = t / 12.0;
             const size_t row = o + (size_t)j * nph;
             const int opp = (nph - i) & m, oppE = (nph - iE) & m;
@@ -213,22 +220,94 @@ __global__ __launch_bounds__(256) void k_assemble(const int4* __restrict__ tiles
                    (sinS * sinS * etaDown / (invDt * etaDown + Cr) -
                     sinN * sinN * etaUp / (invDt * etaUp + Cr));
         }
-        rhs = sinTheta * invDt - div;
-        if (bN) rhs += wN * gmax;
-        if (bS) rhs += wS * gmax;
-        if (bW) rhs += wW * gmax;
-        if (bE) rhs += wE * gmax;
+        r.rhs = sinTheta * invDt - div;
+        if (bN) r.rhs += wN * gmax;
+        if (bS) r.rhs += wS * gmax;
+        if (bW) r.rhs += wW * gmax;
+        if (bE) r.rhs += wE * gmax;
     }
-    const double wNe = (brk || bN) ? 0.0 : wN;
-    const double wWe = (brk || bW) ? 0.0 : wW;
-    sys.wN64[c] = wNe;
-    sys.wW64[c] = wWe;
-    sys.diag64[c] = diag;
-    sys.b64[c] = rhs;
-    sys.x64[c] = brk ? gmax : gamma;      // x0 = Gamma* (bubble.cu:1480-1486)
-    sys.wN[c] = (Ti)wNe;
-    sys.wW[c] = (Ti)wWe;
-    sys.diag[c] = (Ti)diag;
+    r.wN = (r.brk || bN) ? 0.0 : wN;
+    r.wS = (r.brk || bS) ? 0.0 : wS;
+    r.wW = (r.brk || bW) ? 0.0 : wW;
+    r.wE = (r.brk || bE) ? 0.0 : wE;
+    // x0 = Gamma* after the broken-cell override (bubble.cu:1325, 1480-1486)
+    r.x0 = r.brk ? gmax : gamma;
+    r.x0N = bN ? gmax : gN;
+    r.x0S = bS ? gmax : gS;
+    r.x0W = bW ? gmax : gW;
+    r.x0E = bE ? gmax : gE;
+    return r;
+}
+
+// Row-block assembly: coefficients and rhs of one theta row, the initial
+// FP64 residual r0 = b - A Gamma*, r_in = (Ti) r0 and the first line solve
+// z = M^-1 r_in. Partials as k_resid. The block of row 0 resets the
+// bubble's solver control block; the broken-cell override of Gamma* is
+// written back to the state as the reference does.
+template <typename T, typename Ti, int CPT>
+__global__ void k_assemble(const int2* __restrict__ rows, const BubbleDev* __restrict__ bub,
+                           const BubbleTime* __restrict__ btime, Tables tab, StatePtrs<T> s,
+                           AirPtrs<T> air, SysPtrs<Ti> sys, T gmax_state,
+                           SolveCtl* __restrict__ ctl, double* __restrict__ part, long stride) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    __shared__ double red[16 * 4];
+    const int2 rw = rows[blockIdx.x];
+    const BubbleDev B = bub[rw.x];
+    const int j = rw.y, nph = B.nph;
+    const double t = btime[rw.x].t;
+    Ti* sa = reinterpret_cast<Ti*>(lds_raw);
+    Ti* sc = sa + nph;
+    Ti* sd = sc + nph;
+    const size_t row = (size_t)B.off + (size_t)j * nph;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = threadIdx.x; i < nph; i += blockDim.x) {
+        const CellSys q = assemble_cell<T>(B, t, tab, s, air, j, i);
+        const size_t c = row + i;
+        const double ax = q.diag * q.x0 - q.wN * q.x0N - q.wS * q.x0S - q.wW * q.x0W -
+                          q.wE * q.x0E;
+        const double r0 = q.rhs - ax;
+        const Ti ri = (Ti)r0;
+        sys.wN64[c] = q.wN;
+        sys.wW64[c] = q.wW;
+        sys.diag64[c] = q.diag;
+        sys.b64[c] = q.rhs;
+        sys.x64[c] = q.x0;
+        sys.wN[c] = (Ti)q.wN;
+        sys.wW[c] = (Ti)q.wW;
+        sys.diag[c] = (Ti)q.diag;
+        sys.r[c] = ri;
+        const Ti dg = (Ti)q.diag;
+        const Ti inv = Ti(1) / dg;
+        sa[i] = -(Ti)q.wW * inv;
+        sc[i] = -(Ti)q.wE * inv;
+        sd[i] = ri * inv;
+        acc[0] += r0 * r0;
+        acc[1] += q.rhs * q.rhs;
+        acc[3] += (double)ri * (double)ri;
+    }
+    __syncthreads();
+    // every thread has read the neighbours' Gamma*: apply the override now
+    for (int i = threadIdx.x; i < nph; i += blockDim.x) {
+        const size_t c = row + i;
+        if (is_broken((double)s.gam[c], (double)s.eta[c], B.gamma_max)) s.gam[c] = gmax_state;
+    }
+    pcr_periodic<Ti, CPT>(sa, sc, sd, nph);
+    for (int i = threadIdx.x; i < nph; i += blockDim.x) {
+        const Ti z = sd[i];
+        sys.z[row + i] = z;
+        acc[2] += (double)sys.r[row + i] * (double)z;
+    }
+    block_sum<4>(acc, red);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) part[q * stride + B.pr0 + j] = acc[q];
+        if (j == 0) {
+            SolveCtl z = {};
+            z.xbuf = 0;
+            z.outers = 1;
+            ctl[rw.x] = z;
+        }
+    }
 }
 
 // -------------------------------------------------------- line solve ----
@@ -419,32 +498,31 @@ __global__ void k_cgstep(const int2* __restrict__ rows, const BubbleDev* __restr
 }
 
 // ----------------------------------------------------------- finalize ----
-// One block per bubble: the verdict from the last FP64 residual evaluated.
-__global__ void k_finalize(const BubbleDev* __restrict__ bub, SolveCtl* __restrict__ ctl,
-                           SolveHist* __restrict__ hist, SolveParams sp,
-                           const double* __restrict__ partR, long stride) {
-    __shared__ double red[16 * 4];
-    const BubbleDev B = bub[blockIdx.x];
+// The verdict from the last FP64 residual evaluated (block-wide; every
+// thread of the block must call it).
+__device__ __forceinline__ void finalize_block(const BubbleDev& B, SolveCtl* __restrict__ C,
+                                               SolveHist* __restrict__ Hh, const SolveParams& sp,
+                                               const double* __restrict__ partR, long stride,
+                                               double* red) {
     double s[2];
     sum_partials<2>(partR, stride, B.pr0, B.nth, s, red);
     if (threadIdx.x == 0) {
-        SolveCtl* C = ctl + blockIdx.x;
         const double res = sp.absolute ? sqrt(s[0]) : sqrt(s[0]) / (s[1] > 0.0 ? sqrt(s[1]) : 1.0);
         C->rel_res = res;
         C->bb = s[1];
         C->converged = res <= sp.tol ? 1 : 0;
-        SolveHist* Hh = hist + blockIdx.x;
         Hh->solves += 1;
         if (!(res <= sp.tol)) Hh->fails += 1;
         if (!(res <= Hh->max_res)) Hh->max_res = res;
     }
 }
 
-__global__ void k_reset_ctl(SolveCtl* __restrict__ ctl, int nb) {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= nb) return;
-    SolveCtl z = {};
-    ctl[b] = z;
+__global__ void k_finalize(const BubbleDev* __restrict__ bub, SolveCtl* __restrict__ ctl,
+                           SolveHist* __restrict__ hist, SolveParams sp,
+                           const double* __restrict__ partR, long stride) {
+    __shared__ double red[16 * 4];
+    const BubbleDev B = bub[blockIdx.x];
+    finalize_block(B, ctl + blockIdx.x, hist + blockIdx.x, sp, partR, stride, red);
 }
 
 // ------------------------------------------------------------- update ----
@@ -455,13 +533,23 @@ __device__ __forceinline__ double clamp_gamma(double g, double gmax) {
 template <typename T>
 __global__ __launch_bounds__(256) void k_update(const int4* __restrict__ tiles,
                                                 const BubbleDev* __restrict__ bub,
-                                                const BubbleTime* __restrict__ btime,
-                                                const SolveCtl* __restrict__ ctl, Tables tab,
+                                                BubbleTime* __restrict__ btime,
+                                                SolveCtl* __restrict__ ctl, Tables tab,
                                                 StatePtrs<T> st, AirPtrs<T> air,
                                                 const double* __restrict__ x0,
-                                                const double* __restrict__ x1, StatePtrs<T> out) {
+                                                const double* __restrict__ x1, StatePtrs<T> out,
+                                                int fin, SolveHist* __restrict__ hist,
+                                                SolveParams sp, const double* __restrict__ partR,
+                                                long stride) {
+    __shared__ double red[16 * 4];
     const int4 tl = tiles[blockIdx.x];
     const BubbleDev B = bub[tl.x];
+    if (tl.y == 0 && tl.z == 0) {
+        // first tile of the bubble: solve verdict, and the time advance that
+        // the next step's advection applies
+        if (fin) finalize_block(B, ctl + tl.x, hist + tl.x, sp, partR, stride, red);
+        if (threadIdx.x == 0 && btime[tl.x].advance) btime[tl.x].pending = 1;
+    }
     const int j = tl.y + (int)threadIdx.x / tl.w;
     const int i = tl.z + (int)threadIdx.x % tl.w;
     if (j >= B.nth || i >= B.nph) return;
@@ -531,11 +619,6 @@ __global__ __launch_bounds__(256) void k_update(const int4* __restrict__ tiles,
     }
 }
 
-__global__ void k_advance_time(BubbleTime* __restrict__ bt, int nb) {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b < nb && bt[b].advance) bt[b].t += bt[b].dt_s;
-}
-
 // Per-row film volume (sum eta sin(theta)) and count of cells with eta > 0.
 template <typename T>
 __global__ void k_row_stats(const int2* __restrict__ rows, const BubbleDev* __restrict__ bub,
@@ -560,17 +643,30 @@ __global__ void k_row_stats(const int2* __restrict__ rows, const BubbleDev* __re
 
 // ------------------------------------------------------------ launchers ----
 template <typename T>
-void launch_advect(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, const Tables& tab,
-                   StatePtrs<T> in, StatePtrs<T> out) {
-    hipLaunchKernelGGL(k_advect<T>, dim3(g.ntiles), dim3(256), 0, s, g.tiles, bub, tab, in, out);
+void launch_advect(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, BubbleTime* bt,
+                   const Tables& tab, StatePtrs<T> in, StatePtrs<T> out) {
+    hipLaunchKernelGGL(k_advect<T>, dim3(g.ntiles), dim3(256), 0, s, g.tiles, bub, bt, tab, in,
+                       out);
 }
 
+template <typename T, typename Ti, int CPT>
+static void assemble_cpt(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub,
+                         const BubbleTime* bt, const Tables& tab, StatePtrs<T> st, AirPtrs<T> air,
+                         SysPtrs<Ti> sys, double gmax_state, SolveCtl* ctl, double* part,
+                         long stride) {
+    hipLaunchKernelGGL((k_assemble<T, Ti, CPT>), dim3(g.nrows), dim3(g.row_threads), g.lds_bytes,
+                       s, g.rows, bub, bt, tab, st, air, sys, (T)gmax_state, ctl, part, stride);
+}
 template <typename T, typename Ti>
 void launch_assemble(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub,
                      const BubbleTime* bt, const Tables& tab, StatePtrs<T> st, AirPtrs<T> air,
-                     SysPtrs<Ti> sys, double gmax_state) {
-    hipLaunchKernelGGL((k_assemble<T, Ti>), dim3(g.ntiles), dim3(256), 0, s, g.tiles, bub, bt, tab,
-                       st, air, sys, (T)gmax_state);
+                     SysPtrs<Ti> sys, double gmax_state, SolveCtl* ctl, double* part, long stride) {
+    switch (g.cpt) {
+        case 1: assemble_cpt<T, Ti, 1>(s, g, bub, bt, tab, st, air, sys, gmax_state, ctl, part, stride); break;
+        case 2: assemble_cpt<T, Ti, 2>(s, g, bub, bt, tab, st, air, sys, gmax_state, ctl, part, stride); break;
+        case 4: assemble_cpt<T, Ti, 4>(s, g, bub, bt, tab, st, air, sys, gmax_state, ctl, part, stride); break;
+        default: assemble_cpt<T, Ti, 8>(s, g, bub, bt, tab, st, air, sys, gmax_state, ctl, part, stride); break;
+    }
 }
 
 template <typename Ti, int CPT>
@@ -624,20 +720,13 @@ void launch_finalize(hipStream_t s, int nb, const BubbleDev* bub, SolveCtl* ctl,
     hipLaunchKernelGGL(k_finalize, dim3(nb), dim3(256), 0, s, bub, ctl, hist, sp, partR, stride);
 }
 
-void launch_reset_ctl(hipStream_t s, SolveCtl* ctl, int nb) {
-    hipLaunchKernelGGL(k_reset_ctl, dim3((nb + 63) / 64), dim3(64), 0, s, ctl, nb);
-}
-
 template <typename T>
-void launch_update(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, const BubbleTime* bt,
-                   const SolveCtl* ctl, const Tables& tab, StatePtrs<T> st, AirPtrs<T> air,
-                   const double* x0, const double* x1, StatePtrs<T> out) {
+void launch_update(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, BubbleTime* bt,
+                   SolveCtl* ctl, const Tables& tab, StatePtrs<T> st, AirPtrs<T> air,
+                   const double* x0, const double* x1, StatePtrs<T> out, int fin, SolveHist* hist,
+                   const SolveParams& sp, const double* partR, long stride) {
     hipLaunchKernelGGL(k_update<T>, dim3(g.ntiles), dim3(256), 0, s, g.tiles, bub, bt, ctl, tab, st,
-                       air, x0, x1, out);
-}
-
-void launch_advance_time(hipStream_t s, BubbleTime* bt, int nb) {
-    hipLaunchKernelGGL(k_advance_time, dim3((nb + 63) / 64), dim3(64), 0, s, bt, nb);
+                       air, x0, x1, out, fin, hist, sp, partR, stride);
 }
 
 template <typename T>
@@ -648,14 +737,15 @@ void launch_row_stats(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, 
 }
 
 // explicit instantiations
-template void launch_advect<float>(hipStream_t, const LaunchGeom&, const BubbleDev*, const Tables&,
-                                   StatePtrs<float>, StatePtrs<float>);
-template void launch_advect<double>(hipStream_t, const LaunchGeom&, const BubbleDev*, const Tables&,
-                                    StatePtrs<double>, StatePtrs<double>);
+template void launch_advect<float>(hipStream_t, const LaunchGeom&, const BubbleDev*, BubbleTime*,
+                                   const Tables&, StatePtrs<float>, StatePtrs<float>);
+template void launch_advect<double>(hipStream_t, const LaunchGeom&, const BubbleDev*, BubbleTime*,
+                                    const Tables&, StatePtrs<double>, StatePtrs<double>);
 #define SF_INST_SYS(T, Ti)                                                                        \
     template void launch_assemble<T, Ti>(hipStream_t, const LaunchGeom&, const BubbleDev*,        \
                                          const BubbleTime*, const Tables&, StatePtrs<T>,          \
-                                         AirPtrs<T>, SysPtrs<Ti>, double);
+                                         AirPtrs<T>, SysPtrs<Ti>, double, SolveCtl*, double*,     \
+                                         long);
 SF_INST_SYS(float, float)
 SF_INST_SYS(float, double)
 SF_INST_SYS(double, double)
@@ -669,14 +759,14 @@ SF_INST_SYS(double, double)
                                     SysPtrs<Ti>, const Ti*, int, const double*, double*, long);
 SF_INST_TI(float)
 SF_INST_TI(double)
-template void launch_update<float>(hipStream_t, const LaunchGeom&, const BubbleDev*,
-                                   const BubbleTime*, const SolveCtl*, const Tables&,
-                                   StatePtrs<float>, AirPtrs<float>, const double*, const double*,
-                                   StatePtrs<float>);
-template void launch_update<double>(hipStream_t, const LaunchGeom&, const BubbleDev*,
-                                    const BubbleTime*, const SolveCtl*, const Tables&,
-                                    StatePtrs<double>, AirPtrs<double>, const double*,
-                                    const double*, StatePtrs<double>);
+template void launch_update<float>(hipStream_t, const LaunchGeom&, const BubbleDev*, BubbleTime*,
+                                   SolveCtl*, const Tables&, StatePtrs<float>, AirPtrs<float>,
+                                   const double*, const double*, StatePtrs<float>, int,
+                                   SolveHist*, const SolveParams&, const double*, long);
+template void launch_update<double>(hipStream_t, const LaunchGeom&, const BubbleDev*, BubbleTime*,
+                                    SolveCtl*, const Tables&, StatePtrs<double>, AirPtrs<double>,
+                                    const double*, const double*, StatePtrs<double>, int,
+                                    SolveHist*, const SolveParams&, const double*, long);
 template void launch_row_stats<float>(hipStream_t, const LaunchGeom&, const BubbleDev*,
                                       const Tables&, const float*, double*, long);
 template void launch_row_stats<double>(hipStream_t, const LaunchGeom&, const BubbleDev*,

@@ -56,12 +56,12 @@ struct LaunchGeom {
 };
 
 template <typename T>
-void launch_advect(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, const Tables& tab,
-                   StatePtrs<T> in, StatePtrs<T> out);
+void launch_advect(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, BubbleTime* bt,
+                   const Tables& tab, StatePtrs<T> in, StatePtrs<T> out);
 template <typename T, typename Ti>
 void launch_assemble(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub,
                      const BubbleTime* bt, const Tables& tab, StatePtrs<T> st, AirPtrs<T> air,
-                     SysPtrs<Ti> sys, double gmax_state);
+                     SysPtrs<Ti> sys, double gmax_state, SolveCtl* ctl, double* part, long stride);
 template <typename Ti>
 void launch_resid(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, SolveCtl* ctl,
                   SysPtrs<Ti> sys, const double* xo, double* xn, int apply_e, int k, double* part,
@@ -76,12 +76,11 @@ void launch_cgstep(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, Sol
                    long stride);
 void launch_finalize(hipStream_t s, int nb, const BubbleDev* bub, SolveCtl* ctl, SolveHist* hist,
                      const SolveParams& sp, const double* partR, long stride);
-void launch_reset_ctl(hipStream_t s, SolveCtl* ctl, int nb);
 template <typename T>
-void launch_update(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, const BubbleTime* bt,
-                   const SolveCtl* ctl, const Tables& tab, StatePtrs<T> st, AirPtrs<T> air,
-                   const double* x0, const double* x1, StatePtrs<T> out);
-void launch_advance_time(hipStream_t s, BubbleTime* bt, int nb);
+void launch_update(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, BubbleTime* bt,
+                   SolveCtl* ctl, const Tables& tab, StatePtrs<T> st, AirPtrs<T> air,
+                   const double* x0, const double* x1, StatePtrs<T> out, int fin, SolveHist* hist,
+                   const SolveParams& sp, const double* partR, long stride);
 template <typename T>
 void launch_row_stats(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, const Tables& tab,
                       const T* eta, double* out, long stride);
