@@ -95,7 +95,7 @@ typedef struct {
     int32_t max_inner;    /* inner PCG iterations per refinement; 0 -> automatic  */
     double inner_rho;     /* inner residual reduction; 0 -> automatic             */
     int32_t advance_time; /* 1: air-flow time advances by dt every step           */
-    int32_t pad_;
+    int32_t inner_method; /* 0 automatic, 1 PCG, 2 Chebyshev (inner solve)        */
 } sf_options;
 
 typedef struct {

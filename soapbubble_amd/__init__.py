@@ -59,7 +59,7 @@ class Options(ctypes.Structure):
                 ("use_graph", ctypes.c_int32), ("tol_absolute", ctypes.c_int32),
                 ("tol", ctypes.c_double), ("max_outer", ctypes.c_int32),
                 ("max_inner", ctypes.c_int32), ("inner_rho", ctypes.c_double),
-                ("advance_time", ctypes.c_int32), ("pad_", ctypes.c_int32)]
+                ("advance_time", ctypes.c_int32), ("inner_method", ctypes.c_int32)]
 
 
 class BubbleStats(ctypes.Structure):
@@ -191,14 +191,15 @@ class SoapFilm:
     def __init__(self, bubbles: Sequence[BubbleDesc] | BubbleDesc, precision: str = "fp64",
                  device: int = 0, use_graph: bool = False, tol: float = 1e-10,
                  tol_absolute: bool = False, advance_time: bool = True, max_outer: int = 0,
-                 max_inner: int = 0, inner_rho: float = 0.0):
+                 max_inner: int = 0, inner_rho: float = 0.0, inner_method: str = "auto"):
         self._lib = load_library()
         if isinstance(bubbles, BubbleDesc):
             bubbles = [bubbles]
         self.bubbles = list(bubbles)
         arr = (BubbleDesc * len(self.bubbles))(*self.bubbles)
         opt = Options(PRECISIONS[precision], device, int(use_graph), int(tol_absolute), tol,
-                      max_outer, max_inner, inner_rho, int(advance_time), 0)
+                      max_outer, max_inner, inner_rho, int(advance_time),
+                      {"auto": 0, "cg": 1, "chebyshev": 2}[inner_method])
         h = ctypes.c_void_p()
         rc = self._lib.sf_create(ctypes.byref(opt), len(self.bubbles), arr, ctypes.byref(h))
         if rc:

@@ -82,11 +82,14 @@ struct Base {
     double* partS = nullptr;
     double* x64[2] = {nullptr, nullptr};
     double *wN64 = nullptr, *wW64 = nullptr, *diag64 = nullptr, *b64 = nullptr;
+    int* rsteps = nullptr;
+    int nxcd = 1;
     SolveParams sp{};
     int max_outer = 3, max_inner = 500;
     bool calibrated = false;
     std::vector<int> calib;   // max inner iterations per refinement seen while calibrating
     std::vector<int> sched;   // inner iterations per refinement of the fixed schedule
+    int method = 1;           // inner solver: 1 PCG, 2 Chebyshev
     bool trace_solve = std::getenv("SF_SOLVE_TRACE") != nullptr;
     int steps_done = 0;
     int stage = 0;            // 0 idle, 1 advected, 2 assembled, 3 solved
@@ -122,7 +125,7 @@ struct Impl : Base {
 
     AirPtrs<T> air() const { return AirPtrs<T>{ua0, va0, au, av}; }
     SysPtrs<Ti> sys(double* x_init) const {
-        return SysPtrs<Ti>{wN64, wW64, diag64, b64, x_init, wN, wW, dg, e, r, z, Ap};
+        return SysPtrs<Ti>{wN64, wW64, diag64, b64, x_init, wN, wW, dg, e, r, z, Ap, rsteps};
     }
     bool fp64_inner() const override { return sizeof(Ti) == 8; }
 
@@ -176,10 +179,31 @@ struct Impl : Base {
     }
 
     // FP64 residual evaluation k >= 1: x64[k&1] = x64[(k-1)&1] + e.
-    void resid(int k) {
+    void resid(int k, Ti* ecorr) {
         const double* xo = x64[(k + 1) & 1];
         double* xn = x64[k & 1];
-        launch_resid<Ti>(stream, geom, bub_d, ctl_d, sys(x64[0]), xo, xn, 1, k, partR, stride);
+        SysPtrs<Ti> s = sys(x64[0]);
+        s.e = ecorr;
+        launch_resid<Ti>(stream, geom, bub_d, ctl_d, s, xo, xn, k, partR, stride);
+    }
+    // Chebyshev buffers: e alternates between e and Ap, d lives in p[0].
+    Ti* cheb_e(int k) { return (k & 1) ? Ap : e; }
+    void cheb(int it) {
+        launch_cheb<Ti>(stream, geom, bub_d, ctl_d, sys(x64[0]), cheb_e(it), cheb_e(it + 1), p[0], it,
+                        sp, partR, partB, stride);
+    }
+    bool cheb_done(std::vector<SolveCtl>& c, std::vector<double>& pb) {
+        SF_HIP(hipMemcpyAsync(pb.data(), partB + stride, sizeof(double) * stride,
+                              hipMemcpyDeviceToHost, stream));
+        SF_HIP(hipStreamSynchronize(stream));
+        for (int b = 0; b < nb; ++b) {
+            if (c[b].outer_done) continue;
+            double rr = 0.0;
+            for (int j = 0; j < bub_h[b].nth; ++j) rr += pb[bub_h[b].pr0 + j];
+            const double tgt = sp.inner_frac * sp.tol;
+            if (!(rr <= tgt * tgt * (sp.absolute ? 1.0 : c[b].bb))) return false;
+        }
+        return true;
     }
     void inner(int it) {
         launch_spmv<Ti>(stream, geom, bub_d, ctl_d, sys(x64[0]), p[(it + 1) & 1], p[it & 1], it, sp,
@@ -204,10 +228,38 @@ struct Impl : Base {
     // the calibrated schedule, every kernel launched, converged bubbles
     // skip the work (graph capture).
     void solve(bool host_driven, bool finalize) override {
-        if (host_driven) {
+        std::vector<int> used;
+        if (method == 2) {
+            // Chebyshev: kernel `it` measures the residual of e_it and
+            // produces e_{it+1}; a refinement of n kernels hands e_{n+1} on
+            if (host_driven) {
+                std::vector<SolveCtl> c(nb);
+                std::vector<double> pb(stride);
+                for (int k = 0; k < max_outer; ++k) {
+                    cheb(1);
+                    const bool done = all_flags(c, false);
+                    if (trace_solve)
+                        std::fprintf(stderr, "[sf solve] outer %d residual %.3e mu %.4f%s\n", k,
+                                     c[0].rel_res, c[0].mu, done ? " converged" : "");
+                    if (done) break;
+                    int n = max_inner;
+                    for (int it = 1; it <= max_inner; ++it) {
+                        if (it > 1) cheb(it);
+                        if (cheb_done(c, pb)) { n = it; break; }
+                    }
+                    used.push_back(n);
+                    if (trace_solve) std::fprintf(stderr, "[sf solve]   chebyshev iterations %d\n", n);
+                    resid(k + 1, cheb_e(n + 1));
+                }
+            } else {
+                for (size_t k = 0; k < sched.size(); ++k) {
+                    for (int it = 1; it <= sched[k]; ++it) cheb(it);
+                    resid((int)k + 1, cheb_e(sched[k] + 1));
+                }
+            }
+        } else if (host_driven) {
             std::vector<SolveCtl> c(nb);
             std::vector<int> before(nb);
-            std::vector<int> used;
             for (int k = 0; k < max_outer; ++k) {
                 launch_spmv<Ti>(stream, geom, bub_d, ctl_d, sys(x64[0]), p[1], p[0], 0, sp, partR,
                                 partB, partA, stride);
@@ -229,17 +281,17 @@ struct Impl : Base {
                 used.push_back(mx);
                 if (trace_solve)
                     std::fprintf(stderr, "[sf solve]   inner iterations %d\n", c[0].iters - before[0]);
-                resid(k + 1);
-            }
-            if (!calibrated) {
-                if (calib.size() < used.size()) calib.resize(used.size(), 0);
-                for (size_t k = 0; k < used.size(); ++k) calib[k] = std::max(calib[k], used[k]);
+                resid(k + 1, e);
             }
         } else {
             for (size_t k = 0; k < sched.size(); ++k) {
                 for (int it = 0; it < sched[k]; ++it) inner(it);
-                resid((int)k + 1);
+                resid((int)k + 1, e);
             }
+        }
+        if (host_driven && !calibrated) {
+            if (calib.size() < used.size()) calib.resize(used.size(), 0);
+            for (size_t k = 0; k < used.size(); ++k) calib[k] = std::max(calib[k], used[k]);
         }
         if (finalize) launch_finalize(stream, nb, bub_d, ctl_d, hist_d, sp, partR, stride);
     }
@@ -312,6 +364,20 @@ struct Impl : Base {
 
 // ------------------------------------------------------------------ setup --
 bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
+
+// Workgroup k of a launch is dispatched to XCD k mod nxcd, and each XCD has
+// its own L2. Work item l (a row or a tile, in spatial order) is therefore
+// placed at launch slot (l mod chunk) * nxcd + l / chunk with chunk =
+// ceil(n / nxcd): every XCD receives one contiguous band of rows, so the
+// neighbour rows that stencils and advection gathers read are mostly in the
+// same L2. Slots past the end hold `pad` (bubble index -1, skipped).
+template <typename X>
+std::vector<X> xcd_order(const std::vector<X>& work, X pad, int nxcd) {
+    const size_t n = work.size(), chunk = (n + nxcd - 1) / nxcd;
+    std::vector<X> out(chunk * nxcd, pad);
+    for (size_t l = 0; l < n; ++l) out[(l % chunk) * nxcd + l / chunk] = work[l];
+    return out;
+}
 
 void derive(const sf_bubble_desc& d, sf_derived& o) { sfh::derive_params(d, o); }
 
@@ -452,6 +518,13 @@ void setup(Base& c, int prec) {
     std::vector<int2> rows;
     for (int b = 0; b < nb; ++b)
         for (int j = 0; j < c.bub_h[b].nth; ++j) rows.push_back(make_int2(b, j));
+    int nxcd = 1;
+    if (hipDeviceGetAttribute(&nxcd, hipDeviceAttributeNumberOfXccs, c.device) != hipSuccess || nxcd < 1)
+        nxcd = 1;
+    if (const char* e = std::getenv("SF_XCD_ORDER")) nxcd = std::max(1, std::atoi(e));
+    c.nxcd = nxcd;
+    tiles = xcd_order(tiles, make_int4(-1, 0, 0, 1), nxcd);
+    rows = xcd_order(rows, make_int2(-1, 0), nxcd);
     int4* tiles_d = dalloc<int4>(tiles.size(), c.owned);
     SF_HIP(hipMemcpy(tiles_d, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice));
     int2* rows_d = dalloc<int2>(rows.size(), c.owned);
@@ -478,7 +551,7 @@ void setup(Base& c, int prec) {
                          std::to_string(lds_limit) + " (use SF_PREC_MIXED for this size)"};
     }
 
-    c.partR = dalloc<double>(4 * (size_t)c.stride, c.owned);
+    c.partR = dalloc<double>(5 * (size_t)c.stride, c.owned);
     c.partA = dalloc<double>((size_t)c.stride, c.owned);
     c.partB = dalloc<double>(2 * (size_t)c.stride, c.owned);
     c.partS = dalloc<double>(2 * (size_t)c.stride, c.owned);
@@ -489,8 +562,12 @@ void setup(Base& c, int prec) {
     c.wW64 = dalloc<double>(n, c.owned);
     c.diag64 = dalloc<double>(n, c.owned);
     c.b64 = dalloc<double>(n, c.owned);
+    c.rsteps = dalloc<int>((size_t)c.stride, c.owned);
 
-    // solver schedule
+    // solver schedule; the inner method: PCG for an FP64 inner solve (the
+    // plain path), Chebyshev for FP32 unless asked otherwise
+    c.method = c.opt.inner_method > 0 ? c.opt.inner_method : (c.fp64_inner() ? 1 : 2);
+    if (c.method != 1 && c.method != 2) throw Status{SF_ERR_INVALID, "unknown inner method"};
     c.sp.tol = c.opt.tol > 0 ? c.opt.tol : 1e-10;
     c.sp.absolute = c.opt.tol_absolute;
     c.sp.inner_frac = 0.5;
@@ -701,15 +778,15 @@ int sf_step(sf_ctx* ctx, int32_t nsteps) {
             // calibration: two host-driven steps fix the refinement schedule
             for (; k < nsteps && k < 2; ++k) eager_step(c, true);
             if (k == 2) {
-                // fixed schedule: the refinements seen, each with one inner
-                // iteration of margin, plus one spare refinement; a bubble
+                // fixed schedule: the inner iterations of each refinement seen
+                // while calibrating, then one spare refinement of two
+                // iterations that catches a step needing more; a bubble
                 // that converged skips the remaining kernels
                 c.sched.clear();
-                for (int n : c.calib) c.sched.push_back(c.opt.max_inner > 0 ? c.opt.max_inner : n + 1);
+                for (int n : c.calib) c.sched.push_back(c.opt.max_inner > 0 ? c.opt.max_inner : std::max(n, 1));
                 if (c.sched.empty()) c.sched.push_back(1);
-                const int last = c.sched.back();
-                c.sched.push_back(std::max(3, last));
-                if (c.opt.max_outer > 0) c.sched.resize((size_t)c.opt.max_outer, std::max(3, last));
+                c.sched.push_back(2);
+                if (c.opt.max_outer > 0) c.sched.resize((size_t)c.opt.max_outer, 2);
                 c.calibrated = true;
                 build_graph(c);
             }

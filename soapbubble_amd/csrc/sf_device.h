@@ -41,6 +41,7 @@ struct SolveCtl {
     double rr_in0;       // |r_in|^2 at the start of the current inner solve
     double rel_res;      // last relative (or absolute) residual, FP64
     double bb;           // |b|^2
+    double mu;           // spectral bound of the line-preconditioned operator
     int outer_done;      // 1 once the FP64 residual met the tolerance
     int inner_done;      // 1 once the inner solve met its reduction
     int iters;           // inner iterations executed
@@ -386,6 +387,29 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* red) {
     __syncthreads();
 }
 
+// Deterministic block maximum of a non-negative value; every thread receives
+// it. `red` must hold 16 doubles of LDS.
+__device__ __forceinline__ double block_max(double v, double* red) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_down(v, o, 64));
+    __syncthreads();
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    double m = 0.0;
+    for (int w = 0; w < nw; ++w) m = fmax(m, red[w]);
+    __syncthreads();
+    return m;
+}
+
+// Maximum of the per-row values [base, base+n) of one slot.
+__device__ __forceinline__ double max_partials(const double* __restrict__ part, int base, int n,
+                                               double* red) {
+    double m = 0.0;
+    for (int r = threadIdx.x; r < n; r += blockDim.x) m = fmax(m, part[base + r]);
+    return block_max(m, red);
+}
+
 // Sum the per-row partials [base, base+n) of NV interleaved slots
 // (layout part[slot * stride + row]).
 template <int NV>
@@ -401,16 +425,45 @@ __device__ __forceinline__ void sum_partials(const double* __restrict__ part, lo
 }
 
 // --------------------------------------------- periodic line solve ----
+// Reciprocal for the preconditioner: FP32 uses the hardware v_rcp_f32
+// (1 ulp; the line solve is a preconditioner, not the residual), FP64 the
+// correctly rounded division.
+__device__ __forceinline__ float pre_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double pre_rcp(double x) { return 1.0 / x; }
+
+// Coupling below which a reduced row is decoupled: below the rounding of the
+// precision, so stopping the reduction there changes z by less than one ulp.
+template <typename Ti> struct PcrTol;
+template <> struct PcrTol<float> { static constexpr float v = 2e-9f; };
+template <> struct PcrTol<double> { static constexpr double v = 1e-18; };
+
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_down(v, o, 64));
+    return __shfl(v, 0, 64);
+}
+
 // Parallel cyclic reduction of one periodic tridiagonal row held in LDS,
-// normalised to unit diagonal: x_i + a_i x_{i-1} + c_i x_{i+1} = d_i.
-// n is a power of two >= 4. After log2(n)-1 reductions the stride reaches
-// n/2, where x_{i-n/2} and x_{i+n/2} are one unknown and the rows pair up
-// into 2x2 systems. On exit sd holds x. CPT bounds the cells per thread.
+// normalised to unit diagonal: x_i + a_i x_{i-1} + c_i x_{i+1} = d_i, n a
+// power of two >= 4. Each reduction doubles the coupling stride and, for a
+// diagonally dominant row, squares the size of the couplings; after
+// log2(n)-1 reductions the stride is n/2, where x_{i-n/2} = x_{i+n/2} and
+// the rows pair up into 2x2 systems. A row whose couplings have all fallen
+// below PcrTol is decoupled early (x_i = d_i).
+// steps < 0: reduce adaptively (a block-wide max per reduction) and return
+// the number of reductions done; steps >= 0: do exactly that many (the
+// count returned by the adaptive solve of the same matrix).
+// On exit sd holds x. CPT bounds the cells per thread.
 template <typename Ti, int CPT>
-__device__ __forceinline__ void pcr_periodic(Ti* sa, Ti* sc, Ti* sd, int n) {
+__device__ __forceinline__ int pcr_periodic(Ti* sa, Ti* sc, Ti* sd, int n, int steps,
+                                            float* red) {
     const int tid = threadIdx.x, nt = blockDim.x, m = n - 1;
+    const int full = 31 - __clz(n) - 1;     // log2(n) - 1
+    int done = 0;
     for (int s = 1; s < n / 2; s <<= 1) {
+        if (steps >= 0 && done >= steps) break;
         Ti na[CPT], nc[CPT], nd[CPT];
+        float mx = 0.0f;
 #pragma unroll
         for (int k = 0; k < CPT; ++k) {
             int i = tid + k * nt;
@@ -419,10 +472,11 @@ __device__ __forceinline__ void pcr_periodic(Ti* sa, Ti* sc, Ti* sd, int n) {
                 Ti ai = sa[i], ci = sc[i], di = sd[i];
                 Ti am = sa[im], cm = sc[im], dm = sd[im];
                 Ti ap = sa[ip], cp = sc[ip], dp = sd[ip];
-                Ti iD = Ti(1) / (Ti(1) - ai * cm - ci * ap);
+                Ti iD = pre_rcp(Ti(1) - ai * cm - ci * ap);
                 na[k] = -ai * am * iD;
                 nc[k] = -ci * cp * iD;
                 nd[k] = (di - ai * dm - ci * dp) * iD;
+                mx = fmaxf(mx, (float)(fabs(na[k]) + fabs(nc[k])));
             }
         }
         __syncthreads();
@@ -431,8 +485,20 @@ __device__ __forceinline__ void pcr_periodic(Ti* sa, Ti* sc, Ti* sd, int n) {
             int i = tid + k * nt;
             if (i < n) { sa[i] = na[k]; sc[i] = nc[k]; sd[i] = nd[k]; }
         }
-        __syncthreads();
+        ++done;
+        if (steps < 0) {
+            mx = wave_max(mx);
+            if ((tid & 63) == 0) red[tid >> 6] = mx;
+            __syncthreads();
+            float bm = 0.0f;
+            for (int w = 0; w < (nt + 63) >> 6; ++w) bm = fmaxf(bm, red[w]);
+            __syncthreads();
+            if (bm < (float)PcrTol<Ti>::v) return done;   // sd already holds x
+        } else {
+            __syncthreads();
+        }
     }
+    if (done < full) return done;                    // truncated: x_i = d_i
     Ti xs[CPT];
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
@@ -440,7 +506,7 @@ __device__ __forceinline__ void pcr_periodic(Ti* sa, Ti* sc, Ti* sd, int n) {
         if (i < n) {
             int j = (i + n / 2) & m;
             Ti ei = sa[i] + sc[i], ej = sa[j] + sc[j];
-            xs[k] = (sd[i] - ei * sd[j]) / (Ti(1) - ei * ej);
+            xs[k] = (sd[i] - ei * sd[j]) * pre_rcp(Ti(1) - ei * ej);
         }
     }
     __syncthreads();
@@ -450,6 +516,7 @@ __device__ __forceinline__ void pcr_periodic(Ti* sa, Ti* sc, Ti* sd, int n) {
         if (i < n) sd[i] = xs[k];
     }
     __syncthreads();
+    return done;
 }
 
 }  // namespace sfk

@@ -33,6 +33,7 @@ __global__ __launch_bounds__(256) void k_advect(const int4* __restrict__ tiles,
                                                 BubbleTime* __restrict__ btime, Tables tab,
                                                 StatePtrs<T> in, StatePtrs<T> out) {
     const int4 tl = tiles[blockIdx.x];
+    if (tl.x < 0) return;
     const BubbleDev B = bub[tl.x];
     // the previous step's time advance lands here, before any kernel of this
     // step reads the time
@@ -241,7 +242,8 @@ __device__ __forceinline__ CellSys assemble_cell(const BubbleDev& B, double t, c
 
 // Row-block assembly: coefficients and rhs of one theta row, the initial
 // FP64 residual r0 = b - A Gamma*, r_in = (Ti) r0 and the first line solve
-// z = M^-1 r_in. Partials as k_resid. The block of row 0 resets the
+// z = M^-1 r_in, whose reduction depth is recorded for the later line solves
+// of the same matrix. Partials as k_resid. The block of row 0 resets the
 // bubble's solver control block; the broken-cell override of Gamma* is
 // written back to the state as the reference does.
 template <typename T, typename Ti, int CPT>
@@ -251,7 +253,9 @@ __global__ void k_assemble(const int2* __restrict__ rows, const BubbleDev* __res
                            SolveCtl* __restrict__ ctl, double* __restrict__ part, long stride) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     __shared__ double red[16 * 4];
+    __shared__ float redf[16];
     const int2 rw = rows[blockIdx.x];
+    if (rw.x < 0) return;
     const BubbleDev B = bub[rw.x];
     const int j = rw.y, nph = B.nph;
     const double t = btime[rw.x].t;
@@ -260,8 +264,14 @@ __global__ void k_assemble(const int2* __restrict__ rows, const BubbleDev* __res
     Ti* sd = sc + nph;
     const size_t row = (size_t)B.off + (size_t)j * nph;
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    double mu = 0.0;
     for (int i = threadIdx.x; i < nph; i += blockDim.x) {
         const CellSys q = assemble_cell<T>(B, t, tab, s, air, j, i);
+        // x^T N x <= sum (wN + wS) x_i^2 and x^T M x >= sum (diag - wW - wE) x_i^2
+        // for the theta couplings N and the phi-line blocks M, so the
+        // eigenvalues of M^-1 A lie in [1 - mu, 1 + mu] with mu the largest
+        // ratio below
+        if (!q.brk) mu = fmax(mu, (q.wN + q.wS) / (q.diag - q.wW - q.wE));
         const size_t c = row + i;
         const double ax = q.diag * q.x0 - q.wN * q.x0N - q.wS * q.x0S - q.wW * q.x0W -
                           q.wE * q.x0E;
@@ -276,8 +286,7 @@ __global__ void k_assemble(const int2* __restrict__ rows, const BubbleDev* __res
         sys.wW[c] = (Ti)q.wW;
         sys.diag[c] = (Ti)q.diag;
         sys.r[c] = ri;
-        const Ti dg = (Ti)q.diag;
-        const Ti inv = Ti(1) / dg;
+        const Ti inv = pre_rcp((Ti)q.diag);
         sa[i] = -(Ti)q.wW * inv;
         sc[i] = -(Ti)q.wE * inv;
         sd[i] = ri * inv;
@@ -286,21 +295,23 @@ __global__ void k_assemble(const int2* __restrict__ rows, const BubbleDev* __res
         acc[3] += (double)ri * (double)ri;
     }
     __syncthreads();
-    // every thread has read the neighbours' Gamma*: apply the override now
     for (int i = threadIdx.x; i < nph; i += blockDim.x) {
         const size_t c = row + i;
         if (is_broken((double)s.gam[c], (double)s.eta[c], B.gamma_max)) s.gam[c] = gmax_state;
     }
-    pcr_periodic<Ti, CPT>(sa, sc, sd, nph);
+    const int steps = pcr_periodic<Ti, CPT>(sa, sc, sd, nph, -1, redf);
     for (int i = threadIdx.x; i < nph; i += blockDim.x) {
         const Ti z = sd[i];
         sys.z[row + i] = z;
         acc[2] += (double)sys.r[row + i] * (double)z;
     }
     block_sum<4>(acc, red);
+    mu = block_max(mu, red);
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) part[q * stride + B.pr0 + j] = acc[q];
+        part[4 * stride + B.pr0 + j] = mu;
+        sys.rsteps[B.pr0 + j] = steps;
         if (j == 0) {
             SolveCtl z = {};
             z.xbuf = 0;
@@ -310,72 +321,66 @@ __global__ void k_assemble(const int2* __restrict__ rows, const BubbleDev* __res
     }
 }
 
-// -------------------------------------------------------- line solve ----
-// Loads the phi-line block of row j into LDS (normalised by the diagonal)
-// with right-hand side r, solves it and returns z in sd.
-template <typename Ti>
-__device__ __forceinline__ void line_setup(Ti* sa, Ti* sc, Ti* sd, const Ti* __restrict__ wW,
-                                           const Ti* __restrict__ dg, size_t row, int i, int m,
-                                           Ti r) {
-    const Ti d = dg[row + i];
-    const Ti inv = Ti(1) / d;
-    sa[i] = -wW[row + i] * inv;
-    sc[i] = -wW[row + ((i + 1) & m)] * inv;
-    sd[i] = r * inv;
-}
-
 // ----------------------------------------------------- FP64 residual ----
-// x_new = x_old + e (the inner correction, when apply_e), r = b - A x_new in
-// FP64, r_in = (Ti) r, z = M^-1 r_in. Partials: |r|^2, |b|^2, r_in.z, |r_in|^2.
+// x_new = x_old + e (the inner correction), r = b - A x_new in FP64,
+// r_in = (Ti) r, z = M^-1 r_in. Partials: |r|^2, |b|^2, r_in.z, |r_in|^2.
 template <typename Ti, int CPT>
 __global__ void k_resid(const int2* __restrict__ rows, const BubbleDev* __restrict__ bub,
                         SolveCtl* __restrict__ ctl, SysPtrs<Ti> sys, const double* __restrict__ xo,
-                        double* __restrict__ xn, int apply_e, int k, double* __restrict__ part,
-                        long stride) {
+                        double* __restrict__ xn, int k, double* __restrict__ part, long stride) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     __shared__ double red[16 * 4];
+    __shared__ float redf[16];
     const int2 rw = rows[blockIdx.x];
+    if (rw.x < 0 || ctl[rw.x].outer_done) return;
     const BubbleDev B = bub[rw.x];
-    if (k > 0 && ctl[rw.x].outer_done) return;
     const int j = rw.y, nph = B.nph, m = B.mask, nth = B.nth;
+    const int steps = sys.rsteps[B.pr0 + j];
     Ti* sa = reinterpret_cast<Ti*>(lds_raw);
     Ti* sc = sa + nph;
     Ti* sd = sc + nph;
     const size_t row = (size_t)B.off + (size_t)j * nph;
     const Ti* __restrict__ e = sys.e;
+    const bool hasN = j > 0, hasS = j < nth - 1;
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int i = threadIdx.x; i < nph; i += blockDim.x) {
-        const size_t c = row + i;
-        const size_t cW = row + ((i - 1) & m), cE = row + ((i + 1) & m);
-        double xc = xo[c], xW = xo[cW], xE = xo[cE];
-        if (apply_e) { xc += (double)e[c]; xW += (double)e[cW]; xE += (double)e[cE]; }
-        double ax = sys.diag64[c] * xc - sys.wW64[c] * xW - sys.wW64[cE] * xE;
-        if (j > 0) {
-            double xN = xo[c - nph];
-            if (apply_e) xN += (double)e[c - nph];
-            ax -= sys.wN64[c] * xN;
+    Ti rin[CPT];
+#pragma unroll
+    for (int q = 0; q < CPT; ++q) {
+        const int i = threadIdx.x + q * blockDim.x;
+        rin[q] = Ti(0);
+        if (i < nph) {
+            const size_t c = row + i;
+            const size_t cW = row + ((i - 1) & m), cE = row + ((i + 1) & m);
+            const double xc = xo[c] + (double)e[c];
+            double ax = sys.diag64[c] * xc - sys.wW64[c] * (xo[cW] + (double)e[cW]) -
+                        sys.wW64[cE] * (xo[cE] + (double)e[cE]);
+            if (hasN) ax -= sys.wN64[c] * (xo[c - nph] + (double)e[c - nph]);
+            if (hasS) ax -= sys.wN64[c + nph] * (xo[c + nph] + (double)e[c + nph]);
+            xn[c] = xc;
+            const double b = sys.b64[c];
+            const double r = b - ax;
+            const Ti ri = (Ti)r;
+            rin[q] = ri;
+            sys.r[c] = ri;
+            const Ti inv = pre_rcp(sys.diag[c]);
+            sa[i] = -sys.wW[c] * inv;
+            sc[i] = -sys.wW[cE] * inv;
+            sd[i] = ri * inv;
+            acc[0] += r * r;
+            acc[1] += b * b;
+            acc[3] += (double)ri * (double)ri;
         }
-        if (j < nth - 1) {
-            double xS = xo[c + nph];
-            if (apply_e) xS += (double)e[c + nph];
-            ax -= sys.wN64[c + nph] * xS;
-        }
-        xn[c] = xc;
-        const double b = sys.b64[c];
-        const double r = b - ax;
-        const Ti ri = (Ti)r;
-        sys.r[c] = ri;
-        line_setup(sa, sc, sd, sys.wW, sys.diag, row, i, m, ri);
-        acc[0] += r * r;
-        acc[1] += b * b;
-        acc[3] += (double)ri * (double)ri;
     }
     __syncthreads();
-    pcr_periodic<Ti, CPT>(sa, sc, sd, nph);
-    for (int i = threadIdx.x; i < nph; i += blockDim.x) {
-        const Ti z = sd[i];
-        sys.z[row + i] = z;
-        acc[2] += (double)sys.r[row + i] * (double)z;
+    pcr_periodic<Ti, CPT>(sa, sc, sd, nph, steps, redf);
+#pragma unroll
+    for (int q = 0; q < CPT; ++q) {
+        const int i = threadIdx.x + q * blockDim.x;
+        if (i < nph) {
+            const Ti z = sd[i];
+            sys.z[row + i] = z;
+            acc[2] += (double)rin[q] * (double)z;
+        }
     }
     block_sum<4>(acc, red);
     if (threadIdx.x == 0) {
@@ -389,6 +394,9 @@ __global__ void k_resid(const int2* __restrict__ rows, const BubbleDev* __restri
 }
 
 // ------------------------------------------------------ inner PCG: A p ----
+// p = z + beta p_old (beta from the r.z partials of the previous step),
+// Ap, partial p.Ap. Iteration 0 of a refinement also decides whether the
+// FP64 residual already meets the tolerance.
 template <typename Ti, int CPT>
 __global__ void k_spmv(const int2* __restrict__ rows, const BubbleDev* __restrict__ bub,
                        SolveCtl* __restrict__ ctl, SysPtrs<Ti> sys, const Ti* __restrict__ pold,
@@ -396,16 +404,20 @@ __global__ void k_spmv(const int2* __restrict__ rows, const BubbleDev* __restric
                        const double* __restrict__ partB, double* __restrict__ partA, long stride) {
     __shared__ double red[16 * 4];
     const int2 rw = rows[blockIdx.x];
-    const BubbleDev B = bub[rw.x];
+    if (rw.x < 0) return;
     SolveCtl* C = ctl + rw.x;
-    if (C->outer_done) return;
+    if (C->outer_done || (it > 0 && C->inner_done)) return;
+    const BubbleDev B = bub[rw.x];
     const int j = rw.y, nph = B.nph, m = B.mask, nth = B.nth;
     const bool leader = (j == 0 && threadIdx.x == 0);
+    const bool hasN = j > 0, hasS = j < nth - 1;
+    const size_t row = (size_t)B.off + (size_t)j * nph;
+    const Ti* __restrict__ z = sys.z;
     double beta = 0.0;
     if (it == 0) {
-        double s[4];
-        sum_partials<4>(partR, stride, B.pr0, nth, s, red);
-        const double rr = s[0], bb = s[1], rz = s[2], rrin = s[3];
+        double sm[4];
+        sum_partials<4>(partR, stride, B.pr0, nth, sm, red);
+        const double rr = sm[0], bb = sm[1], rz = sm[2], rrin = sm[3];
         const double res = sp.absolute ? sqrt(rr) : sqrt(rr) / (bb > 0.0 ? sqrt(bb) : 1.0);
         if (res <= sp.tol) {
             if (leader) { C->outer_done = 1; C->rel_res = res; }
@@ -419,10 +431,9 @@ __global__ void k_spmv(const int2* __restrict__ rows, const BubbleDev* __restric
             C->bb = bb;
         }
     } else {
-        if (C->inner_done) return;
-        double s[2];
-        sum_partials<2>(partB, stride, B.pr0, nth, s, red);
-        const double rz = s[0], rr = s[1];
+        double sm[2];
+        sum_partials<2>(partB, stride, B.pr0, nth, sm, red);
+        const double rz = sm[0], rr = sm[1];
         const double tgt = sp.inner_frac * sp.tol;
         const double bbe = sp.absolute ? 1.0 : C->bb;
         if (rr <= sp.inner_rho2 * C->rr_in0 || rr <= tgt * tgt * bbe) {
@@ -432,20 +443,42 @@ __global__ void k_spmv(const int2* __restrict__ rows, const BubbleDev* __restric
         beta = rz / C->rz[(it - 1) & 1];
         if (leader) C->rz[it & 1] = rz;
     }
-    const size_t row = (size_t)B.off + (size_t)j * nph;
+    // operands (the decision above comes first, so a converged solve reads none)
+    Ti zc[CPT], zw[CPT], ze[CPT], zn[CPT], zs[CPT];
+    Ti pc[CPT], pw[CPT], pe[CPT], pn[CPT], ps[CPT];
+    Ti dg[CPT], wn[CPT], wsv[CPT], ww[CPT], we[CPT];
+#pragma unroll
+    for (int q = 0; q < CPT; ++q) {
+        const int i = threadIdx.x + q * blockDim.x;
+        zc[q] = zw[q] = ze[q] = zn[q] = zs[q] = Ti(0);
+        pc[q] = pw[q] = pe[q] = pn[q] = ps[q] = Ti(0);
+        dg[q] = wn[q] = wsv[q] = ww[q] = we[q] = Ti(0);
+        if (i < nph) {
+            const size_t c = row + i, cW = row + ((i - 1) & m), cE = row + ((i + 1) & m);
+            zc[q] = z[c]; zw[q] = z[cW]; ze[q] = z[cE];
+            dg[q] = sys.diag[c]; ww[q] = sys.wW[c]; we[q] = sys.wW[cE];
+            if (hasN) { zn[q] = z[c - nph]; wn[q] = sys.wN[c]; }
+            if (hasS) { zs[q] = z[c + nph]; wsv[q] = sys.wN[c + nph]; }
+            if (it > 0) {
+                pc[q] = pold[c]; pw[q] = pold[cW]; pe[q] = pold[cE];
+                if (hasN) pn[q] = pold[c - nph];
+                if (hasS) ps[q] = pold[c + nph];
+            }
+        }
+    }
     const Ti bt = (Ti)beta;
-    const Ti* __restrict__ z = sys.z;
-    auto pv = [&](size_t q) -> Ti { return it == 0 ? z[q] : z[q] + bt * pold[q]; };
     double acc[1] = {0.0};
-    for (int i = threadIdx.x; i < nph; i += blockDim.x) {
-        const size_t c = row + i, cW = row + ((i - 1) & m), cE = row + ((i + 1) & m);
-        const Ti pc = pv(c);
-        Ti ap = sys.diag[c] * pc - sys.wW[c] * pv(cW) - sys.wW[cE] * pv(cE);
-        if (j > 0) ap -= sys.wN[c] * pv(c - nph);
-        if (j < nth - 1) ap -= sys.wN[c + nph] * pv(c + nph);
-        pnew[c] = pc;
-        sys.Ap[c] = ap;
-        acc[0] += (double)pc * (double)ap;
+#pragma unroll
+    for (int q = 0; q < CPT; ++q) {
+        const int i = threadIdx.x + q * blockDim.x;
+        if (i < nph) {
+            const Ti p0 = zc[q] + bt * pc[q];
+            const Ti ap = dg[q] * p0 - ww[q] * (zw[q] + bt * pw[q]) - we[q] * (ze[q] + bt * pe[q]) -
+                          wn[q] * (zn[q] + bt * pn[q]) - wsv[q] * (zs[q] + bt * ps[q]);
+            pnew[row + i] = p0;
+            sys.Ap[row + i] = ap;
+            acc[0] += (double)p0 * (double)ap;
+        }
     }
     block_sum<1>(acc, red);
     if (threadIdx.x == 0) partA[B.pr0 + j] = acc[0];
@@ -459,40 +492,169 @@ __global__ void k_cgstep(const int2* __restrict__ rows, const BubbleDev* __restr
                          long stride) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     __shared__ double red[16 * 4];
+    __shared__ float redf[16];
     const int2 rw = rows[blockIdx.x];
-    const BubbleDev B = bub[rw.x];
+    if (rw.x < 0) return;
     SolveCtl* C = ctl + rw.x;
     if (C->outer_done || C->inner_done) return;
+    const BubbleDev B = bub[rw.x];
     const int j = rw.y, nph = B.nph, m = B.mask, nth = B.nth;
-    double s[1];
-    sum_partials<1>(partA, stride, B.pr0, nth, s, red);
-    const double alpha = C->rz[it & 1] / s[0];
-    const Ti al = (Ti)alpha;
+    const int steps = sys.rsteps[B.pr0 + j];
     Ti* sa = reinterpret_cast<Ti*>(lds_raw);
     Ti* sc = sa + nph;
     Ti* sd = sc + nph;
     const size_t row = (size_t)B.off + (size_t)j * nph;
+    Ti pv[CPT], apv[CPT], ev[CPT], rv[CPT], ww[CPT], we[CPT], dg[CPT];
+#pragma unroll
+    for (int q = 0; q < CPT; ++q) {
+        const int i = threadIdx.x + q * blockDim.x;
+        pv[q] = apv[q] = ev[q] = rv[q] = ww[q] = we[q] = Ti(0);
+        dg[q] = Ti(1);
+        if (i < nph) {
+            const size_t c = row + i;
+            pv[q] = p[c];
+            apv[q] = sys.Ap[c];
+            if (it > 0) ev[q] = sys.e[c];
+            rv[q] = sys.r[c];
+            ww[q] = sys.wW[c];
+            we[q] = sys.wW[row + ((i + 1) & m)];
+            dg[q] = sys.diag[c];
+        }
+    }
+    double sm[1];
+    sum_partials<1>(partA, stride, B.pr0, nth, sm, red);
+    const Ti al = (Ti)(C->rz[it & 1] / sm[0]);
     double acc[2] = {0.0, 0.0};
-    for (int i = threadIdx.x; i < nph; i += blockDim.x) {
-        const size_t c = row + i;
-        const Ti pc = p[c];
-        sys.e[c] = it == 0 ? al * pc : sys.e[c] + al * pc;
-        const Ti r = sys.r[c] - al * sys.Ap[c];
-        sys.r[c] = r;
-        line_setup(sa, sc, sd, sys.wW, sys.diag, row, i, m, r);
-        acc[1] += (double)r * (double)r;
+#pragma unroll
+    for (int q = 0; q < CPT; ++q) {
+        const int i = threadIdx.x + q * blockDim.x;
+        if (i < nph) {
+            const size_t c = row + i;
+            sys.e[c] = ev[q] + al * pv[q];
+            const Ti r = rv[q] - al * apv[q];
+            rv[q] = r;
+            sys.r[c] = r;
+            const Ti inv = pre_rcp(dg[q]);
+            sa[i] = -ww[q] * inv;
+            sc[i] = -we[q] * inv;
+            sd[i] = r * inv;
+            acc[1] += (double)r * (double)r;
+        }
     }
     __syncthreads();
-    pcr_periodic<Ti, CPT>(sa, sc, sd, nph);
-    for (int i = threadIdx.x; i < nph; i += blockDim.x) {
-        const Ti z = sd[i];
-        sys.z[row + i] = z;
-        acc[0] += (double)sys.r[row + i] * (double)z;
+    pcr_periodic<Ti, CPT>(sa, sc, sd, nph, steps, redf);
+#pragma unroll
+    for (int q = 0; q < CPT; ++q) {
+        const int i = threadIdx.x + q * blockDim.x;
+        if (i < nph) {
+            const Ti z = sd[i];
+            sys.z[row + i] = z;
+            acc[0] += (double)rv[q] * (double)z;
+        }
     }
     block_sum<2>(acc, red);
     if (threadIdx.x == 0) {
         partB[B.pr0 + j] = acc[0];
         partB[stride + B.pr0 + j] = acc[1];
+        if (j == 0) C->iters += 1;
+    }
+}
+
+// ------------------------------------------- inner Chebyshev iteration ----
+// Chebyshev acceleration of the phi-line Jacobi iteration for A e = r_in
+// (Saad, Iterative Methods for Sparse Linear Systems, Alg. 12.1), with the
+// spectrum of M^-1 A inside [1 - mu, 1 + mu] (mu from the assembly, widened
+// slightly). No inner products: one kernel per iteration. Iteration k >= 1
+// reads e_k (k = 1: e_1 = d_0 = z_0 = M^-1 r_in) and d_{k-1}, forms
+// r_k = r_in - A e_k, z_k = M^-1 r_k, rho_k = 1 / (2 sigma - rho_{k-1}),
+// d_k = rho_k rho_{k-1} d_{k-1} + (2 rho_k / delta) z_k, e_{k+1} = e_k + d_k.
+// Partial |r_k|^2 goes to partB slot 1 (calibration and tracing).
+template <typename Ti, int CPT>
+__global__ void k_cheb(const int2* __restrict__ rows, const BubbleDev* __restrict__ bub,
+                       SolveCtl* __restrict__ ctl, SysPtrs<Ti> sys, const Ti* __restrict__ ek,
+                       Ti* __restrict__ enext, Ti* __restrict__ dv, int it, SolveParams sp,
+                       const double* __restrict__ partR, double* __restrict__ partB, long stride) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    __shared__ double red[16 * 4];
+    __shared__ float redf[16];
+    const int2 rw = rows[blockIdx.x];
+    if (rw.x < 0) return;
+    SolveCtl* C = ctl + rw.x;
+    if (C->outer_done) return;
+    const BubbleDev B = bub[rw.x];
+    const int j = rw.y, nph = B.nph, m = B.mask, nth = B.nth;
+    const bool leader = (j == 0 && threadIdx.x == 0);
+    double mu;
+    if (it == 1) {
+        double sm[2];
+        sum_partials<2>(partR, stride, B.pr0, nth, sm, red);
+        const double res = sp.absolute ? sqrt(sm[0]) : sqrt(sm[0]) / (sm[1] > 0.0 ? sqrt(sm[1]) : 1.0);
+        if (res <= sp.tol) {
+            if (leader) { C->outer_done = 1; C->rel_res = res; }
+            return;
+        }
+        mu = max_partials(partR + 4 * stride, B.pr0, nth, red);
+        if (leader) {
+            C->mu = mu;
+            C->rel_res = res;
+            C->bb = sm[1];
+        }
+    } else {
+        mu = C->mu;
+    }
+    const double delta = fmax(mu * 1.02 + 1e-3, 1e-4);
+    const double sigma = 1.0 / delta;
+    double rho_prev = delta, rho = delta;      // rho_0 = 1 / sigma
+    for (int q = 1; q <= it; ++q) {
+        rho_prev = rho;
+        rho = 1.0 / (2.0 * sigma - rho_prev);
+    }
+    const Ti cd = (Ti)(rho * rho_prev), cz = (Ti)(2.0 * rho / delta);
+    const int steps = sys.rsteps[B.pr0 + j];
+    Ti* sa = reinterpret_cast<Ti*>(lds_raw);
+    Ti* sc = sa + nph;
+    Ti* sd = sc + nph;
+    const size_t row = (size_t)B.off + (size_t)j * nph;
+    const bool hasN = j > 0, hasS = j < nth - 1;
+    const Ti* __restrict__ e = it == 1 ? sys.z : ek;
+    const Ti* __restrict__ dprev = it == 1 ? sys.z : dv;
+    Ti ev[CPT], dvv[CPT];
+    double acc = 0.0;
+#pragma unroll
+    for (int q = 0; q < CPT; ++q) {
+        const int i = threadIdx.x + q * blockDim.x;
+        ev[q] = dvv[q] = Ti(0);
+        if (i < nph) {
+            const size_t c = row + i, cW = row + ((i - 1) & m), cE = row + ((i + 1) & m);
+            const Ti ec = e[c];
+            Ti ax = sys.diag[c] * ec - sys.wW[c] * e[cW] - sys.wW[cE] * e[cE];
+            if (hasN) ax -= sys.wN[c] * e[c - nph];
+            if (hasS) ax -= sys.wN[c + nph] * e[c + nph];
+            const Ti r = sys.r[c] - ax;
+            ev[q] = ec;
+            dvv[q] = dprev[c];
+            const Ti inv = pre_rcp(sys.diag[c]);
+            sa[i] = -sys.wW[c] * inv;
+            sc[i] = -sys.wW[cE] * inv;
+            sd[i] = r * inv;
+            acc += (double)r * (double)r;
+        }
+    }
+    __syncthreads();
+    pcr_periodic<Ti, CPT>(sa, sc, sd, nph, steps, redf);
+#pragma unroll
+    for (int q = 0; q < CPT; ++q) {
+        const int i = threadIdx.x + q * blockDim.x;
+        if (i < nph) {
+            const Ti dn = cd * dvv[q] + cz * sd[i];
+            dv[row + i] = dn;
+            enext[row + i] = ev[q] + dn;
+        }
+    }
+    double a1[1] = {acc};
+    block_sum<1>(a1, red);
+    if (threadIdx.x == 0) {
+        partB[stride + B.pr0 + j] = a1[0];
         if (j == 0) C->iters += 1;
     }
 }
@@ -543,6 +705,7 @@ __global__ __launch_bounds__(256) void k_update(const int4* __restrict__ tiles,
                                                 long stride) {
     __shared__ double red[16 * 4];
     const int4 tl = tiles[blockIdx.x];
+    if (tl.x < 0) return;
     const BubbleDev B = bub[tl.x];
     if (tl.y == 0 && tl.z == 0) {
         // first tile of the bubble: solve verdict, and the time advance that
@@ -626,6 +789,7 @@ __global__ void k_row_stats(const int2* __restrict__ rows, const BubbleDev* __re
                             long stride) {
     __shared__ double red[16 * 4];
     const int2 rw = rows[blockIdx.x];
+    if (rw.x < 0) return;
     const BubbleDev B = bub[rw.x];
     const size_t row = (size_t)B.off + (size_t)rw.y * B.nph;
     double acc[2] = {0.0, 0.0};
@@ -671,29 +835,40 @@ void launch_assemble(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub,
 
 template <typename Ti, int CPT>
 static void resid_cpt(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, SolveCtl* ctl,
-                      SysPtrs<Ti> sys, const double* xo, double* xn, int apply_e, int k,
-                      double* part, long stride) {
+                      SysPtrs<Ti> sys, const double* xo, double* xn, int k, double* part,
+                      long stride) {
     hipLaunchKernelGGL((k_resid<Ti, CPT>), dim3(g.nrows), dim3(g.row_threads), g.lds_bytes, s,
-                       g.rows, bub, ctl, sys, xo, xn, apply_e, k, part, stride);
+                       g.rows, bub, ctl, sys, xo, xn, k, part, stride);
 }
 template <typename Ti>
 void launch_resid(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, SolveCtl* ctl,
-                  SysPtrs<Ti> sys, const double* xo, double* xn, int apply_e, int k, double* part,
+                  SysPtrs<Ti> sys, const double* xo, double* xn, int k, double* part,
                   long stride) {
     switch (g.cpt) {
-        case 1: resid_cpt<Ti, 1>(s, g, bub, ctl, sys, xo, xn, apply_e, k, part, stride); break;
-        case 2: resid_cpt<Ti, 2>(s, g, bub, ctl, sys, xo, xn, apply_e, k, part, stride); break;
-        case 4: resid_cpt<Ti, 4>(s, g, bub, ctl, sys, xo, xn, apply_e, k, part, stride); break;
-        default: resid_cpt<Ti, 8>(s, g, bub, ctl, sys, xo, xn, apply_e, k, part, stride); break;
+        case 1: resid_cpt<Ti, 1>(s, g, bub, ctl, sys, xo, xn, k, part, stride); break;
+        case 2: resid_cpt<Ti, 2>(s, g, bub, ctl, sys, xo, xn, k, part, stride); break;
+        case 4: resid_cpt<Ti, 4>(s, g, bub, ctl, sys, xo, xn, k, part, stride); break;
+        default: resid_cpt<Ti, 8>(s, g, bub, ctl, sys, xo, xn, k, part, stride); break;
     }
 }
 
+template <typename Ti, int CPT>
+static void spmv_cpt(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, SolveCtl* ctl,
+                     SysPtrs<Ti> sys, const Ti* pold, Ti* pnew, int it, const SolveParams& sp,
+                     const double* partR, const double* partB, double* partA, long stride) {
+    hipLaunchKernelGGL((k_spmv<Ti, CPT>), dim3(g.nrows), dim3(g.row_threads), 0, s, g.rows, bub,
+                       ctl, sys, pold, pnew, it, sp, partR, partB, partA, stride);
+}
 template <typename Ti>
 void launch_spmv(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, SolveCtl* ctl,
                  SysPtrs<Ti> sys, const Ti* pold, Ti* pnew, int it, const SolveParams& sp,
                  const double* partR, const double* partB, double* partA, long stride) {
-    hipLaunchKernelGGL((k_spmv<Ti, 1>), dim3(g.nrows), dim3(g.row_threads), 0, s, g.rows, bub, ctl,
-                       sys, pold, pnew, it, sp, partR, partB, partA, stride);
+    switch (g.cpt) {
+        case 1: spmv_cpt<Ti, 1>(s, g, bub, ctl, sys, pold, pnew, it, sp, partR, partB, partA, stride); break;
+        case 2: spmv_cpt<Ti, 2>(s, g, bub, ctl, sys, pold, pnew, it, sp, partR, partB, partA, stride); break;
+        case 4: spmv_cpt<Ti, 4>(s, g, bub, ctl, sys, pold, pnew, it, sp, partR, partB, partA, stride); break;
+        default: spmv_cpt<Ti, 8>(s, g, bub, ctl, sys, pold, pnew, it, sp, partR, partB, partA, stride); break;
+    }
 }
 
 template <typename Ti, int CPT>
@@ -712,6 +887,25 @@ void launch_cgstep(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, Sol
         case 2: cgstep_cpt<Ti, 2>(s, g, bub, ctl, sys, p, it, partA, partB, stride); break;
         case 4: cgstep_cpt<Ti, 4>(s, g, bub, ctl, sys, p, it, partA, partB, stride); break;
         default: cgstep_cpt<Ti, 8>(s, g, bub, ctl, sys, p, it, partA, partB, stride); break;
+    }
+}
+
+template <typename Ti, int CPT>
+static void cheb_cpt(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, SolveCtl* ctl,
+                     SysPtrs<Ti> sys, const Ti* ek, Ti* enext, Ti* dv, int it, const SolveParams& sp,
+                     const double* partR, double* partB, long stride) {
+    hipLaunchKernelGGL((k_cheb<Ti, CPT>), dim3(g.nrows), dim3(g.row_threads), g.lds_bytes, s,
+                       g.rows, bub, ctl, sys, ek, enext, dv, it, sp, partR, partB, stride);
+}
+template <typename Ti>
+void launch_cheb(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, SolveCtl* ctl,
+                 SysPtrs<Ti> sys, const Ti* ek, Ti* enext, Ti* dv, int it, const SolveParams& sp,
+                 const double* partR, double* partB, long stride) {
+    switch (g.cpt) {
+        case 1: cheb_cpt<Ti, 1>(s, g, bub, ctl, sys, ek, enext, dv, it, sp, partR, partB, stride); break;
+        case 2: cheb_cpt<Ti, 2>(s, g, bub, ctl, sys, ek, enext, dv, it, sp, partR, partB, stride); break;
+        case 4: cheb_cpt<Ti, 4>(s, g, bub, ctl, sys, ek, enext, dv, it, sp, partR, partB, stride); break;
+        default: cheb_cpt<Ti, 8>(s, g, bub, ctl, sys, ek, enext, dv, it, sp, partR, partB, stride); break;
     }
 }
 
@@ -751,12 +945,15 @@ SF_INST_SYS(float, double)
 SF_INST_SYS(double, double)
 #define SF_INST_TI(Ti)                                                                            \
     template void launch_resid<Ti>(hipStream_t, const LaunchGeom&, const BubbleDev*, SolveCtl*,   \
-                                   SysPtrs<Ti>, const double*, double*, int, int, double*, long); \
+                                   SysPtrs<Ti>, const double*, double*, int, double*, long);      \
     template void launch_spmv<Ti>(hipStream_t, const LaunchGeom&, const BubbleDev*, SolveCtl*,    \
                                   SysPtrs<Ti>, const Ti*, Ti*, int, const SolveParams&,           \
                                   const double*, const double*, double*, long);                   \
     template void launch_cgstep<Ti>(hipStream_t, const LaunchGeom&, const BubbleDev*, SolveCtl*,  \
-                                    SysPtrs<Ti>, const Ti*, int, const double*, double*, long);
+                                    SysPtrs<Ti>, const Ti*, int, const double*, double*, long);   \
+    template void launch_cheb<Ti>(hipStream_t, const LaunchGeom&, const BubbleDev*, SolveCtl*,    \
+                                  SysPtrs<Ti>, const Ti*, Ti*, Ti*, int, const SolveParams&,      \
+                                  const double*, double*, long);
 SF_INST_TI(float)
 SF_INST_TI(double)
 template void launch_update<float>(hipStream_t, const LaunchGeom&, const BubbleDev*, BubbleTime*,

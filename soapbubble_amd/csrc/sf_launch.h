@@ -35,6 +35,7 @@ struct SysPtrs {
     Ti* r;           // inner residual
     Ti* z;           // preconditioned residual
     Ti* Ap;
+    int* rsteps;     // line-solve reductions per row (set by the assembly)
 };
 
 struct SolveParams {
@@ -64,8 +65,7 @@ void launch_assemble(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub,
                      SysPtrs<Ti> sys, double gmax_state, SolveCtl* ctl, double* part, long stride);
 template <typename Ti>
 void launch_resid(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, SolveCtl* ctl,
-                  SysPtrs<Ti> sys, const double* xo, double* xn, int apply_e, int k, double* part,
-                  long stride);
+                  SysPtrs<Ti> sys, const double* xo, double* xn, int k, double* part, long stride);
 template <typename Ti>
 void launch_spmv(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, SolveCtl* ctl,
                  SysPtrs<Ti> sys, const Ti* pold, Ti* pnew, int it, const SolveParams& sp,
@@ -74,6 +74,10 @@ template <typename Ti>
 void launch_cgstep(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, SolveCtl* ctl,
                    SysPtrs<Ti> sys, const Ti* p, int it, const double* partA, double* partB,
                    long stride);
+template <typename Ti>
+void launch_cheb(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, SolveCtl* ctl,
+                 SysPtrs<Ti> sys, const Ti* ek, Ti* enext, Ti* dv, int it, const SolveParams& sp,
+                 const double* partR, double* partB, long stride);
 void launch_finalize(hipStream_t s, int nb, const BubbleDev* bub, SolveCtl* ctl, SolveHist* hist,
                      const SolveParams& sp, const double* partR, long stride);
 template <typename T>

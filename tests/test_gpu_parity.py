@@ -60,16 +60,18 @@ def test_stages_fp64(oracle):
             assert rel_l2(got[k], new[k]) < 1e-10, (k, rel_l2(got[k], new[k]))
 
 
-@pytest.mark.parametrize("prec,tol", [("fp64", 1e-9), ("fp32_adv", 5e-5), ("mixed", 5e-5)])
-def test_steps_c0(oracle, prec, tol):
+@pytest.mark.parametrize("prec,method,tol", [("fp64", "auto", 1e-9), ("fp64", "chebyshev", 1e-9),
+                                             ("fp32_adv", "auto", 5e-5), ("mixed", "auto", 5e-5),
+                                             ("mixed", "cg", 5e-5)])
+def test_steps_c0(oracle, prec, method, tol):
     """64 x 128, dt = 0.005, 20 steps (the CPU-runnable workload)."""
-    _say(f"20 steps {prec} 64x128")
+    _say(f"20 steps {prec}/{method} 64x128")
     n, dt, steps = 64, 0.005, 20
     st, ua, va = make_case(n, 0)
     p = oracle.params(n, dt)
     ref = copy_state(st)
     oracle.step(p, ref, ua, va, steps)
-    with sb.SoapFilm(sb.bubble(n, dt), prec) as ctx:
+    with sb.SoapFilm(sb.bubble(n, dt), prec, inner_method=method) as ctx:
         load(ctx, 0, st, ua, va)
         ctx.step(steps)
         got = ctx.state(0)
