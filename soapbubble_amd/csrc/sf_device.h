@@ -95,10 +95,49 @@ __device__ __forceinline__ float tfloor(float x) { return floorf(x); }
 __device__ __forceinline__ double tfloor(double x) { return floor(x); }
 __device__ __forceinline__ float tsqrt(float x) { return sqrtf(x); }
 __device__ __forceinline__ double tsqrt(double x) { return sqrt(x); }
-__device__ __forceinline__ float tatan2(float y, float x) { return atan2f(y, x); }
-__device__ __forceinline__ double tatan2(double y, double x) { return atan2(y, x); }
 __device__ __forceinline__ bool tisnan(float x) { return __builtin_isnan(x); }
 __device__ __forceinline__ bool tisnan(double x) { return __builtin_isnan(x); }
+
+// FP32 atan2 for the traced points: octant reduction to a = min/max in
+// [0, 1] and atan(a) = a P(a^2), P of degree 8 fitted to atan(a)/a on [0, 1]
+// by iteratively reweighted least squares on the relative error (Lawson's
+// algorithm); maximum absolute error 1.2e-7 rad evaluated in FP32, i.e. a
+// few 1e-5 of a cell at nTheta = 1024, below the FP32 resolution of the
+// index-space position. FP64 keeps the library atan2.
+__device__ __forceinline__ float tatan2(float y, float x) {
+    const float ax = fabsf(x), ay = fabsf(y);
+    const float mx = fmaxf(ax, ay), mn = fminf(ax, ay);
+    const float a = mx > 0.0f ? mn * __builtin_amdgcn_rcpf(mx) : 0.0f;
+    const float s = a * a;
+    float p = 0.002850157907232642f;
+    p = p * s - 0.01606966182589531f;
+    p = p * s + 0.0426931269466877f;
+    p = p * s - 0.07504424452781677f;
+    p = p * s + 0.10640992224216461f;
+    p = p * s - 0.1420365869998932f;
+    p = p * s + 0.1999262124300003f;
+    p = p * s - 0.3333307206630707f;
+    p = p * s + 1.0f;
+    float r = a * p;
+    if (ay > ax) r = 1.57079632679489662f - r;
+    if (x < 0.0f) r = 3.14159265358979324f - r;
+    return copysignf(r, y);
+}
+__device__ __forceinline__ double tatan2(double y, double x) { return atan2(y, x); }
+
+// Normalisation of a direction: returns |v| and 1/|v| from |v|^2. FP32 uses
+// the hardware reciprocal square root; FP64 the correctly rounded sqrt and
+// division.
+__device__ __forceinline__ void norm_inv(float s, float& n, float& inv) {
+    inv = __builtin_amdgcn_rsqf(s);
+    n = s > 0.0f ? s * inv : s;   // |0| = 0 as length() gives, not 0 * inf
+}
+__device__ __forceinline__ void norm_inv(double s, double& n, double& inv) {
+    n = sqrt(s);
+    inv = 1.0 / n;
+}
+__device__ __forceinline__ float trcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double trcp(double x) { return 1.0 / x; }
 
 // sin/cos of the great-circle step. The step is a small angle (|u| dt), so
 // FP32 uses the Taylor series to x^7 / x^6 below 1/16 rad (truncation below
@@ -144,7 +183,7 @@ struct Field {
     int nph, mask, maxrow;
     __device__ __forceinline__ T at(int row, int col) const {
         row = row < 0 ? 0 : (row > maxrow ? maxrow : row);
-        return p[(size_t)row * nph + (col & mask)];
+        return p[(unsigned)(row * nph + (col & mask))];
     }
     __device__ __forceinline__ T belt(int row, int lo, T a) const {
         return lerp_nan(at(row, lo), at(row, lo + 1), a);
@@ -186,6 +225,31 @@ __device__ T sample_centered(const Field<T>& f, int nth, T rx, T ry) {
     T lower = f.belt(tI, pI, aP);
     T higher = f.belt(tI + 1, pI, aP);
     return lerp_nan(lower, higher, aT);
+}
+
+// sampleCentered of two fields at one point (the eta and Gamma samples of
+// advectionAllCentered, bubble.cu:957-960) sharing the index computation.
+template <typename T>
+__device__ void sample_centered2(const Field<T>& f, const Field<T>& g, int nth, T rx, T ry, T& of,
+                                 T& og) {
+    T x = rx - T(0.5), y = ry;
+    const int nph = f.nph;
+    bool fl = validate_coord(x, y, nth, nph);
+    int tI, pI; T aT, aP;
+    split_index(x, y, nph, tI, pI, aT, aP);
+    if (tI == 0 && fl) {
+        of = lerp_nan(f.belt(0, pI + nph / 2, aP), f.belt(0, pI, aP), aT);
+        og = lerp_nan(g.belt(0, pI + nph / 2, aP), g.belt(0, pI, aP), aT);
+        return;
+    }
+    if (fl) tI -= 1;
+    if (tI == nth - 1) {
+        of = lerp_nan(f.belt(tI, pI, aP), f.belt(tI, pI + nph / 2, aP), aT);
+        og = lerp_nan(g.belt(tI, pI, aP), g.belt(tI, pI + nph / 2, aP), aT);
+        return;
+    }
+    of = lerp_nan(f.belt(tI, pI, aP), f.belt(tI + 1, pI, aP), aT);
+    og = lerp_nan(g.belt(tI, pI, aP), g.belt(tI + 1, pI, aP), aT);
 }
 
 // sampleVPhi (bubble.cu:178-240): values taken across the pole flip sign.
@@ -281,7 +345,7 @@ __device__ __forceinline__ void cart_to_index(T x, T y, T z, T invh, T& ti, T& p
 template <typename T>
 __device__ __forceinline__ void tangent_vec(T x, T y, T z, T rho, T vT, T vP, T& o0, T& o1, T& o2) {
     T cp = T(1), sp = T(0);
-    if (rho > T(0)) { T ir = T(1) / rho; cp = x * ir; sp = y * ir; }
+    if (rho > T(0)) { T ir = trcp(rho); cp = x * ir; sp = y * ir; }
     o0 = vT * z * cp - vP * sp;
     o1 = vT * z * sp + vP * cp;
     o2 = -vT * rho;
@@ -294,9 +358,9 @@ __device__ __forceinline__ bool circle_mid(const VelCtx<T>& c, T sinT, T cosT, T
     T u1 = guT * (cosT * sinP) + guP * cosP;
     T u2 = -guT * sinT;
     o.w0 = cosP * sinT; o.w1 = sinP * sinT; o.w2 = cosT;
-    o.unorm = tsqrt(u0 * u0 + u1 * u1 + u2 * u2);
+    T inv;
+    norm_inv(u0 * u0 + u1 * u1 + u2 * u2, o.unorm, inv);
     if (!(o.unorm > T(0))) return false;
-    T inv = T(1) / o.unorm;
     o.u0 = u0 * inv; o.u1 = u1 * inv; o.u2 = u2 * inv;
     T s, co;
     sincos_step(T(-0.5) * o.unorm * c.dt, &s, &co);
@@ -316,9 +380,8 @@ __device__ __forceinline__ bool circle_mid(const VelCtx<T>& c, T sinT, T cosT, T
     u0 = mguP * o.u0 + mguT * v0;
     u1 = mguP * o.u1 + mguT * v1;
     u2 = mguP * o.u2 + mguT * v2;
-    o.unorm = tsqrt(u0 * u0 + u1 * u1 + u2 * u2);
+    norm_inv(u0 * u0 + u1 * u1 + u2 * u2, o.unorm, inv);
     if (!(o.unorm > T(0))) return false;
-    inv = T(1) / o.unorm;
     o.u0 = u0 * inv; o.u1 = u1 * inv; o.u2 = u2 * inv;
     o.v0 = o.w1 * o.u2 - o.w2 * o.u1;
     o.v1 = o.w2 * o.u0 - o.w0 * o.u2;

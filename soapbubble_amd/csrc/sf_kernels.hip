@@ -58,25 +58,41 @@ __global__ __launch_bounds__(256) void k_advect(const int4* __restrict__ tiles,
     const T sinTc = TabSel<T>::sTc(tab)[B.rt0 + j], cosTc = TabSel<T>::cTc(tab)[B.rt0 + j];
     const T sinPc = TabSel<T>::sPc(tab)[B.ct0 + i], cosPc = TabSel<T>::cPc(tab)[B.ct0 + i];
 
+    // Start-point samples. Away from the two polar rows the sample positions
+    // are grid-aligned, and the samplers reduce to their interior branch with
+    // weights 0 or 1/2 (exactly the values the general path computes); the
+    // polar rows go through the general samplers.
+    const bool interior = j > 0 && j < B.nth - 1;
+
     // eta, Gamma at the cell centre (j+1/2, i): traceGreatCircle + two samples
     {
         const T gx = (T)j + T(0.5), gy = (T)i;
-        T guT = sample_vtheta(v.uth, B.nth, gx, gy);
-        T guP = sample_vphi(v.uph, B.nth, gx, gy);
+        T guT, guP;
+        if (interior) {
+            guT = lerp_nan(v.uth.belt(j - 1, i, T(0)), v.uth.belt(j, i, T(0)), T(0.5));
+            guP = lerp_nan(v.uph.belt(j, i, T(0.5)), v.uph.belt(j + 1, i, T(0.5)), T(0));
+        } else {
+            guT = sample_vtheta(v.uth, B.nth, gx, gy);
+            guP = sample_vphi(v.uph, B.nth, gx, gy);
+        }
         Circle<T> o;
         T tx = gx, ty = gy;
         if (circle_mid(v, sinTc, cosTc, sinPc, cosPc, guT, guP, o)) {
             T s, co, px, py, pz, rho;
             circle_depart(v, o, s, co, px, py, pz, tx, ty, rho);
         }
-        out.eta[B.off + c] = sample_centered(feta, B.nth, tx, ty);
-        out.gam[B.off + c] = sample_centered(fgam, B.nth, tx, ty);
+        T en, gn;
+        sample_centered2(feta, fgam, B.nth, tx, ty, en, gn);
+        out.eta[B.off + c] = en;
+        out.gam[B.off + c] = gn;
     }
     // u_phi at the west face (j+1/2, i-1/2)
     {
         const T gx = (T)j + T(0.5), gy = (T)i - T(0.5);
         const T sinP = TabSel<T>::sPh(tab)[B.ct0 + i], cosP = TabSel<T>::cPh(tab)[B.ct0 + i];
-        T guT = sample_vtheta(v.uth, B.nth, gx, gy);
+        T guT = interior
+                    ? lerp_nan(v.uth.belt(j - 1, i - 1, T(0.5)), v.uth.belt(j, i - 1, T(0.5)), T(0.5))
+                    : sample_vtheta(v.uth, B.nth, gx, gy);
         T guP = in.uph[B.off + c];
         Circle<T> o;
         T res;
@@ -89,12 +105,12 @@ __global__ __launch_bounds__(256) void k_advect(const int4* __restrict__ tiles,
         }
         out.uph[B.off + c] = res;
     }
-    // u_theta at the south face (j+1, i)
+    // u_theta at the south face (j+1, i); sampleVPhi there is always the
+    // interior branch with weights 1/2
     if (j < B.nth - 1) {
-        const T gx = (T)(j + 1), gy = (T)i;
         const T sinT = TabSel<T>::sTf(tab)[B.rt0 + j + 1], cosT = TabSel<T>::cTf(tab)[B.rt0 + j + 1];
         T guT = in.uth[B.off + c];
-        T guP = sample_vphi(v.uph, B.nth, gx, gy);
+        T guP = lerp_nan(v.uph.belt(j, i, T(0.5)), v.uph.belt(j + 1, i, T(0.5)), T(0.5));
         Circle<T> o;
         T res;
         if (!circle_mid(v, sinT, cosT, sinPc, cosPc, guT, guP, o)) {
