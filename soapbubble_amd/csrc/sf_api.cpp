@@ -81,7 +81,7 @@ struct Base {
     double* partB = nullptr;
     double* partS = nullptr;
     double* x64[2] = {nullptr, nullptr};
-    double *wN64 = nullptr, *wW64 = nullptr, *diag64 = nullptr, *b64 = nullptr;
+    double* b64 = nullptr;
     int* rsteps = nullptr;
     int nxcd = 1;
     SolveParams sp{};
@@ -125,7 +125,7 @@ struct Impl : Base {
 
     AirPtrs<T> air() const { return AirPtrs<T>{ua0, va0, au, av}; }
     SysPtrs<Ti> sys(double* x_init) const {
-        return SysPtrs<Ti>{wN64, wW64, diag64, b64, x_init, wN, wW, dg, e, r, z, Ap, rsteps};
+        return SysPtrs<Ti>{b64, x_init, wN, wW, dg, e, r, z, Ap, rsteps};
     }
     bool fp64_inner() const override { return sizeof(Ti) == 8; }
 
@@ -184,7 +184,7 @@ struct Impl : Base {
         double* xn = x64[k & 1];
         SysPtrs<Ti> s = sys(x64[0]);
         s.e = ecorr;
-        launch_resid<Ti>(stream, geom, bub_d, ctl_d, s, xo, xn, k, partR, stride);
+        launch_resid<T, Ti>(stream, geom, bub_d, tab, nxt, ctl_d, s, xo, xn, k, partR, stride);
     }
     // Chebyshev buffers: e alternates between e and Ap, d lives in p[0].
     Ti* cheb_e(int k) { return (k & 1) ? Ap : e; }
@@ -345,9 +345,9 @@ struct Impl : Base {
             case SF_FIELD_ETA_STAR: fetch(nxt.eta, o, N, h); break;
             case SF_FIELD_GAMMA_STAR: fetch(nxt.gam, o, N, h); break;
             case SF_FIELD_RHS: fetch(b64, o, N, h); break;
-            case SF_FIELD_DIAG: fetch(diag64, o, N, h); break;
-            case SF_FIELD_WNORTH: fetch(wN64, o, N, h); break;
-            case SF_FIELD_WWEST: fetch(wW64, o, N, h); break;
+            case SF_FIELD_DIAG: fetch(dg, o, N, h); break;
+            case SF_FIELD_WNORTH: fetch(wN, o, N, h); break;
+            case SF_FIELD_WWEST: fetch(wW, o, N, h); break;
             case SF_FIELD_SOLUTION: {
                 SolveCtl c;
                 SF_HIP(hipMemcpyAsync(&c, ctl_d + b, sizeof(c), hipMemcpyDeviceToHost, stream));
@@ -533,16 +533,20 @@ void setup(Base& c, int prec) {
     c.geom.ntiles = (int)tiles.size();
     c.geom.rows = rows_d;
     c.geom.nrows = (int)rows.size();
-    int nt = std::max(64, std::min(1024, nph_max / 4));
-    nt = (nt + 63) / 64 * 64;
-    int cpt = (nph_max + nt - 1) / nt;
-    int cp2 = 1;
-    while (cp2 < cpt) cp2 <<= 1;
-    if (cp2 > 8) throw Status{SF_ERR_UNSUPPORTED, "nPhi above 8192 is not supported"};
+    // row blocks: C consecutive cells per thread (4, or 8 above nPhi 4096,
+    // 2 for small rows), one thread per C cells of the widest row; every
+    // row must hold at least 4 C cells for the reduced line system
+    int nph_min = nph_max;
+    for (int b = 0; b < nb; ++b) nph_min = std::min(nph_min, c.bub_h[b].nph);
+    int cpt = nph_max > 4096 ? 8 : (nph_max <= 128 ? 2 : 4);
+    if (nph_min < 4 * cpt) cpt = 2;
+    if (nph_min < 4 * cpt || nph_max / cpt > 1024)
+        throw Status{SF_ERR_UNSUPPORTED, "bubble resolutions of one context span too wide a range"};
+    const int nt = std::max(64, (nph_max / cpt + 63) / 64 * 64);
     c.geom.row_threads = nt;
-    c.geom.cpt = cp2;
+    c.geom.cpt = cpt;
     const size_t esz = c.fp64_inner() ? 8 : 4;
-    c.geom.lds_bytes = 3 * (size_t)nph_max * esz;
+    c.geom.lds_bytes = 6 * (size_t)(nph_max / cpt) * esz;
     const size_t lds_limit = (size_t)prop.sharedMemPerBlock - 1024;
     if (c.geom.lds_bytes > lds_limit) {
         throw Status{SF_ERR_UNSUPPORTED,
@@ -558,9 +562,6 @@ void setup(Base& c, int prec) {
     const size_t n = (size_t)c.ncells;
     c.x64[0] = dalloc<double>(n, c.owned);
     c.x64[1] = dalloc<double>(n, c.owned);
-    c.wN64 = dalloc<double>(n, c.owned);
-    c.wW64 = dalloc<double>(n, c.owned);
-    c.diag64 = dalloc<double>(n, c.owned);
     c.b64 = dalloc<double>(n, c.owned);
     c.rsteps = dalloc<int>((size_t)c.stride, c.owned);
 

@@ -153,7 +153,9 @@ struct CellSys {
     bool brk;
 };
 
-template <typename T>
+// RHS = false: the operator row only (the FP64 residual recomputes it from
+// eta and Gamma* instead of storing it).
+template <typename T, bool RHS>
 __device__ __forceinline__ CellSys assemble_cell(const BubbleDev& B, double t, const Tables& tab,
                                                  const StatePtrs<T>& s, const AirPtrs<T>& air,
                                                  int j, int i) {
@@ -192,56 +194,59 @@ __device__ __forceinline__ CellSys assemble_cell(const BubbleDev& B, double t, c
     const double wE = s2 * csc * MDt / (etaRight + CrDt);
     const double wS = s2 * sinS * MDt / (etaDown + CrDt);
 
+    r.rhs = 0.0;
     if (r.brk) {
         r.diag = 1.0;
         r.rhs = gmax;
     } else {
         r.diag = sinTheta * invDt / gamma - (((-wN) + (-wW)) + (-wE) + (-wS));
-        double uW = (double)s.uph[c], uE = (double)s.uph[cE];
-        double vN = j > 0 ? (double)s.uth[c - nph] : 0.0;
-        double vS = j < nth - 1 ? (double)s.uth[c] : 0.0;
-        if (__builtin_isnan(vN)) vN = 0.0;
-        if (__builtin_isnan(vS)) vS = 0.0;
-        if (__builtin_isnan(uW)) uW = 0.0;
-        if (__builtin_isnan(uE)) uE = 0.0;
-        double div = B.invh * (uE * etaRight / (etaRight + Cr * dt) -
-                               uW * etaLeft / (etaLeft + Cr * dt) +
-                               (vS * sinS * etaDown / (etaDown + Cr * dt) -
-                                vN * sinN * etaUp / (etaUp + Cr * dt)));
-        if (B.airflow) {
-            const double kt = 3.0 - fabs(t - 3.0);
-            const double bl = t / 12.0;
-            const size_t row = o + (size_t)j * nph;
-            const int opp = (nph - i) & m, oppE = (nph - iE) & m;
-            const double uAirW = 0.5 * (air_at(air.ua0, air.au, row + i, kt) * bl +
-                                        air_at(air.ua0, air.au, row + opp, kt) * (1 - bl));
-            const double uAirE = 0.5 * (air_at(air.ua0, air.au, row + iE, kt) * bl +
-                                        air_at(air.ua0, air.au, row + oppE, kt) * (1 - bl));
-            double vAirN = 0.0, vAirS = 0.0;
-            if (j > 0) {
-                const size_t rn = row - nph;
-                vAirN = 0.5 * (air_at(air.va0, air.av, rn + i, kt) * bl -
-                               air_at(air.va0, air.av, rn + opp, kt) * (1 - bl));
+        if (RHS) {
+            double uW = (double)s.uph[c], uE = (double)s.uph[cE];
+            double vN = j > 0 ? (double)s.uth[c - nph] : 0.0;
+            double vS = j < nth - 1 ? (double)s.uth[c] : 0.0;
+            if (__builtin_isnan(vN)) vN = 0.0;
+            if (__builtin_isnan(vS)) vS = 0.0;
+            if (__builtin_isnan(uW)) uW = 0.0;
+            if (__builtin_isnan(uE)) uE = 0.0;
+            double div = B.invh * (uE * etaRight / (etaRight + Cr * dt) -
+                                   uW * etaLeft / (etaLeft + Cr * dt) +
+                                   (vS * sinS * etaDown / (etaDown + Cr * dt) -
+                                    vN * sinN * etaUp / (etaUp + Cr * dt)));
+            if (B.airflow) {
+                const double kt = 3.0 - fabs(t - 3.0);
+                const double bl = t / 12.0;
+                const size_t row = o + (size_t)j * nph;
+                const int opp = (nph - i) & m, oppE = (nph - iE) & m;
+                const double uAirW = 0.5 * (air_at(air.ua0, air.au, row + i, kt) * bl +
+                                            air_at(air.ua0, air.au, row + opp, kt) * (1 - bl));
+                const double uAirE = 0.5 * (air_at(air.ua0, air.au, row + iE, kt) * bl +
+                                            air_at(air.ua0, air.au, row + oppE, kt) * (1 - bl));
+                double vAirN = 0.0, vAirS = 0.0;
+                if (j > 0) {
+                    const size_t rn = row - nph;
+                    vAirN = 0.5 * (air_at(air.va0, air.av, rn + i, kt) * bl -
+                                   air_at(air.va0, air.av, rn + opp, kt) * (1 - bl));
+                }
+                if (j < nth - 1) {
+                    vAirS = 0.5 * (air_at(air.va0, air.av, row + i, kt) * bl -
+                                   air_at(air.va0, air.av, row + opp, kt) * (1 - bl));
+                }
+                div += B.invh * (uAirE / (etaRight / Cr * invDt + 1.0) -
+                                 uAirW / (etaLeft / Cr * invDt + 1.0) +
+                                 vAirS * sinS / (etaDown / Cr * invDt + 1.0) -
+                                 vAirN * sinN / (etaUp / Cr * invDt + 1.0));
             }
-            if (j < nth - 1) {
-                vAirS = 0.5 * (air_at(air.va0, air.av, row + i, kt) * bl -
-                               air_at(air.va0, air.av, row + opp, kt) * (1 - bl));
+            if (B.gravity) {
+                div += B.gs * B.invh *
+                       (sinS * sinS * etaDown / (invDt * etaDown + Cr) -
+                        sinN * sinN * etaUp / (invDt * etaUp + Cr));
             }
-            div += B.invh * (uAirE / (etaRight / Cr * invDt + 1.0) -
-                             uAirW / (etaLeft / Cr * invDt + 1.0) +
-                             vAirS * sinS / (etaDown / Cr * invDt + 1.0) -
-                             vAirN * sinN / (etaUp / Cr * invDt + 1.0));
+            r.rhs = sinTheta * invDt - div;
+            if (bN) r.rhs += wN * gmax;
+            if (bS) r.rhs += wS * gmax;
+            if (bW) r.rhs += wW * gmax;
+            if (bE) r.rhs += wE * gmax;
         }
-        if (B.gravity) {
-            div += B.gs * B.invh *
-                   (sinS * sinS * etaDown / (invDt * etaDown + Cr) -
-                    sinN * sinN * etaUp / (invDt * etaUp + Cr));
-        }
-        r.rhs = sinTheta * invDt - div;
-        if (bN) r.rhs += wN * gmax;
-        if (bS) r.rhs += wS * gmax;
-        if (bW) r.rhs += wW * gmax;
-        if (bE) r.rhs += wE * gmax;
     }
     r.wN = (r.brk || bN) ? 0.0 : wN;
     r.wS = (r.brk || bS) ? 0.0 : wS;
@@ -256,13 +261,117 @@ __device__ __forceinline__ CellSys assemble_cell(const BubbleDev& B, double t, c
     return r;
 }
 
+// ------------------------------------------------- blocked row layout ----
+// A row block gives thread t the C consecutive cells [C t, C t + C) of its
+// theta row: row loads and stores are one contiguous vector per thread, and
+// the phi-line solve eliminates each thread's interior cells locally.
+template <int C, typename X>
+__device__ __forceinline__ void ld_row(const X* __restrict__ p, X (&v)[C]) {
+    const X* q = (const X*)__builtin_assume_aligned(p, sizeof(X) * C);
+#pragma unroll
+    for (int k = 0; k < C; ++k) v[k] = q[k];
+}
+template <int C, typename X>
+__device__ __forceinline__ void st_row(X* __restrict__ p, const X (&v)[C]) {
+    X* q = (X*)__builtin_assume_aligned(p, sizeof(X) * C);
+#pragma unroll
+    for (int k = 0; k < C; ++k) q[k] = v[k];
+}
+
+// Periodic phi-line solve of one row, normalised to unit diagonal
+// (x_i + a_i x_{i-1} + c_i x_{i+1} = d_i), n a power of two, thread t owning
+// cells [C t, C t + C) in registers (threads t >= n / C hold nothing but
+// take part in the barriers). Partition method: each thread expresses its
+// interior cells k < C-1 as x_k = al_k + be_k X_{t-1} + ga_k X_t, where X_t is
+// its last cell, by a Thomas sweep with three right-hand sides; substituting
+// the neighbours' interior cells into the last-cell equations leaves a
+// periodic tridiagonal system in the R = n / C unknowns X_t, solved by
+// parallel cyclic reduction in LDS (steps as pcr_periodic); back-substitution
+// recovers the interior cells. On exit d holds x. `lds` holds 6 R values.
+template <typename Ti, int C>
+__device__ __forceinline__ int line_solve(Ti (&a)[C], Ti (&c)[C], Ti (&d)[C], int n, int steps,
+                                          Ti* lds, float* redf) {
+    const int tid = threadIdx.x, R = n / C;
+    const bool act = tid < R;
+    Ti* ra = lds;
+    Ti* rc = ra + R;
+    Ti* rd = rc + R;
+    Ti* xa = rd + R;
+    Ti* xb = xa + R;
+    Ti* xg = xb + R;
+    Ti al[C], be[C], ga[C];
+#pragma unroll
+    for (int k = 0; k < C; ++k) al[k] = be[k] = ga[k] = Ti(0);
+    if (C > 1 && act) {
+        constexpr int NI = C - 1;
+        Ti cp[C], pa[C], pb[C], pg[C];
+        cp[0] = NI > 1 ? c[0] : Ti(0);
+        pa[0] = d[0];
+        pb[0] = -a[0];
+        pg[0] = NI == 1 ? -c[0] : Ti(0);
+#pragma unroll
+        for (int k = 1; k < NI; ++k) {
+            const Ti im = pre_rcp(Ti(1) - a[k] * cp[k - 1]);
+            cp[k] = (k < NI - 1 ? c[k] : Ti(0)) * im;
+            pa[k] = (d[k] - a[k] * pa[k - 1]) * im;
+            pb[k] = (-a[k] * pb[k - 1]) * im;
+            pg[k] = ((k == NI - 1 ? -c[k] : Ti(0)) - a[k] * pg[k - 1]) * im;
+        }
+        al[NI - 1] = pa[NI - 1];
+        be[NI - 1] = pb[NI - 1];
+        ga[NI - 1] = pg[NI - 1];
+#pragma unroll
+        for (int k = NI - 2; k >= 0; --k) {
+            al[k] = pa[k] - cp[k] * al[k + 1];
+            be[k] = pb[k] - cp[k] * be[k + 1];
+            ga[k] = pg[k] - cp[k] * ga[k + 1];
+        }
+        xa[tid] = al[0];
+        xb[tid] = be[0];
+        xg[tid] = ga[0];
+    }
+    __syncthreads();
+    if (act) {
+        constexpr int L = C - 1;
+        Ti A, Bd, Cc, D;
+        if (C > 1) {
+            const int tn = (tid + 1) & (R - 1);
+            const Ti a0n = xa[tn], b0n = xb[tn], g0n = xg[tn];
+            A = a[L] * be[L > 0 ? L - 1 : 0];
+            Bd = Ti(1) + a[L] * ga[L > 0 ? L - 1 : 0] + c[L] * b0n;
+            Cc = c[L] * g0n;
+            D = d[L] - a[L] * al[L > 0 ? L - 1 : 0] - c[L] * a0n;
+        } else {
+            A = a[0];
+            Bd = Ti(1);
+            Cc = c[0];
+            D = d[0];
+        }
+        const Ti ib = pre_rcp(Bd);
+        ra[tid] = A * ib;
+        rc[tid] = Cc * ib;
+        rd[tid] = D * ib;
+    }
+    __syncthreads();
+    const int done = pcr_periodic<Ti, 1>(ra, rc, rd, R, steps, redf);
+    if (act) {
+        const Ti Xt = rd[tid], Xp = rd[(tid - 1) & (R - 1)];
+#pragma unroll
+        for (int k = 0; k < C - 1; ++k) d[k] = al[k] + be[k] * Xp + ga[k] * Xt;
+        d[C - 1] = Xt;
+    }
+    __syncthreads();
+    return done;
+}
+
 // Row-block assembly: coefficients and rhs of one theta row, the initial
-// FP64 residual r0 = b - A Gamma*, r_in = (Ti) r0 and the first line solve
-// z = M^-1 r_in, whose reduction depth is recorded for the later line solves
-// of the same matrix. Partials as k_resid. The block of row 0 resets the
-// bubble's solver control block; the broken-cell override of Gamma* is
-// written back to the state as the reference does.
-template <typename T, typename Ti, int CPT>
+// FP64 residual r0 = b - A Gamma*, r_in = (Ti) r0, the spectral bound mu of
+// the line-preconditioned operator and the first line solve z = M^-1 r_in,
+// whose reduction depth is recorded for the later line solves of the same
+// matrix. Partials as k_resid. The block of row 0 resets the bubble's
+// solver control block; the broken-cell override of Gamma* is written back
+// to the state as the reference does.
+template <typename T, typename Ti, int C>
 __global__ void k_assemble(const int2* __restrict__ rows, const BubbleDev* __restrict__ bub,
                            const BubbleTime* __restrict__ btime, Tables tab, StatePtrs<T> s,
                            AirPtrs<T> air, SysPtrs<Ti> sys, T gmax_state,
@@ -274,52 +383,63 @@ __global__ void k_assemble(const int2* __restrict__ rows, const BubbleDev* __res
     if (rw.x < 0) return;
     const BubbleDev B = bub[rw.x];
     const int j = rw.y, nph = B.nph;
+    const bool act = (int)threadIdx.x < nph / C;
+    const int i0 = C * threadIdx.x;
     const double t = btime[rw.x].t;
-    Ti* sa = reinterpret_cast<Ti*>(lds_raw);
-    Ti* sc = sa + nph;
-    Ti* sd = sc + nph;
     const size_t row = (size_t)B.off + (size_t)j * nph;
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     double mu = 0.0;
-    for (int i = threadIdx.x; i < nph; i += blockDim.x) {
-        const CellSys q = assemble_cell<T>(B, t, tab, s, air, j, i);
-        // x^T N x <= sum (wN + wS) x_i^2 and x^T M x >= sum (diag - wW - wE) x_i^2
-        // for the theta couplings N and the phi-line blocks M, so the
-        // eigenvalues of M^-1 A lie in [1 - mu, 1 + mu] with mu the largest
-        // ratio below
-        if (!q.brk) mu = fmax(mu, (q.wN + q.wS) / (q.diag - q.wW - q.wE));
-        const size_t c = row + i;
-        const double ax = q.diag * q.x0 - q.wN * q.x0N - q.wS * q.x0S - q.wW * q.x0W -
-                          q.wE * q.x0E;
-        const double r0 = q.rhs - ax;
-        const Ti ri = (Ti)r0;
-        sys.wN64[c] = q.wN;
-        sys.wW64[c] = q.wW;
-        sys.diag64[c] = q.diag;
-        sys.b64[c] = q.rhs;
-        sys.x64[c] = q.x0;
-        sys.wN[c] = (Ti)q.wN;
-        sys.wW[c] = (Ti)q.wW;
-        sys.diag[c] = (Ti)q.diag;
-        sys.r[c] = ri;
-        const Ti inv = pre_rcp((Ti)q.diag);
-        sa[i] = -(Ti)q.wW * inv;
-        sc[i] = -(Ti)q.wE * inv;
-        sd[i] = ri * inv;
-        acc[0] += r0 * r0;
-        acc[1] += q.rhs * q.rhs;
-        acc[3] += (double)ri * (double)ri;
+    Ti a[C], cc[C], d[C], rin[C];
+#pragma unroll
+    for (int k = 0; k < C; ++k) a[k] = cc[k] = d[k] = rin[k] = Ti(0);
+    if (act) {
+        double bv[C], xv[C];
+        Ti wn[C], ww[C], dg[C];
+#pragma unroll
+        for (int k = 0; k < C; ++k) {
+            const CellSys q = assemble_cell<T, true>(B, t, tab, s, air, j, i0 + k);
+            const double ax = q.diag * q.x0 - q.wN * q.x0N - q.wS * q.x0S - q.wW * q.x0W -
+                              q.wE * q.x0E;
+            const double r0 = q.rhs - ax;
+            // x^T N x <= sum (wN + wS) x_i^2 and x^T M x >= sum (diag - wW - wE)
+            // x_i^2 for the theta couplings N and the phi-line blocks M, so
+            // the eigenvalues of M^-1 A lie in [1 - mu, 1 + mu] with mu the
+            // largest ratio below
+            if (!q.brk) mu = fmax(mu, (q.wN + q.wS) / (q.diag - q.wW - q.wE));
+            rin[k] = (Ti)r0;
+            bv[k] = q.rhs;
+            xv[k] = q.x0;
+            wn[k] = (Ti)q.wN;
+            ww[k] = (Ti)q.wW;
+            dg[k] = (Ti)q.diag;
+            const Ti inv = pre_rcp(dg[k]);
+            a[k] = -ww[k] * inv;
+            cc[k] = -(Ti)q.wE * inv;
+            d[k] = rin[k] * inv;
+            acc[0] += r0 * r0;
+            acc[1] += q.rhs * q.rhs;
+            acc[3] += (double)rin[k] * (double)rin[k];
+        }
+        st_row<C>(sys.b64 + row + i0, bv);
+        st_row<C>(sys.x64 + row + i0, xv);
+        st_row<C>(sys.wN + row + i0, wn);
+        st_row<C>(sys.wW + row + i0, ww);
+        st_row<C>(sys.diag + row + i0, dg);
+        st_row<C>(sys.r + row + i0, rin);
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < nph; i += blockDim.x) {
-        const size_t c = row + i;
-        if (is_broken((double)s.gam[c], (double)s.eta[c], B.gamma_max)) s.gam[c] = gmax_state;
+    if (act) {
+#pragma unroll
+        for (int k = 0; k < C; ++k) {
+            const size_t c = row + i0 + k;
+            if (is_broken((double)s.gam[c], (double)s.eta[c], B.gamma_max)) s.gam[c] = gmax_state;
+        }
     }
-    const int steps = pcr_periodic<Ti, CPT>(sa, sc, sd, nph, -1, redf);
-    for (int i = threadIdx.x; i < nph; i += blockDim.x) {
-        const Ti z = sd[i];
-        sys.z[row + i] = z;
-        acc[2] += (double)sys.r[row + i] * (double)z;
+    const int steps = line_solve<Ti, C>(a, cc, d, nph, -1, reinterpret_cast<Ti*>(lds_raw), redf);
+    if (act) {
+        st_row<C>(sys.z + row + i0, d);
+#pragma unroll
+        for (int k = 0; k < C; ++k) acc[2] += (double)rin[k] * (double)d[k];
     }
     block_sum<4>(acc, red);
     mu = block_max(mu, red);
@@ -337,13 +457,66 @@ __global__ void k_assemble(const int2* __restrict__ rows, const BubbleDev* __res
     }
 }
 
+// The five-point operator of the inner solve applied to v on this
+// thread's cells: out_k = diag v_k - wW v_{k-1} - wE v_{k+1} - wN vN_k - wS vS_k.
+template <typename Ti, int C>
+struct RowOp {
+    Ti dg[C], ww[C], wwE, wn[C], wsv[C];
+    __device__ __forceinline__ void load(const SysPtrs<Ti>& sys, size_t row, int i0, int m,
+                                         int nph, bool hasN, bool hasS) {
+        ld_row<C>(sys.diag + row + i0, dg);
+        ld_row<C>(sys.wW + row + i0, ww);
+        wwE = sys.wW[row + ((i0 + C) & m)];
+#pragma unroll
+        for (int k = 0; k < C; ++k) wn[k] = wsv[k] = Ti(0);
+        if (hasN) ld_row<C>(sys.wN + row + i0, wn);
+        if (hasS) ld_row<C>(sys.wN + row + nph + i0, wsv);
+    }
+    __device__ __forceinline__ Ti wE(int k) const { return k == C - 1 ? wwE : ww[k + 1]; }
+    __device__ __forceinline__ void apply(const Ti (&v)[C], Ti vW, Ti vE, const Ti (&vN)[C],
+                                          const Ti (&vS)[C], Ti (&out)[C]) const {
+#pragma unroll
+        for (int k = 0; k < C; ++k) {
+            const Ti l = k == 0 ? vW : v[k - 1];
+            const Ti r = k == C - 1 ? vE : v[k + 1];
+            out[k] = dg[k] * v[k] - ww[k] * l - wE(k) * r - wn[k] * vN[k] - wsv[k] * vS[k];
+        }
+    }
+    __device__ __forceinline__ void line(const Ti (&r)[C], Ti (&a)[C], Ti (&c)[C], Ti (&d)[C]) const {
+#pragma unroll
+        for (int k = 0; k < C; ++k) {
+            const Ti inv = pre_rcp(dg[k]);
+            a[k] = -ww[k] * inv;
+            c[k] = -wE(k) * inv;
+            d[k] = r[k] * inv;
+        }
+    }
+};
+
+// A row vector of this thread's cells with its west/east and north/south
+// neighbours (zero where the row has no neighbour).
+template <typename Ti, int C>
+__device__ __forceinline__ void ld_stencil(const Ti* __restrict__ v, size_t row, int i0, int m,
+                                           int nph, bool hasN, bool hasS, Ti (&c)[C], Ti& w, Ti& e,
+                                           Ti (&n)[C], Ti (&s)[C]) {
+    ld_row<C>(v + row + i0, c);
+    w = v[row + ((i0 - 1) & m)];
+    e = v[row + ((i0 + C) & m)];
+#pragma unroll
+    for (int k = 0; k < C; ++k) n[k] = s[k] = Ti(0);
+    if (hasN) ld_row<C>(v + row - nph + i0, n);
+    if (hasS) ld_row<C>(v + row + nph + i0, s);
+}
+
 // ----------------------------------------------------- FP64 residual ----
-// x_new = x_old + e (the inner correction), r = b - A x_new in FP64,
+// x_new = x_old + e (the inner correction), r = b - A x_new in FP64 with the
+// operator recomputed from eta and Gamma* (bitwise the assembly's),
 // r_in = (Ti) r, z = M^-1 r_in. Partials: |r|^2, |b|^2, r_in.z, |r_in|^2.
-template <typename Ti, int CPT>
+template <typename T, typename Ti, int C>
 __global__ void k_resid(const int2* __restrict__ rows, const BubbleDev* __restrict__ bub,
-                        SolveCtl* __restrict__ ctl, SysPtrs<Ti> sys, const double* __restrict__ xo,
-                        double* __restrict__ xn, int k, double* __restrict__ part, long stride) {
+                        Tables tab, StatePtrs<T> s, SolveCtl* __restrict__ ctl, SysPtrs<Ti> sys,
+                        const double* __restrict__ xo, double* __restrict__ xn, int k,
+                        double* __restrict__ part, long stride) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     __shared__ double red[16 * 4];
     __shared__ float redf[16];
@@ -351,52 +524,51 @@ __global__ void k_resid(const int2* __restrict__ rows, const BubbleDev* __restri
     if (rw.x < 0 || ctl[rw.x].outer_done) return;
     const BubbleDev B = bub[rw.x];
     const int j = rw.y, nph = B.nph, m = B.mask, nth = B.nth;
+    const bool act = (int)threadIdx.x < nph / C;
+    const int i0 = C * threadIdx.x;
     const int steps = sys.rsteps[B.pr0 + j];
-    Ti* sa = reinterpret_cast<Ti*>(lds_raw);
-    Ti* sc = sa + nph;
-    Ti* sd = sc + nph;
     const size_t row = (size_t)B.off + (size_t)j * nph;
-    const Ti* __restrict__ e = sys.e;
     const bool hasN = j > 0, hasS = j < nth - 1;
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    Ti rin[CPT];
+    Ti a[C], cc[C], d[C], rin[C];
 #pragma unroll
-    for (int q = 0; q < CPT; ++q) {
-        const int i = threadIdx.x + q * blockDim.x;
-        rin[q] = Ti(0);
-        if (i < nph) {
-            const size_t c = row + i;
-            const size_t cW = row + ((i - 1) & m), cE = row + ((i + 1) & m);
-            const double xc = xo[c] + (double)e[c];
-            double ax = sys.diag64[c] * xc - sys.wW64[c] * (xo[cW] + (double)e[cW]) -
-                        sys.wW64[cE] * (xo[cE] + (double)e[cE]);
-            if (hasN) ax -= sys.wN64[c] * (xo[c - nph] + (double)e[c - nph]);
-            if (hasS) ax -= sys.wN64[c + nph] * (xo[c + nph] + (double)e[c + nph]);
-            xn[c] = xc;
-            const double b = sys.b64[c];
-            const double r = b - ax;
-            const Ti ri = (Ti)r;
-            rin[q] = ri;
-            sys.r[c] = ri;
-            const Ti inv = pre_rcp(sys.diag[c]);
-            sa[i] = -sys.wW[c] * inv;
-            sc[i] = -sys.wW[cE] * inv;
-            sd[i] = ri * inv;
-            acc[0] += r * r;
-            acc[1] += b * b;
-            acc[3] += (double)ri * (double)ri;
+    for (int q = 0; q < C; ++q) a[q] = cc[q] = d[q] = rin[q] = Ti(0);
+    if (act) {
+        double xc[C], xN[C], xS[C], xW, xE;
+        ld_stencil<double, C>(xo, row, i0, m, nph, hasN, hasS, xc, xW, xE, xN, xS);
+        Ti ec[C], eN[C], eS[C], eW, eE;
+        ld_stencil<Ti, C>(sys.e, row, i0, m, nph, hasN, hasS, ec, eW, eE, eN, eS);
+        double bv[C];
+        ld_row<C>(sys.b64 + row + i0, bv);
+        double xnew[C];
+#pragma unroll
+        for (int q = 0; q < C; ++q) xnew[q] = xc[q] + (double)ec[q];
+        const double xWn = xW + (double)eW, xEn = xE + (double)eE;
+#pragma unroll
+        for (int q = 0; q < C; ++q) {
+            const CellSys w = assemble_cell<T, false>(B, 0.0, tab, s, AirPtrs<T>{}, j, i0 + q);
+            const double l = q == 0 ? xWn : xnew[q - 1];
+            const double r = q == C - 1 ? xEn : xnew[q + 1];
+            double ax = w.diag * xnew[q] - w.wW * l - w.wE * r;
+            if (hasN) ax -= w.wN * (xN[q] + (double)eN[q]);
+            if (hasS) ax -= w.wS * (xS[q] + (double)eS[q]);
+            const double res = bv[q] - ax;
+            rin[q] = (Ti)res;
+            acc[0] += res * res;
+            acc[1] += bv[q] * bv[q];
+            acc[3] += (double)rin[q] * (double)rin[q];
         }
+        st_row<C>(xn + row + i0, xnew);
+        st_row<C>(sys.r + row + i0, rin);
+        RowOp<Ti, C> op;
+        op.load(sys, row, i0, m, nph, hasN, hasS);
+        op.line(rin, a, cc, d);
     }
-    __syncthreads();
-    pcr_periodic<Ti, CPT>(sa, sc, sd, nph, steps, redf);
+    line_solve<Ti, C>(a, cc, d, nph, steps, reinterpret_cast<Ti*>(lds_raw), redf);
+    if (act) {
+        st_row<C>(sys.z + row + i0, d);
 #pragma unroll
-    for (int q = 0; q < CPT; ++q) {
-        const int i = threadIdx.x + q * blockDim.x;
-        if (i < nph) {
-            const Ti z = sd[i];
-            sys.z[row + i] = z;
-            acc[2] += (double)rin[q] * (double)z;
-        }
+        for (int q = 0; q < C; ++q) acc[2] += (double)rin[q] * (double)d[q];
     }
     block_sum<4>(acc, red);
     if (threadIdx.x == 0) {
@@ -413,7 +585,7 @@ __global__ void k_resid(const int2* __restrict__ rows, const BubbleDev* __restri
 // p = z + beta p_old (beta from the r.z partials of the previous step),
 // Ap, partial p.Ap. Iteration 0 of a refinement also decides whether the
 // FP64 residual already meets the tolerance.
-template <typename Ti, int CPT>
+template <typename Ti, int C>
 __global__ void k_spmv(const int2* __restrict__ rows, const BubbleDev* __restrict__ bub,
                        SolveCtl* __restrict__ ctl, SysPtrs<Ti> sys, const Ti* __restrict__ pold,
                        Ti* __restrict__ pnew, int it, SolveParams sp, const double* __restrict__ partR,
@@ -421,14 +593,15 @@ __global__ void k_spmv(const int2* __restrict__ rows, const BubbleDev* __restric
     __shared__ double red[16 * 4];
     const int2 rw = rows[blockIdx.x];
     if (rw.x < 0) return;
-    SolveCtl* C = ctl + rw.x;
-    if (C->outer_done || (it > 0 && C->inner_done)) return;
+    SolveCtl* Cb = ctl + rw.x;
+    if (Cb->outer_done || (it > 0 && Cb->inner_done)) return;
     const BubbleDev B = bub[rw.x];
     const int j = rw.y, nph = B.nph, m = B.mask, nth = B.nth;
     const bool leader = (j == 0 && threadIdx.x == 0);
     const bool hasN = j > 0, hasS = j < nth - 1;
+    const bool act = (int)threadIdx.x < nph / C;
+    const int i0 = C * threadIdx.x;
     const size_t row = (size_t)B.off + (size_t)j * nph;
-    const Ti* __restrict__ z = sys.z;
     double beta = 0.0;
     if (it == 0) {
         double sm[4];
@@ -436,72 +609,61 @@ __global__ void k_spmv(const int2* __restrict__ rows, const BubbleDev* __restric
         const double rr = sm[0], bb = sm[1], rz = sm[2], rrin = sm[3];
         const double res = sp.absolute ? sqrt(rr) : sqrt(rr) / (bb > 0.0 ? sqrt(bb) : 1.0);
         if (res <= sp.tol) {
-            if (leader) { C->outer_done = 1; C->rel_res = res; }
+            if (leader) { Cb->outer_done = 1; Cb->rel_res = res; }
             return;
         }
         if (leader) {
-            C->rr_in0 = rrin;
-            C->inner_done = 0;
-            C->rz[0] = rz;
-            C->rel_res = res;
-            C->bb = bb;
+            Cb->rr_in0 = rrin;
+            Cb->inner_done = 0;
+            Cb->rz[0] = rz;
+            Cb->rel_res = res;
+            Cb->bb = bb;
         }
     } else {
         double sm[2];
         sum_partials<2>(partB, stride, B.pr0, nth, sm, red);
         const double rz = sm[0], rr = sm[1];
         const double tgt = sp.inner_frac * sp.tol;
-        const double bbe = sp.absolute ? 1.0 : C->bb;
-        if (rr <= sp.inner_rho2 * C->rr_in0 || rr <= tgt * tgt * bbe) {
-            if (leader) C->inner_done = 1;
+        const double bbe = sp.absolute ? 1.0 : Cb->bb;
+        if (rr <= sp.inner_rho2 * Cb->rr_in0 || rr <= tgt * tgt * bbe) {
+            if (leader) Cb->inner_done = 1;
             return;
         }
-        beta = rz / C->rz[(it - 1) & 1];
-        if (leader) C->rz[it & 1] = rz;
+        beta = rz / Cb->rz[(it - 1) & 1];
+        if (leader) Cb->rz[it & 1] = rz;
     }
-    // operands (the decision above comes first, so a converged solve reads none)
-    Ti zc[CPT], zw[CPT], ze[CPT], zn[CPT], zs[CPT];
-    Ti pc[CPT], pw[CPT], pe[CPT], pn[CPT], ps[CPT];
-    Ti dg[CPT], wn[CPT], wsv[CPT], ww[CPT], we[CPT];
-#pragma unroll
-    for (int q = 0; q < CPT; ++q) {
-        const int i = threadIdx.x + q * blockDim.x;
-        zc[q] = zw[q] = ze[q] = zn[q] = zs[q] = Ti(0);
-        pc[q] = pw[q] = pe[q] = pn[q] = ps[q] = Ti(0);
-        dg[q] = wn[q] = wsv[q] = ww[q] = we[q] = Ti(0);
-        if (i < nph) {
-            const size_t c = row + i, cW = row + ((i - 1) & m), cE = row + ((i + 1) & m);
-            zc[q] = z[c]; zw[q] = z[cW]; ze[q] = z[cE];
-            dg[q] = sys.diag[c]; ww[q] = sys.wW[c]; we[q] = sys.wW[cE];
-            if (hasN) { zn[q] = z[c - nph]; wn[q] = sys.wN[c]; }
-            if (hasS) { zs[q] = z[c + nph]; wsv[q] = sys.wN[c + nph]; }
-            if (it > 0) {
-                pc[q] = pold[c]; pw[q] = pold[cW]; pe[q] = pold[cE];
-                if (hasN) pn[q] = pold[c - nph];
-                if (hasS) ps[q] = pold[c + nph];
-            }
-        }
-    }
-    const Ti bt = (Ti)beta;
     double acc[1] = {0.0};
+    if (act) {
+        const Ti bt = (Ti)beta;
+        Ti zc[C], zN[C], zS[C], zW, zE;
+        ld_stencil<Ti, C>(sys.z, row, i0, m, nph, hasN, hasS, zc, zW, zE, zN, zS);
+        if (it > 0) {
+            Ti pc[C], pN[C], pS[C], pW, pE;
+            ld_stencil<Ti, C>(pold, row, i0, m, nph, hasN, hasS, pc, pW, pE, pN, pS);
 #pragma unroll
-    for (int q = 0; q < CPT; ++q) {
-        const int i = threadIdx.x + q * blockDim.x;
-        if (i < nph) {
-            const Ti p0 = zc[q] + bt * pc[q];
-            const Ti ap = dg[q] * p0 - ww[q] * (zw[q] + bt * pw[q]) - we[q] * (ze[q] + bt * pe[q]) -
-                          wn[q] * (zn[q] + bt * pn[q]) - wsv[q] * (zs[q] + bt * ps[q]);
-            pnew[row + i] = p0;
-            sys.Ap[row + i] = ap;
-            acc[0] += (double)p0 * (double)ap;
+            for (int q = 0; q < C; ++q) {
+                zc[q] += bt * pc[q];
+                zN[q] += bt * pN[q];
+                zS[q] += bt * pS[q];
+            }
+            zW += bt * pW;
+            zE += bt * pE;
         }
+        RowOp<Ti, C> op;
+        op.load(sys, row, i0, m, nph, hasN, hasS);
+        Ti ap[C];
+        op.apply(zc, zW, zE, zN, zS, ap);
+        st_row<C>(pnew + row + i0, zc);
+        st_row<C>(sys.Ap + row + i0, ap);
+#pragma unroll
+        for (int q = 0; q < C; ++q) acc[0] += (double)zc[q] * (double)ap[q];
     }
     block_sum<1>(acc, red);
     if (threadIdx.x == 0) partA[B.pr0 + j] = acc[0];
 }
 
 // -------------------------------------------- inner PCG: update + M^-1 ----
-template <typename Ti, int CPT>
+template <typename Ti, int C>
 __global__ void k_cgstep(const int2* __restrict__ rows, const BubbleDev* __restrict__ bub,
                          SolveCtl* __restrict__ ctl, SysPtrs<Ti> sys, const Ti* __restrict__ p,
                          int it, const double* __restrict__ partA, double* __restrict__ partB,
@@ -511,68 +673,50 @@ __global__ void k_cgstep(const int2* __restrict__ rows, const BubbleDev* __restr
     __shared__ float redf[16];
     const int2 rw = rows[blockIdx.x];
     if (rw.x < 0) return;
-    SolveCtl* C = ctl + rw.x;
-    if (C->outer_done || C->inner_done) return;
+    SolveCtl* Cb = ctl + rw.x;
+    if (Cb->outer_done || Cb->inner_done) return;
     const BubbleDev B = bub[rw.x];
     const int j = rw.y, nph = B.nph, m = B.mask, nth = B.nth;
+    const bool act = (int)threadIdx.x < nph / C;
+    const int i0 = C * threadIdx.x;
     const int steps = sys.rsteps[B.pr0 + j];
-    Ti* sa = reinterpret_cast<Ti*>(lds_raw);
-    Ti* sc = sa + nph;
-    Ti* sd = sc + nph;
     const size_t row = (size_t)B.off + (size_t)j * nph;
-    Ti pv[CPT], apv[CPT], ev[CPT], rv[CPT], ww[CPT], we[CPT], dg[CPT];
-#pragma unroll
-    for (int q = 0; q < CPT; ++q) {
-        const int i = threadIdx.x + q * blockDim.x;
-        pv[q] = apv[q] = ev[q] = rv[q] = ww[q] = we[q] = Ti(0);
-        dg[q] = Ti(1);
-        if (i < nph) {
-            const size_t c = row + i;
-            pv[q] = p[c];
-            apv[q] = sys.Ap[c];
-            if (it > 0) ev[q] = sys.e[c];
-            rv[q] = sys.r[c];
-            ww[q] = sys.wW[c];
-            we[q] = sys.wW[row + ((i + 1) & m)];
-            dg[q] = sys.diag[c];
-        }
-    }
     double sm[1];
     sum_partials<1>(partA, stride, B.pr0, nth, sm, red);
-    const Ti al = (Ti)(C->rz[it & 1] / sm[0]);
+    const Ti al = (Ti)(Cb->rz[it & 1] / sm[0]);
     double acc[2] = {0.0, 0.0};
+    Ti a[C], cc[C], d[C], rv[C];
 #pragma unroll
-    for (int q = 0; q < CPT; ++q) {
-        const int i = threadIdx.x + q * blockDim.x;
-        if (i < nph) {
-            const size_t c = row + i;
-            sys.e[c] = ev[q] + al * pv[q];
-            const Ti r = rv[q] - al * apv[q];
-            rv[q] = r;
-            sys.r[c] = r;
-            const Ti inv = pre_rcp(dg[q]);
-            sa[i] = -ww[q] * inv;
-            sc[i] = -we[q] * inv;
-            sd[i] = r * inv;
-            acc[1] += (double)r * (double)r;
+    for (int q = 0; q < C; ++q) a[q] = cc[q] = d[q] = rv[q] = Ti(0);
+    if (act) {
+        Ti pv[C], apv[C], ev[C];
+        ld_row<C>(p + row + i0, pv);
+        ld_row<C>(sys.Ap + row + i0, apv);
+        ld_row<C>(sys.r + row + i0, rv);
+        if (it > 0) ld_row<C>(sys.e + row + i0, ev);
+#pragma unroll
+        for (int q = 0; q < C; ++q) {
+            ev[q] = (it > 0 ? ev[q] : Ti(0)) + al * pv[q];
+            rv[q] = rv[q] - al * apv[q];
+            acc[1] += (double)rv[q] * (double)rv[q];
         }
+        st_row<C>(sys.e + row + i0, ev);
+        st_row<C>(sys.r + row + i0, rv);
+        RowOp<Ti, C> op;
+        op.load(sys, row, i0, m, nph, j > 0, j < nth - 1);
+        op.line(rv, a, cc, d);
     }
-    __syncthreads();
-    pcr_periodic<Ti, CPT>(sa, sc, sd, nph, steps, redf);
+    line_solve<Ti, C>(a, cc, d, nph, steps, reinterpret_cast<Ti*>(lds_raw), redf);
+    if (act) {
+        st_row<C>(sys.z + row + i0, d);
 #pragma unroll
-    for (int q = 0; q < CPT; ++q) {
-        const int i = threadIdx.x + q * blockDim.x;
-        if (i < nph) {
-            const Ti z = sd[i];
-            sys.z[row + i] = z;
-            acc[0] += (double)rv[q] * (double)z;
-        }
+        for (int q = 0; q < C; ++q) acc[0] += (double)rv[q] * (double)d[q];
     }
     block_sum<2>(acc, red);
     if (threadIdx.x == 0) {
         partB[B.pr0 + j] = acc[0];
         partB[stride + B.pr0 + j] = acc[1];
-        if (j == 0) C->iters += 1;
+        if (j == 0) Cb->iters += 1;
     }
 }
 
@@ -585,7 +729,7 @@ __global__ void k_cgstep(const int2* __restrict__ rows, const BubbleDev* __restr
 // r_k = r_in - A e_k, z_k = M^-1 r_k, rho_k = 1 / (2 sigma - rho_{k-1}),
 // d_k = rho_k rho_{k-1} d_{k-1} + (2 rho_k / delta) z_k, e_{k+1} = e_k + d_k.
 // Partial |r_k|^2 goes to partB slot 1 (calibration and tracing).
-template <typename Ti, int CPT>
+template <typename Ti, int C>
 __global__ void k_cheb(const int2* __restrict__ rows, const BubbleDev* __restrict__ bub,
                        SolveCtl* __restrict__ ctl, SysPtrs<Ti> sys, const Ti* __restrict__ ek,
                        Ti* __restrict__ enext, Ti* __restrict__ dv, int it, SolveParams sp,
@@ -595,8 +739,8 @@ __global__ void k_cheb(const int2* __restrict__ rows, const BubbleDev* __restric
     __shared__ float redf[16];
     const int2 rw = rows[blockIdx.x];
     if (rw.x < 0) return;
-    SolveCtl* C = ctl + rw.x;
-    if (C->outer_done) return;
+    SolveCtl* Cb = ctl + rw.x;
+    if (Cb->outer_done) return;
     const BubbleDev B = bub[rw.x];
     const int j = rw.y, nph = B.nph, m = B.mask, nth = B.nth;
     const bool leader = (j == 0 && threadIdx.x == 0);
@@ -606,17 +750,17 @@ __global__ void k_cheb(const int2* __restrict__ rows, const BubbleDev* __restric
         sum_partials<2>(partR, stride, B.pr0, nth, sm, red);
         const double res = sp.absolute ? sqrt(sm[0]) : sqrt(sm[0]) / (sm[1] > 0.0 ? sqrt(sm[1]) : 1.0);
         if (res <= sp.tol) {
-            if (leader) { C->outer_done = 1; C->rel_res = res; }
+            if (leader) { Cb->outer_done = 1; Cb->rel_res = res; }
             return;
         }
         mu = max_partials(partR + 4 * stride, B.pr0, nth, red);
         if (leader) {
-            C->mu = mu;
-            C->rel_res = res;
-            C->bb = sm[1];
+            Cb->mu = mu;
+            Cb->rel_res = res;
+            Cb->bb = sm[1];
         }
     } else {
-        mu = C->mu;
+        mu = Cb->mu;
     }
     const double delta = fmax(mu * 1.02 + 1e-3, 1e-4);
     const double sigma = 1.0 / delta;
@@ -627,51 +771,49 @@ __global__ void k_cheb(const int2* __restrict__ rows, const BubbleDev* __restric
     }
     const Ti cd = (Ti)(rho * rho_prev), cz = (Ti)(2.0 * rho / delta);
     const int steps = sys.rsteps[B.pr0 + j];
-    Ti* sa = reinterpret_cast<Ti*>(lds_raw);
-    Ti* sc = sa + nph;
-    Ti* sd = sc + nph;
     const size_t row = (size_t)B.off + (size_t)j * nph;
     const bool hasN = j > 0, hasS = j < nth - 1;
+    const bool act = (int)threadIdx.x < nph / C;
+    const int i0 = C * threadIdx.x;
     const Ti* __restrict__ e = it == 1 ? sys.z : ek;
     const Ti* __restrict__ dprev = it == 1 ? sys.z : dv;
-    Ti ev[CPT], dvv[CPT];
+    Ti ev[C], dvv[C], a[C], cc[C], d[C];
+#pragma unroll
+    for (int q = 0; q < C; ++q) ev[q] = dvv[q] = a[q] = cc[q] = d[q] = Ti(0);
     double acc = 0.0;
+    if (act) {
+        Ti eN[C], eS[C], eW, eE;
+        ld_stencil<Ti, C>(e, row, i0, m, nph, hasN, hasS, ev, eW, eE, eN, eS);
+        ld_row<C>(dprev + row + i0, dvv);
+        Ti rin[C];
+        ld_row<C>(sys.r + row + i0, rin);
+        RowOp<Ti, C> op;
+        op.load(sys, row, i0, m, nph, hasN, hasS);
+        Ti ax[C], r[C];
+        op.apply(ev, eW, eE, eN, eS, ax);
 #pragma unroll
-    for (int q = 0; q < CPT; ++q) {
-        const int i = threadIdx.x + q * blockDim.x;
-        ev[q] = dvv[q] = Ti(0);
-        if (i < nph) {
-            const size_t c = row + i, cW = row + ((i - 1) & m), cE = row + ((i + 1) & m);
-            const Ti ec = e[c];
-            Ti ax = sys.diag[c] * ec - sys.wW[c] * e[cW] - sys.wW[cE] * e[cE];
-            if (hasN) ax -= sys.wN[c] * e[c - nph];
-            if (hasS) ax -= sys.wN[c + nph] * e[c + nph];
-            const Ti r = sys.r[c] - ax;
-            ev[q] = ec;
-            dvv[q] = dprev[c];
-            const Ti inv = pre_rcp(sys.diag[c]);
-            sa[i] = -sys.wW[c] * inv;
-            sc[i] = -sys.wW[cE] * inv;
-            sd[i] = r * inv;
-            acc += (double)r * (double)r;
+        for (int q = 0; q < C; ++q) {
+            r[q] = rin[q] - ax[q];
+            acc += (double)r[q] * (double)r[q];
         }
+        op.line(r, a, cc, d);
     }
-    __syncthreads();
-    pcr_periodic<Ti, CPT>(sa, sc, sd, nph, steps, redf);
+    line_solve<Ti, C>(a, cc, d, nph, steps, reinterpret_cast<Ti*>(lds_raw), redf);
+    if (act) {
+        Ti dn[C], en[C];
 #pragma unroll
-    for (int q = 0; q < CPT; ++q) {
-        const int i = threadIdx.x + q * blockDim.x;
-        if (i < nph) {
-            const Ti dn = cd * dvv[q] + cz * sd[i];
-            dv[row + i] = dn;
-            enext[row + i] = ev[q] + dn;
+        for (int q = 0; q < C; ++q) {
+            dn[q] = cd * dvv[q] + cz * d[q];
+            en[q] = ev[q] + dn[q];
         }
+        st_row<C>(dv + row + i0, dn);
+        st_row<C>(enext + row + i0, en);
     }
     double a1[1] = {acc};
     block_sum<1>(a1, red);
     if (threadIdx.x == 0) {
         partB[stride + B.pr0 + j] = a1[0];
-        if (j == 0) C->iters += 1;
+        if (j == 0) Cb->iters += 1;
     }
 }
 
@@ -829,100 +971,66 @@ void launch_advect(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, Bub
                        out);
 }
 
-template <typename T, typename Ti, int CPT>
-static void assemble_cpt(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub,
-                         const BubbleTime* bt, const Tables& tab, StatePtrs<T> st, AirPtrs<T> air,
-                         SysPtrs<Ti> sys, double gmax_state, SolveCtl* ctl, double* part,
-                         long stride) {
-    hipLaunchKernelGGL((k_assemble<T, Ti, CPT>), dim3(g.nrows), dim3(g.row_threads), g.lds_bytes,
-                       s, g.rows, bub, bt, tab, st, air, sys, (T)gmax_state, ctl, part, stride);
-}
+#define SF_CASES(CALL)                                 \
+    switch (g.cpt) {                                   \
+        case 2: CALL(2); break;                        \
+        case 4: CALL(4); break;                        \
+        default: CALL(8); break;                       \
+    }
+
 template <typename T, typename Ti>
 void launch_assemble(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub,
                      const BubbleTime* bt, const Tables& tab, StatePtrs<T> st, AirPtrs<T> air,
                      SysPtrs<Ti> sys, double gmax_state, SolveCtl* ctl, double* part, long stride) {
-    switch (g.cpt) {
-        case 1: assemble_cpt<T, Ti, 1>(s, g, bub, bt, tab, st, air, sys, gmax_state, ctl, part, stride); break;
-        case 2: assemble_cpt<T, Ti, 2>(s, g, bub, bt, tab, st, air, sys, gmax_state, ctl, part, stride); break;
-        case 4: assemble_cpt<T, Ti, 4>(s, g, bub, bt, tab, st, air, sys, gmax_state, ctl, part, stride); break;
-        default: assemble_cpt<T, Ti, 8>(s, g, bub, bt, tab, st, air, sys, gmax_state, ctl, part, stride); break;
-    }
+#define SF_CALL(C)                                                                                \
+    hipLaunchKernelGGL((k_assemble<T, Ti, C>), dim3(g.nrows), dim3(g.row_threads), g.lds_bytes, s, \
+                       g.rows, bub, bt, tab, st, air, sys, (T)gmax_state, ctl, part, stride)
+    SF_CASES(SF_CALL)
+#undef SF_CALL
 }
 
-template <typename Ti, int CPT>
-static void resid_cpt(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, SolveCtl* ctl,
-                      SysPtrs<Ti> sys, const double* xo, double* xn, int k, double* part,
-                      long stride) {
-    hipLaunchKernelGGL((k_resid<Ti, CPT>), dim3(g.nrows), dim3(g.row_threads), g.lds_bytes, s,
-                       g.rows, bub, ctl, sys, xo, xn, k, part, stride);
-}
-template <typename Ti>
-void launch_resid(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, SolveCtl* ctl,
-                  SysPtrs<Ti> sys, const double* xo, double* xn, int k, double* part,
-                  long stride) {
-    switch (g.cpt) {
-        case 1: resid_cpt<Ti, 1>(s, g, bub, ctl, sys, xo, xn, k, part, stride); break;
-        case 2: resid_cpt<Ti, 2>(s, g, bub, ctl, sys, xo, xn, k, part, stride); break;
-        case 4: resid_cpt<Ti, 4>(s, g, bub, ctl, sys, xo, xn, k, part, stride); break;
-        default: resid_cpt<Ti, 8>(s, g, bub, ctl, sys, xo, xn, k, part, stride); break;
-    }
+template <typename T, typename Ti>
+void launch_resid(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, const Tables& tab,
+                  StatePtrs<T> st, SolveCtl* ctl, SysPtrs<Ti> sys, const double* xo, double* xn,
+                  int k, double* part, long stride) {
+#define SF_CALL(C)                                                                              \
+    hipLaunchKernelGGL((k_resid<T, Ti, C>), dim3(g.nrows), dim3(g.row_threads), g.lds_bytes, s, \
+                       g.rows, bub, tab, st, ctl, sys, xo, xn, k, part, stride)
+    SF_CASES(SF_CALL)
+#undef SF_CALL
 }
 
-template <typename Ti, int CPT>
-static void spmv_cpt(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, SolveCtl* ctl,
-                     SysPtrs<Ti> sys, const Ti* pold, Ti* pnew, int it, const SolveParams& sp,
-                     const double* partR, const double* partB, double* partA, long stride) {
-    hipLaunchKernelGGL((k_spmv<Ti, CPT>), dim3(g.nrows), dim3(g.row_threads), 0, s, g.rows, bub,
-                       ctl, sys, pold, pnew, it, sp, partR, partB, partA, stride);
-}
 template <typename Ti>
 void launch_spmv(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, SolveCtl* ctl,
                  SysPtrs<Ti> sys, const Ti* pold, Ti* pnew, int it, const SolveParams& sp,
                  const double* partR, const double* partB, double* partA, long stride) {
-    switch (g.cpt) {
-        case 1: spmv_cpt<Ti, 1>(s, g, bub, ctl, sys, pold, pnew, it, sp, partR, partB, partA, stride); break;
-        case 2: spmv_cpt<Ti, 2>(s, g, bub, ctl, sys, pold, pnew, it, sp, partR, partB, partA, stride); break;
-        case 4: spmv_cpt<Ti, 4>(s, g, bub, ctl, sys, pold, pnew, it, sp, partR, partB, partA, stride); break;
-        default: spmv_cpt<Ti, 8>(s, g, bub, ctl, sys, pold, pnew, it, sp, partR, partB, partA, stride); break;
-    }
+#define SF_CALL(C)                                                                        \
+    hipLaunchKernelGGL((k_spmv<Ti, C>), dim3(g.nrows), dim3(g.row_threads), 0, s, g.rows, \
+                       bub, ctl, sys, pold, pnew, it, sp, partR, partB, partA, stride)
+    SF_CASES(SF_CALL)
+#undef SF_CALL
 }
 
-template <typename Ti, int CPT>
-static void cgstep_cpt(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, SolveCtl* ctl,
-                       SysPtrs<Ti> sys, const Ti* p, int it, const double* partA, double* partB,
-                       long stride) {
-    hipLaunchKernelGGL((k_cgstep<Ti, CPT>), dim3(g.nrows), dim3(g.row_threads), g.lds_bytes, s,
-                       g.rows, bub, ctl, sys, p, it, partA, partB, stride);
-}
 template <typename Ti>
 void launch_cgstep(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, SolveCtl* ctl,
                    SysPtrs<Ti> sys, const Ti* p, int it, const double* partA, double* partB,
                    long stride) {
-    switch (g.cpt) {
-        case 1: cgstep_cpt<Ti, 1>(s, g, bub, ctl, sys, p, it, partA, partB, stride); break;
-        case 2: cgstep_cpt<Ti, 2>(s, g, bub, ctl, sys, p, it, partA, partB, stride); break;
-        case 4: cgstep_cpt<Ti, 4>(s, g, bub, ctl, sys, p, it, partA, partB, stride); break;
-        default: cgstep_cpt<Ti, 8>(s, g, bub, ctl, sys, p, it, partA, partB, stride); break;
-    }
+#define SF_CALL(C)                                                                               \
+    hipLaunchKernelGGL((k_cgstep<Ti, C>), dim3(g.nrows), dim3(g.row_threads), g.lds_bytes, s,   \
+                       g.rows, bub, ctl, sys, p, it, partA, partB, stride)
+    SF_CASES(SF_CALL)
+#undef SF_CALL
 }
 
-template <typename Ti, int CPT>
-static void cheb_cpt(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, SolveCtl* ctl,
-                     SysPtrs<Ti> sys, const Ti* ek, Ti* enext, Ti* dv, int it, const SolveParams& sp,
-                     const double* partR, double* partB, long stride) {
-    hipLaunchKernelGGL((k_cheb<Ti, CPT>), dim3(g.nrows), dim3(g.row_threads), g.lds_bytes, s,
-                       g.rows, bub, ctl, sys, ek, enext, dv, it, sp, partR, partB, stride);
-}
 template <typename Ti>
 void launch_cheb(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, SolveCtl* ctl,
                  SysPtrs<Ti> sys, const Ti* ek, Ti* enext, Ti* dv, int it, const SolveParams& sp,
                  const double* partR, double* partB, long stride) {
-    switch (g.cpt) {
-        case 1: cheb_cpt<Ti, 1>(s, g, bub, ctl, sys, ek, enext, dv, it, sp, partR, partB, stride); break;
-        case 2: cheb_cpt<Ti, 2>(s, g, bub, ctl, sys, ek, enext, dv, it, sp, partR, partB, stride); break;
-        case 4: cheb_cpt<Ti, 4>(s, g, bub, ctl, sys, ek, enext, dv, it, sp, partR, partB, stride); break;
-        default: cheb_cpt<Ti, 8>(s, g, bub, ctl, sys, ek, enext, dv, it, sp, partR, partB, stride); break;
-    }
+#define SF_CALL(C)                                                                             \
+    hipLaunchKernelGGL((k_cheb<Ti, C>), dim3(g.nrows), dim3(g.row_threads), g.lds_bytes, s,   \
+                       g.rows, bub, ctl, sys, ek, enext, dv, it, sp, partR, partB, stride)
+    SF_CASES(SF_CALL)
+#undef SF_CALL
 }
 
 void launch_finalize(hipStream_t s, int nb, const BubbleDev* bub, SolveCtl* ctl, SolveHist* hist,
@@ -955,13 +1063,14 @@ template void launch_advect<double>(hipStream_t, const LaunchGeom&, const Bubble
     template void launch_assemble<T, Ti>(hipStream_t, const LaunchGeom&, const BubbleDev*,        \
                                          const BubbleTime*, const Tables&, StatePtrs<T>,          \
                                          AirPtrs<T>, SysPtrs<Ti>, double, SolveCtl*, double*,     \
-                                         long);
+                                         long);                                                   \
+    template void launch_resid<T, Ti>(hipStream_t, const LaunchGeom&, const BubbleDev*,           \
+                                      const Tables&, StatePtrs<T>, SolveCtl*, SysPtrs<Ti>,        \
+                                      const double*, double*, int, double*, long);
 SF_INST_SYS(float, float)
 SF_INST_SYS(float, double)
 SF_INST_SYS(double, double)
 #define SF_INST_TI(Ti)                                                                            \
-    template void launch_resid<Ti>(hipStream_t, const LaunchGeom&, const BubbleDev*, SolveCtl*,   \
-                                   SysPtrs<Ti>, const double*, double*, int, double*, long);      \
     template void launch_spmv<Ti>(hipStream_t, const LaunchGeom&, const BubbleDev*, SolveCtl*,    \
                                   SysPtrs<Ti>, const Ti*, Ti*, int, const SolveParams&,           \
                                   const double*, const double*, double*, long);                   \

@@ -23,12 +23,9 @@ struct AirPtrs {
 
 template <typename Ti>
 struct SysPtrs {
-    double* wN64;    // north-face weight of each cell (row 0: 0)
-    double* wW64;    // west-face weight of each cell
-    double* diag64;  // diagonal
     double* b64;     // right-hand side
     double* x64;     // initial guess written by the assembly
-    Ti* wN;          // inner-precision copies of the operator
+    Ti* wN;          // north-face weight of each cell (row 0: 0)
     Ti* wW;
     Ti* diag;
     Ti* e;           // inner correction
@@ -52,8 +49,8 @@ struct LaunchGeom {
     const int2* rows;    // row blocks: (bubble, row)
     int nrows;
     int row_threads;     // threads of a row block
-    int cpt;             // cells per thread bound of the line solve
-    size_t lds_bytes;    // dynamic LDS of a row block
+    int cpt;             // consecutive cells per thread of a row block (2, 4 or 8)
+    size_t lds_bytes;    // dynamic LDS of a row block (the reduced line system)
 };
 
 template <typename T>
@@ -63,9 +60,10 @@ template <typename T, typename Ti>
 void launch_assemble(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub,
                      const BubbleTime* bt, const Tables& tab, StatePtrs<T> st, AirPtrs<T> air,
                      SysPtrs<Ti> sys, double gmax_state, SolveCtl* ctl, double* part, long stride);
-template <typename Ti>
-void launch_resid(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, SolveCtl* ctl,
-                  SysPtrs<Ti> sys, const double* xo, double* xn, int k, double* part, long stride);
+template <typename T, typename Ti>
+void launch_resid(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, const Tables& tab,
+                  StatePtrs<T> st, SolveCtl* ctl, SysPtrs<Ti> sys, const double* xo, double* xn,
+                  int k, double* part, long stride);
 template <typename Ti>
 void launch_spmv(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, SolveCtl* ctl,
                  SysPtrs<Ti> sys, const Ti* pold, Ti* pnew, int it, const SolveParams& sp,
