@@ -146,121 +146,6 @@ __device__ __forceinline__ bool is_broken(double gamma, double eta, double gmax)
 // rhs gamma_max; its coupling to unbroken neighbours is moved to their
 // right-hand sides (x_broken = gamma_max is known), which leaves the
 // solution unchanged and keeps the operator symmetric.
-struct CellSys {
-    double wN, wS, wW, wE;   // effective face weights (0 across broken cells)
-    double diag, rhs;
-    double x0, x0N, x0S, x0W, x0E;  // initial guess Gamma* at the cell and neighbours
-    bool brk;
-};
-
-// RHS = false: the operator row only (the FP64 residual recomputes it from
-// eta and Gamma* instead of storing it).
-template <typename T, bool RHS>
-__device__ __forceinline__ CellSys assemble_cell(const BubbleDev& B, double t, const Tables& tab,
-                                                 const StatePtrs<T>& s, const AirPtrs<T>& air,
-                                                 int j, int i) {
-    const int nph = B.nph, m = B.mask, nth = B.nth;
-    const size_t o = (size_t)B.off;
-    const size_t c = o + (size_t)j * nph + i;
-    const int iE = (i + 1) & m, iW = (i - 1) & m;
-    const double gmax = B.gamma_max;
-    const T* eta = s.eta;
-    const T* gam = s.gam;
-    const double e = (double)eta[c];
-    const double gamma = (double)gam[c];
-    CellSys r;
-    r.brk = is_broken(gamma, e, gmax);
-
-    const size_t cW = o + (size_t)j * nph + iW, cE = o + (size_t)j * nph + iE;
-    const size_t cN = j > 0 ? c - nph : o + (size_t)((i + nth) & m);
-    const size_t cS = j < nth - 1 ? c + nph : o + (size_t)j * nph + ((i + nth) & m);
-    const double etaW = (double)eta[cW], etaE = (double)eta[cE];
-    const double etaN = (double)eta[cN], etaS = (double)eta[cS];
-    const double gW = (double)gam[cW], gE = (double)gam[cE];
-    const double gN = (double)gam[cN], gS = (double)gam[cS];
-    const bool bW = is_broken(gW, etaW, gmax);
-    const bool bE = is_broken(gE, etaE, gmax);
-    const bool bN = j > 0 && is_broken(gN, etaN, gmax);
-    const bool bS = j < nth - 1 && is_broken(gS, etaS, gmax);
-
-    const double dt = B.dt, invDt = 1.0 / dt, Cr = B.Cr;
-    const double CrDt = Cr * dt, MDt = B.M * dt, s2 = B.invh * B.invh;
-    const double sinN = tab.sinFace[B.rt0 + j], sinS = tab.sinFace[B.rt0 + j + 1];
-    const double sinTheta = tab.sinTc[B.rt0 + j], csc = tab.cscTc[B.rt0 + j];
-    const double etaUp = (etaN + e) / 2.0, etaLeft = (etaW + e) / 2.0;
-    const double etaRight = (etaE + e) / 2.0, etaDown = (etaS + e) / 2.0;
-    const double wN = s2 * sinN * MDt / (etaUp + CrDt);
-    const double wW = s2 * csc * MDt / (etaLeft + CrDt);
-    const double wE = s2 * csc * MDt / (etaRight + CrDt);
-    const double wS = s2 * sinS * MDt / (etaDown + CrDt);
-
-    r.rhs = 0.0;
-    if (r.brk) {
-        r.diag = 1.0;
-        r.rhs = gmax;
-    } else {
-        r.diag = sinTheta * invDt / gamma - (((-wN) + (-wW)) + (-wE) + (-wS));
-        if (RHS) {
-            double uW = (double)s.uph[c], uE = (double)s.uph[cE];
-            double vN = j > 0 ? (double)s.uth[c - nph] : 0.0;
-            double vS = j < nth - 1 ? (double)s.uth[c] : 0.0;
-            if (__builtin_isnan(vN)) vN = 0.0;
-            if (__builtin_isnan(vS)) vS = 0.0;
-            if (__builtin_isnan(uW)) uW = 0.0;
-            if (__builtin_isnan(uE)) uE = 0.0;
-            double div = B.invh * (uE * etaRight / (etaRight + Cr * dt) -
-                                   uW * etaLeft / (etaLeft + Cr * dt) +
-                                   (vS * sinS * etaDown / (etaDown + Cr * dt) -
-                                    vN * sinN * etaUp / (etaUp + Cr * dt)));
-            if (B.airflow) {
-                const double kt = 3.0 - fabs(t - 3.0);
-                const double bl = t / 12.0;
-                const size_t row = o + (size_t)j * nph;
-                const int opp = (nph - i) & m, oppE = (nph - iE) & m;
-                const double uAirW = 0.5 * (air_at(air.ua0, air.au, row + i, kt) * bl +
-                                            air_at(air.ua0, air.au, row + opp, kt) * (1 - bl));
-                const double uAirE = 0.5 * (air_at(air.ua0, air.au, row + iE, kt) * bl +
-                                            air_at(air.ua0, air.au, row + oppE, kt) * (1 - bl));
-                double vAirN = 0.0, vAirS = 0.0;
-                if (j > 0) {
-                    const size_t rn = row - nph;
-                    vAirN = 0.5 * (air_at(air.va0, air.av, rn + i, kt) * bl -
-                                   air_at(air.va0, air.av, rn + opp, kt) * (1 - bl));
-                }
-                if (j < nth - 1) {
-                    vAirS = 0.5 * (air_at(air.va0, air.av, row + i, kt) * bl -
-                                   air_at(air.va0, air.av, row + opp, kt) * (1 - bl));
-                }
-                div += B.invh * (uAirE / (etaRight / Cr * invDt + 1.0) -
-                                 uAirW / (etaLeft / Cr * invDt + 1.0) +
-                                 vAirS * sinS / (etaDown / Cr * invDt + 1.0) -
-                                 vAirN * sinN / (etaUp / Cr * invDt + 1.0));
-            }
-            if (B.gravity) {
-                div += B.gs * B.invh *
-                       (sinS * sinS * etaDown / (invDt * etaDown + Cr) -
-                        sinN * sinN * etaUp / (invDt * etaUp + Cr));
-            }
-            r.rhs = sinTheta * invDt - div;
-            if (bN) r.rhs += wN * gmax;
-            if (bS) r.rhs += wS * gmax;
-            if (bW) r.rhs += wW * gmax;
-            if (bE) r.rhs += wE * gmax;
-        }
-    }
-    r.wN = (r.brk || bN) ? 0.0 : wN;
-    r.wS = (r.brk || bS) ? 0.0 : wS;
-    r.wW = (r.brk || bW) ? 0.0 : wW;
-    r.wE = (r.brk || bE) ? 0.0 : wE;
-    // x0 = Gamma* after the broken-cell override (bubble.cu:1325, 1480-1486)
-    r.x0 = r.brk ? gmax : gamma;
-    r.x0N = bN ? gmax : gN;
-    r.x0S = bS ? gmax : gS;
-    r.x0W = bW ? gmax : gW;
-    r.x0E = bE ? gmax : gE;
-    return r;
-}
-
 // ------------------------------------------------- blocked row layout ----
 // A row block gives thread t the C consecutive cells [C t, C t + C) of its
 // theta row: row loads and stores are one contiguous vector per thread, and
@@ -276,6 +161,159 @@ __device__ __forceinline__ void st_row(X* __restrict__ p, const X (&v)[C]) {
     X* q = (X*)__builtin_assume_aligned(p, sizeof(X) * C);
 #pragma unroll
     for (int k = 0; k < C; ++k) q[k] = v[k];
+}
+
+// Operator (and optionally right-hand side) of the C consecutive cells
+// [i0, i0 + C) of row j, computed from row-vector loads. Every phi face is
+// evaluated once and shared by its two cells; each expression is the one of
+// concentrationLinearSystemKernel (bubble.cu:1330-1472), so a face weight
+// has the same value whichever cell computes it.
+template <int C>
+struct RowSys {
+    double wN[C], wS[C], wF[C + 1];   // effective weights; wF[k] is the face west of cell k
+    double diag[C], rhs[C];
+    double x0[C], x0N[C], x0S[C], x0W, x0E;
+    bool brk[C];
+};
+
+template <typename T, int C, bool RHS>
+__device__ __forceinline__ void row_system(const BubbleDev& B, double t, const Tables& tab,
+                                           const StatePtrs<T>& s, const AirPtrs<T>& air, int j,
+                                           int i0, RowSys<C>& q) {
+    const int nph = B.nph, m = B.mask, nth = B.nth;
+    const size_t row = (size_t)B.off + (size_t)j * nph;
+    const double gmax = B.gamma_max;
+    const bool hasN = j > 0, hasS = j < nth - 1;
+    // neighbours across the poles are the phi + pi cells of the same row
+    // (bubble.cu:1384-1401); their face weights vanish with sin(theta_face)
+    const size_t rowN = hasN ? row - nph : row, rowS = hasS ? row + nph : row;
+    const int iNS = hasN ? i0 : (i0 + nth) & m;
+    const int iSS = hasS ? i0 : (i0 + nth) & m;
+    T ev[C], eNv[C], eSv[C], gv[C], gNv[C], gSv[C];
+    ld_row<C>(s.eta + row + i0, ev);
+    ld_row<C>(s.gam + row + i0, gv);
+    ld_row<C>(s.eta + rowN + (hasN ? i0 : iNS), eNv);
+    ld_row<C>(s.gam + rowN + (hasN ? i0 : iNS), gNv);
+    ld_row<C>(s.eta + rowS + (hasS ? i0 : iSS), eSv);
+    ld_row<C>(s.gam + rowS + (hasS ? i0 : iSS), gSv);
+    const double eW = (double)s.eta[row + ((i0 - 1) & m)], gW = (double)s.gam[row + ((i0 - 1) & m)];
+    const double eE = (double)s.eta[row + ((i0 + C) & m)], gE = (double)s.gam[row + ((i0 + C) & m)];
+
+    const double dt = B.dt, invDt = 1.0 / dt, Cr = B.Cr;
+    const double CrDt = Cr * dt, MDt = B.M * dt, s2 = B.invh * B.invh;
+    const double sinN = tab.sinFace[B.rt0 + j], sinS = tab.sinFace[B.rt0 + j + 1];
+    const double sinTheta = tab.sinTc[B.rt0 + j], csc = tab.cscTc[B.rt0 + j];
+
+    bool bW = is_broken(gW, eW, gmax), bE = is_broken(gE, eE, gmax);
+    bool bN[C], bS[C];
+    double e[C], g[C];
+#pragma unroll
+    for (int k = 0; k < C; ++k) {
+        e[k] = (double)ev[k];
+        g[k] = (double)gv[k];
+        q.brk[k] = is_broken(g[k], e[k], gmax);
+        bN[k] = hasN && is_broken((double)gNv[k], (double)eNv[k], gmax);
+        bS[k] = hasS && is_broken((double)gSv[k], (double)eSv[k], gmax);
+    }
+    // phi faces: face k lies between cells k-1 and k
+    double wFr[C + 1], etaF[C + 1];
+#pragma unroll
+    for (int k = 0; k <= C; ++k) {
+        const double l = k == 0 ? eW : e[k - 1];
+        const double r = k == C ? eE : e[k];
+        etaF[k] = (l + r) / 2.0;
+        wFr[k] = s2 * csc * MDt / (etaF[k] + CrDt);
+        const bool bl = k == 0 ? bW : q.brk[k - 1];
+        const bool br = k == C ? bE : q.brk[k];
+        q.wF[k] = (bl || br) ? 0.0 : wFr[k];
+    }
+    double uv[C + 1], vNv[C], vSv[C];
+    if (RHS) {
+        T u[C], vn[C], vs[C];
+        ld_row<C>(s.uph + row + i0, u);
+#pragma unroll
+        for (int k = 0; k < C; ++k) { uv[k] = (double)u[k]; vNv[k] = vSv[k] = 0.0; }
+        uv[C] = (double)s.uph[row + ((i0 + C) & m)];
+        if (hasN) {
+            ld_row<C>(s.uth + row - nph + i0, vn);
+#pragma unroll
+            for (int k = 0; k < C; ++k) vNv[k] = (double)vn[k];
+        }
+        if (hasS) {
+            ld_row<C>(s.uth + row + i0, vs);
+#pragma unroll
+            for (int k = 0; k < C; ++k) vSv[k] = (double)vs[k];
+        }
+    }
+    const double kt = 3.0 - fabs(t - 3.0);
+    const double bl = t / 12.0;
+#pragma unroll
+    for (int k = 0; k < C; ++k) {
+        const double eN = (double)eNv[k], eS = (double)eSv[k];
+        const double etaUp = (eN + e[k]) / 2.0, etaDown = (eS + e[k]) / 2.0;
+        const double etaLeft = etaF[k], etaRight = etaF[k + 1];
+        const double wN = s2 * sinN * MDt / (etaUp + CrDt);
+        const double wS = s2 * sinS * MDt / (etaDown + CrDt);
+        const double wW = wFr[k], wE = wFr[k + 1];
+        const bool brk = q.brk[k];
+        q.wN[k] = (brk || bN[k]) ? 0.0 : wN;
+        q.wS[k] = (brk || bS[k]) ? 0.0 : wS;
+        q.x0[k] = brk ? gmax : g[k];
+        q.x0N[k] = bN[k] ? gmax : (double)gNv[k];
+        q.x0S[k] = bS[k] ? gmax : (double)gSv[k];
+        q.rhs[k] = 0.0;
+        if (brk) {
+            q.diag[k] = 1.0;
+            q.rhs[k] = gmax;
+            continue;
+        }
+        q.diag[k] = sinTheta * invDt / g[k] - (((-wN) + (-wW)) + (-wE) + (-wS));
+        if (!RHS) continue;
+        const int i = i0 + k, iE = (i + 1) & m;
+        double uW = uv[k], uE = uv[k + 1], vN = vNv[k], vS = vSv[k];
+        if (__builtin_isnan(vN)) vN = 0.0;
+        if (__builtin_isnan(vS)) vS = 0.0;
+        if (__builtin_isnan(uW)) uW = 0.0;
+        if (__builtin_isnan(uE)) uE = 0.0;
+        double div = B.invh * (uE * etaRight / (etaRight + Cr * dt) -
+                               uW * etaLeft / (etaLeft + Cr * dt) +
+                               (vS * sinS * etaDown / (etaDown + Cr * dt) -
+                                vN * sinN * etaUp / (etaUp + Cr * dt)));
+        if (B.airflow) {
+            const int opp = (nph - i) & m, oppE = (nph - iE) & m;
+            const double uAirW = 0.5 * (air_at(air.ua0, air.au, row + i, kt) * bl +
+                                        air_at(air.ua0, air.au, row + opp, kt) * (1 - bl));
+            const double uAirE = 0.5 * (air_at(air.ua0, air.au, row + iE, kt) * bl +
+                                        air_at(air.ua0, air.au, row + oppE, kt) * (1 - bl));
+            double vAirN = 0.0, vAirS = 0.0;
+            if (hasN) {
+                const size_t rn = row - nph;
+                vAirN = 0.5 * (air_at(air.va0, air.av, rn + i, kt) * bl -
+                               air_at(air.va0, air.av, rn + opp, kt) * (1 - bl));
+            }
+            if (hasS) {
+                vAirS = 0.5 * (air_at(air.va0, air.av, row + i, kt) * bl -
+                               air_at(air.va0, air.av, row + opp, kt) * (1 - bl));
+            }
+            div += B.invh * (uAirE / (etaRight / Cr * invDt + 1.0) -
+                             uAirW / (etaLeft / Cr * invDt + 1.0) +
+                             vAirS * sinS / (etaDown / Cr * invDt + 1.0) -
+                             vAirN * sinN / (etaUp / Cr * invDt + 1.0));
+        }
+        if (B.gravity) {
+            div += B.gs * B.invh *
+                   (sinS * sinS * etaDown / (invDt * etaDown + Cr) -
+                    sinN * sinN * etaUp / (invDt * etaUp + Cr));
+        }
+        double rr = sinTheta * invDt - div;
+        if (bN[k]) rr += wN * gmax;
+        if (bS[k]) rr += wS * gmax;
+        if (k == 0 ? bW : q.brk[k - 1]) rr += wW * gmax;
+        if (k == C - 1 ? bE : q.brk[k + 1]) rr += wE * gmax;
+        q.rhs[k] = rr;
+    }
+    q.x0W = bW ? gmax : gW;
+    q.x0E = bE ? gmax : gE;
 }
 
 // Periodic phi-line solve of one row, normalised to unit diagonal
@@ -393,34 +431,37 @@ __global__ void k_assemble(const int2* __restrict__ rows, const BubbleDev* __res
 #pragma unroll
     for (int k = 0; k < C; ++k) a[k] = cc[k] = d[k] = rin[k] = Ti(0);
     if (act) {
-        double bv[C], xv[C];
+        RowSys<C> q;
+        row_system<T, C, true>(B, t, tab, s, air, j, i0, q);
+        double xv[C];
         Ti wn[C], ww[C], dg[C];
 #pragma unroll
         for (int k = 0; k < C; ++k) {
-            const CellSys q = assemble_cell<T, true>(B, t, tab, s, air, j, i0 + k);
-            const double ax = q.diag * q.x0 - q.wN * q.x0N - q.wS * q.x0S - q.wW * q.x0W -
-                              q.wE * q.x0E;
-            const double r0 = q.rhs - ax;
+            const double wW = q.wF[k], wE = q.wF[k + 1];
+            const double xl = k == 0 ? q.x0W : q.x0[k - 1];
+            const double xr = k == C - 1 ? q.x0E : q.x0[k + 1];
+            const double ax = q.diag[k] * q.x0[k] - q.wN[k] * q.x0N[k] - q.wS[k] * q.x0S[k] -
+                              wW * xl - wE * xr;
+            const double r0 = q.rhs[k] - ax;
             // x^T N x <= sum (wN + wS) x_i^2 and x^T M x >= sum (diag - wW - wE)
             // x_i^2 for the theta couplings N and the phi-line blocks M, so
             // the eigenvalues of M^-1 A lie in [1 - mu, 1 + mu] with mu the
             // largest ratio below
-            if (!q.brk) mu = fmax(mu, (q.wN + q.wS) / (q.diag - q.wW - q.wE));
+            if (!q.brk[k]) mu = fmax(mu, (q.wN[k] + q.wS[k]) / (q.diag[k] - wW - wE));
             rin[k] = (Ti)r0;
-            bv[k] = q.rhs;
-            xv[k] = q.x0;
-            wn[k] = (Ti)q.wN;
-            ww[k] = (Ti)q.wW;
-            dg[k] = (Ti)q.diag;
+            xv[k] = q.x0[k];
+            wn[k] = (Ti)q.wN[k];
+            ww[k] = (Ti)wW;
+            dg[k] = (Ti)q.diag[k];
             const Ti inv = pre_rcp(dg[k]);
             a[k] = -ww[k] * inv;
-            cc[k] = -(Ti)q.wE * inv;
+            cc[k] = -(Ti)wE * inv;
             d[k] = rin[k] * inv;
             acc[0] += r0 * r0;
-            acc[1] += q.rhs * q.rhs;
+            acc[1] += q.rhs[k] * q.rhs[k];
             acc[3] += (double)rin[k] * (double)rin[k];
         }
-        st_row<C>(sys.b64 + row + i0, bv);
+        st_row<C>(sys.b64 + row + i0, q.rhs);
         st_row<C>(sys.x64 + row + i0, xv);
         st_row<C>(sys.wN + row + i0, wn);
         st_row<C>(sys.wW + row + i0, ww);
@@ -544,14 +585,15 @@ __global__ void k_resid(const int2* __restrict__ rows, const BubbleDev* __restri
 #pragma unroll
         for (int q = 0; q < C; ++q) xnew[q] = xc[q] + (double)ec[q];
         const double xWn = xW + (double)eW, xEn = xE + (double)eE;
+        RowSys<C> w;
+        row_system<T, C, false>(B, 0.0, tab, s, AirPtrs<T>{}, j, i0, w);
 #pragma unroll
         for (int q = 0; q < C; ++q) {
-            const CellSys w = assemble_cell<T, false>(B, 0.0, tab, s, AirPtrs<T>{}, j, i0 + q);
             const double l = q == 0 ? xWn : xnew[q - 1];
             const double r = q == C - 1 ? xEn : xnew[q + 1];
-            double ax = w.diag * xnew[q] - w.wW * l - w.wE * r;
-            if (hasN) ax -= w.wN * (xN[q] + (double)eN[q]);
-            if (hasS) ax -= w.wS * (xS[q] + (double)eS[q]);
+            double ax = w.diag[q] * xnew[q] - w.wF[q] * l - w.wF[q + 1] * r;
+            if (hasN) ax -= w.wN[q] * (xN[q] + (double)eN[q]);
+            if (hasS) ax -= w.wS[q] * (xS[q] + (double)eS[q]);
             const double res = bv[q] - ax;
             rin[q] = (Ti)res;
             acc[0] += res * res;
