@@ -317,8 +317,20 @@ struct Impl : Base {
             default: throw Status{SF_ERR_INVALID, "field cannot be uploaded"};
         }
         std::vector<T> tmp(N, T(0));
-        for (size_t k = 0; k < n; ++k) tmp[k] = (T)h[k];
+        bool nan = false;
+        for (size_t k = 0; k < n; ++k) {
+            tmp[k] = (T)h[k];
+            nan |= std::isnan(h[k]);
+        }
         SF_HIP(hipMemcpyAsync(dst + B.off, tmp.data(), N * sizeof(T), hipMemcpyHostToDevice, stream));
+        if (nan && field <= SF_FIELD_GAMMA) {
+            // the advection samples this bubble with NaN-skipping interpolation
+            BubbleTime bt;
+            SF_HIP(hipMemcpyAsync(&bt, time_d + b, sizeof(bt), hipMemcpyDeviceToHost, stream));
+            SF_HIP(hipStreamSynchronize(stream));
+            bt.nans = 1;
+            SF_HIP(hipMemcpyAsync(time_d + b, &bt, sizeof(bt), hipMemcpyHostToDevice, stream));
+        }
         SF_HIP(hipStreamSynchronize(stream));
     }
 
@@ -493,7 +505,7 @@ void setup(Base& c, int prec) {
         ct += B.nph;
         pr += B.nth;
         nph_max = std::max(nph_max, B.nph);
-        c.time_h[b] = BubbleTime{0.0, d.dt * 1.0, c.opt.advance_time, 0};
+        c.time_h[b] = BubbleTime{0.0, d.dt * 1.0, c.opt.advance_time, 0, 0, 0};
     }
     c.ncells = off;
     c.nrow_total = pr;

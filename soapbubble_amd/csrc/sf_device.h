@@ -32,6 +32,8 @@ struct BubbleTime {
     double dt_s;
     int advance;
     int pending;
+    int nans;   // sticky: set once a NaN has been written into the state fields
+    int pad_;
 };
 
 // Per-bubble control block of the iterative solve. Every field is written by
@@ -153,11 +155,19 @@ __device__ __forceinline__ void sincos_step(float x, float* s, float* c) {
 }
 __device__ __forceinline__ void sincos_step(double x, double* s, double* c) { sincos(x, s, c); }
 
-// bubbleLerp (bubble.cu:153-160): NaN operands are skipped.
+// bubbleLerp (bubble.cu:153-160): NaN operands are skipped. With NANS =
+// false the fields are known to hold no NaN (a per-bubble flag that every
+// writer of a NaN sets) and the same expression is evaluated without the
+// checks.
 template <typename T>
 __device__ __forceinline__ T lerp_nan(T from, T to, T alpha) {
     if (tisnan(from)) return to;
     if (tisnan(to)) return from;
+    return (T(1) - alpha) * from + alpha * to;
+}
+template <typename T, bool NANS>
+__device__ __forceinline__ T lerp_s(T from, T to, T alpha) {
+    if (NANS) return lerp_nan(from, to, alpha);
     return (T(1) - alpha) * from + alpha * to;
 }
 
@@ -181,13 +191,19 @@ template <typename T>
 struct Field {
     const T* __restrict__ p;   // bubble base
     int nph, mask, maxrow;
+    // rows are clamped into the field so that no position, however
+    // malformed, addresses memory outside it
     __device__ __forceinline__ T at(int row, int col) const {
-        row = row < 0 ? 0 : (row > maxrow ? maxrow : row);
+        row = min(max(row, 0), maxrow);
         return p[(unsigned)(row * nph + (col & mask))];
     }
+    template <bool NANS>
     __device__ __forceinline__ T belt(int row, int lo, T a) const {
-        return lerp_nan(at(row, lo), at(row, lo + 1), a);
+        row = min(max(row, 0), maxrow);
+        const T* r = p + (unsigned)(row * nph);
+        return lerp_s<T, NANS>(r[lo & mask], r[(lo + 1) & mask], a);
     }
+    __device__ __forceinline__ T belt(int row, int lo, T a) const { return belt<true>(row, lo, a); }
 };
 
 // Split a sample position into indices and weights; the phi index is
@@ -204,7 +220,7 @@ __device__ __forceinline__ void split_index(T x, T y, int nph, int& tI, int& pI,
 }
 
 // sampleCentered (bubble.cu:320-376)
-template <typename T>
+template <typename T, bool NANS = true>
 __device__ T sample_centered(const Field<T>& f, int nth, T rx, T ry) {
     T x = rx - T(0.5), y = ry;
     const int nph = f.nph;
@@ -212,24 +228,24 @@ __device__ T sample_centered(const Field<T>& f, int nth, T rx, T ry) {
     int tI, pI; T aT, aP;
     split_index(x, y, nph, tI, pI, aT, aP);
     if (tI == 0 && fl) {
-        T higher = f.belt(0, pI, aP);
-        T lower = f.belt(0, pI + nph / 2, aP);
-        return lerp_nan(lower, higher, aT);
+        T higher = f.template belt<NANS>(0, pI, aP);
+        T lower = f.template belt<NANS>(0, pI + nph / 2, aP);
+        return lerp_s<T, NANS>(lower, higher, aT);
     }
     if (fl) tI -= 1;
     if (tI == nth - 1) {
-        T lower = f.belt(tI, pI, aP);
-        T higher = f.belt(tI, pI + nph / 2, aP);
-        return lerp_nan(lower, higher, aT);
+        T lower = f.template belt<NANS>(tI, pI, aP);
+        T higher = f.template belt<NANS>(tI, pI + nph / 2, aP);
+        return lerp_s<T, NANS>(lower, higher, aT);
     }
-    T lower = f.belt(tI, pI, aP);
-    T higher = f.belt(tI + 1, pI, aP);
-    return lerp_nan(lower, higher, aT);
+    T lower = f.template belt<NANS>(tI, pI, aP);
+    T higher = f.template belt<NANS>(tI + 1, pI, aP);
+    return lerp_s<T, NANS>(lower, higher, aT);
 }
 
 // sampleCentered of two fields at one point (the eta and Gamma samples of
 // advectionAllCentered, bubble.cu:957-960) sharing the index computation.
-template <typename T>
+template <typename T, bool NANS = true>
 __device__ void sample_centered2(const Field<T>& f, const Field<T>& g, int nth, T rx, T ry, T& of,
                                  T& og) {
     T x = rx - T(0.5), y = ry;
@@ -237,23 +253,20 @@ __device__ void sample_centered2(const Field<T>& f, const Field<T>& g, int nth, 
     bool fl = validate_coord(x, y, nth, nph);
     int tI, pI; T aT, aP;
     split_index(x, y, nph, tI, pI, aT, aP);
+    int r0 = tI, r1 = tI + 1, c0 = pI, c1 = pI;
     if (tI == 0 && fl) {
-        of = lerp_nan(f.belt(0, pI + nph / 2, aP), f.belt(0, pI, aP), aT);
-        og = lerp_nan(g.belt(0, pI + nph / 2, aP), g.belt(0, pI, aP), aT);
-        return;
+        r0 = 0; r1 = 0; c0 = pI + nph / 2;
+    } else {
+        if (fl) tI -= 1;
+        if (tI == nth - 1) { r0 = tI; r1 = tI; c1 = pI + nph / 2; }
+        else { r0 = tI; r1 = tI + 1; }
     }
-    if (fl) tI -= 1;
-    if (tI == nth - 1) {
-        of = lerp_nan(f.belt(tI, pI, aP), f.belt(tI, pI + nph / 2, aP), aT);
-        og = lerp_nan(g.belt(tI, pI, aP), g.belt(tI, pI + nph / 2, aP), aT);
-        return;
-    }
-    of = lerp_nan(f.belt(tI, pI, aP), f.belt(tI + 1, pI, aP), aT);
-    og = lerp_nan(g.belt(tI, pI, aP), g.belt(tI + 1, pI, aP), aT);
+    of = lerp_s<T, NANS>(f.template belt<NANS>(r0, c0, aP), f.template belt<NANS>(r1, c1, aP), aT);
+    og = lerp_s<T, NANS>(g.template belt<NANS>(r0, c0, aP), g.template belt<NANS>(r1, c1, aP), aT);
 }
 
 // sampleVPhi (bubble.cu:178-240): values taken across the pole flip sign.
-template <typename T>
+template <typename T, bool NANS = true>
 __device__ T sample_vphi(const Field<T>& f, int nth, T rx, T ry) {
     T x = rx - T(0.5), y = ry + T(0.5);
     const int nph = f.nph;
@@ -261,20 +274,20 @@ __device__ T sample_vphi(const Field<T>& f, int nth, T rx, T ry) {
     int tI, pI; T aT, aP;
     split_index(x, y, nph, tI, pI, aT, aP);
     if (tI == 0 && fl) {
-        T higher = -f.belt(0, pI, aP);
-        T lower = f.belt(0, pI + nph / 2, aP);
-        return lerp_nan(lower, higher, aT);
+        T higher = -f.template belt<NANS>(0, pI, aP);
+        T lower = f.template belt<NANS>(0, pI + nph / 2, aP);
+        return lerp_s<T, NANS>(lower, higher, aT);
     }
     if (fl) tI -= 1;
     if (tI == nth - 1) {
-        T lower = f.belt(tI, pI, aP);
-        T higher = -f.belt(tI, pI + nph / 2, aP);
-        T v = lerp_nan(lower, higher, aT);
+        T lower = f.template belt<NANS>(tI, pI, aP);
+        T higher = -f.template belt<NANS>(tI, pI + nph / 2, aP);
+        T v = lerp_s<T, NANS>(lower, higher, aT);
         return fl ? -v : v;
     }
-    T lower = f.belt(tI, pI, aP);
-    T higher = f.belt(tI + 1, pI, aP);
-    T v = lerp_nan(lower, higher, aT);
+    T lower = f.template belt<NANS>(tI, pI, aP);
+    T higher = f.template belt<NANS>(tI + 1, pI, aP);
+    T v = lerp_s<T, NANS>(lower, higher, aT);
     return fl ? -v : v;
 }
 
@@ -282,7 +295,7 @@ __device__ T sample_vphi(const Field<T>& f, int nth, T rx, T ry) {
 // interpolate between the first/last row and its phi+pi mirror with half
 // weight. A sample exactly at theta = pi (which the reference reads past
 // the array for) takes the south-cap branch.
-template <typename T>
+template <typename T, bool NANS = true>
 __device__ T sample_vtheta(const Field<T>& f, int nth, T rx, T ry) {
     T x = rx - T(1), y = ry;
     const int nph = f.nph;
@@ -297,20 +310,20 @@ __device__ T sample_vtheta(const Field<T>& f, int nth, T rx, T ry) {
         tI -= 2;
     }
     if (tI == 0 && fl && rx > T(-1)) {
-        T higher = -f.belt(0, pI, aP);
-        T lower = f.belt(0, pI + nph / 2, aP);
-        return lerp_nan(lower, higher, T(0.5) * aT);
+        T higher = -f.template belt<NANS>(0, pI, aP);
+        T lower = f.template belt<NANS>(0, pI + nph / 2, aP);
+        return lerp_s<T, NANS>(lower, higher, T(0.5) * aT);
     }
     if (tI >= nth - 1 && !fl) { tI = nth - 2; aT = T(1); }
     if (tI == nth - 2) {
-        T lower = f.belt(tI, pI, aP);
-        T higher = -f.belt(tI, pI + nph / 2, aP);
-        T v = lerp_nan(lower, higher, T(0.5) * aT);
+        T lower = f.template belt<NANS>(tI, pI, aP);
+        T higher = -f.template belt<NANS>(tI, pI + nph / 2, aP);
+        T v = lerp_s<T, NANS>(lower, higher, T(0.5) * aT);
         return fl ? -v : v;
     }
-    T lower = f.belt(tI, pI, aP);
-    T higher = f.belt(tI + 1, pI, aP);
-    T v = lerp_nan(lower, higher, aT);
+    T lower = f.template belt<NANS>(tI, pI, aP);
+    T higher = f.template belt<NANS>(tI + 1, pI, aP);
+    T v = lerp_s<T, NANS>(lower, higher, aT);
     return fl ? -v : v;
 }
 
@@ -321,11 +334,11 @@ __device__ T sample_vtheta(const Field<T>& f, int nth, T rx, T ry) {
 // 787-867). Positions on the sphere are kept Cartesian: the sin/cos of a
 // traced point's theta and phi are its z, rho, x/rho and y/rho, which is
 // the reference's cos(acos(z)), sin(acos(z)), cos(atan2(y,x)), sin(atan2(y,x))
-// without the round trip, and theta = atan2(rho, z) stays accurate at the
-// poles in FP32 where acos(z) does not.
+// without the round trip.
 template <typename T>
 struct Circle {
     T w0, w1, w2, u0, u1, u2, v0, v1, v2, unorm;
+    T gx, gy, sT, cT, sP, cP;   // start point: index coordinates and trig
 };
 
 template <typename T>
@@ -335,11 +348,41 @@ struct VelCtx {
     T invh, dt;
 };
 
+// atan2(n, d) for the small angle between a traced point and its start
+// point: the odd series t - t^3/3 + t^5/5 of atan(t), t = n/d, below
+// |t| = 1/10 (truncation under 1.5e-8 rad), the general atan2 otherwise
+// (the phi change of a trace that passes close to a pole).
+__device__ __forceinline__ float small_atan2(float n, float d) {
+    if (d > 0.0f && fabsf(n) < 0.1f * d) {
+        const float t = n * __builtin_amdgcn_rcpf(d), t2 = t * t;
+        return t * (1.0f + t2 * (-1.0f / 3.0f + t2 * 0.2f));
+    }
+    return tatan2(n, d);
+}
+
+// Index-space coordinates of a traced point (x, y, z) and rho = sin(theta).
+// FP64: theta = atan2(rho, z), phi = atan2(y, x). FP32: theta and phi as the
+// start point's plus the angles turned from it, theta - theta0 =
+// atan2(rho z0 - z rho0, z z0 + rho rho0) in the meridian half-plane and
+// phi - phi0 = atan2(y x0 - x y0, x x0 + y y0) (x0, y0 the start point's
+// cos phi0, sin phi0): the same angles, with the grid-exact start index
+// carrying the magnitude so FP32 keeps the small increment's precision.
 template <typename T>
-__device__ __forceinline__ void cart_to_index(T x, T y, T z, T invh, T& ti, T& pi_, T& rho) {
-    rho = tsqrt(x * x + y * y);
-    ti = tatan2(rho, z) * invh;
-    pi_ = tatan2(y, x) * invh;
+__device__ __forceinline__ void point_index(const Circle<T>& o, T x, T y, T z, T invh, T& ti,
+                                            T& pi_, T& rho);
+template <>
+__device__ __forceinline__ void point_index(const Circle<double>& o, double x, double y, double z,
+                                            double invh, double& ti, double& pi_, double& rho) {
+    rho = sqrt(x * x + y * y);
+    ti = atan2(rho, z) * invh;
+    pi_ = atan2(y, x) * invh;
+}
+template <>
+__device__ __forceinline__ void point_index(const Circle<float>& o, float x, float y, float z,
+                                            float invh, float& ti, float& pi_, float& rho) {
+    rho = sqrtf(x * x + y * y);
+    ti = o.gx + small_atan2(rho * o.cT - z * o.sT, z * o.cT + rho * o.sT) * invh;
+    pi_ = o.gy + small_atan2(y * o.cP - x * o.sP, x * o.cP + y * o.sP) * invh;
 }
 
 template <typename T>
@@ -351,9 +394,10 @@ __device__ __forceinline__ void tangent_vec(T x, T y, T z, T rho, T vT, T vP, T&
     o2 = -vT * rho;
 }
 
-template <typename T>
-__device__ __forceinline__ bool circle_mid(const VelCtx<T>& c, T sinT, T cosT, T sinP, T cosP,
-                                           T guT, T guP, Circle<T>& o) {
+template <typename T, bool NANS = true>
+__device__ __forceinline__ bool circle_mid(const VelCtx<T>& c, T gx, T gy, T sinT, T cosT, T sinP,
+                                           T cosP, T guT, T guP, Circle<T>& o) {
+    o.gx = gx; o.gy = gy; o.sT = sinT; o.cT = cosT; o.sP = sinP; o.cP = cosP;
     T u0 = guT * (cosT * cosP) - guP * sinP;
     T u1 = guT * (cosT * sinP) + guP * cosP;
     T u2 = -guT * sinT;
@@ -366,9 +410,9 @@ __device__ __forceinline__ bool circle_mid(const VelCtx<T>& c, T sinT, T cosT, T
     sincos_step(T(-0.5) * o.unorm * c.dt, &s, &co);
     T mx = o.u0 * s + o.w0 * co, my = o.u1 * s + o.w1 * co, mz = o.u2 * s + o.w2 * co;
     T mti, mpi, rho;
-    cart_to_index(mx, my, mz, c.invh, mti, mpi, rho);
-    T muT = sample_vtheta(c.uth, c.nth, mti, mpi);
-    T muP = sample_vphi(c.uph, c.nth, mti, mpi);
+    point_index(o, mx, my, mz, c.invh, mti, mpi, rho);
+    T muT = sample_vtheta<T, NANS>(c.uth, c.nth, mti, mpi);
+    T muP = sample_vphi<T, NANS>(c.uph, c.nth, mti, mpi);
     T m0, m1, m2;
     tangent_vec(mx, my, mz, rho, muT, muP, m0, m1, m2);
     T uc0 = o.u0 * co - o.w0 * s, uc1 = o.u1 * co - o.w1 * s, uc2 = o.u2 * co - o.w2 * s;
@@ -398,18 +442,18 @@ __device__ __forceinline__ void circle_depart(const VelCtx<T>& c, const Circle<T
     px = o.u0 * s + o.w0 * co;
     py = o.u1 * s + o.w1 * co;
     pz = o.u2 * s + o.w2 * co;
-    cart_to_index(px, py, pz, c.invh, ti, pi_, rho);
+    point_index(o, px, py, pz, c.invh, ti, pi_, rho);
 }
 
 // Velocity at the departure point transported back along the circle into
 // the start frame (bubble.cu:745-766, 881-902).
-template <typename T>
+template <typename T, bool NANS = true>
 __device__ __forceinline__ void transport_back(const VelCtx<T>& c, const Circle<T>& o, T& r0, T& r1,
                                                T& r2) {
     T s, co, px, py, pz, ti, pi_, rho;
     circle_depart(c, o, s, co, px, py, pz, ti, pi_, rho);
-    T puT = sample_vtheta(c.uth, c.nth, ti, pi_);
-    T puP = sample_vphi(c.uph, c.nth, ti, pi_);
+    T puT = sample_vtheta<T, NANS>(c.uth, c.nth, ti, pi_);
+    T puP = sample_vphi<T, NANS>(c.uph, c.nth, ti, pi_);
     T p0, p1, p2;
     tangent_vec(px, py, pz, rho, puT, puP, p0, p1, p2);
     T uc0 = o.u0 * co - o.w0 * s, uc1 = o.u1 * co - o.w1 * s, uc2 = o.u2 * co - o.w2 * s;

@@ -27,25 +27,14 @@ constexpr double kEps = 1e-7;   // bubble.cu:28
 constexpr double kThresh = 0.0; // bubble.cu:27
 
 // ------------------------------------------------------------ advection ----
-template <typename T>
-__global__ __launch_bounds__(256) void k_advect(const int4* __restrict__ tiles,
-                                                const BubbleDev* __restrict__ bub,
-                                                BubbleTime* __restrict__ btime, Tables tab,
-                                                StatePtrs<T> in, StatePtrs<T> out) {
-    const int4 tl = tiles[blockIdx.x];
-    if (tl.x < 0) return;
-    const BubbleDev B = bub[tl.x];
-    // the previous step's time advance lands here, before any kernel of this
-    // step reads the time
-    if (tl.y == 0 && tl.z == 0 && threadIdx.x == 0 && btime[tl.x].pending) {
-        btime[tl.x].t += btime[tl.x].dt_s;
-        btime[tl.x].pending = 0;
-    }
-    const int j = tl.y + (int)threadIdx.x / tl.w;
-    const int i = tl.z + (int)threadIdx.x % tl.w;
-    if (j >= B.nth || i >= B.nph) return;
+// One cell: eta and Gamma at the centre (j+1/2, i), u_phi at the west face
+// (j+1/2, i-1/2), u_theta at the south face (j+1, i). NANS selects the
+// reference's NaN-skipping interpolation; without it the fields hold no NaN.
+template <typename T, bool NANS>
+__device__ __forceinline__ void advect_cell(const BubbleDev& B, const Tables& tab,
+                                            const StatePtrs<T>& in, const StatePtrs<T>& out, int j,
+                                            int i) {
     const size_t c = (size_t)j * B.nph + i;
-
     VelCtx<T> v;
     v.uth = Field<T>{in.uth + B.off, B.nph, B.mask, B.nth - 2};
     v.uph = Field<T>{in.uph + B.off, B.nph, B.mask, B.nth - 1};
@@ -64,64 +53,91 @@ __global__ __launch_bounds__(256) void k_advect(const int4* __restrict__ tiles,
     // polar rows go through the general samplers.
     const bool interior = j > 0 && j < B.nth - 1;
 
-    // eta, Gamma at the cell centre (j+1/2, i): traceGreatCircle + two samples
+    // eta, Gamma at the cell centre: traceGreatCircle + two samples
     {
         const T gx = (T)j + T(0.5), gy = (T)i;
         T guT, guP;
         if (interior) {
-            guT = lerp_nan(v.uth.belt(j - 1, i, T(0)), v.uth.belt(j, i, T(0)), T(0.5));
-            guP = lerp_nan(v.uph.belt(j, i, T(0.5)), v.uph.belt(j + 1, i, T(0.5)), T(0));
+            guT = lerp_s<T, NANS>(v.uth.template belt<NANS>(j - 1, i, T(0)),
+                                  v.uth.template belt<NANS>(j, i, T(0)), T(0.5));
+            guP = lerp_s<T, NANS>(v.uph.template belt<NANS>(j, i, T(0.5)),
+                                  v.uph.template belt<NANS>(j + 1, i, T(0.5)), T(0));
         } else {
-            guT = sample_vtheta(v.uth, B.nth, gx, gy);
-            guP = sample_vphi(v.uph, B.nth, gx, gy);
+            guT = sample_vtheta<T, NANS>(v.uth, B.nth, gx, gy);
+            guP = sample_vphi<T, NANS>(v.uph, B.nth, gx, gy);
         }
         Circle<T> o;
         T tx = gx, ty = gy;
-        if (circle_mid(v, sinTc, cosTc, sinPc, cosPc, guT, guP, o)) {
+        if (circle_mid<T, NANS>(v, gx, gy, sinTc, cosTc, sinPc, cosPc, guT, guP, o)) {
             T s, co, px, py, pz, rho;
             circle_depart(v, o, s, co, px, py, pz, tx, ty, rho);
         }
         T en, gn;
-        sample_centered2(feta, fgam, B.nth, tx, ty, en, gn);
+        sample_centered2<T, NANS>(feta, fgam, B.nth, tx, ty, en, gn);
         out.eta[B.off + c] = en;
         out.gam[B.off + c] = gn;
     }
-    // u_phi at the west face (j+1/2, i-1/2)
+    // u_phi at the west face
     {
         const T gx = (T)j + T(0.5), gy = (T)i - T(0.5);
         const T sinP = TabSel<T>::sPh(tab)[B.ct0 + i], cosP = TabSel<T>::cPh(tab)[B.ct0 + i];
-        T guT = interior
-                    ? lerp_nan(v.uth.belt(j - 1, i - 1, T(0.5)), v.uth.belt(j, i - 1, T(0.5)), T(0.5))
-                    : sample_vtheta(v.uth, B.nth, gx, gy);
+        T guT = interior ? lerp_s<T, NANS>(v.uth.template belt<NANS>(j - 1, i - 1, T(0.5)),
+                                           v.uth.template belt<NANS>(j, i - 1, T(0.5)), T(0.5))
+                         : sample_vtheta<T, NANS>(v.uth, B.nth, gx, gy);
         T guP = in.uph[B.off + c];
         Circle<T> o;
         T res;
-        if (!circle_mid(v, sinTc, cosTc, sinP, cosP, guT, guP, o)) {
+        if (!circle_mid<T, NANS>(v, gx, gy, sinTc, cosTc, sinP, cosP, guT, guP, o)) {
             res = o.unorm;
         } else {
             T r0, r1, r2;
-            transport_back(v, o, r0, r1, r2);
+            transport_back<T, NANS>(v, o, r0, r1, r2);
             res = -sinP * r0 + cosP * r1;
         }
         out.uph[B.off + c] = res;
     }
-    // u_theta at the south face (j+1, i); sampleVPhi there is always the
-    // interior branch with weights 1/2
+    // u_theta at the south face; sampleVPhi there is always the interior
+    // branch with weights 1/2
     if (j < B.nth - 1) {
+        const T gx = (T)j + T(1), gy = (T)i;
         const T sinT = TabSel<T>::sTf(tab)[B.rt0 + j + 1], cosT = TabSel<T>::cTf(tab)[B.rt0 + j + 1];
         T guT = in.uth[B.off + c];
-        T guP = lerp_nan(v.uph.belt(j, i, T(0.5)), v.uph.belt(j + 1, i, T(0.5)), T(0.5));
+        T guP = lerp_s<T, NANS>(v.uph.template belt<NANS>(j, i, T(0.5)),
+                                v.uph.template belt<NANS>(j + 1, i, T(0.5)), T(0.5));
         Circle<T> o;
         T res;
-        if (!circle_mid(v, sinT, cosT, sinPc, cosPc, guT, guP, o)) {
+        if (!circle_mid<T, NANS>(v, gx, gy, sinT, cosT, sinPc, cosPc, guT, guP, o)) {
             res = o.unorm;
         } else {
             T r0, r1, r2;
-            transport_back(v, o, r0, r1, r2);
+            transport_back<T, NANS>(v, o, r0, r1, r2);
             res = r0 * (cosT * cosPc) + r1 * (cosT * sinPc) - r2 * sinT;
         }
         out.uth[B.off + c] = res;
     }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_advect(const int4* __restrict__ tiles,
+                                                const BubbleDev* __restrict__ bub,
+                                                BubbleTime* __restrict__ btime, Tables tab,
+                                                StatePtrs<T> in, StatePtrs<T> out) {
+    const int4 tl = tiles[blockIdx.x];
+    if (tl.x < 0) return;
+    const BubbleDev B = bub[tl.x];
+    // the previous step's time advance lands here, before any kernel of this
+    // step reads the time
+    if (tl.y == 0 && tl.z == 0 && threadIdx.x == 0 && btime[tl.x].pending) {
+        btime[tl.x].t += btime[tl.x].dt_s;
+        btime[tl.x].pending = 0;
+    }
+    const int j = tl.y + (int)threadIdx.x / tl.w;
+    const int i = tl.z + (int)threadIdx.x % tl.w;
+    if (j >= B.nth || i >= B.nph) return;
+    if (btime[tl.x].nans)
+        advect_cell<T, true>(B, tab, in, out, j, i);
+    else
+        advect_cell<T, false>(B, tab, in, out, j, i);
 }
 
 // ---------------------------------------------------------- air flow ----
@@ -936,12 +952,14 @@ __global__ __launch_bounds__(256) void k_update(const int4* __restrict__ tiles,
         div = (1.0 - fmin(g, 2 * ga) / ga) / dt;
     }
     const double eta = (double)st.eta[c];
+    bool nan_out;
     // applyforceThickness (bubble.cu:2041-2063)
     {
         double v = eta * (1 - dt * div);
         if (B.evaporation) v += B.eva * dt;
         out.eta[c] = (T)fmax(kThresh, v);
         out.gam[c] = (T)gnew;
+        nan_out = __builtin_isnan(gnew);
     }
     // applyforcevelphiKernel (bubble.cu:1879-1935)
     {
@@ -960,6 +978,7 @@ __global__ __launch_bounds__(256) void k_update(const int4* __restrict__ tiles,
             res = (u1 / dt + (f1 + f2 + 0.0 + 0.0)) / (1. / dt + Cr * invEta);
         }
         out.uph[c] = (T)res;
+        nan_out |= __builtin_isnan(res);
     }
     // applyforcevelthetaKernel (bubble.cu:1819-1876)
     if (j < nth - 1) {
@@ -979,7 +998,10 @@ __global__ __launch_bounds__(256) void k_update(const int4* __restrict__ tiles,
             res = (v1 / dt + f1 + f2 + f3 + 0.0) / (1. / dt + Cr * invEta);
         }
         out.uth[c] = (T)res;
+        nan_out |= __builtin_isnan(res);
     }
+    // the advection of the next step takes its NaN-skipping path from here on
+    if (nan_out) btime[tl.x].nans = 1;
 }
 
 // Per-row film volume (sum eta sin(theta)) and count of cells with eta > 0.
