@@ -149,3 +149,41 @@ def test_upload_nan_sets_flag(oracle):
             ng, nr = np.isnan(got), np.isnan(ref[k])
             assert (ng == nr).all(), k
             assert rel_l2(got[~ng], ref[k][~nr]) < 1e-12, k
+
+
+@pytest.mark.parametrize("name", ["g8", "g16", "c0"])
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-9), ("mixed", 5e-5)])
+def test_golden_fixtures(name, prec, tol):
+    """The HIP step against the committed oracle states (tests/golden)."""
+    from golden_cases import FIELDS, load as load_golden
+    case, (st, ua, va), outs = load_golden(name)
+    _say(f"golden {name} {prec}")
+    with sb.SoapFilm(sb.bubble(case["n_theta"], case["dt"]), prec,
+                     tol=1e-12 if prec == "fp64" else 1e-10) as ctx:
+        load(ctx, 0, st, ua, va)
+        done = 0
+        for s in case["steps"]:
+            ctx.step(s - done)
+            done = s
+            got = ctx.state(0)
+            for k in FIELDS:
+                e = rel_l2(got[k], outs[s][k])
+                assert e < tol, (s, k, e)
+
+
+def test_thickness_exr_roundtrip(tmp_path):
+    """write_thickness_img (solver.cu:541-593): eta H 2e5 as HALF in B, G, R,
+    uncompressed; read back with an independent reader."""
+    from exr_reader import read_exr
+    n = 32
+    st, ua, va = make_case(n, 21)
+    with sb.SoapFilm(sb.bubble(n, 0.005), "fp64") as ctx:
+        load(ctx, 0, st, ua, va)
+        path = str(tmp_path / "frame0000.exr")
+        ctx.write_thickness_exr(0, path)
+    attrs, chans, img = read_exr(path)
+    assert chans == ["B", "G", "R"]
+    want = (st["eta"] * 1e-6 * 2e5).reshape(n, 2 * n).astype(np.float32).astype(np.float16)
+    want = want.astype(np.float32)
+    for c in chans:
+        assert np.array_equal(img[c], want), c

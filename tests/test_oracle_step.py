@@ -1,0 +1,399 @@
+"""Pins of the oracle's step stages (CPU only): advection invariants, the air
+flow's closed form, the assembled system's structure and known answers, the
+solve against a dense direct solve, the force updates' per-cell closed forms
+and the derived parameters."""
+import math
+
+import numpy as np
+import pytest
+
+from cases import make_case
+
+SEED = 1234
+
+
+def _rand_state(n, seed, u_scale=0.6):
+    st, ua, va = make_case(n, seed, u_scale=u_scale)
+    return st, ua, va
+
+
+# ------------------------------------------------------------ advection ----
+def test_advect_zero_velocity_is_identity(oracle):
+    n = 16
+    st, _, _ = _rand_state(n, 1, u_scale=0.0)
+    p = oracle.params(n, 0.01)
+    out = oracle.advect(p, st)
+    for k in ("u_theta", "u_phi", "eta", "gamma"):
+        assert np.array_equal(out[k], st[k]), k
+
+
+def test_advect_preserves_constants(oracle):
+    n = 16
+    st, _, _ = _rand_state(n, 2, u_scale=1.5)
+    st["eta"][:] = 0.73
+    st["gamma"][:] = 1.9
+    p = oracle.params(n, 0.02)
+    out = oracle.advect(p, st)
+    assert np.abs(out["eta"] - 0.73).max() <= 1e-15
+    assert np.abs(out["gamma"] - 1.9).max() <= 1e-14
+
+
+def _mirror_phi(st, n):
+    """phi -> -phi: centres i -> -i, u_phi faces i-1/2 -> 1/2-i (index 1-i),
+    u_phi changes sign, u_theta does not."""
+    nph = 2 * n
+    i = np.arange(nph)
+    c = (-i) % nph
+    f = (1 - i) % nph
+    out = {}
+    out["eta"] = st["eta"].reshape(n, nph)[:, c].ravel()
+    out["gamma"] = st["gamma"].reshape(n, nph)[:, c].ravel()
+    out["u_theta"] = st["u_theta"].reshape(n - 1, nph)[:, c].ravel()
+    out["u_phi"] = -st["u_phi"].reshape(n, nph)[:, f].ravel()
+    return out
+
+
+def _mirror_theta(st, n):
+    """theta -> pi - theta: rows j -> n-1-j (u_theta rows j -> n-2-j),
+    u_theta changes sign, u_phi does not."""
+    nph = 2 * n
+    out = {}
+    for k in ("eta", "gamma", "u_phi"):
+        out[k] = st[k].reshape(n, nph)[::-1].ravel().copy()
+    out["u_theta"] = -st["u_theta"].reshape(n - 1, nph)[::-1].ravel()
+    return out
+
+
+@pytest.mark.parametrize("mirror", [_mirror_phi, _mirror_theta])
+def test_advect_reflection_symmetry(oracle, mirror):
+    """Advection commutes with the reflections of the sphere that map the
+    grid to itself."""
+    n = 16
+    st, _, _ = _rand_state(n, 3, u_scale=1.2)
+    p = oracle.params(n, 0.02)
+    a = mirror(oracle.advect(p, st), n)
+    b = oracle.advect(p, mirror(st, n))
+    for k in ("u_theta", "u_phi", "eta", "gamma"):
+        scale = np.abs(b[k]).max()
+        assert np.abs(a[k] - b[k]).max() <= 1e-11 * scale, k
+
+
+def test_advect_along_meridian_rotation(oracle):
+    """Rotation about the x axis, u = omega x_hat x r: u_theta = -omega
+    sin(phi), u_phi = -omega cos(theta) cos(phi). On the meridian phi = pi/2
+    (grid column nth/2) the flow is the great circle itself at speed omega
+    (a geodesic, so the advected velocity is exactly the velocity), u_phi
+    interpolates to exactly zero there, and eta is carried along the column
+    from theta + omega dt. The advected u_theta on that column is -omega and
+    the advected eta is the bilinear sample of eta at the departure point."""
+    from test_oracle_samplers import CENTRED, ghost_sample
+    n, dt, om = 32, 0.02, 0.9
+    nph = 2 * n
+    h = math.pi / n
+    jt, it = np.meshgrid(np.arange(n - 1), np.arange(nph), indexing="ij")
+    uth = (-om * np.sin(it * h)).ravel()
+    jp, ip = np.meshgrid(np.arange(n), np.arange(nph), indexing="ij")
+    uph = (-om * np.cos((jp + 0.5) * h) * np.cos((ip - 0.5) * h)).ravel()
+    rng = np.random.default_rng(2)
+    eta = np.repeat(rng.uniform(0.5, 1.5, n), nph)       # zonal
+    st = {"u_theta": uth, "u_phi": uph, "eta": eta, "gamma": eta.copy()}
+    p = oracle.params(n, dt)
+    out = oracle.advect(p, st)
+    col = n // 2
+    shift = om * dt / h
+    for j in range(3, n - 5):
+        c = j * nph + col
+        assert abs(out["u_theta"][c] + om) <= 1e-12, j
+        want = ghost_sample(CENTRED, eta, n, j + 0.5 + shift, col)
+        assert abs(out["eta"][c] - want) <= 1e-12, j
+
+
+# ------------------------------------------------------------- air flow ----
+def _air_closed_form(n, t):
+    """airFlowU/V (bubble.cu:2301-2380) with wind (0, cos pi/2, sin pi/2):
+    normal = normalize(x x wind) = -y, and 0.5 sin(beta) projectedWind =
+    0.5 normal x r, so the corridor is the solid-body rotation 0.5 (n x r)
+    = 0.5 (-z, 0, x), tapered by 1 - smoothstep(0.2, 0.4, |beta - pi/2|),
+    beta = acos(n . r), and scaled by 3 - |t - 3|. Its components are
+    0.5 cos(theta) sin(phi) along e_phi and -0.5 cos(phi) along e_theta."""
+    h = math.pi / n
+    nph = 2 * n
+
+    def taper(th, ph):
+        beta = np.arccos(np.clip(-np.sin(th) * np.sin(ph), -1, 1))
+        x = np.abs(beta - math.pi / 2)
+        y = np.clip((x - 0.2) / 0.2, 0, 1)
+        return 1 - y * y * (3 - 2 * y)
+
+    kt = 3 - abs(t - 3)
+    jp, ip = np.meshgrid(np.arange(n), np.arange(nph), indexing="ij")
+    th, ph = (jp + 0.5) * h, (ip - 0.5) * h
+    du = 0.5 * np.cos(th) * np.sin(ph) * taper(th, ph) * kt
+    jt, it = np.meshgrid(np.arange(n - 1), np.arange(nph), indexing="ij")
+    th, ph = (jt + 1.0) * h, it * h
+    dv = -0.5 * np.cos(ph) * taper(th, ph) * kt
+    return du.ravel(), dv.ravel()
+
+
+@pytest.mark.parametrize("t", [0.0, 0.7, 2.9, 4.5])
+def test_airflow_closed_form(oracle, t):
+    n = 32
+    _, ua, va = _rand_state(n, 4)
+    p = oracle.params(n, 0.01, time_s=t)
+    u, v = oracle.airflow(p, ua, va)
+    du, dv = _air_closed_form(n, t)
+    # smoothstep is evaluated in float by the reference
+    assert np.abs(u - ua - du).max() <= 2e-7 * max(1.0, t)
+    assert np.abs(v - va - dv).max() <= 2e-7 * max(1.0, t)
+    if t == 0.0:
+        assert np.array_equal(u, ua) and np.array_equal(v, va)
+
+
+def test_airflow_nan_to_zero(oracle):
+    n = 16
+    _, ua, va = _rand_state(n, 5)
+    ua[7] = np.nan
+    va[3] = np.nan
+    p = oracle.params(n, 0.01, time_s=1.0)
+    u, v = oracle.airflow(p, ua, va)
+    assert u[7] == 0.0 and v[3] == 0.0
+
+
+# ------------------------------------------------------------- assembly ----
+def _dense(oracle, p, val):
+    n = p.n_theta
+    N = n * 2 * n
+    col = oracle.col_ind(p)
+    A = np.zeros((N, N))
+    rows = np.repeat(np.arange(N), 5)
+    np.add.at(A, (rows, col), val)
+    return A
+
+
+def test_col_ind_pattern(oracle):
+    """initLinearSystem (solver.cu:25-60): north, west, centre, east, south;
+    phi periodic; across the poles the phi + pi cell of the same row."""
+    n = 8
+    nph = 2 * n
+    p = oracle.params(n, 0.01)
+    col = oracle.col_ind(p).reshape(n, nph, 5)
+    for j in range(n):
+        for i in range(nph):
+            c = j * nph + i
+            north = (j - 1) * nph + i if j > 0 else j * nph + (i + n) % nph
+            south = (j + 1) * nph + i if j < n - 1 else j * nph + (i + n) % nph
+            assert list(col[j, i]) == [north, j * nph + (i - 1) % nph, c,
+                                        j * nph + (i + 1) % nph, south], (j, i)
+
+
+def test_assembly_symmetric_rowsum_spd(oracle):
+    n = 8
+    st, ua, va = _rand_state(n, 6)
+    p = oracle.params(n, 0.01, time_s=0.4)
+    uair, vair = oracle.airflow(p, ua, va)
+    val, rhs, gam = oracle.assemble(p, st, uair, vair)
+    A = _dense(oracle, p, val)
+    assert np.abs(A - A.T).max() <= 1e-13 * np.abs(A).max()
+    h = math.pi / n
+    sin_c = np.repeat(np.sin((np.arange(n) + 0.5) * h), 2 * n)
+    # A 1 = sin(theta) / (dt Gamma*) row by row
+    assert np.allclose(A.sum(axis=1), sin_c / (p.dt * gam), rtol=1e-12, atol=0)
+    np.linalg.cholesky(A)
+    off = np.abs(A - np.diag(np.diag(A))).sum(axis=1)
+    assert (np.diag(A) > off).all()
+
+
+def test_assembly_known_answer(oracle):
+    """u = 0, no air, no gravity, Gamma* constant: the solution is Gamma*."""
+    n = 16
+    st, ua, va = _rand_state(n, 7, u_scale=0.0)
+    st["gamma"][:] = 1.3
+    p = oracle.params(n, 0.01, airflow=False)
+    zero_u, zero_v = np.zeros_like(ua), np.zeros_like(va)
+    val, rhs, gam = oracle.assemble(p, st, zero_u, zero_v)
+    h = math.pi / n
+    sin_c = np.repeat(np.sin((np.arange(n) + 0.5) * h), 2 * n)
+    assert np.allclose(rhs, sin_c / p.dt, rtol=1e-14, atol=0)
+    x, s, rc = oracle.solve(p, val, rhs, np.ones_like(rhs))
+    assert rc == 0
+    assert np.abs(x - 1.3).max() <= 1e-10
+
+
+def _ylm(l, m, th, ph):
+    from scipy.special import sph_harm_y
+    y = sph_harm_y(l, abs(m), th, ph)
+    if m > 0:
+        return math.sqrt(2) * (-1) ** m * y.real
+    if m < 0:
+        return math.sqrt(2) * (-1) ** m * y.imag
+    return y.real
+
+
+@pytest.mark.parametrize("l,m", [(2, 1), (3, -2), (4, 0)])
+def test_assembly_laplacian_consistency(oracle, l, m):
+    """With eta constant the off-diagonal part of A is -K sin(theta) times the
+    discrete Laplace-Beltrami operator, K = M dt / (eta + Cr dt); on Y_lm it
+    approaches K l (l+1) sin(theta) Y_lm at O(h^2)."""
+    errs = []
+    for n in (16, 32, 64):
+        nph = 2 * n
+        h = math.pi / n
+        st, ua, va = _rand_state(n, 8, u_scale=0.0)
+        eta = 0.9
+        st["eta"][:] = eta
+        st["gamma"][:] = 1.0
+        p = oracle.params(n, 0.01, airflow=False)
+        val, rhs, gam = oracle.assemble(p, st, np.zeros_like(ua), np.zeros_like(va))
+        jj, ii = np.meshgrid(np.arange(n), np.arange(nph), indexing="ij")
+        th, ph = (jj + 0.5) * h, ii * h
+        f = _ylm(l, m, th, ph).ravel()
+        sin_c = np.sin(th).ravel()
+        Lf = oracle.spmv(p, val, f) - sin_c / p.dt * f
+        K = p.M * p.dt / (eta + p.Cr * p.dt)
+        want = K * l * (l + 1) * sin_c * f
+        errs.append(np.sqrt(np.mean((Lf - want) ** 2)) / np.sqrt(np.mean(want ** 2)))
+    assert errs[-1] < 1e-2
+    assert errs[0] / errs[1] > 3.0 and errs[1] / errs[2] > 3.0, errs
+
+
+def test_assembly_broken_cell(oracle):
+    """A broken cell (eta == 0 or Gamma* > gamma_max - eps) gets an identity
+    row with rhs gamma_max, and Gamma* := gamma_max (bubble.cu:1453-1466)."""
+    n = 8
+    nph = 2 * n
+    st, ua, va = _rand_state(n, 9)
+    st["eta"][3 * nph + 4] = 0.0
+    p = oracle.params(n, 0.01)
+    st["gamma"][5 * nph + 9] = p.gamma_max
+    uair, vair = oracle.airflow(p, ua, va)
+    val, rhs, gam = oracle.assemble(p, st, uair, vair)
+    for c in (3 * nph + 4, 5 * nph + 9):
+        assert list(val[5 * c:5 * c + 5]) == [0.0, 0.0, 1.0, 0.0, 0.0]
+        assert rhs[c] == p.gamma_max and gam[c] == p.gamma_max
+
+
+# ---------------------------------------------------------------- solve ----
+@pytest.mark.parametrize("broken", [False, True])
+def test_solve_matches_dense(oracle, broken):
+    n = 16
+    nph = 2 * n
+    st, ua, va = _rand_state(n, 10, u_scale=1.0)
+    if broken:
+        st["eta"][4 * nph + 2:4 * nph + 6] = 0.0
+    p = oracle.params(n, 0.01, time_s=1.3)
+    uair, vair = oracle.airflow(p, ua, va)
+    adv = oracle.advect(p, st)
+    val, rhs, gam = oracle.assemble(p, adv, uair, vair)
+    A = _dense(oracle, p, val)
+    x_dense = np.linalg.solve(A, rhs)
+    x, s, rc = oracle.solve(p, val, rhs, gam)
+    assert rc == 0 and s.converged == 1
+    assert np.linalg.norm(rhs - A @ x) / np.linalg.norm(rhs) <= 1e-12
+    assert np.linalg.norm(x - x_dense) / np.linalg.norm(x_dense) <= 1e-10
+
+
+def test_solve_absolute_tolerance(oracle):
+    """PCGF_F.json's ABSOLUTE criterion: ||b - A x|| <= tol."""
+    n = 16
+    st, ua, va = _rand_state(n, 11, u_scale=1.0)
+    p = oracle.params(n, 0.01, tol=1e-7, tol_absolute=True)
+    uair, vair = oracle.airflow(p, ua, va)
+    val, rhs, gam = oracle.assemble(p, st, uair, vair)
+    x, s, rc = oracle.solve(p, val, rhs, gam)
+    A = _dense(oracle, p, val)
+    assert rc == 0 and np.linalg.norm(rhs - A @ x) <= 1e-7
+
+
+# --------------------------------------------------------------- update ----
+def test_update_closed_forms(oracle):
+    """Constant eta, Gamma^{n+1} = Gamma* (1 - delta): div = delta / dt, so
+    eta^{n+1} = eta (1 - delta); with a phi-independent Gamma the phi friction
+    update is u = (u/dt + Cr/eta u_air) / (1/dt + Cr/eta) (bubble.cu:1879-1935,
+    2041-2063, 1276-1295)."""
+    n = 16
+    nph = 2 * n
+    h = math.pi / n
+    st, ua, va = _rand_state(n, 12, u_scale=1.0)
+    eta = 0.8
+    st["eta"][:] = eta
+    st["gamma"][:] = 2.0
+    p = oracle.params(n, 0.01, time_s=0.9, gravity=True)
+    uair, vair = oracle.airflow(p, ua, va)
+    rng = np.random.default_rng(0)
+    delta = rng.uniform(-0.5, 0.5, n)               # per theta row
+    gnext = (2.0 * (1 - delta))[:, None].repeat(nph, axis=1).ravel()
+    out = oracle.update(p, st, gnext, uair, vair)
+    dt, Cr, M = p.dt, p.Cr, p.M
+    assert np.allclose(out["eta"], eta * (1 - np.repeat(delta, nph)), rtol=1e-13, atol=1e-15)
+    inv = 1.0 / eta
+    want_u = (st["u_phi"] / dt + Cr * inv * uair) / (1 / dt + Cr * inv)
+    assert np.allclose(out["u_phi"], want_u, rtol=1e-13, atol=1e-14)
+    # theta: gradient of Gamma between rows j and j+1, gravity g sin(theta_face)
+    g = gnext.reshape(n, nph)
+    dG = ((g[1:] - g[:-1]) / h).ravel()
+    sin_f = np.repeat(np.sin((np.arange(n - 1) + 1.0) * h), nph)
+    want_v = (st["u_theta"] / dt - M * inv * dG + Cr * inv * vair + p.gs * sin_f) / (1 / dt + Cr * inv)
+    assert np.allclose(out["u_theta"], want_v, rtol=1e-12, atol=1e-13)
+
+
+def test_update_broken_and_growth(oracle):
+    n = 8
+    nph = 2 * n
+    st, ua, va = _rand_state(n, 13)
+    p = oracle.params(n, 0.01)
+    uair, vair = oracle.airflow(p, ua, va)
+    gnext = st["gamma"].copy()
+    gnext[5] = p.gamma_max               # broken: eta -> 0, Gamma -> gamma_max
+    gnext[6] = -1.0                      # non-positive: broken as well
+    gnext[7] = 3.0 * st["gamma"][7]      # min(G, 2 G*) / G* = 2: div = -1/dt
+    out = oracle.update(p, st, gnext, uair, vair)
+    assert out["eta"][5] == 0.0 and out["gamma"][5] == p.gamma_max
+    assert out["eta"][6] == 0.0 and out["gamma"][6] == p.gamma_max
+    assert abs(out["eta"][7] - 2.0 * st["eta"][7]) <= 1e-14
+    # evaporation adds eva dt to every cell and clamps at 0
+    pe = oracle.params(n, 0.01, evaporation=True)
+    g2 = st["gamma"].copy()
+    oute = oracle.update(pe, st, g2, uair, vair)
+    assert np.allclose(oute["eta"], np.maximum(0.0, st["eta"] + pe.eva * pe.dt), rtol=1e-14)
+
+
+def test_update_nan_on_empty_faces(oracle):
+    """Two zero-thickness cells make 1/eta_face infinite: the face velocity
+    becomes NaN (bubble.cu:1849, 1907)."""
+    n = 8
+    nph = 2 * n
+    st, ua, va = _rand_state(n, 14)
+    st["eta"][2 * nph + 3] = st["eta"][2 * nph + 4] = 0.0
+    p = oracle.params(n, 0.01)
+    uair, vair = oracle.airflow(p, ua, va)
+    out = oracle.update(p, st, st["gamma"].copy(), uair, vair)
+    assert np.isnan(out["u_phi"][2 * nph + 4])
+    assert np.isfinite(out["u_phi"][2 * nph + 3]) and np.isfinite(out["u_phi"][2 * nph + 5])
+
+
+# ----------------------------------------------------- derived numbers ----
+def test_derived_parameters(oracle):
+    """Bubble::Bubble / run (bubble.cu:2573-2580, 2627-2638) with the
+    config.txt defaults: M = Gm R T / (3 rho H U^2), Cr = rho_a sqrt(nu_a r /
+    U) / (rho H), gamma_max = sigma_a / (R T) / Gm, gs = g r / U^2."""
+    p = oracle.params(64, 0.005, evaporation=True, gravity=True)
+    R, T, Gm, rho, H, U, r = 8.3144598, 298.15, 1e-6, 1e3, 1e-6, 1.0, 1.0
+    assert abs(p.M - Gm * R * T / (3 * rho * H * U * U)) <= 1e-15
+    assert abs(p.M - 0.826319) <= 5e-7
+    assert abs(p.Cr - 1.184 * math.sqrt(1.562e-5 * r / U) / (rho * H)) <= 1e-12
+    assert abs(p.Cr - 4.679422) <= 5e-7
+    assert abs(p.gamma_max - 0.07275 / (R * T) / Gm) <= 1e-12
+    assert abs(p.gamma_max - 29.347029) <= 5e-7
+    assert p.gs == 9.8 and abs(p.dt - 0.005) <= 1e-18
+    assert abs(p.eva - (-3e-8 * r / (H * U))) <= 1e-15
+
+
+def test_volume_quadrature(oracle):
+    """sum sin(theta_j) h^2 over the grid is the midpoint rule for the area
+    of the unit sphere, 4 pi (1 + O(h^2))."""
+    errs = []
+    for n in (16, 32, 64):
+        p = oracle.params(n, 0.01)
+        v = oracle.volume(p, np.ones(2 * n * n))
+        errs.append(abs(v - 4 * math.pi))
+    assert errs[-1] < 2e-3 and errs[0] / errs[1] > 3.9 and errs[1] / errs[2] > 3.9
