@@ -69,6 +69,20 @@ def advect_bytes(n_theta, esz):
     return esz * (2 * (n_theta - 1) * nph + 2 * n_theta * nph + 4 * n_theta * nph)
 
 
+def copy_calibration(mib):
+    """A plain device copy of known size: its FETCH_SIZE / WRITE_SIZE counter
+    readings against the bytes it must move give the counters' scale."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": 4 * n, "launches": 3}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -80,11 +94,19 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--profile", choices=("graph", "eager"), default=None,
+                    help="run only the headline steps (for rocprofv3), graph or eager launches")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="run only a device-to-device copy of MIB MiB three times (torch), the "
+                         "reference kernel that scales the profiler's HBM byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(bubbles would shard across GPUs without communication).", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
 
     import soapbubble_amd as sb
 
@@ -93,6 +115,15 @@ def main():
     cells = n * nph
     seed = 2000
     st0, ua, va = make_inputs(n, seed, 64, 12, 0.05)
+
+    if args.profile:
+        ctx = new_ctx("mixed", n, dt, st0, ua, va, args.profile == "graph")
+        el = timed_run(ctx, args.warmup, args.steps)
+        s = ctx.stats(0)
+        ctx.close()
+        print(json.dumps({"profile": args.profile, "steps": args.steps, "warmup": args.warmup,
+                          "ms_per_step": el * 1e3 / args.steps, "failed_solves": s["failed_solves"]}))
+        return
 
     # headline: mixed precision, hipGraph per step
     ctx = new_ctx("mixed", n, dt, st0, ua, va, True)

@@ -206,6 +206,57 @@ struct Field {
     __device__ __forceinline__ T belt(int row, int lo, T a) const { return belt<true>(row, lo, a); }
 };
 
+// ------------------------------------------- FP32 interior fast path ----
+// Away from the polar rows every sampler reduces to one bilinear
+// interpolation between two adjacent rows with no reflection (validateCoord
+// leaves theta alone, no pole branch applies). For FP32 fields without NaNs
+// that case is evaluated directly: the phi wrap by one conditional add or
+// subtract (y + nPhi stays unrounded where the reference's fmod(y + nPhi,
+// nPhi) would round it at the magnitude of nPhi), byte offsets into a
+// buffer resource, and the next row through the scalar offset. A buffer
+// load outside the field returns 0, so no position, however malformed, can
+// read outside it.
+struct FastIdx {
+    int off, off1;   // byte offsets of the lower row's two phi neighbours
+    float aT, aP;
+};
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t field_rsrc(const float* p, int rows, int nph) {
+    return __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, rows * nph * 4, 0x00020000);
+}
+
+__device__ __forceinline__ float bload(__amdgpu_buffer_rsrc_t r, int off, int soff) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, soff, 0));
+}
+
+// x, y: sampler-relative index coordinates, x in [0, rows - 1)
+__device__ __forceinline__ FastIdx fast_index(float x, float y, int nph, int mask) {
+    const float tp = (float)nph;
+    y = y < 0.0f ? y + tp : y;
+    y = y >= tp ? y - tp : y;
+    const float fx = floorf(x), fy = floorf(y);
+    FastIdx q;
+    q.aT = x - fx;
+    q.aP = y - fy;
+    const int tI = (int)fx, pI = (int)fy;
+    q.off = (tI * nph + pI) * 4;
+    q.off1 = pI == mask ? q.off - mask * 4 : q.off + 4;
+    return q;
+}
+
+__device__ __forceinline__ float fast_bilerp(__amdgpu_buffer_rsrc_t r, const FastIdx& q, int row_bytes) {
+    const float v00 = bload(r, q.off, 0), v01 = bload(r, q.off1, 0);
+    const float v10 = bload(r, q.off, row_bytes), v11 = bload(r, q.off1, row_bytes);
+    const float lo = (1.0f - q.aP) * v00 + q.aP * v01;
+    const float hi = (1.0f - q.aP) * v10 + q.aP * v11;
+    return (1.0f - q.aT) * lo + q.aT * hi;
+}
+
+template <typename T, bool NANS>
+struct FastPath { static constexpr bool on = false; };
+template <>
+struct FastPath<float, false> { static constexpr bool on = true; };
+
 // Split a sample position into indices and weights; the phi index is
 // clamped into range so a NaN position cannot address memory outside the
 // field.
@@ -222,6 +273,11 @@ __device__ __forceinline__ void split_index(T x, T y, int nph, int& tI, int& pI,
 // sampleCentered (bubble.cu:320-376)
 template <typename T, bool NANS = true>
 __device__ T sample_centered(const Field<T>& f, int nth, T rx, T ry) {
+    if constexpr (FastPath<T, NANS>::on) {
+        if (rx >= 0.5f && rx < (float)nth - 0.5f)
+            return fast_bilerp(field_rsrc(f.p, nth, f.nph), fast_index(rx - 0.5f, ry, f.nph, f.mask),
+                               f.nph * 4);
+    }
     T x = rx - T(0.5), y = ry;
     const int nph = f.nph;
     bool fl = validate_coord(x, y, nth, nph);
@@ -248,6 +304,14 @@ __device__ T sample_centered(const Field<T>& f, int nth, T rx, T ry) {
 template <typename T, bool NANS = true>
 __device__ void sample_centered2(const Field<T>& f, const Field<T>& g, int nth, T rx, T ry, T& of,
                                  T& og) {
+    if constexpr (FastPath<T, NANS>::on) {
+        if (rx >= 0.5f && rx < (float)nth - 0.5f) {
+            const FastIdx q = fast_index(rx - 0.5f, ry, f.nph, f.mask);
+            of = fast_bilerp(field_rsrc(f.p, nth, f.nph), q, f.nph * 4);
+            og = fast_bilerp(field_rsrc(g.p, nth, g.nph), q, g.nph * 4);
+            return;
+        }
+    }
     T x = rx - T(0.5), y = ry;
     const int nph = f.nph;
     bool fl = validate_coord(x, y, nth, nph);
@@ -268,6 +332,11 @@ __device__ void sample_centered2(const Field<T>& f, const Field<T>& g, int nth, 
 // sampleVPhi (bubble.cu:178-240): values taken across the pole flip sign.
 template <typename T, bool NANS = true>
 __device__ T sample_vphi(const Field<T>& f, int nth, T rx, T ry) {
+    if constexpr (FastPath<T, NANS>::on) {
+        if (rx >= 0.5f && rx < (float)nth - 0.5f)
+            return fast_bilerp(field_rsrc(f.p, nth, f.nph),
+                               fast_index(rx - 0.5f, ry + 0.5f, f.nph, f.mask), f.nph * 4);
+    }
     T x = rx - T(0.5), y = ry + T(0.5);
     const int nph = f.nph;
     bool fl = validate_coord(x, y, nth, nph);
@@ -297,6 +366,11 @@ __device__ T sample_vphi(const Field<T>& f, int nth, T rx, T ry) {
 // the array for) takes the south-cap branch.
 template <typename T, bool NANS = true>
 __device__ T sample_vtheta(const Field<T>& f, int nth, T rx, T ry) {
+    if constexpr (FastPath<T, NANS>::on) {
+        if (rx >= 1.0f && rx < (float)nth - 1.0f)
+            return fast_bilerp(field_rsrc(f.p, nth - 1, f.nph),
+                               fast_index(rx - 1.0f, ry, f.nph, f.mask), f.nph * 4);
+    }
     T x = rx - T(1), y = ry;
     const int nph = f.nph;
     bool fl = validate_coord(x, y, nth, nph);
