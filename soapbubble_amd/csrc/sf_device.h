@@ -138,6 +138,16 @@ __device__ __forceinline__ void norm_inv(double s, double& n, double& inv) {
     n = sqrt(s);
     inv = 1.0 / n;
 }
+// FP64 reciprocal of a positive finite y: the hardware estimate refined by
+// two Newton steps r += r (1 - y r), each of which squares the relative
+// error; the result is within an ulp or two of 1 / y.
+__device__ __forceinline__ double rcp64(double y) {
+    double r = __builtin_amdgcn_rcp(y);
+    double e = fma(-y, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-y, r, 1.0);
+    return fma(r, e, r);
+}
 __device__ __forceinline__ float trcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ double trcp(double x) { return 1.0 / x; }
 

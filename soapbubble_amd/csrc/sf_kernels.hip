@@ -219,6 +219,7 @@ __device__ __forceinline__ void row_system(const BubbleDev& B, double t, const T
     const double CrDt = Cr * dt, MDt = B.M * dt, s2 = B.invh * B.invh;
     const double sinN = tab.sinFace[B.rt0 + j], sinS = tab.sinFace[B.rt0 + j + 1];
     const double sinTheta = tab.sinTc[B.rt0 + j], csc = tab.cscTc[B.rt0 + j];
+    const double cW = s2 * csc * MDt, cN = s2 * sinN * MDt, cS = s2 * sinS * MDt;
 
     bool bW = is_broken(gW, eW, gmax), bE = is_broken(gE, eE, gmax);
     bool bN[C], bS[C];
@@ -231,19 +232,28 @@ __device__ __forceinline__ void row_system(const BubbleDev& B, double t, const T
         bN[k] = hasN && is_broken((double)gNv[k], (double)eNv[k], gmax);
         bS[k] = hasS && is_broken((double)gSv[k], (double)eSv[k], gmax);
     }
+    // Every face term of concentrationLinearSystemKernel has the factor
+    // 1 / (eta_f + Cr dt): the weight s2 sin M dt / (eta_f + Cr dt), the
+    // film term u eta_f / (eta_f + Cr dt), the air term u_air / (eta_f / Cr
+    // / dt + 1) = u_air Cr dt / (eta_f + Cr dt) and the gravity term
+    // eta_f / (eta_f / dt + Cr) = eta_f dt / (eta_f + Cr dt). One reciprocal
+    // per face serves all of them (a few ulp from the reference's divisions).
     // phi faces: face k lies between cells k-1 and k
-    double wFr[C + 1], etaF[C + 1];
+    double wFr[C + 1], etaF[C + 1], invF[C + 1];
 #pragma unroll
     for (int k = 0; k <= C; ++k) {
         const double l = k == 0 ? eW : e[k - 1];
         const double r = k == C ? eE : e[k];
         etaF[k] = (l + r) / 2.0;
-        wFr[k] = s2 * csc * MDt / (etaF[k] + CrDt);
+        invF[k] = rcp64(etaF[k] + CrDt);
+        wFr[k] = cW * invF[k];
         const bool bl = k == 0 ? bW : q.brk[k - 1];
         const bool br = k == C ? bE : q.brk[k];
         q.wF[k] = (bl || br) ? 0.0 : wFr[k];
     }
-    double uv[C + 1], vNv[C], vSv[C];
+    double uv[C + 1], vNv[C], vSv[C], uAirF[C + 1];
+    const double kt = 3.0 - fabs(t - 3.0);
+    const double bl = t / 12.0;
     if (RHS) {
         T u[C], vn[C], vs[C];
         ld_row<C>(s.uph + row + i0, u);
@@ -260,16 +270,26 @@ __device__ __forceinline__ void row_system(const BubbleDev& B, double t, const T
 #pragma unroll
             for (int k = 0; k < C; ++k) vSv[k] = (double)vs[k];
         }
+        if (B.airflow) {
+            // the t/12 blend with the phi mirror (bubble.cu:1364-1379), per
+            // phi face, shared by the two cells of the face
+#pragma unroll
+            for (int k = 0; k <= C; ++k) {
+                const int i = (i0 + k) & m, opp = (nph - i) & m;
+                uAirF[k] = 0.5 * (air_at(air.ua0, air.au, row + i, kt) * bl +
+                                  air_at(air.ua0, air.au, row + opp, kt) * (1 - bl));
+            }
+        }
     }
-    const double kt = 3.0 - fabs(t - 3.0);
-    const double bl = t / 12.0;
 #pragma unroll
     for (int k = 0; k < C; ++k) {
         const double eN = (double)eNv[k], eS = (double)eSv[k];
         const double etaUp = (eN + e[k]) / 2.0, etaDown = (eS + e[k]) / 2.0;
         const double etaLeft = etaF[k], etaRight = etaF[k + 1];
-        const double wN = s2 * sinN * MDt / (etaUp + CrDt);
-        const double wS = s2 * sinS * MDt / (etaDown + CrDt);
+        const double invU = rcp64(etaUp + CrDt), invD = rcp64(etaDown + CrDt);
+        const double invL = invF[k], invR = invF[k + 1];
+        const double wN = cN * invU;
+        const double wS = cS * invD;
         const double wW = wFr[k], wE = wFr[k + 1];
         const bool brk = q.brk[k];
         q.wN[k] = (brk || bN[k]) ? 0.0 : wN;
@@ -285,22 +305,16 @@ __device__ __forceinline__ void row_system(const BubbleDev& B, double t, const T
         }
         q.diag[k] = sinTheta * invDt / g[k] - (((-wN) + (-wW)) + (-wE) + (-wS));
         if (!RHS) continue;
-        const int i = i0 + k, iE = (i + 1) & m;
+        const int i = i0 + k;
         double uW = uv[k], uE = uv[k + 1], vN = vNv[k], vS = vSv[k];
         if (__builtin_isnan(vN)) vN = 0.0;
         if (__builtin_isnan(vS)) vS = 0.0;
         if (__builtin_isnan(uW)) uW = 0.0;
         if (__builtin_isnan(uE)) uE = 0.0;
-        double div = B.invh * (uE * etaRight / (etaRight + Cr * dt) -
-                               uW * etaLeft / (etaLeft + Cr * dt) +
-                               (vS * sinS * etaDown / (etaDown + Cr * dt) -
-                                vN * sinN * etaUp / (etaUp + Cr * dt)));
+        double div = B.invh * (uE * etaRight * invR - uW * etaLeft * invL +
+                               (vS * sinS * etaDown * invD - vN * sinN * etaUp * invU));
         if (B.airflow) {
-            const int opp = (nph - i) & m, oppE = (nph - iE) & m;
-            const double uAirW = 0.5 * (air_at(air.ua0, air.au, row + i, kt) * bl +
-                                        air_at(air.ua0, air.au, row + opp, kt) * (1 - bl));
-            const double uAirE = 0.5 * (air_at(air.ua0, air.au, row + iE, kt) * bl +
-                                        air_at(air.ua0, air.au, row + oppE, kt) * (1 - bl));
+            const int opp = (nph - i) & m;
             double vAirN = 0.0, vAirS = 0.0;
             if (hasN) {
                 const size_t rn = row - nph;
@@ -311,15 +325,12 @@ __device__ __forceinline__ void row_system(const BubbleDev& B, double t, const T
                 vAirS = 0.5 * (air_at(air.va0, air.av, row + i, kt) * bl -
                                air_at(air.va0, air.av, row + opp, kt) * (1 - bl));
             }
-            div += B.invh * (uAirE / (etaRight / Cr * invDt + 1.0) -
-                             uAirW / (etaLeft / Cr * invDt + 1.0) +
-                             vAirS * sinS / (etaDown / Cr * invDt + 1.0) -
-                             vAirN * sinN / (etaUp / Cr * invDt + 1.0));
+            div += B.invh * CrDt *
+                   (uAirF[k + 1] * invR - uAirF[k] * invL + vAirS * sinS * invD - vAirN * sinN * invU);
         }
         if (B.gravity) {
-            div += B.gs * B.invh *
-                   (sinS * sinS * etaDown / (invDt * etaDown + Cr) -
-                    sinN * sinN * etaUp / (invDt * etaUp + Cr));
+            div += B.gs * B.invh * dt *
+                   (sinS * sinS * etaDown * invD - sinN * sinN * etaUp * invU);
         }
         double rr = sinTheta * invDt - div;
         if (bN[k]) rr += wN * gmax;
