@@ -28,6 +28,28 @@
  *     identity rows as assembled; their known value gamma_max is moved to
  *     the right-hand side of the neighbouring rows, which leaves the
  *     solution unchanged and makes the system symmetric.
+ *
+ * How each function is pinned (tests/test_oracle_samplers.py,
+ * tests/test_oracle_step.py, tests/test_golden.py):
+ *  samplers     a ghost-row bilinear restatement on 8x16 and 16x32 grids,
+ *               constants, exact node values, periodicity, NaN skipping
+ *  trace        zero velocity, uniform flow along the equator and a
+ *               meridian (exact), O(h^2) convergence for solid-body rotation
+ *  advect       zero-velocity identity, constants, the two grid reflections,
+ *               a geodesic flow along a grid meridian (exact)
+ *  airflow      closed form: 0.5 (3 - |t - 3|) taper(beta) (n x r), t = 0
+ *  col_ind      the neighbour pattern
+ *  assemble     symmetry, row sums, Cholesky, known answer, O(h^2) Y_lm
+ *               consistency, divergence-free cancellation, closed forms of
+ *               the film, air and gravity terms, broken rows
+ *  solve        dense direct solves (with and without broken cells)
+ *  update       per-cell closed forms (friction, div, broken, evaporation)
+ *  derive       closed forms and the published parameter values
+ *  volume       midpoint-rule area of the sphere, O(h^2)
+ * Parity unpinned (reference text only): the phi-mirror half of the t/12
+ * air blend (only its own-index half is pinned, at t = 12); the u_theta
+ * sample exactly at the north pole; floorf_mode (D2) and the float
+ * sinf/cosf of D1, which the oracle does not reproduce.
  */
 #include "sf_oracle.h"
 
