@@ -187,3 +187,49 @@ def test_thickness_exr_roundtrip(tmp_path):
     want = want.astype(np.float32)
     for c in chans:
         assert np.array_equal(img[c], want), c
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-9), ("mixed", 5e-5)])
+def test_packed_ensemble(oracle, prec, tol):
+    """Bubbles of different sizes, steps, air and Marangoni numbers in one
+    context (the packed ensemble): each matches its own oracle run."""
+    specs = [(16, 0.005, 1.0, 31), (64, 0.004, 1.7, 32), (32, 0.002, 0.6, 33), (16, 0.003, 1.2, 34)]
+    _say(f"ensemble of {len(specs)} bubbles {prec}")
+    steps = 5
+    descs, inputs, refs = [], [], []
+    for n, dt, ms, seed in specs:
+        st, ua, va = make_case(n, seed, u_scale=0.7)
+        descs.append(sb.bubble(n, dt, M_scale=ms))
+        inputs.append((st, ua, va))
+        p = oracle.params(n, dt, M_scale=ms)
+        ref = copy_state(st)
+        oracle.step(p, ref, ua, va, steps)
+        refs.append(ref)
+    with sb.SoapFilm(descs, prec) as ctx:
+        for b, (st, ua, va) in enumerate(inputs):
+            load(ctx, b, st, ua, va)
+        ctx.step(steps)
+        for b, ref in enumerate(refs):
+            got = ctx.state(b)
+            s = ctx.stats(b)
+            assert s["failed_solves"] == 0
+            for k in ("u_theta", "u_phi", "eta", "gamma"):
+                e = rel_l2(got[k], ref[k])
+                assert e < tol, (b, k, e)
+
+
+def test_graph_replay_matches_eager():
+    """The captured step replayed many times equals eager stepping."""
+    n, dt = 64, 0.004
+    st, ua, va = make_case(n, 41, u_scale=0.5)
+    out = []
+    for use_graph in (False, True):
+        with sb.SoapFilm(sb.bubble(n, dt), "mixed", use_graph=use_graph) as ctx:
+            load(ctx, 0, st, ua, va)
+            for _ in range(4):
+                ctx.step(3)
+                ctx.synchronize()
+            out.append(ctx.state(0))
+            assert ctx.stats(0)["failed_solves"] == 0
+    for k in ("u_theta", "u_phi", "eta", "gamma"):
+        assert rel_l2(out[1][k], out[0][k]) < 1e-6, k
