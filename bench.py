@@ -192,6 +192,13 @@ def main():
         err_oracle.update({kk: rel_l2(g[kk], st[kk]) for kk in ("eta", "gamma", "u_theta",
                                                                  "u_phi")})
 
+    # HBM bytes per advection launch measured by rocprofv3 for this workload
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "hbm_traffic_1024x2048_mixed.json")
+    if n == 1024 and os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f)["kernels"]["k_advect<float>"]["traffic_bytes"]
+
     out = {
         "metric": "cell-steps/s (advect+implicit solve) at stated err vs FP64 oracle; % HBM peak",
         "value": value,
@@ -211,7 +218,7 @@ def main():
                                 "iterative refinement to rel. residual 1e-10"},
         "roofline": {"bound": "hbm", "kernel": "k_advect<float>", "achieved": adv_gbs,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": adv_gbs / HBM_PEAK_GBS,
-                     "traffic": None, "bytes_per_launch": adv_bytes, "avg_ms": adv_ms},
+                     "traffic": traffic, "bytes_per_launch": adv_bytes, "avg_ms": adv_ms},
         "cpu_baseline": cpu,
         "stage_ms": stage_ms,
         "solver": {"max_residual": mstats["max_residual"], "failed_solves": mstats["failed_solves"],
