@@ -153,6 +153,11 @@ int sf_solve(sf_ctx* ctx);
 int sf_update(sf_ctx* ctx);
 
 int sf_synchronize(sf_ctx* ctx);
+/* Average device time of one advection launch over nlaunch back-to-back
+ * launches between two events on the context's stream (the launches
+ * recompute the advected fields from the unchanged state; call at the start
+ * of a step). A measurement aid with no counterpart in the reference. */
+int sf_time_advection(sf_ctx* ctx, int32_t nlaunch, double* avg_ms);
 /* Solver statistics and Solver::isBroken (solver.cu:296-298, 568-574). */
 int sf_get_stats(sf_ctx* ctx, int32_t bubble, sf_bubble_stats* out);
 /* Device time per stage, accumulated since creation or the last reset. */
@@ -173,6 +178,16 @@ int sf_derive_params(const sf_bubble_desc* desc, sf_derived* out);
  * initAirflowfromPic (initializer.cu:50-135). Arrays sized as the fields. */
 int sf_synth_fields(int32_t n_theta, uint64_t seed, int32_t l_max, int32_t l_air, double noise,
                     double air_amp, double* eta, double* gamma, double* uair0, double* vair0);
+/* initThicknessfromPic / initAirflowfromPic (initializer.cu:50-135), host
+ * only: an uncompressed HALF/FLOAT EXR texture (channels B, G, R) resized to
+ * nPhi x nTheta by bilinear interpolation with aligned pixel centres and
+ * replicated borders (OpenCV INTER_LINEAR semantics); eta = (B - 0.5) 0.3 +
+ * 0.5; uair0 = (G_i + G_{i-1}) / 2 * radius; vair0 = -(R_j + R_{j+1}) / 2 *
+ * radius. As in the reference, a missing or unreadable file leaves the
+ * output as it was; *loaded gets bit 0 for the thickness and bit 1 for the
+ * air texture actually read. Any output pointer may be NULL. */
+int sf_texture_fields(const char* thickness_exr, const char* air_exr, int32_t n_theta,
+                      double radius, double* eta, double* uair0, double* vair0, int32_t* loaded);
 /* Real spherical harmonic Y_lm(theta, phi), orthonormal on the unit sphere. */
 double sf_real_ylm(int32_t l, int32_t m, double theta, double phi);
 

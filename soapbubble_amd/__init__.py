@@ -36,8 +36,9 @@ STATUS = {0: "SF_OK", 1: "SF_ERR_INVALID", 2: "SF_ERR_HIP", 3: "SF_ERR_UNSUPPORT
 EXPORTS = (
     "sf_create", "sf_destroy", "sf_upload_field", "sf_download_field", "sf_set_time",
     "sf_set_dt", "sf_step", "sf_advect", "sf_assemble", "sf_solve", "sf_update",
-    "sf_synchronize", "sf_get_stats", "sf_get_timing", "sf_reset_timing", "sf_last_error",
-    "sf_derive_params", "sf_synth_fields", "sf_real_ylm", "sf_write_thickness_exr",
+    "sf_synchronize", "sf_time_advection", "sf_get_stats", "sf_get_timing", "sf_reset_timing", "sf_last_error",
+    "sf_derive_params", "sf_synth_fields", "sf_texture_fields", "sf_real_ylm",
+    "sf_write_thickness_exr",
 )
 
 
@@ -106,6 +107,7 @@ def load_library(path: str = LIB_PATH):
         "sf_solve": (ctypes.c_int, [vp]),
         "sf_update": (ctypes.c_int, [vp]),
         "sf_synchronize": (ctypes.c_int, [vp]),
+        "sf_time_advection": (ctypes.c_int, [vp, ctypes.c_int32, P(ctypes.c_double)]),
         "sf_get_stats": (ctypes.c_int, [vp, ctypes.c_int32, P(BubbleStats)]),
         "sf_get_timing": (ctypes.c_int, [vp, P(Timing)]),
         "sf_reset_timing": (ctypes.c_int, [vp]),
@@ -114,6 +116,8 @@ def load_library(path: str = LIB_PATH):
         "sf_synth_fields": (ctypes.c_int, [ctypes.c_int32, ctypes.c_uint64, ctypes.c_int32,
                                            ctypes.c_int32, ctypes.c_double, ctypes.c_double,
                                            dp, dp, dp, dp]),
+        "sf_texture_fields": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int32,
+                                             ctypes.c_double, dp, dp, dp, P(ctypes.c_int32)]),
         "sf_real_ylm": (ctypes.c_double, [ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
                                           ctypes.c_double]),
         "sf_write_thickness_exr": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.c_char_p]),
@@ -167,6 +171,26 @@ def synth_fields(n_theta: int, seed: int, l_max: int = 4, l_air: int = 6, noise:
                              _ptr(out["gamma"]), _ptr(out["uair0"]), _ptr(out["vair0"]))
     if rc:
         raise SoapFilmError(rc, "sf_synth_fields")
+    return out
+
+
+def texture_fields(n_theta: int, thickness_exr: str | None = None, air_exr: str | None = None,
+                   radius: float = 1.0) -> dict:
+    """Fields from the reference's init textures (host only). Starts from
+    the reference's defaults (eta = 0.5, air = 0) where a texture is absent
+    or unreadable; "loaded" tells which textures were read."""
+    lib = load_library()
+    nph = 2 * n_theta
+    out = {"eta": np.full(n_theta * nph, 0.5), "uair0": np.zeros(n_theta * nph),
+           "vair0": np.zeros((n_theta - 1) * nph)}
+    got = ctypes.c_int32(0)
+    enc = lambda s: s.encode() if s else None
+    rc = lib.sf_texture_fields(enc(thickness_exr), enc(air_exr), n_theta, radius,
+                               _ptr(out["eta"]), _ptr(out["uair0"]), _ptr(out["vair0"]),
+                               ctypes.byref(got))
+    if rc:
+        raise SoapFilmError(rc, "sf_texture_fields")
+    out["loaded"] = got.value
     return out
 
 
@@ -257,6 +281,12 @@ class SoapFilm:
 
     def update(self):
         self._check(self._lib.sf_update(self._h))
+
+    def time_advection(self, nlaunch: int = 20) -> float:
+        """Average device time (ms) of one advection launch."""
+        out = ctypes.c_double()
+        self._check(self._lib.sf_time_advection(self._h, nlaunch, ctypes.byref(out)))
+        return out.value
 
     def synchronize(self):
         self._check(self._lib.sf_synchronize(self._h))

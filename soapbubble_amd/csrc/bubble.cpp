@@ -1,0 +1,324 @@
+// C++ Solver / Bubble surface over the C ABI (see bubble.h).
+#include "bubble.h"
+
+#include <cctype>
+#include <cmath>
+#include <cstdio>
+#include <filesystem>
+#include <fstream>
+#include <iostream>
+#include <map>
+#include <memory>
+#include <sstream>
+
+namespace sfb {
+
+// ------------------------------------------------------------- config ----
+bool read_config(const std::string& path, Config& c, std::string& err) {
+    std::ifstream fin(path);
+    if (!fin) {
+        err = "cannot open " + path;
+        return false;
+    }
+    std::string attrib;
+    auto num = [&](const char* what, auto& v) {
+        if (!(fin >> attrib >> v)) {
+            err = std::string("config entry ") + what + " is missing or malformed";
+            return false;
+        }
+        return true;
+    };
+    // kernel/main.cu:22-38, in order
+    if (!num("radius", c.radius) || !num("H", c.H) || !num("U", c.U) || !num("cm", c.c_m) ||
+        !num("Gammam", c.Gamma_m) || !num("T", c.T) || !num("Ds", c.Ds) || !num("rm", c.rm) ||
+        !num("numberOfGridsInTheta", c.nTheta) || !num("timePerStep", c.dt) ||
+        !num("timePerFrame", c.DT) || !num("frameNumbers", c.frames) ||
+        !num("outputDir", c.outputDir) || !num("thicknessInit", c.thicknessImage) ||
+        !num("AMGconfig", c.AMGconfig) || !num("blendCoeff", c.blendCoeff) ||
+        !num("gpu", c.device))
+        return false;
+    if (c.thicknessImage == "null") c.thicknessImage = "";   // main.cu:40-43
+    return true;
+}
+
+// --------------------------------------------------------------- JSON ----
+namespace {
+
+struct Json {
+    enum Kind { Null, Bool, Num, Str, Arr, Obj } kind = Null;
+    double num = 0;
+    bool b = false;
+    std::string str;
+    std::vector<Json> arr;
+    std::vector<std::pair<std::string, Json>> obj;
+    const Json* get(const std::string& k) const {
+        for (auto& kv : obj)
+            if (kv.first == k) return &kv.second;
+        return nullptr;
+    }
+};
+
+struct JsonParser {
+    const std::string& s;
+    size_t p = 0;
+    std::string err;
+    explicit JsonParser(const std::string& t) : s(t) {}
+    void ws() {
+        while (p < s.size() && std::isspace((unsigned char)s[p])) ++p;
+    }
+    bool lit(const char* w) {
+        const size_t n = std::char_traits<char>::length(w);
+        if (s.compare(p, n, w) == 0) { p += n; return true; }
+        return false;
+    }
+    bool string(std::string& out) {
+        if (p >= s.size() || s[p] != '"') return false;
+        ++p;
+        out.clear();
+        while (p < s.size() && s[p] != '"') {
+            if (s[p] == '\\' && p + 1 < s.size()) ++p;
+            out += s[p++];
+        }
+        if (p >= s.size()) return false;
+        ++p;
+        return true;
+    }
+    bool value(Json& v, int depth) {
+        if (depth > 64) return false;
+        ws();
+        if (p >= s.size()) return false;
+        const char ch = s[p];
+        if (ch == '{') {
+            ++p;
+            v.kind = Json::Obj;
+            ws();
+            if (p < s.size() && s[p] == '}') { ++p; return true; }
+            for (;;) {
+                ws();
+                std::string k;
+                if (!string(k)) return false;
+                ws();
+                if (p >= s.size() || s[p] != ':') return false;
+                ++p;
+                Json e;
+                if (!value(e, depth + 1)) return false;
+                v.obj.emplace_back(k, std::move(e));
+                ws();
+                if (p < s.size() && s[p] == ',') { ++p; continue; }
+                if (p < s.size() && s[p] == '}') { ++p; return true; }
+                return false;
+            }
+        }
+        if (ch == '[') {
+            ++p;
+            v.kind = Json::Arr;
+            ws();
+            if (p < s.size() && s[p] == ']') { ++p; return true; }
+            for (;;) {
+                Json e;
+                if (!value(e, depth + 1)) return false;
+                v.arr.push_back(std::move(e));
+                ws();
+                if (p < s.size() && s[p] == ',') { ++p; continue; }
+                if (p < s.size() && s[p] == ']') { ++p; return true; }
+                return false;
+            }
+        }
+        if (ch == '"') { v.kind = Json::Str; return string(v.str); }
+        if (lit("true")) { v.kind = Json::Bool; v.b = true; return true; }
+        if (lit("false")) { v.kind = Json::Bool; return true; }
+        if (lit("null")) { v.kind = Json::Null; return true; }
+        const char* b = s.c_str() + p;
+        char* e = nullptr;
+        v.num = std::strtod(b, &e);
+        if (e == b) return false;
+        v.kind = Json::Num;
+        p += (size_t)(e - b);
+        return true;
+    }
+};
+
+}  // namespace
+
+bool read_amg_settings(const std::string& path, SolverSettings& out, std::string& err) {
+    std::ifstream f(path);
+    if (!f) {
+        err = "cannot open " + path;
+        return false;
+    }
+    std::stringstream ss;
+    ss << f.rdbuf();
+    const std::string text = ss.str();
+    JsonParser jp(text);
+    Json root;
+    if (!jp.value(root, 0) || root.kind != Json::Obj) {
+        err = "malformed JSON in " + path;
+        return false;
+    }
+    const Json* sol = root.get("solver");
+    if (!sol || sol->kind != Json::Obj) {
+        err = "no \"solver\" object in " + path;
+        return false;
+    }
+    if (const Json* t = sol->get("tolerance"); t && t->kind == Json::Num) out.tolerance = t->num;
+    if (const Json* c = sol->get("convergence"); c && c->kind == Json::Str)
+        out.absolute = c->str == "ABSOLUTE";
+    if (const Json* m = sol->get("max_iters"); m && m->kind == Json::Num) out.max_iters = (int)m->num;
+    return true;
+}
+
+// ------------------------------------------------------------- Solver ----
+void Solver::check(int rc) {
+    if (rc != SF_OK) throw Error(rc, sf_last_error(ctx_));
+}
+
+Solver::Solver(size_t nPhi, size_t nTheta, double radius, double dt, double H, int device,
+               const std::string& AMGconfig, sf_precision precision, double Gamma_m, double T)
+    : nPhi_(nPhi), nTheta_(nTheta), radius_(radius), dt_(dt) {
+    if (nPhi != 2 * nTheta) throw Error(SF_ERR_UNSUPPORTED, "nPhi must be 2 nTheta");
+    SolverSettings st;
+    std::string err;
+    if (!AMGconfig.empty() && !read_amg_settings(AMGconfig, st, err))
+        std::cerr << err << ", using the defaults" << std::endl;
+    std::cout << "AMG config file: " << AMGconfig << " (tolerance " << st.tolerance
+              << (st.absolute ? " ABSOLUTE" : " RELATIVE") << ", max_iters " << st.max_iters << ")"
+              << std::endl;
+    sf_bubble_desc d{};
+    d.n_theta = (int32_t)nTheta;
+    // gravity, evaporation and airflow are defined in header.cuh:94-96
+    d.gravity = 1;
+    d.evaporation = 1;
+    d.airflow = 1;
+    d.radius = radius;
+    d.H = H;
+    d.U = 1.0;          // Bubble assumes U = 1 (bubble.cu:2566)
+    d.Gamma_m = Gamma_m;
+    d.T = T;
+    d.dt = dt;
+    d.M_scale = 1.0;
+    sf_options o{};
+    o.precision = precision;
+    o.device = device;
+    o.use_graph = 0;
+    o.tol = st.tolerance;
+    o.tol_absolute = st.absolute ? 1 : 0;
+    o.max_inner = st.max_iters;
+    o.advance_time = 0;   // the caller sets the frame-start time
+    check(sf_create(&o, 1, &d, &ctx_));
+}
+
+Solver::~Solver() {
+    if (ctx_) sf_destroy(ctx_);
+}
+
+void Solver::initThicknessfromPic(const std::string& path) {
+    const size_t N = nPhi_ * nTheta_;
+    std::vector<double> eta(N, 0.5);
+    int32_t got = 0;
+    check(sf_texture_fields(path.c_str(), nullptr, (int32_t)nTheta_, radius_, eta.data(), nullptr,
+                            nullptr, &got));
+    if (!(got & 1)) std::cout << "No thickness image provided, initialize with eta = 0.5" << std::endl;
+    check(sf_upload_field(ctx_, 0, SF_FIELD_ETA, eta.data()));
+}
+
+void Solver::initAirflowfromPic(const std::string& path) {
+    const size_t N = nPhi_ * nTheta_;
+    std::vector<double> u(N, 0.0), v(N - nPhi_, 0.0);
+    int32_t got = 0;
+    check(sf_texture_fields(nullptr, path.c_str(), (int32_t)nTheta_, radius_, nullptr, u.data(),
+                            v.data(), &got));
+    if (!(got & 2)) std::cout << "No airflow image provided, initialize with zero" << std::endl;
+    check(sf_upload_field(ctx_, 0, SF_FIELD_UAIR0, u.data()));
+    check(sf_upload_field(ctx_, 0, SF_FIELD_VAIR0, v.data()));
+}
+
+void Solver::initWithConst(sf_field field, double val) {
+    const size_t n = (field == SF_FIELD_UTHETA || field == SF_FIELD_VAIR0) ? (nTheta_ - 1) * nPhi_
+                                                                           : nTheta_ * nPhi_;
+    std::vector<double> h(n, val);
+    check(sf_upload_field(ctx_, 0, field, h.data()));
+}
+
+void Solver::stepForward() {
+    check(sf_step(ctx_, 1));
+    sf_bubble_stats s;
+    check(sf_get_stats(ctx_, 0, &s));
+    if (!s.converged) broken_ = true;   // AlgebraicMultiGridCG (bubble.cu:1500-1507)
+}
+
+void Solver::stepForward(double dt) {
+    check(sf_set_dt(ctx_, 0, dt));
+    stepForward();
+    check(sf_set_dt(ctx_, 0, dt_));
+}
+
+bool Solver::isBroken() {
+    sf_bubble_stats s;
+    check(sf_get_stats(ctx_, 0, &s));
+    return broken_ || s.broken;
+}
+
+void Solver::setTime(double t) { check(sf_set_time(ctx_, 0, t)); }
+
+void Solver::write_thickness_img(const std::string& s, int frame) {
+    char name[32];
+    std::snprintf(name, sizeof name, "frame%04d.exr", frame);
+    check(sf_write_thickness_exr(ctx_, 0, (s + name).c_str()));
+}
+
+void Solver::copyToCPU(sf_field field, std::vector<double>& out) {
+    const size_t n = (field == SF_FIELD_UTHETA || field == SF_FIELD_VAIR0 ||
+                      field == SF_FIELD_UTHETA_STAR)
+                         ? (nTheta_ - 1) * nPhi_
+                         : nTheta_ * nPhi_;
+    out.assign(n, 0.0);
+    check(sf_download_field(ctx_, 0, field, out.data()));
+}
+
+// ------------------------------------------------------------- Bubble ----
+Bubble::Bubble(double radius, double H, double U, double c_m, double Gamma_m, double T, double Ds,
+               double rm, size_t nTheta, double dt, double DT, int frames, std::string outputDir,
+               std::string thicknessImage, int device, std::string AMGconfig, double blendCoeff)
+    : radius_(radius), H_(H), Gamma_m_(Gamma_m), T_(T), dt_(dt), DT_(DT), nTheta_(nTheta),
+      frames_(frames), device_(device), outputDir_(std::move(outputDir)),
+      thicknessImage_(std::move(thicknessImage)), AMGconfig_(std::move(AMGconfig)) {
+    // U is taken as 1 as in the reference (bubble.cu:2566); c_m, Ds, rm and
+    // blendCoeff do not enter the plain step
+    (void)U; (void)c_m; (void)Ds; (void)rm; (void)blendCoeff;
+    if (!outputDir_.empty() && outputDir_.back() != '/') outputDir_ += '/';
+}
+
+Bubble::Bubble(const Config& c)
+    : Bubble(c.radius, c.H, c.U, c.c_m, c.Gamma_m, c.T, c.Ds, c.rm, (size_t)c.nTheta, c.dt, c.DT,
+             c.frames, c.outputDir, c.thicknessImage, c.device, c.AMGconfig, c.blendCoeff) {}
+
+int Bubble::run() {
+    if (!outputDir_.empty()) std::filesystem::create_directories(outputDir_);
+    Solver solver(2 * nTheta_, nTheta_, radius_, dt_, H_, device_, AMGconfig_, precision, Gamma_m_,
+                  T_);
+    solver.initThicknessfromPic(thicknessImage_);
+    solver.initAirflowfromPic(airImage);
+    solver.write_thickness_img(outputDir_, 0);
+
+    const double eps = 1e-7;   // bubble.cu:28
+    double T = 0.0;
+    int i = 1;
+    for (; i < frames_; ++i) {
+        solver.setTime(T);
+        while ((T + dt_) <= i * DT_ && !solver.isBroken()) {
+            solver.stepForward();
+            T += dt_;
+        }
+        if ((T + eps) < i * DT_ && !solver.isBroken()) solver.stepForward(i * DT_ - T);
+        if (solver.isBroken()) {
+            std::cerr << "Film is broken." << std::endl;
+            break;
+        }
+        T = i * DT_;
+        if (verbose) std::cout << "Frame " << i << " is ready" << std::endl;
+        solver.write_thickness_img(outputDir_, i);
+    }
+    return i;
+}
+
+}  // namespace sfb

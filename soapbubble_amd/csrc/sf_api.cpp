@@ -821,6 +821,25 @@ int sf_step(sf_ctx* ctx, int32_t nsteps) {
     });
 }
 
+int sf_time_advection(sf_ctx* ctx, int32_t nlaunch, double* avg_ms) {
+    return guarded(ctx, [&] {
+        if (!ctx || !ctx->impl || !avg_ms) throw Status{SF_ERR_INVALID, "null argument"};
+        if (nlaunch <= 0) throw Status{SF_ERR_INVALID, "launch count must be positive"};
+        Base& c = *ctx->impl;
+        if (c.stage != 0) throw Status{SF_ERR_ORDER, "sf_time_advection must start a step"};
+        // every launch recomputes the advected fields from the same state
+        c.advect();
+        SF_HIP(hipEventRecord(c.ev[0], c.stream));
+        for (int k = 0; k < nlaunch; ++k) c.advect();
+        SF_HIP(hipEventRecord(c.ev[5], c.stream));
+        SF_HIP(hipEventSynchronize(c.ev[5]));
+        float ms = 0.f;
+        SF_HIP(hipEventElapsedTime(&ms, c.ev[0], c.ev[5]));
+        *avg_ms = (double)ms / nlaunch;
+        return (int)SF_OK;
+    });
+}
+
 int sf_advect(sf_ctx* ctx) {
     return guarded(ctx, [&] {
         if (!ctx || !ctx->impl) throw Status{SF_ERR_INVALID, "null context"};

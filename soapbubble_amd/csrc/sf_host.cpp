@@ -468,7 +468,87 @@ bool read_exr_channel(const std::string& path, const std::string& channel, int* 
 
 }  // namespace sfh
 
+namespace sfh {
+
+namespace {
+
+// Bilinear resize of one float channel, W x H -> w x h: destination pixel
+// centre x + 1/2 maps to source x_s = (x + 1/2) W / w - 1/2; taps outside
+// the image are clamped to the border pixel.
+std::vector<float> resize_linear(const std::vector<float>& s, int W, int H, int w, int h) {
+    std::vector<float> out((size_t)w * h);
+    auto tap = [](double f, int n, int& i0, int& i1, double& a) {
+        double fl = std::floor(f);
+        a = f - fl;
+        i0 = (int)fl;
+        i1 = i0 + 1;
+        if (i0 < 0) { i0 = i1 = 0; a = 0.0; }
+        if (i1 > n - 1) { i1 = n - 1; if (i0 > n - 1) i0 = n - 1; }
+        if (i0 == i1) a = 0.0;
+    };
+    for (int y = 0; y < h; ++y) {
+        int y0, y1;
+        double ay;
+        tap((y + 0.5) * (double)H / h - 0.5, H, y0, y1, ay);
+        for (int x = 0; x < w; ++x) {
+            int x0, x1;
+            double ax;
+            tap((x + 0.5) * (double)W / w - 0.5, W, x0, x1, ax);
+            const double lo = (1 - ax) * s[(size_t)y0 * W + x0] + ax * s[(size_t)y0 * W + x1];
+            const double hi = (1 - ax) * s[(size_t)y1 * W + x0] + ax * s[(size_t)y1 * W + x1];
+            out[(size_t)y * w + x] = (float)((1 - ay) * lo + ay * hi);
+        }
+    }
+    return out;
+}
+
+bool load_channel(const char* path, const char* ch, int nph, int nth, std::vector<float>& out) {
+    int W = 0, H = 0;
+    std::vector<float> raw;
+    if (!read_exr_channel(path, ch, &W, &H, &raw)) return false;
+    out = resize_linear(raw, W, H, nph, nth);
+    return true;
+}
+
+}  // namespace
+
+void texture_fields(const char* thickness_path, const char* air_path, int nth, double radius,
+                    double* eta, double* uair0, double* vair0, int* loaded) {
+    const int nph = 2 * nth;
+    int got = 0;
+    std::vector<float> B, G, R;
+    if (eta && thickness_path && *thickness_path && load_channel(thickness_path, "B", nph, nth, B)) {
+        for (size_t k = 0; k < B.size(); ++k) {
+            double C = (double)B[k];
+            eta[k] = (C - 0.5) * 0.3 + 0.5;
+        }
+        got |= 1;
+    }
+    if ((uair0 || vair0) && air_path && *air_path && load_channel(air_path, "G", nph, nth, G) &&
+        load_channel(air_path, "R", nph, nth, R)) {
+        for (int j = 0; j < nth; ++j)
+            for (int i = 0; i < nph; ++i) {
+                const size_t c = (size_t)j * nph + i, w = (size_t)j * nph + (i + nph - 1) % nph;
+                if (uair0) uair0[c] = 0.5 * (double)(G[c] + G[w]) * radius;
+                if (vair0 && j < nth - 1) vair0[c] = -0.5 * (double)(R[c] + R[c + nph]) * radius;
+            }
+        got |= 2;
+    }
+    if (loaded) *loaded = got;
+}
+
+}  // namespace sfh
+
 extern "C" {
+
+int sf_texture_fields(const char* thickness_exr, const char* air_exr, int32_t n_theta,
+                      double radius, double* eta, double* uair0, double* vair0, int32_t* loaded) {
+    if (n_theta < 4 || (n_theta & (n_theta - 1))) return SF_ERR_UNSUPPORTED;
+    int got = 0;
+    sfh::texture_fields(thickness_exr, air_exr, n_theta, radius, eta, uair0, vair0, &got);
+    if (loaded) *loaded = got;
+    return SF_OK;
+}
 
 int sf_synth_fields(int32_t n_theta, uint64_t seed, int32_t l_max, int32_t l_air, double noise,
                     double air_amp, double* eta, double* gamma, double* uair0, double* vair0) {

@@ -33,4 +33,15 @@ bool write_thickness_exr(const std::string& path, int nth, int nph, const double
 bool read_exr_channel(const std::string& path, const std::string& channel, int* width,
                       int* height, std::vector<float>* data);
 
+// initThicknessfromPic / initAirflowfromPic (initializer.cu:50-135): the
+// B, G, R channels of an EXR texture resized to nPhi x nTheta by bilinear
+// interpolation with pixel centres aligned and replicated borders (the
+// semantics of OpenCV's INTER_LINEAR resize, which the reference calls);
+// eta = (B - 0.5) 0.3 + 0.5; u_air = (G_i + G_{i-1}) / 2 * radius on u_phi
+// faces; v_air = -(R_j + R_{j+1}) / 2 * radius on u_theta faces. A null or
+// empty path, or a file that cannot be read, leaves the output untouched
+// and clears its bit of *loaded (bit 0 thickness, bit 1 air).
+void texture_fields(const char* thickness_path, const char* air_path, int nth, double radius,
+                    double* eta, double* uair0, double* vair0, int* loaded);
+
 }  // namespace sfh

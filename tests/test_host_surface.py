@@ -1,0 +1,146 @@
+"""The host surface on the CPU: init textures (initThicknessfromPic /
+initAirflowfromPic, initializer.cu:50-135) through sf_texture_fields, and
+the soapBubble driver's argument, config.txt and AMG-config handling up to
+the point where it needs a GPU."""
+import math
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import soapbubble_amd as sb
+from exr_reader import read_exr
+from exr_writer import write_exr
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(ROOT, "soapbubble_amd", "soapBubble")
+
+
+def _resize_linear(img, w, h):
+    """Independent restatement of a bilinear resize with aligned pixel
+    centres and replicated borders, vectorised over the image."""
+    H, W = img.shape
+
+    def taps(n_out, n_in):
+        f = (np.arange(n_out) + 0.5) * n_in / n_out - 0.5
+        i0 = np.floor(f).astype(int)
+        a = f - i0
+        lo = i0 < 0
+        i0 = np.where(lo, 0, i0)
+        a = np.where(lo, 0.0, a)
+        i1 = np.minimum(i0 + 1, n_in - 1)
+        i0 = np.minimum(i0, n_in - 1)
+        a = np.where(i0 == i1, 0.0, a)
+        return i0, i1, a
+
+    y0, y1, ay = taps(h, H)
+    x0, x1, ax = taps(w, W)
+    img = img.astype(np.float64)
+    top = img[y0][:, x0] * (1 - ax) + img[y0][:, x1] * ax
+    bot = img[y1][:, x0] * (1 - ax) + img[y1][:, x1] * ax
+    return (top * (1 - ay)[:, None] + bot * ay[:, None]).astype(np.float32)
+
+
+def _fields_from(B, G, R, radius):
+    """The texture-to-field formulas of initializer.cu:64-67, 120-126."""
+    h, w = B.shape
+    eta = (B.astype(np.float64) - 0.5) * 0.3 + 0.5
+    u = 0.5 * (G + np.roll(G, 1, axis=1)).astype(np.float32).astype(np.float64) * radius
+    v = -0.5 * (R[:-1] + R[1:]).astype(np.float32).astype(np.float64) * radius
+    return eta.ravel(), u.ravel(), v.ravel()
+
+
+@pytest.mark.parametrize("src", [(24, 12), (16, 8), (100, 37)])
+def test_texture_fields_match_restatement(tmp_path, src):
+    W, H = src
+    rng = np.random.default_rng(W)
+    ch = {c: rng.uniform(-1, 1, (H, W)).astype(np.float16).astype(np.float32) for c in "BGR"}
+    path = str(tmp_path / "tex.exr")
+    write_exr(path, ch)
+    n = 8
+    radius = 0.05
+    f = sb.texture_fields(n, path, path, radius)
+    assert f["loaded"] == 3
+    rs = {c: _resize_linear(ch[c], 2 * n, n) for c in "BGR"}
+    eta, u, v = _fields_from(rs["B"], rs["G"], rs["R"], radius)
+    assert np.abs(f["eta"] - eta).max() <= 1e-7
+    assert np.abs(f["uair0"] - u).max() <= 1e-8
+    assert np.abs(f["vair0"] - v).max() <= 1e-8
+    if (W, H) == (16, 8):   # same size: the resize is the identity
+        assert np.array_equal(rs["B"], ch["B"])
+
+
+def test_texture_fields_missing_files_keep_defaults(tmp_path):
+    f = sb.texture_fields(8, str(tmp_path / "none.exr"), None)
+    assert f["loaded"] == 0
+    assert np.all(f["eta"] == 0.5) and np.all(f["uair0"] == 0) and np.all(f["vair0"] == 0)
+
+
+REF_INIT = "/root/reference/init"
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_INIT), reason="reference textures not present")
+def test_reference_textures_halving():
+    """The reference's 1024 x 512 textures at nTheta = 256: halving with
+    aligned centres is the mean of 2 x 2 blocks, read here with the tests'
+    own EXR reader."""
+    _, chans, img = read_exr(os.path.join(REF_INIT, "midFrequency.exr"))
+    f = sb.texture_fields(256, os.path.join(REF_INIT, "midFrequency.exr"),
+                          os.path.join(REF_INIT, "velocityfield_512.exr"), 0.05)
+    assert f["loaded"] == 3
+    B = img["B"].astype(np.float64)
+    Bh = 0.25 * (B[0::2, 0::2] + B[0::2, 1::2] + B[1::2, 0::2] + B[1::2, 1::2])
+    assert np.abs(f["eta"] - ((Bh - 0.5) * 0.3 + 0.5).ravel()).max() <= 1e-6
+    _, _, air = read_exr(os.path.join(REF_INIT, "velocityfield_512.exr"))
+    G = air["G"].astype(np.float64)
+    Gh = 0.25 * (G[0::2, 0::2] + G[0::2, 1::2] + G[1::2, 0::2] + G[1::2, 1::2])
+    u = 0.5 * (Gh + np.roll(Gh, 1, axis=1)) * 0.05
+    assert np.abs(f["uair0"] - u.ravel()).max() <= 1e-7
+
+
+# ------------------------------------------------------------ the driver ----
+def _run(*args, cwd=None):
+    return subprocess.run([CLI, *args], capture_output=True, text=True, timeout=60, cwd=cwd)
+
+
+def _config(tmp_path, n_theta=32, **kw):
+    vals = dict(radius=0.05, H=1e-6, U=1, cm=0.5, Gammam=1e-6, T=298.15, Ds=0.0, rm=5e-9,
+                numberOfGridsInTheta=n_theta, timePerStep=0.005, timePerFrame=0.012,
+                frameNumbers=3, outputDir=str(tmp_path / "out"), thicknessInit="null",
+                AMGconfig=str(tmp_path / "amg.json"), blendCoeff=0.5, gpu=0)
+    vals.update(kw)
+    path = tmp_path / "config.txt"
+    path.write_text("".join(f"{k} {v}\n" for k, v in vals.items()))
+    return str(path)
+
+
+def _amg(tmp_path, tol="1e-07", conv="ABSOLUTE", iters=100):
+    (tmp_path / "amg.json").write_text(
+        '{"config_version": 2, "solver": {"preconditioner": {"solver": "AMG", "max_iters": 1},'
+        f' "solver": "PCGF", "max_iters": {iters}, "convergence": "{conv}",'
+        f' "tolerance": {tol}, "norm": "L2"}}}}')
+
+
+def test_cli_usage():
+    r = _run()
+    assert r.returncode == 255 and "config.txt" in r.stdout
+
+
+def test_cli_malformed_config(tmp_path):
+    p = tmp_path / "config.txt"
+    p.write_text("radius 0.05\nH 0.000001\nU one\n")
+    r = _run(str(p))
+    assert r.returncode == 2 and "U is missing or malformed" in r.stderr
+    r = _run(str(tmp_path / "absent.txt"))
+    assert r.returncode == 2 and "cannot open" in r.stderr
+
+
+def test_cli_rejects_grid_before_gpu(tmp_path):
+    _amg(tmp_path, tol="1e-09", conv="RELATIVE_INI", iters=37)
+    r = _run(_config(tmp_path, n_theta=48))
+    assert r.returncode == 3 and "power of two" in r.stderr
+    # the outer solver's settings, not the preconditioner's max_iters = 1
+    assert "tolerance 1e-09 RELATIVE, max_iters 37" in r.stdout
+    r = _run(_config(tmp_path), "single")
+    assert r.returncode == 2 and "fp64 or mixed" in r.stderr
