@@ -194,6 +194,13 @@ double sf_real_ylm(int32_t l, int32_t m, double theta, double phi);
 /* write_thickness_img (solver.cu:541-593): eta * H * 2e5 as a HALF RGB EXR
  * (channels B, G, R, uncompressed). */
 int sf_write_thickness_exr(sf_ctx* ctx, int32_t bubble, const char* path);
+/* Solver::write_image (solver.cu:338-385), host only: width x height float
+ * planes r, g, b (row-major) written as an uncompressed scanline OpenEXR with
+ * HALF channels B, G, R, byte for byte as the reference's tinyexr call
+ * writes it (its header attributes, and its float-to-half rounding: nearest,
+ * ties away from zero). */
+int sf_write_image_exr(const char* path, int32_t width, int32_t height, const float* r,
+                       const float* g, const float* b);
 
 #ifdef __cplusplus
 }

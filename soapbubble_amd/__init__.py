@@ -38,7 +38,7 @@ EXPORTS = (
     "sf_set_dt", "sf_step", "sf_advect", "sf_assemble", "sf_solve", "sf_update",
     "sf_synchronize", "sf_time_advection", "sf_get_stats", "sf_get_timing", "sf_reset_timing", "sf_last_error",
     "sf_derive_params", "sf_synth_fields", "sf_texture_fields", "sf_real_ylm",
-    "sf_write_thickness_exr",
+    "sf_write_thickness_exr", "sf_write_image_exr",
 )
 
 
@@ -121,6 +121,9 @@ def load_library(path: str = LIB_PATH):
         "sf_real_ylm": (ctypes.c_double, [ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
                                           ctypes.c_double]),
         "sf_write_thickness_exr": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.c_char_p]),
+        "sf_write_image_exr": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, ctypes.c_int32,
+                                              P(ctypes.c_float), P(ctypes.c_float),
+                                              P(ctypes.c_float)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -192,6 +195,18 @@ def texture_fields(n_theta: int, thickness_exr: str | None = None, air_exr: str 
         raise SoapFilmError(rc, "sf_texture_fields")
     out["loaded"] = got.value
     return out
+
+
+def write_image_exr(path: str, r, g, b) -> None:
+    """write_image (solver.cu:338-385): three (height, width) planes as a
+    HALF B, G, R OpenEXR (host only)."""
+    lib = load_library()
+    planes = [np.ascontiguousarray(x, dtype=np.float32) for x in (r, g, b)]
+    h, w = planes[0].shape
+    fp = [x.ctypes.data_as(ctypes.POINTER(ctypes.c_float)) for x in planes]
+    rc = lib.sf_write_image_exr(path.encode(), w, h, *fp)
+    if rc:
+        raise SoapFilmError(rc, f"cannot write {path}")
 
 
 def real_ylm(l: int, m: int, theta: float, phi: float) -> float:

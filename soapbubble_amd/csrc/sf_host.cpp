@@ -262,26 +262,31 @@ void synth_fields(int nth, uint64_t seed, int lmax, int lair, double noise, doub
 // ------------------------------------------------------------------ EXR ----
 namespace {
 
-// IEEE binary32 -> binary16, round to nearest even.
+// binary32 -> binary16 as the reference's EXR writer converts (tinyexr,
+// float_to_half_full): truncate the significand and add one unit whenever
+// the highest dropped bit is set (round to nearest, ties away from zero);
+// binary32 subnormals become signed zero, overflow becomes infinity, NaN a
+// quiet NaN.
 uint16_t float_to_half(float f) {
     uint32_t x;
     std::memcpy(&x, &f, 4);
-    const uint32_t sign = (x >> 16) & 0x8000u;
-    const uint32_t absx = x & 0x7FFFFFFFu;
-    if (absx >= 0x7F800000u) return (uint16_t)(sign | 0x7C00u | (absx > 0x7F800000u ? 0x200u : 0u));
-    if (absx >= 0x477FF000u) return (uint16_t)(sign | 0x7C00u);  // overflow -> inf
-    if (absx < 0x38800000u) {                                    // subnormal half or zero
-        if (absx < 0x33000000u) return (uint16_t)sign;
-        const uint32_t mant = (absx & 0x007FFFFFu) | 0x00800000u;
-        const int shift = 126 - (int)(absx >> 23) + 14;
-        uint32_t r = mant >> shift;
-        const uint32_t rem = mant & ((1u << shift) - 1u), half = 1u << (shift - 1);
-        if (rem > half || (rem == half && (r & 1u))) ++r;
+    const uint16_t sign = (uint16_t)((x >> 16) & 0x8000u);
+    const int e = (int)((x >> 23) & 0xFFu);
+    const uint32_t mant = x & 0x7FFFFFu;
+    if (e == 0) return sign;
+    if (e == 255) return (uint16_t)(sign | 0x7C00u | (mant ? 0x200u : 0u));
+    const int ne = e - 127 + 15;
+    if (ne >= 31) return (uint16_t)(sign | 0x7C00u);
+    if (ne <= 0) {
+        const int shift = 14 - ne;   // hidden bit included
+        if (shift > 24) return sign;
+        const uint32_t m = mant | 0x800000u;
+        uint32_t r = m >> shift;
+        if ((m >> (shift - 1)) & 1u) ++r;
         return (uint16_t)(sign | r);
     }
-    uint32_t r = ((absx - 0x38000000u) >> 13);
-    const uint32_t rem = absx & 0x1FFFu;
-    if (rem > 0x1000u || (rem == 0x1000u && (r & 1u))) ++r;
+    uint32_t r = ((uint32_t)ne << 10) | (mant >> 13);
+    if (mant & 0x1000u) ++r;   // may carry into the exponent, up to infinity
     return (uint16_t)(sign | r);
 }
 
@@ -321,9 +326,9 @@ void attr(std::string& s, const char* name, const char* type, const std::string&
 
 }  // namespace
 
-bool write_thickness_exr(const std::string& path, int nth, int nph, const double* eta, double H) {
-    // write_thickness_img (solver.cu:560-582): gray = eta * H * 2e5 in B, G, R
-    const int w = nph, hgt = nth;
+bool write_image_exr(const std::string& path, int w, int hgt, const float* r, const float* g,
+                     const float* b) {
+    // write_image (solver.cu:338-385): channels B, G, R, HALF, uncompressed
     std::string hdr;
     const uint32_t magic = 20000630u;
     hdr.append(reinterpret_cast<const char*>(&magic), 4);
@@ -351,7 +356,9 @@ bool write_thickness_exr(const std::string& path, int nth, int nph, const double
     std::string v2;
     putf(v2, 0.0f); putf(v2, 0.0f);
     attr(hdr, "screenWindowCenter", "v2f", v2);
-    attr(hdr, "screenWindowWidth", "float", f1);
+    std::string fw;
+    putf(fw, (float)w);   // the image width, as the reference's writer stores it
+    attr(hdr, "screenWindowWidth", "float", fw);
     hdr += '\0';
     const uint64_t block = 8 + (uint64_t)w * 3 * 2;
     std::string out = hdr;
@@ -364,13 +371,23 @@ bool write_thickness_exr(const std::string& path, int nth, int nph, const double
     for (int y = 0; y < hgt; ++y) {
         put32(out, y);
         put32(out, w * 3 * 2);
-        for (int i = 0; i < w; ++i) line[i] = float_to_half((float)(eta[(size_t)y * nph + i] * H * 2e5));
-        for (int c = 0; c < 3; ++c) out.append(reinterpret_cast<const char*>(line.data()), 2 * (size_t)w);
+        for (const float* plane : {b, g, r}) {
+            for (int i = 0; i < w; ++i) line[i] = float_to_half(plane[(size_t)y * w + i]);
+            out.append(reinterpret_cast<const char*>(line.data()), 2 * (size_t)w);
+        }
     }
     std::ofstream f(path, std::ios::binary);
     if (!f) return false;
     f.write(out.data(), (std::streamsize)out.size());
     return (bool)f;
+}
+
+bool write_thickness_exr(const std::string& path, int nth, int nph, const double* eta, double H) {
+    // write_thickness_img (solver.cu:560-582): eta * H * 2e5 in all three
+    // channels
+    std::vector<float> gray((size_t)nth * nph);
+    for (size_t k = 0; k < gray.size(); ++k) gray[k] = (float)(eta[k] * H * 2e5);
+    return write_image_exr(path, nph, nth, gray.data(), gray.data(), gray.data());
 }
 
 bool read_exr_channel(const std::string& path, const std::string& channel, int* width,
@@ -540,6 +557,12 @@ void texture_fields(const char* thickness_path, const char* air_path, int nth, d
 }  // namespace sfh
 
 extern "C" {
+
+int sf_write_image_exr(const char* path, int32_t width, int32_t height, const float* r,
+                       const float* g, const float* b) {
+    if (!path || !r || !g || !b || width <= 0 || height <= 0) return SF_ERR_INVALID;
+    return sfh::write_image_exr(path, width, height, r, g, b) ? SF_OK : SF_ERR_INVALID;
+}
 
 int sf_texture_fields(const char* thickness_exr, const char* air_exr, int32_t n_theta,
                       double radius, double* eta, double* uair0, double* vair0, int32_t* loaded) {

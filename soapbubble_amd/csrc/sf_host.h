@@ -29,6 +29,10 @@ void synth_fields(int nth, uint64_t seed, int lmax, int lair, double noise, doub
 // HALF-float RGB OpenEXR, scanline, uncompressed, channels B,G,R
 // (write_image, solver.cu:338-385); value eta * H * 2e5 in every channel.
 bool write_thickness_exr(const std::string& path, int nth, int nph, const double* eta, double H);
+// write_image (solver.cu:338-385): planes r, g, b of w x h floats as HALF
+// channels B, G, R.
+bool write_image_exr(const std::string& path, int w, int h, const float* r, const float* g,
+                     const float* b);
 // Reads one channel of an uncompressed HALF/FLOAT scanline EXR.
 bool read_exr_channel(const std::string& path, const std::string& channel, int* width,
                       int* height, std::vector<float>* data);

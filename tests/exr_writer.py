@@ -24,7 +24,7 @@ def write_exr(path, channels, half=True):
     hdr += _attr("lineOrder", "lineOrder", b"\0")
     hdr += _attr("pixelAspectRatio", "float", struct.pack("<f", 1.0))
     hdr += _attr("screenWindowCenter", "v2f", struct.pack("<ff", 0.0, 0.0))
-    hdr += _attr("screenWindowWidth", "float", struct.pack("<f", 1.0))
+    hdr += _attr("screenWindowWidth", "float", struct.pack("<f", float(w)))
     hdr += b"\0"
     dt = np.dtype("<f2") if half else np.dtype("<f4")
     line = w * len(names) * dt.itemsize

@@ -144,3 +144,69 @@ def test_cli_rejects_grid_before_gpu(tmp_path):
     assert "tolerance 1e-09 RELATIVE, max_iters 37" in r.stdout
     r = _run(_config(tmp_path), "single")
     assert r.returncode == 2 and "fp64 or mixed" in r.stderr
+
+
+# ------------------------------------------------------------ EXR output ----
+def _half_away(x):
+    """binary32 -> binary16 bits, round to nearest with ties away from zero
+    (the reference writer's rule), binary32 subnormals to zero."""
+    b = np.asarray(x, dtype=np.float32).view(np.uint32).astype(np.int64)
+    sign = (b >> 16) & 0x8000
+    e = (b >> 23) & 0xFF
+    m = b & 0x7FFFFF
+    out = np.zeros_like(b)
+    for k in range(b.size):
+        ek, mk, s = int(e.flat[k]), int(m.flat[k]), int(sign.flat[k])
+        if ek == 0:
+            r = 0
+        elif ek == 255:
+            r = 0x7C00 | (0x200 if mk else 0)
+        else:
+            ne = ek - 127 + 15
+            if ne >= 31:
+                r = 0x7C00
+            elif ne <= 0:
+                sh = 14 - ne
+                mm = mk | 0x800000
+                r = 0 if sh > 24 else (mm >> sh) + ((mm >> (sh - 1)) & 1)
+            else:
+                r = ((ne << 10) | (mk >> 13)) + (1 if mk & 0x1000 else 0)
+        out.flat[k] = s | r
+    return out.astype(np.uint16)
+
+
+def _raw_half_plane(path, w, h, ch_index):
+    data = open(path, "rb").read()
+    hdr_end = data.index(b"screenWindowWidth\0float\0") + len(b"screenWindowWidth\0float\0") + 8 + 1
+    out = np.empty((h, w), np.uint16)
+    for y in range(h):
+        off = hdr_end + 8 * h + y * (8 + 6 * w) + 8 + ch_index * 2 * w
+        out[y] = np.frombuffer(data, "<u2", w, off)
+    return out
+
+
+def test_write_image_bytes_match_layout(tmp_path):
+    """Away from rounding ties the file is byte-identical to the tests'
+    independent writer of the published layout."""
+    h, w = 6, 10
+    rng = np.random.default_rng(1)
+    r, g, b = (rng.uniform(-2, 2, (h, w)).astype(np.float32) for _ in range(3))
+    p1, p2 = str(tmp_path / "a.exr"), str(tmp_path / "b.exr")
+    sb.write_image_exr(p1, r, g, b)
+    write_exr(p2, {"R": r, "G": g, "B": b})
+    assert open(p1, "rb").read() == open(p2, "rb").read()
+    attrs, chans, img = read_exr(p1)
+    assert chans == ["B", "G", "R"]
+    assert np.frombuffer(attrs["screenWindowWidth"][1], "<f4")[0] == w
+
+
+def test_write_image_half_rounding(tmp_path):
+    ties = np.float32([1 + 2 ** -11, -(1 + 2 ** -11), 1 + 3 * 2 ** -11, 2 ** -15 + 2 ** -25,
+                       2 ** -14 * 0.75, 65520.0, 70000.0, 1e-40, np.nan, np.inf, 0.1, 3.14159])
+    plane = np.tile(ties, (2, 1))
+    p = str(tmp_path / "t.exr")
+    sb.write_image_exr(p, plane, plane, plane)
+    got = _raw_half_plane(p, plane.shape[1], 2, 0)[0]
+    assert np.array_equal(got, _half_away(ties)), (got, _half_away(ties))
+    # ties round away from zero where round-to-even would not
+    assert got[0] == 0x3C01 and np.float16(ties[0]).view(np.uint16) == 0x3C00
