@@ -545,14 +545,15 @@ void setup(Base& c, int prec) {
     c.geom.ntiles = (int)tiles.size();
     c.geom.rows = rows_d;
     c.geom.nrows = (int)rows.size();
-    // row blocks: C consecutive cells per thread (4, or 8 above nPhi 4096,
-    // 2 for small rows), one thread per C cells of the widest row; every
-    // row must hold at least 4 C cells for the reduced line system
+    // row blocks: C consecutive cells per thread (4, doubled until the
+    // widest row needs at most kRowThreadsMax threads; 2 for small rows), one
+    // thread per C cells of the widest row; every row must hold at least 4 C
+    // cells for the reduced line system
     int nph_min = nph_max;
     for (int b = 0; b < nb; ++b) nph_min = std::min(nph_min, c.bub_h[b].nph);
-    int cpt = nph_max > 4096 ? 8 : (nph_max <= 128 ? 2 : 4);
-    if (nph_min < 4 * cpt) cpt = 2;
-    if (nph_min < 4 * cpt || nph_max / cpt > 1024)
+    int cpt = nph_max <= 128 ? 2 : 4;
+    while (nph_max / cpt > kRowThreadsMax && cpt < 16) cpt *= 2;
+    if (nph_min < 4 * cpt || nph_max / cpt > kRowThreadsMax)
         throw Status{SF_ERR_UNSUPPORTED, "bubble resolutions of one context span too wide a range"};
     const int nt = std::max(64, (nph_max / cpt + 63) / 64 * 64);
     c.geom.row_threads = nt;
@@ -792,14 +793,15 @@ int sf_step(sf_ctx* ctx, int32_t nsteps) {
             for (; k < nsteps && k < 2; ++k) eager_step(c, true);
             if (k == 2) {
                 // fixed schedule: the inner iterations of each refinement seen
-                // while calibrating, then one spare refinement of two
-                // iterations that catches a step needing more; a bubble
-                // that converged skips the remaining kernels
+                // while calibrating, then one spare refinement of one
+                // iteration (a further factor of about ten) that catches a
+                // step needing more; a bubble that converged skips the
+                // remaining kernels
                 c.sched.clear();
                 for (int n : c.calib) c.sched.push_back(c.opt.max_inner > 0 ? c.opt.max_inner : std::max(n, 1));
                 if (c.sched.empty()) c.sched.push_back(1);
-                c.sched.push_back(2);
-                if (c.opt.max_outer > 0) c.sched.resize((size_t)c.opt.max_outer, 2);
+                c.sched.push_back(1);
+                if (c.opt.max_outer > 0) c.sched.resize((size_t)c.opt.max_outer, 1);
                 c.calibrated = true;
                 build_graph(c);
             }

@@ -43,6 +43,10 @@ struct SolveParams {
     int pad_;
 };
 
+// Threads of a row block at most: one per C consecutive cells of the widest
+// row, C in {2, 4, 8, 16}.
+constexpr int kRowThreadsMax = 512;
+
 struct LaunchGeom {
     const int4* tiles;   // cell tiles: (bubble, row, first column, width)
     int ntiles;
