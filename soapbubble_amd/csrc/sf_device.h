@@ -95,7 +95,9 @@ template <> struct TabSel<float> {
 // ---------------------------------------------------------------- math ----
 __device__ __forceinline__ float tfloor(float x) { return floorf(x); }
 __device__ __forceinline__ double tfloor(double x) { return floor(x); }
-__device__ __forceinline__ float tsqrt(float x) { return sqrtf(x); }
+// FP32: the hardware square root (within an ulp) without the
+// correctly-rounded refinement sequence sqrtf expands to
+__device__ __forceinline__ float tsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ double tsqrt(double x) { return sqrt(x); }
 __device__ __forceinline__ bool tisnan(float x) { return __builtin_isnan(x); }
 __device__ __forceinline__ bool tisnan(double x) { return __builtin_isnan(x); }
@@ -464,7 +466,7 @@ __device__ __forceinline__ void point_index(const Circle<double>& o, double x, d
 template <>
 __device__ __forceinline__ void point_index(const Circle<float>& o, float x, float y, float z,
                                             float invh, float& ti, float& pi_, float& rho) {
-    rho = sqrtf(x * x + y * y);
+    rho = tsqrt(x * x + y * y);
     ti = o.gx + small_atan2(rho * o.cT - z * o.sT, z * o.cT + rho * o.sT) * invh;
     pi_ = o.gy + small_atan2(y * o.cP - x * o.sP, x * o.cP + y * o.sP) * invh;
 }
