@@ -34,6 +34,18 @@ class Stats(ctypes.Structure):
                 ("volume", ctypes.c_double)]
 
 
+MAPS = ("fwd_t", "fwd_p", "bwd_t", "bwd_p", "bwdprev_t", "bwdprev_p",
+        "eta_init", "eta_initlast", "eta_delta", "eta_deltalast",
+        "gam_init", "gam_initlast", "gam_delta", "gam_deltalast")
+
+
+class Bimocq(ctypes.Structure):
+    _fields_ = [(k, ctypes.POINTER(ctypes.c_double)) for k in MAPS] + [
+        ("blend", ctypes.c_double), ("threshold", ctypes.c_double),
+        ("multisample", ctypes.c_int32), ("reinit_count", ctypes.c_int32),
+        ("last_distortion", ctypes.c_double)]
+
+
 _lib = None
 
 
@@ -66,6 +78,18 @@ def lib():
         L.sfo_volume.restype = ctypes.c_double
         L.sfo_volume.argtypes = [P, dp]
         L.sfo_num_threads.restype = ctypes.c_int
+        B = ctypes.POINTER(Bimocq)
+        L.sfo_lerp_coords.argtypes = [P, dp, dp, ctypes.c_double, dp]
+        L.sfo_sample_mapping.argtypes = [P, dp, dp, ctypes.c_double, ctypes.c_double, dp]
+        L.sfo_map_update.argtypes = [P, dp, dp, ctypes.c_double, dp, dp, dp, dp]
+        L.sfo_bimocq_init.argtypes = [P, dp, dp, B]
+        L.sfo_advect_bimocq.argtypes = [P, dp, B, dp]
+        L.sfo_accumulate.argtypes = [P, dp, dp, dp, dp, ctypes.c_int]
+        L.sfo_distortion.restype = ctypes.c_double
+        L.sfo_distortion.argtypes = [P, B]
+        L.sfo_bimocq_reinit.argtypes = [P, dp, dp, B]
+        L.sfo_bimocq_step.restype = ctypes.c_int
+        L.sfo_bimocq_step.argtypes = [P] + [dp] * 6 + [B, ctypes.POINTER(Stats)]
         _lib = L
     return _lib
 
@@ -184,3 +208,73 @@ def volume(p: Params, eta) -> float:
 
 def num_threads() -> int:
     return lib().sfo_num_threads()
+
+
+# ---------------------------------------------------------------- BIMOCQ ----
+class BimocqState:
+    """Owns the BIMOCQ arrays of one bubble (numpy) and the ctypes view. The
+    reference swaps array pointers at reinitialisation; the arrays are read
+    back through the struct after every call."""
+
+    def __init__(self, p: Params, eta, gam, blend=0.5, threshold=4.0, multisample=True):
+        N = p.n_theta * 2 * p.n_theta
+        self.arr = {k: np.zeros(N) for k in MAPS}
+        self.s = Bimocq(*[_p(self.arr[k]) for k in MAPS], blend, threshold, int(multisample), 0, 0.0)
+        lib().sfo_bimocq_init(ctypes.byref(p), _p(eta), _p(gam), ctypes.byref(self.s))
+        self.N = N
+
+    def view(self, k):
+        """The array currently bound to field k (after pointer swaps)."""
+        ptr = getattr(self.s, k)
+        return np.ctypeslib.as_array(ptr, shape=(self.N,))
+
+    @property
+    def reinit_count(self):
+        return self.s.reinit_count
+
+    @property
+    def distortion(self):
+        return self.s.last_distortion
+
+
+def lerp_coords(p: Params, a, b, alpha):
+    out = np.zeros(2)
+    lib().sfo_lerp_coords(ctypes.byref(p), _p(np.asarray(a, float)), _p(np.asarray(b, float)),
+                          alpha, _p(out))
+    return out
+
+
+def sample_mapping(p: Params, mt, mp, rx, ry):
+    out = np.zeros(2)
+    lib().sfo_sample_mapping(ctypes.byref(p), _p(mt), _p(mp), rx, ry, _p(out))
+    return out
+
+
+def map_update(p: Params, uth, uph, dt, mt, mp):
+    ot, op = np.empty_like(mt), np.empty_like(mp)
+    lib().sfo_map_update(ctypes.byref(p), _p(uth), _p(uph), dt, _p(mt), _p(mp), _p(ot), _p(op))
+    return ot, op
+
+
+def advect_bimocq(p: Params, eta, b: BimocqState):
+    out = np.empty_like(eta)
+    lib().sfo_advect_bimocq(ctypes.byref(p), _p(eta), ctypes.byref(b.s), _p(out))
+    return out
+
+
+def bimocq_reinit(p: Params, st: dict, b: BimocqState):
+    lib().sfo_bimocq_reinit(ctypes.byref(p), _p(st["eta"]), _p(st["gamma"]), ctypes.byref(b.s))
+
+
+def bimocq_step(p: Params, st: dict, uair0, vair0, b: BimocqState, nsteps: int = 1,
+                advance_time: bool = True):
+    out = []
+    for _ in range(nsteps):
+        s = Stats()
+        rc = lib().sfo_bimocq_step(ctypes.byref(p), _p(st["u_theta"]), _p(st["u_phi"]),
+                                   _p(st["eta"]), _p(st["gamma"]), _p(uair0), _p(vair0),
+                                   ctypes.byref(b.s), ctypes.byref(s))
+        out.append((rc, s.iters, s.rel_residual, b.distortion, b.reinit_count))
+        if advance_time:
+            p.time_s += p.dt * p.radius
+    return out

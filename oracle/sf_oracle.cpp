@@ -374,6 +374,66 @@ void cyclic_tridiag(int n, const double* a, const double* b, const double* c,
     for (int i = 0; i < n; ++i) x[i] -= fact * z[i];
 }
 
+
+/* ------------------------------------------------------------ BIMOCQ ---- */
+struct P2 { double x, y; };   /* (theta, phi) index coordinates */
+
+/* lerpCoords, bubble.cu:512-536: spherical interpolation of two index-space
+ * points along their great circle (Rodrigues rotation about k). */
+P2 lerp_coords(const Grid& g, P2 from, P2 to, double alpha) {
+    const double h = kPi / g.nth;
+    const double ft = from.x * h, fp = from.y * h, tt = to.x * h, tp = to.y * h;
+    V3 f3 = normalize(V3{std::cos(fp) * std::sin(ft), std::sin(fp) * std::sin(ft), std::cos(ft)});
+    V3 t3 = normalize(V3{std::cos(tp) * std::sin(tt), std::sin(tp) * std::sin(tt), std::cos(tt)});
+    V3 k = normalize(cross(f3, t3));
+    if (std::isnan(k.x)) return from;
+    double span = safe_acos(dot(f3, t3)) * alpha;
+    V3 r = f3 * std::cos(span) + cross(k, f3) * std::sin(span) + k * dot(k, f3) * (1 - std::cos(span));
+    return P2{safe_acos(r.z) / h, std::atan2(r.y, r.x) / h};
+}
+
+/* sampleMapping, bubble.cu:539-603. The corner selection follows the
+ * reference's branches; the phi indices wrap modulo nPhi. */
+P2 sample_mapping(const Grid& g, const double* mt, const double* mp, P2 raw) {
+    double x = raw.x - 0.5, y = raw.y;
+    const bool fl = validate_coord(g, x, y);
+    const long n = g.nph;
+    int pI = (int)g.flr(y), tI = (int)g.flr(x);
+    const double aP = y - pI, aT = x - tI;
+    P2 ll{0, 0}, lr{0, 0}, hl{0, 0}, hr{0, 0};
+    auto at = [&](int row, long col) { size_t c = (size_t)row * n + umod(col, n); return P2{mt[c], mp[c]}; };
+    if (fl) {
+        if (tI == 0) {
+            hl = at(0, pI); hr = at(0, pI + 1);
+            ll = at(0, pI + n / 2); lr = at(0, pI + 1 + n / 2);
+        } else {
+            tI -= 1;
+        }
+    }
+    if (tI == g.nth - 1) {
+        ll = at(tI, pI); lr = at(tI, pI + 1);
+        hl = at(tI, pI + n / 2); hr = at(tI, pI + 1 + n / 2);
+    } else if (tI != 0 || !fl) {
+        ll = at(tI, pI); lr = at(tI, pI + 1);
+        hl = at(tI + 1, pI); hr = at(tI + 1, pI + 1);
+    }
+    return lerp_coords(g, lerp_coords(g, ll, lr, aP), lerp_coords(g, hl, hr, aP), aT);
+}
+
+/* dist, bubble.cu:60-79: great-circle distance in grid cells */
+double grid_dist(const Grid& g, P2 a, P2 b) {
+    const double h = kPi / g.nth;
+    V3 a3 = normalize(V3{std::cos(a.y * h) * std::sin(a.x * h), std::sin(a.y * h) * std::sin(a.x * h), std::cos(a.x * h)});
+    V3 b3 = normalize(V3{std::cos(b.y * h) * std::sin(b.x * h), std::sin(b.y * h) * std::sin(b.x * h), std::cos(b.x * h)});
+    return g.invh * safe_acos(dot(a3, b3));
+}
+
+/* MULTISAMPLE (header.cuh:98): five samples around the point
+ * (bubble.cu:983-995) */
+const double kMsW[5] = {0.125, 0.125, 0.125, 0.125, 0.5};
+const double kMsDx[5] = {-0.25, 0.25, -0.25, 0.25, 0.0};
+const double kMsDy[5] = {-0.25, -0.25, 0.25, 0.25, 0.0};
+
 }  // namespace
 
 extern "C" {
@@ -841,6 +901,253 @@ int sfo_step(const sfo_params* p, double* uth, double* uph, double* eta,
     sfo_update(p, uth1.data(), uph1.data(), eta1.data(), gam1.data(), x.data(),
                uair.data(), vair.data(), uth, uph, eta);
     std::copy(x.begin(), x.end(), gam);
+    s->volume = sfo_volume(p, eta);
+    int any = 0;
+    for (size_t k = 0; k < N; ++k) if (eta[k] > 0) { any = 1; break; }
+    s->broken = !any || rc != 0;
+    return rc;
+}
+
+
+/* ------------------------------------------------------------ BIMOCQ ---- */
+void sfo_lerp_coords(const sfo_params* p, const double* from2, const double* to2, double alpha,
+                     double* out2) {
+    Grid g(p);
+    P2 r = lerp_coords(g, P2{from2[0], from2[1]}, P2{to2[0], to2[1]}, alpha);
+    out2[0] = r.x;
+    out2[1] = r.y;
+}
+
+void sfo_sample_mapping(const sfo_params* p, const double* mt, const double* mp, double rx,
+                        double ry, double* out2) {
+    Grid g(p);
+    P2 r = sample_mapping(g, mt, mp, P2{rx, ry});
+    out2[0] = r.x;
+    out2[1] = r.y;
+}
+
+/* updateMappingKernel, bubble.cu:606-633: out(x) = map(trace(x, dt)), the
+ * result through validateCoord. */
+void sfo_map_update(const sfo_params* p, const double* uth, const double* uph, double dt,
+                    const double* mt, const double* mp, double* out_t, double* out_p) {
+    Grid g(p);
+    const int nth = g.nth, nph = g.nph;
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < nth; ++j)
+        for (int i = 0; i < nph; ++i) {
+            double bx, by;
+            trace_great_circle(g, uth, uph, dt, j + 0.5, (double)i, bx, by);
+            P2 s = sample_mapping(g, mt, mp, P2{bx, by});
+            validate_coord(g, s.x, s.y);
+            out_t[(size_t)j * nph + i] = s.x;
+            out_p[(size_t)j * nph + i] = s.y;
+        }
+}
+
+/* initMapping, solver.cu:12-22 */
+static void init_mapping(const Grid& g, double* mt, double* mp) {
+    for (int j = 0; j < g.nth; ++j)
+        for (int i = 0; i < g.nph; ++i) {
+            mt[(size_t)j * g.nph + i] = j + 0.5;
+            mp[(size_t)j * g.nph + i] = (double)i;
+        }
+}
+
+void sfo_bimocq_init(const sfo_params* p, const double* eta, const double* gam, sfo_bimocq* b) {
+    Grid g(p);
+    const size_t N = (size_t)g.nth * g.nph;
+    init_mapping(g, b->fwd_t, b->fwd_p);
+    init_mapping(g, b->bwd_t, b->bwd_p);
+    init_mapping(g, b->bwdprev_t, b->bwdprev_p);
+    for (size_t k = 0; k < N; ++k) {
+        b->eta_init[k] = b->eta_initlast[k] = eta[k];
+        b->gam_init[k] = b->gam_initlast[k] = gam[k];
+        b->eta_delta[k] = b->eta_deltalast[k] = 0.0;
+        b->gam_delta[k] = b->gam_deltalast[k] = 0.0;
+    }
+    b->reinit_count = 0;
+    b->last_distortion = 0.0;
+}
+
+/* advectionCenteredBimocq, bubble.cu:967-1013 */
+void sfo_advect_bimocq(const sfo_params* p, const double* eta, const sfo_bimocq* b,
+                       double* eta_out) {
+    Grid g(p);
+    const int nth = g.nth, nph = g.nph;
+    const int ns = b->multisample ? 5 : 1;
+    const double bl = b->blend;
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < nth; ++j)
+        for (int i = 0; i < nph; ++i) {
+            const size_t c = (size_t)j * nph + i;
+            if (eta[c] == kThresh) { eta_out[c] = kThresh; continue; }
+            double a = 0.0;
+            for (int s = 0; s < ns; ++s) {
+                const int q = ns == 5 ? s : 4;
+                const double w = ns == 5 ? kMsW[q] : 1.0;
+                P2 pos{j + 0.5 + kMsDx[q], i + kMsDy[q]};
+                P2 ip = sample_mapping(g, b->bwd_t, b->bwd_p, pos);
+                P2 lp = sample_mapping(g, b->bwdprev_t, b->bwdprev_p, ip);
+                a += (1.0 - bl) * w * (sample_centered(g, b->eta_initlast, lp.x, lp.y) +
+                                       sample_centered(g, b->eta_delta, ip.x, ip.y) +
+                                       sample_centered(g, b->eta_deltalast, lp.x, lp.y));
+                a += bl * w * (sample_centered(g, b->eta_init, ip.x, ip.y) +
+                               sample_centered(g, b->eta_delta, ip.x, ip.y));
+            }
+            eta_out[c] = std::fmax(kThresh, a);
+        }
+}
+
+/* accumulateChangesCentered, bubble.cu:2081-2114: delta(x) += sum_s w_s
+ * change(fwd(x + d_s)) */
+void sfo_accumulate(const sfo_params* p, const double* fwd_t, const double* fwd_p,
+                    const double* change, double* delta, int multisample) {
+    Grid g(p);
+    const int nth = g.nth, nph = g.nph;
+    const int ns = multisample ? 5 : 1;
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < nth; ++j)
+        for (int i = 0; i < nph; ++i) {
+            const size_t c = (size_t)j * nph + i;
+            for (int s = 0; s < ns; ++s) {
+                const int q = ns == 5 ? s : 4;
+                const double w = ns == 5 ? kMsW[q] : 1.0;
+                P2 ip = sample_mapping(g, fwd_t, fwd_p, P2{j + 0.5 + kMsDx[q], i + kMsDy[q]});
+                delta[c] += w * sample_centered(g, change, ip.x, ip.y);
+            }
+        }
+}
+
+/* estimateDistortion, bubble.cu:1133-1168: the larger of max |x - bwd(fwd(x))|
+ * and max |x - fwd(bwd(x))| in grid cells */
+double sfo_distortion(const sfo_params* p, const sfo_bimocq* b) {
+    Grid g(p);
+    const int nth = g.nth, nph = g.nph;
+    double mx = 0.0;
+#pragma omp parallel for schedule(static) reduction(max : mx)
+    for (int j = 0; j < nth; ++j)
+        for (int i = 0; i < nph; ++i) {
+            const size_t c = (size_t)j * nph + i;
+            P2 x{j + 0.5, (double)i};
+            P2 a = sample_mapping(g, b->bwd_t, b->bwd_p, P2{b->fwd_t[c], b->fwd_p[c]});
+            P2 e = sample_mapping(g, b->fwd_t, b->fwd_p, P2{b->bwd_t[c], b->bwd_p[c]});
+            mx = std::max(mx, std::max((double)(float)grid_dist(g, x, a),
+                                       (double)(float)grid_dist(g, x, e)));
+        }
+    return mx;
+}
+
+/* reInitializeMapping with its error correction (bubble.cu:2509-2564,
+ * correctCentered1/2 at bubble.cu:2180-2280). The neighbourhood bounds and
+ * the clamp are evaluated in float as the reference's fminf/fmaxf/clamp
+ * do. Deviation D6: the 3x3 neighbourhood wraps phi and reflects theta = -1
+ * across the pole like theta = nTheta (the reference's validateId leaves
+ * index -1 negative and reads before the row). */
+void sfo_bimocq_reinit(const sfo_params* p, double* eta, double* gam, sfo_bimocq* b) {
+    Grid g(p);
+    const int nth = g.nth, nph = g.nph;
+    const size_t N = (size_t)nth * nph;
+    std::vector<double> eerr(N), gerr(N), eta0(eta, eta + N), gam0(gam, gam + N);
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < nth; ++j)
+        for (int i = 0; i < nph; ++i) {
+            const size_t c = (size_t)j * nph + i;
+            double te = 0.0, tg = 0.0;
+            for (int q = 0; q < 5; ++q) {
+                P2 ip = sample_mapping(g, b->fwd_t, b->fwd_p, P2{j + 0.5 + kMsDx[q], i + kMsDy[q]});
+                te += kMsW[q] * sample_centered(g, eta0.data(), ip.x, ip.y);
+                tg += kMsW[q] * sample_centered(g, gam0.data(), ip.x, ip.y);
+            }
+            eerr[c] = (te - b->eta_delta[c] - b->eta_init[c]) * 0.5;
+            gerr[c] = (tg - b->gam_delta[c] - b->gam_init[c]) * 0.5;
+        }
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < nth; ++j)
+        for (int i = 0; i < nph; ++i) {
+            const size_t c = (size_t)j * nph + i;
+            double ve = eta0[c], vg = gam0[c];
+            for (int q = 0; q < 5; ++q) {
+                P2 ip = sample_mapping(g, b->bwd_t, b->bwd_p, P2{j + 0.5 + kMsDx[q], i + kMsDy[q]});
+                ve -= kMsW[q] * sample_centered(g, eerr.data(), ip.x, ip.y);
+                vg -= kMsW[q] * sample_centered(g, gerr.data(), ip.x, ip.y);
+            }
+            auto clamp_local = [&](const std::vector<double>& in, double v) {
+                float lo = (float)in[c], hi = lo;
+                for (int t = -1; t <= 1; ++t)
+                    for (int s = -1; s <= 1; ++s) {
+                        int jj = j + t, ii = i + s;
+                        if (jj < 0 || jj >= nth) { jj = jj < 0 ? 0 : nth - 1; ii += nth; }
+                        const float cv = (float)in[(size_t)jj * nph + umod(ii, nph)];
+                        lo = std::fmin(lo, cv);
+                        hi = std::fmax(hi, cv);
+                    }
+                const float cl = std::fmax(lo, std::fmin((float)v, hi));
+                return std::fmax(kThresh, (double)cl);
+            };
+            eta[c] = clamp_local(eta0, ve);
+            gam[c] = clamp_local(gam0, vg);
+        }
+    std::swap(b->eta_initlast, b->eta_init);
+    std::swap(b->eta_deltalast, b->eta_delta);
+    std::swap(b->gam_initlast, b->gam_init);
+    std::swap(b->gam_deltalast, b->gam_delta);
+    std::swap(b->bwdprev_t, b->bwd_t);
+    std::swap(b->bwdprev_p, b->bwd_p);
+    for (size_t k = 0; k < N; ++k) {
+        b->eta_init[k] = eta[k];
+        b->gam_init[k] = gam[k];
+        b->eta_delta[k] = 0.0;
+        b->gam_delta[k] = 0.0;
+    }
+    init_mapping(g, b->fwd_t, b->fwd_p);
+    init_mapping(g, b->bwd_t, b->bwd_p);
+    b->reinit_count += 1;
+}
+
+/* Solver::stepForward with BIMOCQ (solver.cu:260-292): mapping updates,
+ * advection (eta by BIMOCQ, Gamma plain, bubble.cu:1188-1205), bodyforce
+ * with the change accumulation (bubble.cu:2481-2502), distortion check and
+ * reinitialisation. */
+int sfo_bimocq_step(const sfo_params* p, double* uth, double* uph, double* eta, double* gam,
+                    const double* uair0, const double* vair0, sfo_bimocq* b, sfo_stats* st) {
+    Grid g(p);
+    const int nth = g.nth, nph = g.nph;
+    const size_t N = (size_t)nth * nph, Nt = (size_t)(nth - 1) * nph;
+    std::vector<double> tt(N), tp(N);
+    sfo_map_update(p, uth, uph, -p->dt, b->fwd_t, b->fwd_p, tt.data(), tp.data());
+    std::copy(tt.begin(), tt.end(), b->fwd_t);
+    std::copy(tp.begin(), tp.end(), b->fwd_p);
+    sfo_map_update(p, uth, uph, p->dt, b->bwd_t, b->bwd_p, tt.data(), tp.data());
+    std::copy(tt.begin(), tt.end(), b->bwd_t);
+    std::copy(tp.begin(), tp.end(), b->bwd_p);
+
+    std::vector<double> uth1(Nt), uph1(N), eta1(N), gam1(N), eta_plain(N);
+    sfo_advect(p, uth, uph, eta, gam, uth1.data(), uph1.data(), eta_plain.data(), gam1.data());
+    sfo_advect_bimocq(p, eta, b, eta1.data());
+
+    std::vector<double> uair(N), vair(Nt);
+    if (p->airflow) sfo_airflow(p, uair0, vair0, uair.data(), vair.data());
+    std::vector<double> val(5 * N), rhs(N);
+    sfo_assemble(p, uth1.data(), uph1.data(), gam1.data(), eta1.data(), uair.data(), vair.data(),
+                 val.data(), rhs.data());
+    std::vector<double> x(gam1);
+    sfo_stats local;
+    sfo_stats* s = st ? st : &local;
+    int rc = sfo_solve(p, val.data(), rhs.data(), x.data(), s);
+    sfo_update(p, uth1.data(), uph1.data(), eta1.data(), gam1.data(), x.data(), uair.data(),
+               vair.data(), uth, uph, eta);
+    std::copy(x.begin(), x.end(), gam);
+    std::vector<double> de(N), dg(N);
+    for (size_t k = 0; k < N; ++k) {
+        de[k] = eta[k] - eta1[k];
+        dg[k] = gam[k] - gam1[k];
+    }
+    sfo_accumulate(p, b->fwd_t, b->fwd_p, de.data(), b->eta_delta, b->multisample);
+    sfo_accumulate(p, b->fwd_t, b->fwd_p, dg.data(), b->gam_delta, b->multisample);
+
+    b->last_distortion = sfo_distortion(p, b);
+    if (b->last_distortion > b->threshold) sfo_bimocq_reinit(p, eta, gam, b);
+
     s->volume = sfo_volume(p, eta);
     int any = 0;
     for (size_t k = 0; k < N; ++k) if (eta[k] > 0) { any = 1; break; }

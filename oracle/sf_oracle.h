@@ -96,6 +96,39 @@ double sfo_volume(const sfo_params* p, const double* eta);
 /* Number of OpenMP threads the oracle uses. */
 int sfo_num_threads(void);
 
+/* ---- BIMOCQ scalar advection (header.cuh:97-98 BIMOCQ, MULTISAMPLE) ----
+ * Caller-owned arrays of N = nTheta nPhi: forward map fwd (initial ->
+ * current positions), backward maps bwd and bwdprev (current -> initial, and
+ * the map of the previous reinitialisation interval), and for eta and Gamma
+ * the initial fields and accumulated changes of the current and previous
+ * interval (Quantity::getGPUInit/Delta/InitLast/DeltaLast). */
+typedef struct {
+    double *fwd_t, *fwd_p, *bwd_t, *bwd_p, *bwdprev_t, *bwdprev_p;
+    double *eta_init, *eta_initlast, *eta_delta, *eta_deltalast;
+    double *gam_init, *gam_initlast, *gam_delta, *gam_deltalast;
+    double blend;            /* blendCoeff (config.txt)                     */
+    double threshold;        /* reinitialisation distortion, cells (4)     */
+    int32_t multisample;     /* 1: five samples per point                  */
+    int32_t reinit_count;    /* out: reinitialisations so far              */
+    double last_distortion;  /* out: distortion after the last step        */
+} sfo_bimocq;
+
+void sfo_lerp_coords(const sfo_params* p, const double* from2, const double* to2, double alpha,
+                     double* out2);
+void sfo_sample_mapping(const sfo_params* p, const double* map_t, const double* map_p,
+                        double rx, double ry, double* out2);
+void sfo_map_update(const sfo_params* p, const double* uth, const double* uph, double dt,
+                    const double* map_t, const double* map_p, double* out_t, double* out_p);
+void sfo_bimocq_init(const sfo_params* p, const double* eta, const double* gam, sfo_bimocq* b);
+void sfo_advect_bimocq(const sfo_params* p, const double* eta, const sfo_bimocq* b,
+                       double* eta_out);
+void sfo_accumulate(const sfo_params* p, const double* fwd_t, const double* fwd_p,
+                    const double* change, double* delta, int multisample);
+double sfo_distortion(const sfo_params* p, const sfo_bimocq* b);
+void sfo_bimocq_reinit(const sfo_params* p, double* eta, double* gam, sfo_bimocq* b);
+int sfo_bimocq_step(const sfo_params* p, double* uth, double* uph, double* eta, double* gam,
+                    const double* uair0, const double* vair0, sfo_bimocq* b, sfo_stats* st);
+
 #ifdef __cplusplus
 }
 #endif
