@@ -96,6 +96,13 @@ typedef struct {
     double inner_rho;     /* inner residual reduction; 0 -> automatic             */
     int32_t advance_time; /* 1: air-flow time advances by dt every step           */
     int32_t inner_method; /* 0 automatic, 1 PCG, 2 Chebyshev (inner solve)        */
+    int32_t scalar_advection; /* 0 plain (advectionAllCentered), 1 BIMOCQ for eta
+                                 (header.cuh:97 BIMOCQ, bubble.cu:1188-1205)      */
+    int32_t multisample;  /* BIMOCQ: 1 five samples per point (header.cuh:98)     */
+    double blend;         /* BIMOCQ: blendCoeff of config.txt (bubble.cu:1008-1011) */
+    double reinit_threshold; /* BIMOCQ: reinitialise the maps when their
+                                distortion exceeds this many cells; 0 -> 4
+                                (solver.cu:279-286)                          */
 } sf_options;
 
 typedef struct {
@@ -109,6 +116,9 @@ typedef struct {
     int32_t broken;       /* 1 if no cell has eta > 0, or a solve failed         */
     int32_t steps;        /* steps taken                                         */
     double time;          /* air-flow time, s                                    */
+    double distortion;    /* BIMOCQ: map distortion after the last step, cells   */
+    int32_t reinits;      /* BIMOCQ: map reinitialisations so far                */
+    int32_t pad_;
 } sf_bubble_stats;
 
 typedef struct {

@@ -60,7 +60,9 @@ class Options(ctypes.Structure):
                 ("use_graph", ctypes.c_int32), ("tol_absolute", ctypes.c_int32),
                 ("tol", ctypes.c_double), ("max_outer", ctypes.c_int32),
                 ("max_inner", ctypes.c_int32), ("inner_rho", ctypes.c_double),
-                ("advance_time", ctypes.c_int32), ("inner_method", ctypes.c_int32)]
+                ("advance_time", ctypes.c_int32), ("inner_method", ctypes.c_int32),
+                ("scalar_advection", ctypes.c_int32), ("multisample", ctypes.c_int32),
+                ("blend", ctypes.c_double), ("reinit_threshold", ctypes.c_double)]
 
 
 class BubbleStats(ctypes.Structure):
@@ -68,7 +70,9 @@ class BubbleStats(ctypes.Structure):
                 ("converged", ctypes.c_int32), ("failed_solves", ctypes.c_int32),
                 ("residual", ctypes.c_double), ("max_residual", ctypes.c_double),
                 ("volume", ctypes.c_double), ("broken", ctypes.c_int32),
-                ("steps", ctypes.c_int32), ("time", ctypes.c_double)]
+                ("steps", ctypes.c_int32), ("time", ctypes.c_double),
+                ("distortion", ctypes.c_double), ("reinits", ctypes.c_int32),
+                ("pad_", ctypes.c_int32)]
 
 
 class Timing(ctypes.Structure):
@@ -230,7 +234,9 @@ class SoapFilm:
     def __init__(self, bubbles: Sequence[BubbleDesc] | BubbleDesc, precision: str = "fp64",
                  device: int = 0, use_graph: bool = False, tol: float = 1e-10,
                  tol_absolute: bool = False, advance_time: bool = True, max_outer: int = 0,
-                 max_inner: int = 0, inner_rho: float = 0.0, inner_method: str = "auto"):
+                 max_inner: int = 0, inner_rho: float = 0.0, inner_method: str = "auto",
+                 scalar_advection: str = "plain", blend: float = 0.5, multisample: bool = True,
+                 reinit_threshold: float = 0.0):
         self._lib = load_library()
         if isinstance(bubbles, BubbleDesc):
             bubbles = [bubbles]
@@ -238,7 +244,9 @@ class SoapFilm:
         arr = (BubbleDesc * len(self.bubbles))(*self.bubbles)
         opt = Options(PRECISIONS[precision], device, int(use_graph), int(tol_absolute), tol,
                       max_outer, max_inner, inner_rho, int(advance_time),
-                      {"auto": 0, "cg": 1, "chebyshev": 2}[inner_method])
+                      {"auto": 0, "cg": 1, "chebyshev": 2}[inner_method],
+                      {"plain": 0, "bimocq": 1}[scalar_advection], int(multisample), blend,
+                      reinit_threshold)
         h = ctypes.c_void_p()
         rc = self._lib.sf_create(ctypes.byref(opt), len(self.bubbles), arr, ctypes.byref(h))
         if rc:

@@ -84,6 +84,12 @@ struct Base {
     double* b64 = nullptr;
     int* rsteps = nullptr;
     int nxcd = 1;
+    // BIMOCQ scalar advection (allocated only when selected); its state is
+    // (re)initialised from the current fields at the first step after an
+    // upload of eta or Gamma
+    bool bimocq = false;
+    bool bimocq_dirty = true;
+    BimocqPtrs bm{};
     SolveParams sp{};
     int max_outer = 3, max_inner = 500;
     bool calibrated = false;
@@ -167,7 +173,38 @@ struct Impl : Base {
         return v;
     }
 
-    void advect() override { launch_advect<T>(stream, geom, bub_d, time_d, tab, cur, nxt); }
+    void alloc_bimocq() {
+        const size_t n = (size_t)ncells;
+        double** arrays[] = {&bm.fwd_t, &bm.fwd_p, &bm.bwd_t, &bm.bwd_p, &bm.prev_t, &bm.prev_p,
+                             &bm.nfwd_t, &bm.nfwd_p, &bm.nbwd_t, &bm.nbwd_p, &bm.e_init,
+                             &bm.e_initlast, &bm.e_delta, &bm.e_deltalast, &bm.g_init,
+                             &bm.g_initlast, &bm.g_delta, &bm.g_deltalast, &bm.chg_e, &bm.chg_g,
+                             &bm.err_e, &bm.err_g, &bm.cpy_e, &bm.cpy_g};
+        for (double** a : arrays) *a = dalloc<double>(n, owned);
+        bm.dist_bits = dalloc<unsigned>((size_t)nb, owned);
+        bm.flag = dalloc<int>((size_t)nb, owned);
+        bm.dist_last = dalloc<double>((size_t)nb, owned);
+        bm.reinits = dalloc<int>((size_t)nb, owned);
+        SF_HIP(hipMemset(bm.dist_bits, 0, sizeof(unsigned) * nb));
+        SF_HIP(hipMemset(bm.flag, 0, sizeof(int) * nb));
+        SF_HIP(hipMemset(bm.dist_last, 0, sizeof(double) * nb));
+        SF_HIP(hipMemset(bm.reinits, 0, sizeof(int) * nb));
+    }
+
+    // Solver::advection (bubble.cu:1171-1214); with BIMOCQ the map updates
+    // of stepForward (solver.cu:265-268) come first and eta is replaced by
+    // its BIMOCQ value.
+    void advect() override {
+        if (bimocq) {
+            if (bimocq_dirty) {
+                launch_bimocq_init<T>(stream, geom, bub_d, cur, bm);
+                bimocq_dirty = false;
+            }
+            launch_bimocq_maps<T>(stream, geom, bub_d, time_d, tab, cur, bm);
+        }
+        launch_advect<T>(stream, geom, bub_d, time_d, tab, cur, nxt);
+        if (bimocq) launch_bimocq_eta<T>(stream, geom, bub_d, cur, nxt, bm);
+    }
 
     // Assembly fused with the first FP64 residual and line solve; x0 lands
     // in x64[0].
@@ -298,7 +335,14 @@ struct Impl : Base {
 
     void update(bool finalize) override {
         launch_update<T>(stream, geom, bub_d, time_d, ctl_d, tab, nxt, air(), x64[0], x64[1], cur,
-                         finalize ? 1 : 0, hist_d, sp, partR, stride);
+                         finalize ? 1 : 0, hist_d, sp, partR, stride,
+                         bimocq ? bm.chg_e : nullptr, bimocq ? bm.chg_g : nullptr);
+        if (bimocq) {
+            // the change accumulation of bodyforce (bubble.cu:2486-2502), then
+            // estimateDistortion and reInitializeMapping (solver.cu:279-286)
+            launch_bimocq_accumulate(stream, geom, bub_d, nb, bm);
+            launch_bimocq_reinit<T>(stream, geom, bub_d, cur, bm);
+        }
     }
 
     // ---- host <-> device copies
@@ -323,6 +367,7 @@ struct Impl : Base {
             nan |= std::isnan(h[k]);
         }
         SF_HIP(hipMemcpyAsync(dst + B.off, tmp.data(), N * sizeof(T), hipMemcpyHostToDevice, stream));
+        if (field == SF_FIELD_ETA || field == SF_FIELD_GAMMA) bimocq_dirty = true;
         if (nan && field <= SF_FIELD_GAMMA) {
             // the advection samples this bubble with NaN-skipping interpolation
             BubbleTime bt;
@@ -462,6 +507,7 @@ template <typename T, typename Ti>
 void finish_setup(Impl<T, Ti>& c) {
     c.alloc_fields();
     upload_air_shapes(c);
+    if (c.bimocq) c.alloc_bimocq();
 }
 
 void setup(Base& c, int prec) {
@@ -585,6 +631,12 @@ void setup(Base& c, int prec) {
     c.sp.tol = c.opt.tol > 0 ? c.opt.tol : 1e-10;
     c.sp.absolute = c.opt.tol_absolute;
     c.sp.inner_frac = 0.5;
+    if (c.opt.scalar_advection < 0 || c.opt.scalar_advection > 1)
+        throw Status{SF_ERR_INVALID, "scalar_advection must be 0 (plain) or 1 (BIMOCQ)"};
+    c.bimocq = c.opt.scalar_advection == 1;
+    c.bm.blend = c.opt.blend;
+    c.bm.threshold = c.opt.reinit_threshold != 0.0 ? c.opt.reinit_threshold : 4.0;
+    c.bm.multisample = c.opt.multisample ? 1 : 0;
     if (c.fp64_inner()) {
         c.sp.inner_rho2 = 0.0;
         c.max_outer = c.opt.max_outer > 0 ? c.opt.max_outer : 3;
@@ -934,6 +986,14 @@ int sf_get_stats(sf_ctx* ctx, int32_t b, sf_bubble_stats* out) {
         out->time = bt.t + (bt.pending ? bt.dt_s : 0.0);
         out->failed_solves = hs.fails;
         out->max_residual = hs.max_res;
+        if (c.bimocq) {
+            double dl = 0.0;
+            int ri = 0;
+            SF_HIP(hipMemcpy(&dl, c.bm.dist_last + b, sizeof(double), hipMemcpyDeviceToHost));
+            SF_HIP(hipMemcpy(&ri, c.bm.reinits + b, sizeof(int), hipMemcpyDeviceToHost));
+            out->distortion = dl;
+            out->reinits = ri;
+        }
         return (int)SF_OK;
     });
 }

@@ -551,6 +551,93 @@ __device__ __forceinline__ void transport_back(const VelCtx<T>& c, const Circle<
     r2 = b * o.u2 + a * o.v2;
 }
 
+// ------------------------------------------------------------ BIMOCQ ----
+// Mapping arithmetic of the BIMOCQ scalar advection, FP64 in every precision
+// mode (the maps hold index-space positions; FP32 would lose the small arc
+// angles acos and atan2 recover).
+struct P2d {
+    double x, y;   // (theta, phi) index coordinates
+};
+
+__device__ __forceinline__ double safe_acos_d(double x) { return acos(fmax(-1.0, fmin(1.0, x))); }
+
+__device__ __forceinline__ void unit_of(double t, double f, double& x, double& y, double& z) {
+    const double st = sin(t), ct = cos(t), sf = sin(f), cf = cos(f);
+    x = cf * st;
+    y = sf * st;
+    z = ct;
+    const double inv = 1.0 / sqrt(x * x + y * y + z * z);
+    x *= inv; y *= inv; z *= inv;
+}
+
+// lerpCoords (bubble.cu:512-536): the point a fraction alpha along the
+// great circle from `from` to `to` (Rodrigues rotation about k = from x to).
+__device__ P2d lerp_coords(double h, P2d from, P2d to, double alpha) {
+    double fx, fy, fz, tx, ty, tz;
+    unit_of(from.x * h, from.y * h, fx, fy, fz);
+    unit_of(to.x * h, to.y * h, tx, ty, tz);
+    double kx = fy * tz - fz * ty, ky = fz * tx - fx * tz, kz = fx * ty - fy * tx;
+    const double kin = 1.0 / sqrt(kx * kx + ky * ky + kz * kz);
+    kx *= kin; ky *= kin; kz *= kin;
+    if (__builtin_isnan(kx)) return from;
+    const double span = safe_acos_d(fx * tx + fy * ty + fz * tz) * alpha;
+    const double cs = cos(span), sn = sin(span);
+    const double kf = kx * fx + ky * fy + kz * fz;
+    const double cx = ky * fz - kz * fy, cy = kz * fx - kx * fz, cz = kx * fy - ky * fx;
+    const double rx = fx * cs + cx * sn + kx * kf * (1 - cs);
+    const double ry = fy * cs + cy * sn + ky * kf * (1 - cs);
+    const double rz = fz * cs + cz * sn + kz * kf * (1 - cs);
+    return P2d{safe_acos_d(rz) / h, atan2(ry, rx) / h};
+}
+
+// sampleMapping (bubble.cu:539-603): bilinear interpolation of a map's
+// index-space positions, each blend a spherical interpolation.
+__device__ P2d sample_mapping(const double* __restrict__ mt, const double* __restrict__ mp,
+                              int nth, int nph, double h, double rx, double ry) {
+    double x = rx - 0.5, y = ry;
+    const bool fl = validate_coord(x, y, nth, nph);
+    const int m = nph - 1;
+    int pI = (int)floor(y), tI = (int)floor(x);
+    const double aP = y - pI, aT = x - tI;
+    if ((unsigned)pI >= (unsigned)nph) pI = pI < 0 ? 0 : nph - 1;
+    auto at = [&](int row, int col) {
+        row = min(max(row, 0), nth - 1);
+        const size_t c = (size_t)row * nph + (col & m);
+        return P2d{mt[c], mp[c]};
+    };
+    P2d ll{0, 0}, lr{0, 0}, hl{0, 0}, hr{0, 0};
+    if (fl) {
+        if (tI == 0) {
+            hl = at(0, pI); hr = at(0, pI + 1);
+            ll = at(0, pI + nph / 2); lr = at(0, pI + 1 + nph / 2);
+        } else {
+            tI -= 1;
+        }
+    }
+    if (tI == nth - 1) {
+        ll = at(tI, pI); lr = at(tI, pI + 1);
+        hl = at(tI, pI + nph / 2); hr = at(tI, pI + 1 + nph / 2);
+    } else if (tI != 0 || !fl) {
+        ll = at(tI, pI); lr = at(tI, pI + 1);
+        hl = at(tI + 1, pI); hr = at(tI + 1, pI + 1);
+    }
+    return lerp_coords(h, lerp_coords(h, ll, lr, aP), lerp_coords(h, hl, hr, aP), aT);
+}
+
+// dist (bubble.cu:60-79): great-circle distance in grid cells
+__device__ __forceinline__ double grid_dist(double h, double invh, P2d a, P2d b) {
+    double ax, ay, az, bx, by, bz;
+    unit_of(a.x * h, a.y * h, ax, ay, az);
+    unit_of(b.x * h, b.y * h, bx, by, bz);
+    return invh * safe_acos_d(ax * bx + ay * by + az * bz);
+}
+
+// MULTISAMPLE (header.cuh:98): five samples around a point with these
+// weights and offsets (bubble.cu:983-995)
+__constant__ const double kMsW[5] = {0.125, 0.125, 0.125, 0.125, 0.5};
+__constant__ const double kMsDx[5] = {-0.25, 0.25, -0.25, 0.25, 0.0};
+__constant__ const double kMsDy[5] = {-0.25, -0.25, 0.25, 0.25, 0.0};
+
 // -------------------------------------------------------- reductions ----
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll

@@ -932,7 +932,8 @@ __global__ __launch_bounds__(256) void k_update(const int4* __restrict__ tiles,
                                                 const double* __restrict__ x1, StatePtrs<T> out,
                                                 int fin, SolveHist* __restrict__ hist,
                                                 SolveParams sp, const double* __restrict__ partR,
-                                                long stride) {
+                                                long stride, double* __restrict__ chg_e,
+                                                double* __restrict__ chg_g) {
     __shared__ double red[16 * 4];
     const int4 tl = tiles[blockIdx.x];
     if (tl.x < 0) return;
@@ -974,6 +975,12 @@ __global__ __launch_bounds__(256) void k_update(const int4* __restrict__ tiles,
         out.eta[c] = (T)fmax(kThresh, v);
         out.gam[c] = (T)gnew;
         nan_out = __builtin_isnan(gnew);
+        if (chg_e) {
+            // the changes of this step for BIMOCQ (bubble.cu:2062, 2486-2490),
+            // from the stored values
+            chg_e[c] = (double)(T)fmax(kThresh, v) - eta;
+            chg_g[c] = (double)(T)gnew - ga;
+        }
     }
     // applyforcevelphiKernel (bubble.cu:1879-1935)
     {
@@ -1039,6 +1046,294 @@ __global__ void k_row_stats(const int2* __restrict__ rows, const BubbleDev* __re
         out[B.pr0 + rw.y] = acc[0] * tab.sinTc[B.rt0 + rw.y];
         out[stride + B.pr0 + rw.y] = acc[1];
     }
+}
+
+// ------------------------------------------------------------ BIMOCQ ----
+// The BIMOCQ scalar advection of eta (header.cuh:97-98 BIMOCQ and
+// MULTISAMPLE): backward and forward maps between the current grid and the
+// positions at the last reinitialisation, eta sampled through them from its
+// initial field plus the changes accumulated since (bubble.cu:967-1013),
+// error-corrected reinitialisation when the maps drift apart
+// (bubble.cu:2509-2564). Gamma keeps the plain advection ("do not apply
+// bimocq on concentration for stability", bubble.cu:1195) but carries its
+// init/changes for the correction. One thread per cell, packed tiles.
+
+// initMapping (solver.cu:12-22) and Init/InitLast = current, changes = 0
+template <typename T>
+__global__ __launch_bounds__(256) void k_bm_init(const int4* __restrict__ tiles,
+                                                 const BubbleDev* __restrict__ bub,
+                                                 StatePtrs<T> cur, BimocqPtrs P) {
+    const int4 tl = tiles[blockIdx.x];
+    if (tl.x < 0) return;
+    const BubbleDev B = bub[tl.x];
+    const int j = tl.y + (int)threadIdx.x / tl.w, i = tl.z + (int)threadIdx.x % tl.w;
+    if (j >= B.nth || i >= B.nph) return;
+    const size_t c = (size_t)B.off + (size_t)j * B.nph + i;
+    P.fwd_t[c] = P.bwd_t[c] = P.prev_t[c] = j + 0.5;
+    P.fwd_p[c] = P.bwd_p[c] = P.prev_p[c] = (double)i;
+    P.e_init[c] = P.e_initlast[c] = (double)cur.eta[c];
+    P.g_init[c] = P.g_initlast[c] = (double)cur.gam[c];
+    P.e_delta[c] = P.e_deltalast[c] = P.g_delta[c] = P.g_deltalast[c] = 0.0;
+}
+
+// traceGreatCircle from the cell centre (j + 1/2, i) over dt (either sign)
+template <typename T, bool NANS>
+__device__ __forceinline__ void trace_centre(const VelCtx<T>& v, const Tables& tab,
+                                             const BubbleDev& B, int j, int i, T& tx, T& ty) {
+    const T gx = (T)j + T(0.5), gy = (T)i;
+    const T sinTc = TabSel<T>::sTc(tab)[B.rt0 + j], cosTc = TabSel<T>::cTc(tab)[B.rt0 + j];
+    const T sinPc = TabSel<T>::sPc(tab)[B.ct0 + i], cosPc = TabSel<T>::cPc(tab)[B.ct0 + i];
+    const T guT = sample_vtheta<T, NANS>(v.uth, B.nth, gx, gy);
+    const T guP = sample_vphi<T, NANS>(v.uph, B.nth, gx, gy);
+    Circle<T> o;
+    tx = gx;
+    ty = gy;
+    if (circle_mid<T, NANS>(v, gx, gy, sinTc, cosTc, sinPc, cosPc, guT, guP, o)) {
+        T s, co, px, py, pz, rho;
+        circle_depart(v, o, s, co, px, py, pz, tx, ty, rho);
+    }
+}
+
+// updateForward / updateBackward (bubble.cu:1049-1118 with
+// updateMappingKernel, bubble.cu:606-633): new map(x) = map(trace(x, -dt))
+// for the forward map and map(trace(x, +dt)) for the backward map.
+template <typename T, bool NANS>
+__device__ __forceinline__ void bm_maps_cell(const BubbleDev& B, const Tables& tab,
+                                             const StatePtrs<T>& cur, const BimocqPtrs& P, int j,
+                                             int i) {
+    VelCtx<T> v;
+    v.uth = Field<T>{cur.uth + B.off, B.nph, B.mask, B.nth - 2};
+    v.uph = Field<T>{cur.uph + B.off, B.nph, B.mask, B.nth - 1};
+    v.nth = B.nth;
+    v.invh = (T)B.invh;
+    const size_t o = (size_t)B.off, c = o + (size_t)j * B.nph + i;
+    for (int dir = 0; dir < 2; ++dir) {
+        v.dt = dir == 0 ? (T)(-B.dt) : (T)B.dt;
+        T tx, ty;
+        trace_centre<T, NANS>(v, tab, B, j, i, tx, ty);
+        const double* mt = dir == 0 ? P.fwd_t + o : P.bwd_t + o;
+        const double* mp = dir == 0 ? P.fwd_p + o : P.bwd_p + o;
+        P2d s = sample_mapping(mt, mp, B.nth, B.nph, B.h, (double)tx, (double)ty);
+        validate_coord(s.x, s.y, B.nth, B.nph);
+        (dir == 0 ? P.nfwd_t : P.nbwd_t)[c] = s.x;
+        (dir == 0 ? P.nfwd_p : P.nbwd_p)[c] = s.y;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_bm_maps(const int4* __restrict__ tiles,
+                                                 const BubbleDev* __restrict__ bub,
+                                                 const BubbleTime* __restrict__ btime, Tables tab,
+                                                 StatePtrs<T> cur, BimocqPtrs P) {
+    const int4 tl = tiles[blockIdx.x];
+    if (tl.x < 0) return;
+    const BubbleDev B = bub[tl.x];
+    const int j = tl.y + (int)threadIdx.x / tl.w, i = tl.z + (int)threadIdx.x % tl.w;
+    if (j >= B.nth || i >= B.nph) return;
+    if (btime[tl.x].nans)
+        bm_maps_cell<T, true>(B, tab, cur, P, j, i);
+    else
+        bm_maps_cell<T, false>(B, tab, cur, P, j, i);
+}
+
+// advectionCenteredBimocq (bubble.cu:967-1013) through the new backward map
+// and the previous interval's backward map; overwrites the plain eta*.
+template <typename T>
+__global__ __launch_bounds__(256) void k_bm_eta(const int4* __restrict__ tiles,
+                                                const BubbleDev* __restrict__ bub,
+                                                StatePtrs<T> cur, StatePtrs<T> out, BimocqPtrs P) {
+    const int4 tl = tiles[blockIdx.x];
+    if (tl.x < 0) return;
+    const BubbleDev B = bub[tl.x];
+    const int j = tl.y + (int)threadIdx.x / tl.w, i = tl.z + (int)threadIdx.x % tl.w;
+    if (j >= B.nth || i >= B.nph) return;
+    const size_t o = (size_t)B.off, c = o + (size_t)j * B.nph + i;
+    if ((double)cur.eta[c] == kThresh) {
+        out.eta[c] = (T)kThresh;
+        return;
+    }
+    auto F = [&](const double* a) { return Field<double>{a + o, B.nph, B.mask, B.nth - 1}; };
+    const Field<double> fil = F(P.e_initlast), fin = F(P.e_init), fd = F(P.e_delta),
+                        fdl = F(P.e_deltalast);
+    const int ns = P.multisample ? 5 : 1;
+    const double bl = P.blend;
+    double a = 0.0;
+    for (int s = 0; s < ns; ++s) {
+        const int q = ns == 5 ? s : 4;
+        const double w = ns == 5 ? kMsW[q] : 1.0;
+        const P2d ip = sample_mapping(P.nbwd_t + o, P.nbwd_p + o, B.nth, B.nph, B.h,
+                                      j + 0.5 + kMsDx[q], i + kMsDy[q]);
+        const P2d lp = sample_mapping(P.prev_t + o, P.prev_p + o, B.nth, B.nph, B.h, ip.x, ip.y);
+        a += (1.0 - bl) * w *
+             (sample_centered<double, true>(fil, B.nth, lp.x, lp.y) +
+              sample_centered<double, true>(fd, B.nth, ip.x, ip.y) +
+              sample_centered<double, true>(fdl, B.nth, lp.x, lp.y));
+        a += bl * w * (sample_centered<double, true>(fin, B.nth, ip.x, ip.y) +
+                       sample_centered<double, true>(fd, B.nth, ip.x, ip.y));
+    }
+    out.eta[c] = (T)fmax(kThresh, a);
+}
+
+// accumulateChangesCentered (bubble.cu:2081-2114) for eta and Gamma through
+// the new forward map, the new maps made current, and the distortion
+// (estimateDistortion, bubble.cu:1133-1168) of the new maps: its block
+// maximum goes to the bubble's running maximum.
+__global__ __launch_bounds__(256) void k_bm_accum(const int4* __restrict__ tiles,
+                                                  const BubbleDev* __restrict__ bub,
+                                                  BimocqPtrs P) {
+    __shared__ float red[8];
+    const int4 tl = tiles[blockIdx.x];
+    if (tl.x < 0) return;
+    const BubbleDev B = bub[tl.x];
+    const int j = tl.y + (int)threadIdx.x / tl.w, i = tl.z + (int)threadIdx.x % tl.w;
+    float dmax = 0.0f;
+    if (j < B.nth && i < B.nph) {
+        const size_t o = (size_t)B.off, c = o + (size_t)j * B.nph + i;
+        const Field<double> ce{P.chg_e + o, B.nph, B.mask, B.nth - 1};
+        const Field<double> cg{P.chg_g + o, B.nph, B.mask, B.nth - 1};
+        const int ns = P.multisample ? 5 : 1;
+        double de = 0.0, dg = 0.0;
+        for (int s = 0; s < ns; ++s) {
+            const int q = ns == 5 ? s : 4;
+            const double w = ns == 5 ? kMsW[q] : 1.0;
+            const P2d ip = sample_mapping(P.nfwd_t + o, P.nfwd_p + o, B.nth, B.nph, B.h,
+                                          j + 0.5 + kMsDx[q], i + kMsDy[q]);
+            de += w * sample_centered<double, true>(ce, B.nth, ip.x, ip.y);
+            dg += w * sample_centered<double, true>(cg, B.nth, ip.x, ip.y);
+        }
+        P.e_delta[c] += de;
+        P.g_delta[c] += dg;
+        const P2d f{P.nfwd_t[c], P.nfwd_p[c]}, b{P.nbwd_t[c], P.nbwd_p[c]};
+        const P2d x{j + 0.5, (double)i};
+        const P2d a = sample_mapping(P.nbwd_t + o, P.nbwd_p + o, B.nth, B.nph, B.h, f.x, f.y);
+        const P2d e = sample_mapping(P.nfwd_t + o, P.nfwd_p + o, B.nth, B.nph, B.h, b.x, b.y);
+        dmax = fmaxf((float)grid_dist(B.h, B.invh, x, a), (float)grid_dist(B.h, B.invh, x, e));
+        P.fwd_t[c] = f.x;
+        P.fwd_p[c] = f.y;
+        P.bwd_t[c] = b.x;
+        P.bwd_p[c] = b.y;
+    }
+    dmax = wave_max(dmax);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = dmax;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float m = 0.0f;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) m = fmaxf(m, red[w]);
+        // non-negative floats order as their bit patterns
+        atomicMax(P.dist_bits + tl.x, __float_as_uint(m));
+    }
+}
+
+// The reinitialisation decision of each bubble; clears the running maximum.
+__global__ void k_bm_decide(int nb, BimocqPtrs P) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    const float d = __uint_as_float(P.dist_bits[b]);
+    P.dist_bits[b] = 0u;
+    P.dist_last[b] = (double)d;
+    const int f = (double)d > P.threshold ? 1 : 0;
+    P.flag[b] = f;
+    P.reinits[b] += f;
+}
+
+// correctCentered1 (bubble.cu:2180-2220): the error of the forward-mapped
+// current fields against init + changes; keeps copies of the fields
+template <typename T>
+__global__ __launch_bounds__(256) void k_bm_correct1(const int4* __restrict__ tiles,
+                                                     const BubbleDev* __restrict__ bub,
+                                                     StatePtrs<T> cur, BimocqPtrs P) {
+    const int4 tl = tiles[blockIdx.x];
+    if (tl.x < 0 || !P.flag[tl.x]) return;
+    const BubbleDev B = bub[tl.x];
+    const int j = tl.y + (int)threadIdx.x / tl.w, i = tl.z + (int)threadIdx.x % tl.w;
+    if (j >= B.nth || i >= B.nph) return;
+    const size_t o = (size_t)B.off, c = o + (size_t)j * B.nph + i;
+    const Field<T> fe{cur.eta + o, B.nph, B.mask, B.nth - 1};
+    const Field<T> fg{cur.gam + o, B.nph, B.mask, B.nth - 1};
+    double te = 0.0, tg = 0.0;
+    for (int q = 0; q < 5; ++q) {
+        const P2d ip = sample_mapping(P.fwd_t + o, P.fwd_p + o, B.nth, B.nph, B.h,
+                                      j + 0.5 + kMsDx[q], i + kMsDy[q]);
+        te += kMsW[q] * (double)sample_centered<T, true>(fe, B.nth, (T)ip.x, (T)ip.y);
+        tg += kMsW[q] * (double)sample_centered<T, true>(fg, B.nth, (T)ip.x, (T)ip.y);
+    }
+    P.err_e[c] = (te - P.e_delta[c] - P.e_init[c]) * 0.5;
+    P.err_g[c] = (tg - P.g_delta[c] - P.g_init[c]) * 0.5;
+    P.cpy_e[c] = (double)cur.eta[c];
+    P.cpy_g[c] = (double)cur.gam[c];
+}
+
+// Bounds of the 3 x 3 neighbourhood of a cell in float, as the reference's
+// fminf/fmaxf take them (bubble.cu:2248-2262); theta = -1 reflects across the
+// pole like theta = nTheta (the reference's validateId leaves -1 negative).
+__device__ __forceinline__ void local_bounds(const double* __restrict__ a, int nth, int nph, int j,
+                                             int i, float& lo, float& hi) {
+    lo = hi = (float)a[(size_t)j * nph + i];
+    for (int t = -1; t <= 1; ++t)
+        for (int s = -1; s <= 1; ++s) {
+            int jj = j + t, ii = i + s;
+            if (jj < 0 || jj >= nth) {
+                jj = jj < 0 ? 0 : nth - 1;
+                ii += nth;
+            }
+            const float v = (float)a[(size_t)jj * nph + (ii & (nph - 1))];
+            lo = fminf(lo, v);
+            hi = fmaxf(hi, v);
+        }
+}
+
+// correctCentered2 (bubble.cu:2223-2280): subtract the backward-mapped error,
+// clamp to the local extrema of the uncorrected field (in float)
+template <typename T>
+__global__ __launch_bounds__(256) void k_bm_correct2(const int4* __restrict__ tiles,
+                                                     const BubbleDev* __restrict__ bub,
+                                                     StatePtrs<T> cur, BimocqPtrs P) {
+    const int4 tl = tiles[blockIdx.x];
+    if (tl.x < 0 || !P.flag[tl.x]) return;
+    const BubbleDev B = bub[tl.x];
+    const int j = tl.y + (int)threadIdx.x / tl.w, i = tl.z + (int)threadIdx.x % tl.w;
+    if (j >= B.nth || i >= B.nph) return;
+    const size_t o = (size_t)B.off, c = o + (size_t)j * B.nph + i;
+    const Field<double> ee{P.err_e + o, B.nph, B.mask, B.nth - 1};
+    const Field<double> eg{P.err_g + o, B.nph, B.mask, B.nth - 1};
+    double ve = P.cpy_e[c], vg = P.cpy_g[c];
+    for (int q = 0; q < 5; ++q) {
+        const P2d ip = sample_mapping(P.bwd_t + o, P.bwd_p + o, B.nth, B.nph, B.h,
+                                      j + 0.5 + kMsDx[q], i + kMsDy[q]);
+        ve -= kMsW[q] * sample_centered<double, true>(ee, B.nth, ip.x, ip.y);
+        vg -= kMsW[q] * sample_centered<double, true>(eg, B.nth, ip.x, ip.y);
+    }
+    float lo, hi;
+    local_bounds(P.cpy_e + o, B.nth, B.nph, j, i, lo, hi);
+    cur.eta[c] = (T)fmax(kThresh, (double)fmaxf(lo, fminf((float)ve, hi)));
+    local_bounds(P.cpy_g + o, B.nth, B.nph, j, i, lo, hi);
+    cur.gam[c] = (T)fmax(kThresh, (double)fmaxf(lo, fminf((float)vg, hi)));
+}
+
+// The rest of reInitializeMapping (bubble.cu:2538-2563): the previous
+// interval takes the current init, changes and backward map; init = the
+// corrected fields, changes = 0, maps = identity.
+template <typename T>
+__global__ __launch_bounds__(256) void k_bm_finish(const int4* __restrict__ tiles,
+                                                   const BubbleDev* __restrict__ bub,
+                                                   StatePtrs<T> cur, BimocqPtrs P) {
+    const int4 tl = tiles[blockIdx.x];
+    if (tl.x < 0 || !P.flag[tl.x]) return;
+    const BubbleDev B = bub[tl.x];
+    const int j = tl.y + (int)threadIdx.x / tl.w, i = tl.z + (int)threadIdx.x % tl.w;
+    if (j >= B.nth || i >= B.nph) return;
+    const size_t c = (size_t)B.off + (size_t)j * B.nph + i;
+    P.e_initlast[c] = P.e_init[c];
+    P.e_deltalast[c] = P.e_delta[c];
+    P.g_initlast[c] = P.g_init[c];
+    P.g_deltalast[c] = P.g_delta[c];
+    P.prev_t[c] = P.bwd_t[c];
+    P.prev_p[c] = P.bwd_p[c];
+    P.e_init[c] = (double)cur.eta[c];
+    P.g_init[c] = (double)cur.gam[c];
+    P.e_delta[c] = P.g_delta[c] = 0.0;
+    P.fwd_t[c] = P.bwd_t[c] = j + 0.5;
+    P.fwd_p[c] = P.bwd_p[c] = (double)i;
 }
 
 // ------------------------------------------------------------ launchers ----
@@ -1121,9 +1416,10 @@ template <typename T>
 void launch_update(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, BubbleTime* bt,
                    SolveCtl* ctl, const Tables& tab, StatePtrs<T> st, AirPtrs<T> air,
                    const double* x0, const double* x1, StatePtrs<T> out, int fin, SolveHist* hist,
-                   const SolveParams& sp, const double* partR, long stride) {
+                   const SolveParams& sp, const double* partR, long stride, double* chg_e,
+                   double* chg_g) {
     hipLaunchKernelGGL(k_update<T>, dim3(g.ntiles), dim3(256), 0, s, g.tiles, bub, bt, ctl, tab, st,
-                       air, x0, x1, out, fin, hist, sp, partR, stride);
+                       air, x0, x1, out, fin, hist, sp, partR, stride, chg_e, chg_g);
 }
 
 template <typename T>
@@ -1163,14 +1459,57 @@ SF_INST_TI(double)
 template void launch_update<float>(hipStream_t, const LaunchGeom&, const BubbleDev*, BubbleTime*,
                                    SolveCtl*, const Tables&, StatePtrs<float>, AirPtrs<float>,
                                    const double*, const double*, StatePtrs<float>, int,
-                                   SolveHist*, const SolveParams&, const double*, long);
+                                   SolveHist*, const SolveParams&, const double*, long, double*,
+                                   double*);
 template void launch_update<double>(hipStream_t, const LaunchGeom&, const BubbleDev*, BubbleTime*,
                                     SolveCtl*, const Tables&, StatePtrs<double>, AirPtrs<double>,
                                     const double*, const double*, StatePtrs<double>, int,
-                                    SolveHist*, const SolveParams&, const double*, long);
+                                    SolveHist*, const SolveParams&, const double*, long, double*,
+                                    double*);
 template void launch_row_stats<float>(hipStream_t, const LaunchGeom&, const BubbleDev*,
                                       const Tables&, const float*, double*, long);
 template void launch_row_stats<double>(hipStream_t, const LaunchGeom&, const BubbleDev*,
                                        const Tables&, const double*, double*, long);
+
+
+template <typename T>
+void launch_bimocq_init(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub,
+                        StatePtrs<T> cur, const BimocqPtrs& P) {
+    hipLaunchKernelGGL(k_bm_init<T>, dim3(g.ntiles), dim3(256), 0, s, g.tiles, bub, cur, P);
+}
+template <typename T>
+void launch_bimocq_maps(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub,
+                        BubbleTime* bt, const Tables& tab, StatePtrs<T> cur, const BimocqPtrs& P) {
+    hipLaunchKernelGGL(k_bm_maps<T>, dim3(g.ntiles), dim3(256), 0, s, g.tiles, bub, bt, tab, cur, P);
+}
+template <typename T>
+void launch_bimocq_eta(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub,
+                       StatePtrs<T> cur, StatePtrs<T> out, const BimocqPtrs& P) {
+    hipLaunchKernelGGL(k_bm_eta<T>, dim3(g.ntiles), dim3(256), 0, s, g.tiles, bub, cur, out, P);
+}
+void launch_bimocq_accumulate(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, int nb,
+                              const BimocqPtrs& P) {
+    hipLaunchKernelGGL(k_bm_accum, dim3(g.ntiles), dim3(256), 0, s, g.tiles, bub, P);
+    hipLaunchKernelGGL(k_bm_decide, dim3((nb + 63) / 64), dim3(64), 0, s, nb, P);
+}
+template <typename T>
+void launch_bimocq_reinit(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub,
+                          StatePtrs<T> cur, const BimocqPtrs& P) {
+    hipLaunchKernelGGL(k_bm_correct1<T>, dim3(g.ntiles), dim3(256), 0, s, g.tiles, bub, cur, P);
+    hipLaunchKernelGGL(k_bm_correct2<T>, dim3(g.ntiles), dim3(256), 0, s, g.tiles, bub, cur, P);
+    hipLaunchKernelGGL(k_bm_finish<T>, dim3(g.ntiles), dim3(256), 0, s, g.tiles, bub, cur, P);
+}
+#define SF_INST_BM(T)                                                                             \
+    template void launch_bimocq_init<T>(hipStream_t, const LaunchGeom&, const BubbleDev*,         \
+                                        StatePtrs<T>, const BimocqPtrs&);                         \
+    template void launch_bimocq_maps<T>(hipStream_t, const LaunchGeom&, const BubbleDev*,         \
+                                        BubbleTime*, const Tables&, StatePtrs<T>,                 \
+                                        const BimocqPtrs&);                                       \
+    template void launch_bimocq_eta<T>(hipStream_t, const LaunchGeom&, const BubbleDev*,          \
+                                       StatePtrs<T>, StatePtrs<T>, const BimocqPtrs&);            \
+    template void launch_bimocq_reinit<T>(hipStream_t, const LaunchGeom&, const BubbleDev*,       \
+                                          StatePtrs<T>, const BimocqPtrs&);
+SF_INST_BM(float)
+SF_INST_BM(double)
 
 }  // namespace sfk

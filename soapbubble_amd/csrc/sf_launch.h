@@ -5,6 +5,25 @@
 
 namespace sfk {
 
+// BIMOCQ state of all bubbles (packed like the fields, FP64): maps (index
+// positions), the new maps of the current step, and for eta and Gamma the
+// initial fields and accumulated changes of the current and the previous
+// reinitialisation interval; per-step changes, errors and copies; per-bubble
+// distortion, reinitialisation flag and counters.
+struct BimocqPtrs {
+    double *fwd_t, *fwd_p, *bwd_t, *bwd_p, *prev_t, *prev_p;
+    double *nfwd_t, *nfwd_p, *nbwd_t, *nbwd_p;
+    double *e_init, *e_initlast, *e_delta, *e_deltalast;
+    double *g_init, *g_initlast, *g_delta, *g_deltalast;
+    double *chg_e, *chg_g, *err_e, *err_g, *cpy_e, *cpy_g;
+    unsigned* dist_bits;   // per bubble: running max distortion (float bits)
+    int* flag;             // per bubble: reinitialise at this step
+    double* dist_last;     // per bubble: distortion of the last step
+    int* reinits;          // per bubble: reinitialisations so far
+    double blend, threshold;
+    int multisample;
+};
+
 template <typename T>
 struct StatePtrs {
     T* uth;
@@ -86,7 +105,23 @@ template <typename T>
 void launch_update(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, BubbleTime* bt,
                    SolveCtl* ctl, const Tables& tab, StatePtrs<T> st, AirPtrs<T> air,
                    const double* x0, const double* x1, StatePtrs<T> out, int fin, SolveHist* hist,
-                   const SolveParams& sp, const double* partR, long stride);
+                   const SolveParams& sp, const double* partR, long stride, double* chg_e,
+                   double* chg_g);
+// BIMOCQ stages (packed over all bubbles)
+template <typename T>
+void launch_bimocq_init(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub,
+                        StatePtrs<T> cur, const BimocqPtrs& P);
+template <typename T>
+void launch_bimocq_maps(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub,
+                        BubbleTime* bt, const Tables& tab, StatePtrs<T> cur, const BimocqPtrs& P);
+template <typename T>
+void launch_bimocq_eta(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub,
+                       StatePtrs<T> cur, StatePtrs<T> out, const BimocqPtrs& P);
+void launch_bimocq_accumulate(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, int nb,
+                              const BimocqPtrs& P);
+template <typename T>
+void launch_bimocq_reinit(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub,
+                          StatePtrs<T> cur, const BimocqPtrs& P);
 template <typename T>
 void launch_row_stats(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, const Tables& tab,
                       const T* eta, double* out, long stride);
