@@ -173,7 +173,8 @@ void Solver::check(int rc) {
 }
 
 Solver::Solver(size_t nPhi, size_t nTheta, double radius, double dt, double H, int device,
-               const std::string& AMGconfig, sf_precision precision, double Gamma_m, double T)
+               const std::string& AMGconfig, sf_precision precision, double Gamma_m, double T,
+               bool bimocq, double blend)
     : nPhi_(nPhi), nTheta_(nTheta), radius_(radius), dt_(dt) {
     if (nPhi != 2 * nTheta) throw Error(SF_ERR_UNSUPPORTED, "nPhi must be 2 nTheta");
     SolverSettings st;
@@ -204,6 +205,10 @@ Solver::Solver(size_t nPhi, size_t nTheta, double radius, double dt, double H, i
     o.tol_absolute = st.absolute ? 1 : 0;
     o.max_inner = st.max_iters;
     o.advance_time = 0;   // the caller sets the frame-start time
+    o.scalar_advection = bimocq ? 1 : 0;
+    o.multisample = 1;
+    o.blend = blend;
+    o.reinit_threshold = 4.0;   // solver.cu:284
     check(sf_create(&o, 1, &d, &ctx_));
 }
 
@@ -279,12 +284,13 @@ void Solver::copyToCPU(sf_field field, std::vector<double>& out) {
 Bubble::Bubble(double radius, double H, double U, double c_m, double Gamma_m, double T, double Ds,
                double rm, size_t nTheta, double dt, double DT, int frames, std::string outputDir,
                std::string thicknessImage, int device, std::string AMGconfig, double blendCoeff)
-    : radius_(radius), H_(H), Gamma_m_(Gamma_m), T_(T), dt_(dt), DT_(DT), nTheta_(nTheta),
+    : radius_(radius), H_(H), Gamma_m_(Gamma_m), T_(T), dt_(dt), DT_(DT), blend_(blendCoeff),
+      nTheta_(nTheta),
       frames_(frames), device_(device), outputDir_(std::move(outputDir)),
       thicknessImage_(std::move(thicknessImage)), AMGconfig_(std::move(AMGconfig)) {
-    // U is taken as 1 as in the reference (bubble.cu:2566); c_m, Ds, rm and
-    // blendCoeff do not enter the plain step
-    (void)U; (void)c_m; (void)Ds; (void)rm; (void)blendCoeff;
+    // U is taken as 1 as in the reference (bubble.cu:2566); c_m, Ds and rm
+    // do not enter the step
+    (void)U; (void)c_m; (void)Ds; (void)rm;
     if (!outputDir_.empty() && outputDir_.back() != '/') outputDir_ += '/';
 }
 
@@ -295,7 +301,7 @@ Bubble::Bubble(const Config& c)
 int Bubble::run() {
     if (!outputDir_.empty()) std::filesystem::create_directories(outputDir_);
     Solver solver(2 * nTheta_, nTheta_, radius_, dt_, H_, device_, AMGconfig_, precision, Gamma_m_,
-                  T_);
+                  T_, bimocq, blend_);
     solver.initThicknessfromPic(thicknessImage_);
     solver.initAirflowfromPic(airImage);
     solver.write_thickness_img(outputDir_, 0);

@@ -49,9 +49,12 @@ public:
     // Solver::Solver (solver.cu:63-160): nPhi = 2 nTheta; dt in seconds.
     // Gamma_m and T (config.txt) reach the reference's kernels as constants
     // uploaded by Bubble::run; here they are constructor arguments.
+    // The scalar advection is BIMOCQ with five samples per point, as
+    // header.cuh:97-98 builds the reference, with blendCoeff `blend`;
+    // bimocq = false selects the plain advectionAllCentered.
     Solver(size_t nPhi, size_t nTheta, double radius, double dt, double H, int device,
            const std::string& AMGconfig, sf_precision precision = SF_PREC_FP64,
-           double Gamma_m = 1e-6, double T = 298.15);
+           double Gamma_m = 1e-6, double T = 298.15, bool bimocq = true, double blend = 0.5);
     ~Solver();
     Solver(const Solver&) = delete;
     Solver& operator=(const Solver&) = delete;
@@ -89,6 +92,7 @@ public:
     // ../init/velocityfield_512.exr relative to the build directory).
     std::string airImage = "../init/velocityfield_512.exr";
     sf_precision precision = SF_PREC_FP64;
+    bool bimocq = true;   // header.cuh:97 defines BIMOCQ
     bool verbose = true;
     // Bubble::run (bubble.cu:2609-2704): frames of DT seconds, each made of
     // steps of dt and a partial last step; the air-flow time is the frame
@@ -97,7 +101,7 @@ public:
     int run();
 
 private:
-    double radius_, H_, Gamma_m_, T_, dt_, DT_;
+    double radius_, H_, Gamma_m_, T_, dt_, DT_, blend_;
     size_t nTheta_;
     int frames_, device_;
     std::string outputDir_, thicknessImage_, AMGconfig_;

@@ -1,18 +1,24 @@
 // soapBubble: the reference's command-line driver (kernel/main.cu:4-56) over
-// libsoapfilm.so. Usage: soapBubble config.txt [fp64|mixed] [air.exr]
+// libsoapfilm.so.
+//   soapBubble config.txt [--precision fp64|mixed] [--advection bimocq|plain]
+//                         [--air airflow.exr]
 #include <cstring>
 #include <iostream>
+#include <string>
 
 #include "bubble.h"
 
+static int usage() {
+    std::cout << "Please provide the path to config.txt as an argument." << std::endl;
+    std::cout << "Usage example: ./soapBubble ../config.txt [--precision fp64|mixed] "
+                 "[--advection bimocq|plain] [--air airflow.exr]"
+              << std::endl;
+    std::cout << "Configuration file was missing, exiting." << std::endl;
+    return -1;
+}
+
 int main(int argc, char** argv) {
-    if (argc < 2 || argc > 4) {
-        std::cout << "Please provide the path to config.txt as an argument." << std::endl;
-        std::cout << "Usage example: ./soapBubble ../config.txt [fp64|mixed] [airflow.exr]"
-                  << std::endl;
-        std::cout << "Configuration file was missing, exiting." << std::endl;
-        return -1;
-    }
+    if (argc < 2 || (argc % 2) != 0) return usage();
     sfb::Config cfg;
     std::string err;
     if (!sfb::read_config(argv[1], cfg, err)) {
@@ -21,15 +27,21 @@ int main(int argc, char** argv) {
     }
     try {
         sfb::Bubble bubble(cfg);
-        if (argc >= 3) {
-            if (std::strcmp(argv[2], "mixed") == 0) {
-                bubble.precision = SF_PREC_MIXED;
-            } else if (std::strcmp(argv[2], "fp64") != 0) {
-                std::cerr << "precision must be fp64 or mixed" << std::endl;
+        for (int k = 2; k + 1 < argc; k += 2) {
+            const std::string key = argv[k], val = argv[k + 1];
+            if (key == "--precision" && (val == "fp64" || val == "mixed")) {
+                bubble.precision = val == "mixed" ? SF_PREC_MIXED : SF_PREC_FP64;
+            } else if (key == "--advection" && (val == "bimocq" || val == "plain")) {
+                bubble.bimocq = val == "bimocq";
+            } else if (key == "--air") {
+                bubble.airImage = val;
+            } else {
+                std::cerr << "unknown option " << key << " " << val
+                          << " (--precision fp64|mixed, --advection bimocq|plain, --air path)"
+                          << std::endl;
                 return 2;
             }
         }
-        if (argc == 4) bubble.airImage = argv[3];
         const int done = bubble.run();
         std::cout << "Frames completed: " << done << std::endl;
     } catch (const sfb::Error& e) {

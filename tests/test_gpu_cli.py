@@ -26,8 +26,9 @@ def _smooth(n, seed):
     return f / np.abs(f).max()
 
 
-def test_cli_frames_match_oracle(tmp_path, oracle):
-    print("soapBubble 32x64, 3 frames of 0.012 s at dt 0.005 s", flush=True)
+@pytest.mark.parametrize("advection", ["plain", "bimocq"])
+def test_cli_frames_match_oracle(tmp_path, oracle, advection):
+    print(f"soapBubble 32x64 {advection}, 3 frames of 0.012 s at dt 0.005 s", flush=True)
     n, radius, H, dt, DT, frames = 32, 0.05, 1e-6, 0.005, 0.012, 3
     thick = str(tmp_path / "thickness.exr")
     air = str(tmp_path / "air.exr")
@@ -36,7 +37,8 @@ def test_cli_frames_match_oracle(tmp_path, oracle):
     _amg(tmp_path, tol="1e-12", conv="RELATIVE")
     cfg = _config(tmp_path, n_theta=n, radius=radius, H=H, timePerStep=dt, timePerFrame=DT,
                   frameNumbers=frames, thicknessInit=thick)
-    r = subprocess.run([CLI, cfg, "fp64", air], capture_output=True, text=True, timeout=120)
+    r = subprocess.run([CLI, cfg, "--precision", "fp64", "--advection", advection, "--air", air],
+                       capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     out = tmp_path / "out"
 
@@ -45,17 +47,27 @@ def test_cli_frames_match_oracle(tmp_path, oracle):
     st = {"u_theta": np.zeros((n - 1) * 2 * n), "u_phi": np.zeros(2 * n * n),
           "eta": f["eta"].copy(), "gamma": np.ones(2 * n * n)}
     kw = dict(radius=radius, H=H, gravity=True, evaporation=True, tol=1e-12)
+    bm = None
+    if advection == "bimocq":
+        # blendCoeff 0.5 of the config, five samples, threshold 4 cells
+        bm = oracle.BimocqState(oracle.params(n, dt, **kw), st["eta"], st["gamma"], blend=0.5)
+
+    def one(p):
+        if bm is None:
+            oracle.step(p, st, f["uair0"], f["vair0"], 1, advance_time=False)
+        else:
+            oracle.bimocq_step(p, st, f["uair0"], f["vair0"], bm, 1, advance_time=False)
+
     frames_ref = [st["eta"].copy()]
     T = 0.0
     for i in range(1, frames):
         t0 = T
         p = oracle.params(n, dt, time_s=t0, **kw)
         while T + dt <= i * DT:
-            oracle.step(p, st, f["uair0"], f["vair0"], 1, advance_time=False)
+            one(p)
             T += dt
         if T + 1e-7 < i * DT:
-            p2 = oracle.params(n, i * DT - T, time_s=t0, **kw)
-            oracle.step(p2, st, f["uair0"], f["vair0"], 1, advance_time=False)
+            one(oracle.params(n, i * DT - T, time_s=t0, **kw))
         T = i * DT
         frames_ref.append(st["eta"].copy())
 

@@ -142,8 +142,10 @@ def test_cli_rejects_grid_before_gpu(tmp_path):
     assert r.returncode == 3 and "power of two" in r.stderr
     # the outer solver's settings, not the preconditioner's max_iters = 1
     assert "tolerance 1e-09 RELATIVE, max_iters 37" in r.stdout
-    r = _run(_config(tmp_path), "single")
-    assert r.returncode == 2 and "fp64 or mixed" in r.stderr
+    r = _run(_config(tmp_path), "--precision", "single")
+    assert r.returncode == 2 and "--precision fp64|mixed" in r.stderr
+    r = _run(_config(tmp_path), "--precision")
+    assert r.returncode == 255
 
 
 # ------------------------------------------------------------ EXR output ----
