@@ -94,6 +94,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-bimocq", action="store_true")
+    ap.add_argument("--bimocq-steps", type=int, default=10)
     ap.add_argument("--profile", choices=("graph", "eager"), default=None,
                     help="run only the headline steps (for rocprofv3), graph or eager launches")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -193,6 +195,34 @@ def main():
         err_oracle.update({kk: rel_l2(g[kk], st[kk]) for kk in ("eta", "gamma", "u_theta",
                                                                  "u_phi")})
 
+    # the reference's default scalar advection (BIMOCQ, five samples per
+    # point, blendCoeff 0.5) on the same input: mixed with a graph, and the
+    # FP64 path for the error
+    bimocq = None
+    if not args.no_bimocq:
+        nb_steps = args.bimocq_steps
+        ctx = sb.SoapFilm(sb.bubble(n, dt), "mixed", use_graph=True, scalar_advection="bimocq")
+        ctx.load_state(0, st0)
+        ctx.upload(0, "uair0", ua)
+        ctx.upload(0, "vair0", va)
+        elb = timed_run(ctx, args.warmup, nb_steps)
+        gb = ctx.state(0)
+        sbm = ctx.stats(0)
+        ctx.close()
+        ctx = sb.SoapFilm(sb.bubble(n, dt), "fp64", scalar_advection="bimocq")
+        ctx.load_state(0, st0)
+        ctx.upload(0, "uair0", ua)
+        ctx.upload(0, "vair0", va)
+        ctx.step(args.warmup + nb_steps)
+        rb = ctx.state(0)
+        ctx.close()
+        bimocq = {"value": cells * nb_steps / elb, "unit": "cell-steps/s",
+                  "ms_per_step": elb * 1e3 / nb_steps, "steps": nb_steps, "warmup": args.warmup,
+                  "distortion": sbm["distortion"], "reinits": sbm["reinits"],
+                  "failed_solves": sbm["failed_solves"],
+                  "err_vs_fp64_gpu": {k: rel_l2(gb[k], rb[k]) for k in ("eta", "gamma",
+                                                                        "u_theta", "u_phi")}}
+
     # HBM bytes per advection launch measured by rocprofv3 for this workload
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "hbm_traffic_1024x2048_mixed.json")
@@ -228,6 +258,7 @@ def main():
         "speedup_vs_fp64": (value / fp64) if fp64 else None,
         "err_vs_fp64_gpu": err_fp64,
         "err_vs_oracle": err_oracle,
+        "bimocq": bimocq,
     }
     print(json.dumps(out))
 

@@ -175,8 +175,12 @@ struct Impl : Base {
 
     void alloc_bimocq() {
         const size_t n = (size_t)ncells;
-        double** arrays[] = {&bm.fwd_t, &bm.fwd_p, &bm.bwd_t, &bm.bwd_p, &bm.prev_t, &bm.prev_p,
-                             &bm.nfwd_t, &bm.nfwd_p, &bm.nbwd_t, &bm.nbwd_p, &bm.e_init,
+        for (UMap* M : {&bm.fwd, &bm.bwd, &bm.prev, &bm.nfwd, &bm.nbwd}) {
+            M->x = dalloc<double>(n, owned);
+            M->y = dalloc<double>(n, owned);
+            M->z = dalloc<double>(n, owned);
+        }
+        double** arrays[] = {&bm.e_init,
                              &bm.e_initlast, &bm.e_delta, &bm.e_deltalast, &bm.g_init,
                              &bm.g_initlast, &bm.g_delta, &bm.g_deltalast, &bm.chg_e, &bm.chg_g,
                              &bm.err_e, &bm.err_g, &bm.cpy_e, &bm.cpy_g};

@@ -553,47 +553,58 @@ __device__ __forceinline__ void transport_back(const VelCtx<T>& c, const Circle<
 
 // ------------------------------------------------------------ BIMOCQ ----
 // Mapping arithmetic of the BIMOCQ scalar advection, FP64 in every precision
-// mode (the maps hold index-space positions; FP32 would lose the small arc
-// angles acos and atan2 recover).
-struct P2d {
-    double x, y;   // (theta, phi) index coordinates
+// mode. The reference keeps a map as index-space (theta, phi) pairs and
+// converts every corner to a unit vector inside each spherical
+// interpolation (lerpCoords, bubble.cu:512-536), and the result back. Here a
+// map holds the unit vectors themselves: the interpolations chain on unit
+// vectors, and only a position that samples a field is converted to index
+// coordinates (acos of z, atan2 of y, x, as lerpCoords ends). The same
+// arithmetic up to rounding, without two sin/cos pairs and an acos/atan2 per
+// interpolation.
+struct U3 {
+    double x, y, z;
+};
+struct UMap {
+    double *x, *y, *z;
 };
 
 __device__ __forceinline__ double safe_acos_d(double x) { return acos(fmax(-1.0, fmin(1.0, x))); }
 
-__device__ __forceinline__ void unit_of(double t, double f, double& x, double& y, double& z) {
-    const double st = sin(t), ct = cos(t), sf = sin(f), cf = cos(f);
-    x = cf * st;
-    y = sf * st;
-    z = ct;
-    const double inv = 1.0 / sqrt(x * x + y * y + z * z);
-    x *= inv; y *= inv; z *= inv;
+__device__ __forceinline__ U3 unit_of_index(double h, double t, double f) {
+    double st, ct, sf, cf;
+    sincos(t * h, &st, &ct);
+    sincos(f * h, &sf, &cf);
+    return U3{cf * st, sf * st, ct};
 }
 
-// lerpCoords (bubble.cu:512-536): the point a fraction alpha along the
-// great circle from `from` to `to` (Rodrigues rotation about k = from x to).
-__device__ P2d lerp_coords(double h, P2d from, P2d to, double alpha) {
-    double fx, fy, fz, tx, ty, tz;
-    unit_of(from.x * h, from.y * h, fx, fy, fz);
-    unit_of(to.x * h, to.y * h, tx, ty, tz);
-    double kx = fy * tz - fz * ty, ky = fz * tx - fx * tz, kz = fx * ty - fy * tx;
+struct P2d {
+    double x, y;   // (theta, phi) index coordinates
+};
+
+__device__ __forceinline__ P2d index_of_unit(double invh, const U3& r) {
+    return P2d{safe_acos_d(r.z) * invh, atan2(r.y, r.x) * invh};
+}
+
+// lerpCoords on unit vectors: the point a fraction alpha along the great
+// circle from f to t (Rodrigues rotation about k = f x t / |f x t|); f when
+// the two coincide (k undefined), as the reference returns `from`.
+__device__ __forceinline__ U3 slerp_u(const U3& f, const U3& t, double alpha) {
+    double kx = f.y * t.z - f.z * t.y, ky = f.z * t.x - f.x * t.z, kz = f.x * t.y - f.y * t.x;
     const double kin = 1.0 / sqrt(kx * kx + ky * ky + kz * kz);
     kx *= kin; ky *= kin; kz *= kin;
-    if (__builtin_isnan(kx)) return from;
-    const double span = safe_acos_d(fx * tx + fy * ty + fz * tz) * alpha;
-    const double cs = cos(span), sn = sin(span);
-    const double kf = kx * fx + ky * fy + kz * fz;
-    const double cx = ky * fz - kz * fy, cy = kz * fx - kx * fz, cz = kx * fy - ky * fx;
-    const double rx = fx * cs + cx * sn + kx * kf * (1 - cs);
-    const double ry = fy * cs + cy * sn + ky * kf * (1 - cs);
-    const double rz = fz * cs + cz * sn + kz * kf * (1 - cs);
-    return P2d{safe_acos_d(rz) / h, atan2(ry, rx) / h};
+    if (__builtin_isnan(kx)) return f;
+    const double span = safe_acos_d(f.x * t.x + f.y * t.y + f.z * t.z) * alpha;
+    double sn, cs;
+    sincos(span, &sn, &cs);
+    const double kf = kx * f.x + ky * f.y + kz * f.z;
+    const double cx = ky * f.z - kz * f.y, cy = kz * f.x - kx * f.z, cz = kx * f.y - ky * f.x;
+    return U3{f.x * cs + cx * sn + kx * kf * (1 - cs), f.y * cs + cy * sn + ky * kf * (1 - cs),
+              f.z * cs + cz * sn + kz * kf * (1 - cs)};
 }
 
-// sampleMapping (bubble.cu:539-603): bilinear interpolation of a map's
-// index-space positions, each blend a spherical interpolation.
-__device__ P2d sample_mapping(const double* __restrict__ mt, const double* __restrict__ mp,
-                              int nth, int nph, double h, double rx, double ry) {
+// sampleMapping (bubble.cu:539-603) on a unit-vector map: the corner
+// selection of the reference, three spherical interpolations.
+__device__ U3 sample_map_u(const UMap& M, size_t off, int nth, int nph, double rx, double ry) {
     double x = rx - 0.5, y = ry;
     const bool fl = validate_coord(x, y, nth, nph);
     const int m = nph - 1;
@@ -602,10 +613,10 @@ __device__ P2d sample_mapping(const double* __restrict__ mt, const double* __res
     if ((unsigned)pI >= (unsigned)nph) pI = pI < 0 ? 0 : nph - 1;
     auto at = [&](int row, int col) {
         row = min(max(row, 0), nth - 1);
-        const size_t c = (size_t)row * nph + (col & m);
-        return P2d{mt[c], mp[c]};
+        const size_t c = off + (size_t)row * nph + (col & m);
+        return U3{M.x[c], M.y[c], M.z[c]};
     };
-    P2d ll{0, 0}, lr{0, 0}, hl{0, 0}, hr{0, 0};
+    U3 ll{0, 0, 1}, lr{0, 0, 1}, hl{0, 0, 1}, hr{0, 0, 1};
     if (fl) {
         if (tI == 0) {
             hl = at(0, pI); hr = at(0, pI + 1);
@@ -621,15 +632,7 @@ __device__ P2d sample_mapping(const double* __restrict__ mt, const double* __res
         ll = at(tI, pI); lr = at(tI, pI + 1);
         hl = at(tI + 1, pI); hr = at(tI + 1, pI + 1);
     }
-    return lerp_coords(h, lerp_coords(h, ll, lr, aP), lerp_coords(h, hl, hr, aP), aT);
-}
-
-// dist (bubble.cu:60-79): great-circle distance in grid cells
-__device__ __forceinline__ double grid_dist(double h, double invh, P2d a, P2d b) {
-    double ax, ay, az, bx, by, bz;
-    unit_of(a.x * h, a.y * h, ax, ay, az);
-    unit_of(b.x * h, b.y * h, bx, by, bz);
-    return invh * safe_acos_d(ax * bx + ay * by + az * bz);
+    return slerp_u(slerp_u(ll, lr, aP), slerp_u(hl, hr, aP), aT);
 }
 
 // MULTISAMPLE (header.cuh:98): five samples around a point with these

@@ -1059,6 +1059,13 @@ __global__ void k_row_stats(const int2* __restrict__ rows, const BubbleDev* __re
 // init/changes for the correction. One thread per cell, packed tiles.
 
 // initMapping (solver.cu:12-22) and Init/InitLast = current, changes = 0
+__device__ __forceinline__ void set_identity(const UMap& M, size_t c, double h, int j, int i) {
+    const U3 u = unit_of_index(h, j + 0.5, (double)i);
+    M.x[c] = u.x;
+    M.y[c] = u.y;
+    M.z[c] = u.z;
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_bm_init(const int4* __restrict__ tiles,
                                                  const BubbleDev* __restrict__ bub,
@@ -1069,8 +1076,9 @@ __global__ __launch_bounds__(256) void k_bm_init(const int4* __restrict__ tiles,
     const int j = tl.y + (int)threadIdx.x / tl.w, i = tl.z + (int)threadIdx.x % tl.w;
     if (j >= B.nth || i >= B.nph) return;
     const size_t c = (size_t)B.off + (size_t)j * B.nph + i;
-    P.fwd_t[c] = P.bwd_t[c] = P.prev_t[c] = j + 0.5;
-    P.fwd_p[c] = P.bwd_p[c] = P.prev_p[c] = (double)i;
+    set_identity(P.fwd, c, B.h, j, i);
+    set_identity(P.bwd, c, B.h, j, i);
+    set_identity(P.prev, c, B.h, j, i);
     P.e_init[c] = P.e_initlast[c] = (double)cur.eta[c];
     P.g_init[c] = P.g_initlast[c] = (double)cur.gam[c];
     P.e_delta[c] = P.e_deltalast[c] = P.g_delta[c] = P.g_deltalast[c] = 0.0;
@@ -1111,12 +1119,11 @@ __device__ __forceinline__ void bm_maps_cell(const BubbleDev& B, const Tables& t
         v.dt = dir == 0 ? (T)(-B.dt) : (T)B.dt;
         T tx, ty;
         trace_centre<T, NANS>(v, tab, B, j, i, tx, ty);
-        const double* mt = dir == 0 ? P.fwd_t + o : P.bwd_t + o;
-        const double* mp = dir == 0 ? P.fwd_p + o : P.bwd_p + o;
-        P2d s = sample_mapping(mt, mp, B.nth, B.nph, B.h, (double)tx, (double)ty);
-        validate_coord(s.x, s.y, B.nth, B.nph);
-        (dir == 0 ? P.nfwd_t : P.nbwd_t)[c] = s.x;
-        (dir == 0 ? P.nfwd_p : P.nbwd_p)[c] = s.y;
+        const U3 s = sample_map_u(dir == 0 ? P.fwd : P.bwd, o, B.nth, B.nph, (double)tx, (double)ty);
+        const UMap& out = dir == 0 ? P.nfwd : P.nbwd;
+        out.x[c] = s.x;
+        out.y[c] = s.y;
+        out.z[c] = s.z;
     }
 }
 
@@ -1161,9 +1168,9 @@ __global__ __launch_bounds__(256) void k_bm_eta(const int4* __restrict__ tiles,
     for (int s = 0; s < ns; ++s) {
         const int q = ns == 5 ? s : 4;
         const double w = ns == 5 ? kMsW[q] : 1.0;
-        const P2d ip = sample_mapping(P.nbwd_t + o, P.nbwd_p + o, B.nth, B.nph, B.h,
-                                      j + 0.5 + kMsDx[q], i + kMsDy[q]);
-        const P2d lp = sample_mapping(P.prev_t + o, P.prev_p + o, B.nth, B.nph, B.h, ip.x, ip.y);
+        const P2d ip = index_of_unit(B.invh, sample_map_u(P.nbwd, o, B.nth, B.nph,
+                                                          j + 0.5 + kMsDx[q], i + kMsDy[q]));
+        const P2d lp = index_of_unit(B.invh, sample_map_u(P.prev, o, B.nth, B.nph, ip.x, ip.y));
         a += (1.0 - bl) * w *
              (sample_centered<double, true>(fil, B.nth, lp.x, lp.y) +
               sample_centered<double, true>(fd, B.nth, ip.x, ip.y) +
@@ -1196,22 +1203,25 @@ __global__ __launch_bounds__(256) void k_bm_accum(const int4* __restrict__ tiles
         for (int s = 0; s < ns; ++s) {
             const int q = ns == 5 ? s : 4;
             const double w = ns == 5 ? kMsW[q] : 1.0;
-            const P2d ip = sample_mapping(P.nfwd_t + o, P.nfwd_p + o, B.nth, B.nph, B.h,
-                                          j + 0.5 + kMsDx[q], i + kMsDy[q]);
+            const P2d ip = index_of_unit(B.invh, sample_map_u(P.nfwd, o, B.nth, B.nph,
+                                                              j + 0.5 + kMsDx[q], i + kMsDy[q]));
             de += w * sample_centered<double, true>(ce, B.nth, ip.x, ip.y);
             dg += w * sample_centered<double, true>(cg, B.nth, ip.x, ip.y);
         }
         P.e_delta[c] += de;
         P.g_delta[c] += dg;
-        const P2d f{P.nfwd_t[c], P.nfwd_p[c]}, b{P.nbwd_t[c], P.nbwd_p[c]};
-        const P2d x{j + 0.5, (double)i};
-        const P2d a = sample_mapping(P.nbwd_t + o, P.nbwd_p + o, B.nth, B.nph, B.h, f.x, f.y);
-        const P2d e = sample_mapping(P.nfwd_t + o, P.nfwd_p + o, B.nth, B.nph, B.h, b.x, b.y);
-        dmax = fmaxf((float)grid_dist(B.h, B.invh, x, a), (float)grid_dist(B.h, B.invh, x, e));
-        P.fwd_t[c] = f.x;
-        P.fwd_p[c] = f.y;
-        P.bwd_t[c] = b.x;
-        P.bwd_p[c] = b.y;
+        const U3 f{P.nfwd.x[c], P.nfwd.y[c], P.nfwd.z[c]};
+        const U3 b{P.nbwd.x[c], P.nbwd.y[c], P.nbwd.z[c]};
+        const P2d fi = index_of_unit(B.invh, f), bi = index_of_unit(B.invh, b);
+        const U3 x = unit_of_index(B.h, j + 0.5, (double)i);
+        const U3 a = sample_map_u(P.nbwd, o, B.nth, B.nph, fi.x, fi.y);
+        const U3 e = sample_map_u(P.nfwd, o, B.nth, B.nph, bi.x, bi.y);
+        // dist (bubble.cu:60-79): great-circle distance in cells
+        const double da = B.invh * safe_acos_d(x.x * a.x + x.y * a.y + x.z * a.z);
+        const double dd = B.invh * safe_acos_d(x.x * e.x + x.y * e.y + x.z * e.z);
+        dmax = fmaxf((float)da, (float)dd);
+        P.fwd.x[c] = f.x; P.fwd.y[c] = f.y; P.fwd.z[c] = f.z;
+        P.bwd.x[c] = b.x; P.bwd.y[c] = b.y; P.bwd.z[c] = b.z;
     }
     dmax = wave_max(dmax);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = dmax;
@@ -1252,8 +1262,8 @@ __global__ __launch_bounds__(256) void k_bm_correct1(const int4* __restrict__ ti
     const Field<T> fg{cur.gam + o, B.nph, B.mask, B.nth - 1};
     double te = 0.0, tg = 0.0;
     for (int q = 0; q < 5; ++q) {
-        const P2d ip = sample_mapping(P.fwd_t + o, P.fwd_p + o, B.nth, B.nph, B.h,
-                                      j + 0.5 + kMsDx[q], i + kMsDy[q]);
+        const P2d ip = index_of_unit(B.invh, sample_map_u(P.fwd, o, B.nth, B.nph,
+                                                          j + 0.5 + kMsDx[q], i + kMsDy[q]));
         te += kMsW[q] * (double)sample_centered<T, true>(fe, B.nth, (T)ip.x, (T)ip.y);
         tg += kMsW[q] * (double)sample_centered<T, true>(fg, B.nth, (T)ip.x, (T)ip.y);
     }
@@ -1298,8 +1308,8 @@ __global__ __launch_bounds__(256) void k_bm_correct2(const int4* __restrict__ ti
     const Field<double> eg{P.err_g + o, B.nph, B.mask, B.nth - 1};
     double ve = P.cpy_e[c], vg = P.cpy_g[c];
     for (int q = 0; q < 5; ++q) {
-        const P2d ip = sample_mapping(P.bwd_t + o, P.bwd_p + o, B.nth, B.nph, B.h,
-                                      j + 0.5 + kMsDx[q], i + kMsDy[q]);
+        const P2d ip = index_of_unit(B.invh, sample_map_u(P.bwd, o, B.nth, B.nph,
+                                                          j + 0.5 + kMsDx[q], i + kMsDy[q]));
         ve -= kMsW[q] * sample_centered<double, true>(ee, B.nth, ip.x, ip.y);
         vg -= kMsW[q] * sample_centered<double, true>(eg, B.nth, ip.x, ip.y);
     }
@@ -1327,13 +1337,14 @@ __global__ __launch_bounds__(256) void k_bm_finish(const int4* __restrict__ tile
     P.e_deltalast[c] = P.e_delta[c];
     P.g_initlast[c] = P.g_init[c];
     P.g_deltalast[c] = P.g_delta[c];
-    P.prev_t[c] = P.bwd_t[c];
-    P.prev_p[c] = P.bwd_p[c];
+    P.prev.x[c] = P.bwd.x[c];
+    P.prev.y[c] = P.bwd.y[c];
+    P.prev.z[c] = P.bwd.z[c];
     P.e_init[c] = (double)cur.eta[c];
     P.g_init[c] = (double)cur.gam[c];
     P.e_delta[c] = P.g_delta[c] = 0.0;
-    P.fwd_t[c] = P.bwd_t[c] = j + 0.5;
-    P.fwd_p[c] = P.bwd_p[c] = (double)i;
+    set_identity(P.fwd, c, B.h, j, i);
+    set_identity(P.bwd, c, B.h, j, i);
 }
 
 // ------------------------------------------------------------ launchers ----

@@ -5,14 +5,13 @@
 
 namespace sfk {
 
-// BIMOCQ state of all bubbles (packed like the fields, FP64): maps (index
-// positions), the new maps of the current step, and for eta and Gamma the
+// BIMOCQ state of all bubbles (packed like the fields, FP64): maps (unit
+// vectors of the mapped positions), the new maps of the current step, and for eta and Gamma the
 // initial fields and accumulated changes of the current and the previous
 // reinitialisation interval; per-step changes, errors and copies; per-bubble
 // distortion, reinitialisation flag and counters.
 struct BimocqPtrs {
-    double *fwd_t, *fwd_p, *bwd_t, *bwd_p, *prev_t, *prev_p;
-    double *nfwd_t, *nfwd_p, *nbwd_t, *nbwd_p;
+    UMap fwd, bwd, prev, nfwd, nbwd;   // maps as unit vectors (x, y, z arrays)
     double *e_init, *e_initlast, *e_delta, *e_deltalast;
     double *g_init, *g_initlast, *g_delta, *g_deltalast;
     double *chg_e, *chg_g, *err_e, *err_g, *cpy_e, *cpy_g;
