@@ -69,6 +69,40 @@ def advect_bytes(n_theta, esz):
     return esz * (2 * (n_theta - 1) * nph + 2 * n_theta * nph + 4 * n_theta * nph)
 
 
+def ensemble(name, groups, steps, warmup, l_max, seed0):
+    """A packed ensemble (one context, one launch per stage for all bubbles,
+    graph replay): groups of (count, n_theta, dt). Per-bubble seed
+    seed0 + 16 b, air amplitude U[0.2, 1.0), Marangoni number M times
+    U[0.5, 2.0)."""
+    import soapbubble_amd as sb
+
+    rng = np.random.default_rng(seed0)
+    descs, fields = [], []
+    b = 0
+    for count, n, dt in groups:
+        for _ in range(count):
+            amp = rng.uniform(0.2, 1.0)
+            ms = rng.uniform(0.5, 2.0)
+            f = sb.synth_fields(n, seed0 + 16 * b, l_max, 12, 0.0, amp)
+            descs.append(sb.bubble(n, dt, M_scale=ms))
+            fields.append((n, f))
+            b += 1
+    cells = sum(n * 2 * n for n, _ in fields)
+    ctx = sb.SoapFilm(descs, "mixed", use_graph=True)
+    for k, (n, f) in enumerate(fields):
+        nph = 2 * n
+        ctx.load_state(k, {"u_theta": np.zeros((n - 1) * nph), "u_phi": np.zeros(n * nph),
+                           "eta": f["eta"], "gamma": f["gamma"]})
+        ctx.upload(k, "uair0", f["uair0"])
+        ctx.upload(k, "vair0", f["vair0"])
+    el = timed_run(ctx, warmup, steps)
+    fails = sum(ctx.stats(k)["failed_solves"] for k in range(len(descs)))
+    ctx.close()
+    return {"workload": name, "bubbles": len(descs), "cells": cells, "steps": steps,
+            "value": cells * steps / el, "unit": "cell-steps/s", "ms_per_step": el * 1e3 / steps,
+            "failed_solves": fails}
+
+
 def copy_calibration(mib):
     """A plain device copy of known size: its FETCH_SIZE / WRITE_SIZE counter
     readings against the bytes it must move give the counters' scale."""
@@ -95,6 +129,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-bimocq", action="store_true")
+    ap.add_argument("--no-ensembles", action="store_true")
     ap.add_argument("--bimocq-steps", type=int, default=10)
     ap.add_argument("--profile", choices=("graph", "eager"), default=None,
                     help="run only the headline steps (for rocprofv3), graph or eager launches")
@@ -223,6 +258,17 @@ def main():
                   "err_vs_fp64_gpu": {k: rel_l2(gb[k], rb[k]) for k in ("eta", "gamma",
                                                                         "u_theta", "u_phi")}}
 
+    # packed ensembles of uneven resolution (batched form), mixed, graph
+    ensembles = None
+    if not args.no_ensembles:
+        ensembles = [
+            ensemble("32 x 64x128 (dt 0.005) + 32 x 128x256 (0.004) + 32 x 256x512 (0.002)",
+                     [(32, 64, 0.005), (32, 128, 0.004), (32, 256, 0.002)], 20, args.warmup,
+                     16, 3000),
+            ensemble("4 x 2048x4096 (dt 0.0005) + 64 x 128x256 (0.004)",
+                     [(4, 2048, 0.0005), (64, 128, 0.004)], 10, args.warmup, 128, 4000),
+        ]
+
     # HBM bytes per advection launch measured by rocprofv3 for this workload
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "hbm_traffic_1024x2048_mixed.json")
@@ -259,6 +305,7 @@ def main():
         "err_vs_fp64_gpu": err_fp64,
         "err_vs_oracle": err_oracle,
         "bimocq": bimocq,
+        "ensembles": ensembles,
     }
     print(json.dumps(out))
 

@@ -75,7 +75,8 @@ struct Base {
     SolveCtl* ctl_d = nullptr;
     SolveHist* hist_d = nullptr;
     Tables tab{};
-    LaunchGeom geom{};
+    LaunchGeom geom{};            // cell tiles
+    std::vector<LaunchGeom> rgeo; // row blocks, one geometry per row width
     double* partR = nullptr;
     double* partA = nullptr;
     double* partB = nullptr;
@@ -215,8 +216,9 @@ struct Impl : Base {
     void assemble() override {
         // the broken-cell value of Gamma* is the state-precision gamma_max,
         // one value for a context whose bubbles share Gamma_m and T
-        launch_assemble<T, Ti>(stream, geom, bub_d, time_d, tab, nxt, air(), sys(x64[0]),
-                               (double)state_gmax(bub_h[0].gamma_max), ctl_d, partR, stride);
+        for (const LaunchGeom& rg : rgeo)
+            launch_assemble<T, Ti>(stream, rg, bub_d, time_d, tab, nxt, air(), sys(x64[0]),
+                                   (double)state_gmax(bub_h[0].gamma_max), ctl_d, partR, stride);
     }
 
     // FP64 residual evaluation k >= 1: x64[k&1] = x64[(k-1)&1] + e.
@@ -225,15 +227,22 @@ struct Impl : Base {
         double* xn = x64[k & 1];
         SysPtrs<Ti> s = sys(x64[0]);
         s.e = ecorr;
-        launch_resid<T, Ti>(stream, geom, bub_d, tab, nxt, ctl_d, s, xo, xn, k, partR, stride);
+        for (const LaunchGeom& rg : rgeo)
+            launch_resid<T, Ti>(stream, rg, bub_d, tab, nxt, ctl_d, s, xo, xn, k, partR, stride);
     }
     // Chebyshev buffers: e alternates between e and Ap, d lives in p[0].
     Ti* cheb_e(int k) { return (k & 1) ? Ap : e; }
     void cheb(int it) {
-        launch_cheb<Ti>(stream, geom, bub_d, ctl_d, sys(x64[0]), cheb_e(it), cheb_e(it + 1), p[0], it,
-                        sp, partR, partB, stride);
+        for (const LaunchGeom& rg : rgeo)
+            launch_cheb<Ti>(stream, rg, bub_d, ctl_d, sys(x64[0]), cheb_e(it), cheb_e(it + 1), p[0], it,
+                            sp, partR, partB, stride);
     }
-    bool cheb_done(std::vector<SolveCtl>& c, std::vector<double>& pb) {
+    // The inner solve of a refinement is done for a bubble when its inner
+    // residual meets the final tolerance (times inner_frac) or has fallen by
+    // the factor inner_rho from the refinement's start, the reduction the
+    // inner precision can deliver; rr0 holds the starting |r_in|^2.
+    bool cheb_done(std::vector<SolveCtl>& c, std::vector<double>& pb,
+                   const std::vector<double>& rr0) {
         SF_HIP(hipMemcpyAsync(pb.data(), partB + stride, sizeof(double) * stride,
                               hipMemcpyDeviceToHost, stream));
         SF_HIP(hipStreamSynchronize(stream));
@@ -242,15 +251,27 @@ struct Impl : Base {
             double rr = 0.0;
             for (int j = 0; j < bub_h[b].nth; ++j) rr += pb[bub_h[b].pr0 + j];
             const double tgt = sp.inner_frac * sp.tol;
-            if (!(rr <= tgt * tgt * (sp.absolute ? 1.0 : c[b].bb))) return false;
+            if (!(rr <= tgt * tgt * (sp.absolute ? 1.0 : c[b].bb) || rr <= sp.inner_rho2 * rr0[b]))
+                return false;
         }
         return true;
     }
+    std::vector<double> inner_start() {
+        std::vector<double> ps(stride), rr0(nb, 0.0);
+        SF_HIP(hipMemcpyAsync(ps.data(), partR + 3 * stride, sizeof(double) * stride,
+                              hipMemcpyDeviceToHost, stream));
+        SF_HIP(hipStreamSynchronize(stream));
+        for (int b = 0; b < nb; ++b)
+            for (int j = 0; j < bub_h[b].nth; ++j) rr0[b] += ps[bub_h[b].pr0 + j];
+        return rr0;
+    }
     void inner(int it) {
-        launch_spmv<Ti>(stream, geom, bub_d, ctl_d, sys(x64[0]), p[(it + 1) & 1], p[it & 1], it, sp,
-                        partR, partB, partA, stride);
-        launch_cgstep<Ti>(stream, geom, bub_d, ctl_d, sys(x64[0]), p[it & 1], it, partA, partB,
-                          stride);
+        for (const LaunchGeom& rg : rgeo)
+            launch_spmv<Ti>(stream, rg, bub_d, ctl_d, sys(x64[0]), p[(it + 1) & 1], p[it & 1], it, sp,
+                            partR, partB, partA, stride);
+        for (const LaunchGeom& rg : rgeo)
+            launch_cgstep<Ti>(stream, rg, bub_d, ctl_d, sys(x64[0]), p[it & 1], it, partA, partB,
+                              stride);
     }
 
     bool all_flags(std::vector<SolveCtl>& c, bool inner_flag) {
@@ -284,9 +305,10 @@ struct Impl : Base {
                                      c[0].rel_res, c[0].mu, done ? " converged" : "");
                     if (done) break;
                     int n = max_inner;
+                    const std::vector<double> rr0 = inner_start();
                     for (int it = 1; it <= max_inner; ++it) {
                         if (it > 1) cheb(it);
-                        if (cheb_done(c, pb)) { n = it; break; }
+                        if (cheb_done(c, pb, rr0)) { n = it; break; }
                     }
                     used.push_back(n);
                     if (trace_solve) std::fprintf(stderr, "[sf solve]   chebyshev iterations %d\n", n);
@@ -302,16 +324,18 @@ struct Impl : Base {
             std::vector<SolveCtl> c(nb);
             std::vector<int> before(nb);
             for (int k = 0; k < max_outer; ++k) {
-                launch_spmv<Ti>(stream, geom, bub_d, ctl_d, sys(x64[0]), p[1], p[0], 0, sp, partR,
-                                partB, partA, stride);
+                for (const LaunchGeom& rg : rgeo)
+                    launch_spmv<Ti>(stream, rg, bub_d, ctl_d, sys(x64[0]), p[1], p[0], 0, sp, partR,
+                                    partB, partA, stride);
                 const bool done = all_flags(c, false);
                 if (trace_solve)
                     std::fprintf(stderr, "[sf solve] outer %d residual %.3e (|b|^2 %.3e)%s\n", k,
                                  c[0].rel_res, c[0].bb, done ? " converged" : "");
                 if (done) break;
                 for (int b = 0; b < nb; ++b) before[b] = c[b].iters;
-                launch_cgstep<Ti>(stream, geom, bub_d, ctl_d, sys(x64[0]), p[0], 0, partA, partB,
-                                  stride);
+                for (const LaunchGeom& rg : rgeo)
+                    launch_cgstep<Ti>(stream, rg, bub_d, ctl_d, sys(x64[0]), p[0], 0, partA, partB,
+                                      stride);
                 for (int it = 1; it < max_inner; ++it) {
                     inner(it);
                     if ((it & 3) == 3 && all_flags(c, true)) break;
@@ -420,7 +444,9 @@ struct Impl : Base {
         }
     }
 
-    void row_stats() override { launch_row_stats<T>(stream, geom, bub_d, tab, cur.eta, partS, stride); }
+    void row_stats() override {
+        for (const LaunchGeom& rg : rgeo) launch_row_stats<T>(stream, rg, bub_d, tab, cur.eta, partS, stride);
+    }
 };
 
 // ------------------------------------------------------------------ setup --
@@ -577,45 +603,53 @@ void setup(Base& c, int prec) {
         for (int j = 0; j < B.nth; j += rpt)
             for (int i = 0; i < B.nph; i += w) tiles.push_back(make_int4(b, j, i, w));
     }
-    std::vector<int2> rows;
-    for (int b = 0; b < nb; ++b)
-        for (int j = 0; j < c.bub_h[b].nth; ++j) rows.push_back(make_int2(b, j));
     int nxcd = 1;
     if (hipDeviceGetAttribute(&nxcd, hipDeviceAttributeNumberOfXccs, c.device) != hipSuccess || nxcd < 1)
         nxcd = 1;
     if (const char* e = std::getenv("SF_XCD_ORDER")) nxcd = std::max(1, std::atoi(e));
     c.nxcd = nxcd;
     tiles = xcd_order(tiles, make_int4(-1, 0, 0, 1), nxcd);
-    rows = xcd_order(rows, make_int2(-1, 0), nxcd);
     int4* tiles_d = dalloc<int4>(tiles.size(), c.owned);
     SF_HIP(hipMemcpy(tiles_d, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice));
-    int2* rows_d = dalloc<int2>(rows.size(), c.owned);
-    SF_HIP(hipMemcpy(rows_d, rows.data(), rows.size() * sizeof(int2), hipMemcpyHostToDevice));
     c.geom.tiles = tiles_d;
     c.geom.ntiles = (int)tiles.size();
-    c.geom.rows = rows_d;
-    c.geom.nrows = (int)rows.size();
-    // row blocks: C consecutive cells per thread (4, doubled until the
-    // widest row needs at most kRowThreadsMax threads; 2 for small rows), one
-    // thread per C cells of the widest row; every row must hold at least 4 C
-    // cells for the reduced line system
-    int nph_min = nph_max;
-    for (int b = 0; b < nb; ++b) nph_min = std::min(nph_min, c.bub_h[b].nph);
-    int cpt = nph_max <= 128 ? 2 : 4;
-    while (nph_max / cpt > kRowThreadsMax && cpt < 16) cpt *= 2;
-    if (nph_min < 4 * cpt || nph_max / cpt > kRowThreadsMax)
-        throw Status{SF_ERR_UNSUPPORTED, "bubble resolutions of one context span too wide a range"};
-    const int nt = std::max(64, (nph_max / cpt + 63) / 64 * 64);
-    c.geom.row_threads = nt;
-    c.geom.cpt = cpt;
+    // Row blocks, one launch per row width: C consecutive cells per thread
+    // (4, or 2 for rows of at most 128 cells, doubled until the row needs at
+    // most kRowThreadsMax threads), one thread per C cells; a row holds at
+    // least 4 C cells for the reduced line system. Grouping the rows by
+    // width keeps a narrow row from occupying a wide row's block.
     const size_t esz = c.fp64_inner() ? 8 : 4;
-    c.geom.lds_bytes = 6 * (size_t)(nph_max / cpt) * esz;
     const size_t lds_limit = (size_t)prop.sharedMemPerBlock - 1024;
-    if (c.geom.lds_bytes > lds_limit) {
-        throw Status{SF_ERR_UNSUPPORTED,
-                     "line solve of nPhi = " + std::to_string(nph_max) + " needs " +
-                         std::to_string(c.geom.lds_bytes) + " B of LDS per block, device allows " +
-                         std::to_string(lds_limit) + " (use SF_PREC_MIXED for this size)"};
+    std::vector<int> widths;
+    for (int b = 0; b < nb; ++b) widths.push_back(c.bub_h[b].nph);
+    std::sort(widths.begin(), widths.end());
+    widths.erase(std::unique(widths.begin(), widths.end()), widths.end());
+    c.rgeo.clear();
+    for (int nph : widths) {
+        std::vector<int2> rows;
+        for (int b = 0; b < nb; ++b)
+            if (c.bub_h[b].nph == nph)
+                for (int j = 0; j < c.bub_h[b].nth; ++j) rows.push_back(make_int2(b, j));
+        rows = xcd_order(rows, make_int2(-1, 0), nxcd);
+        int2* rows_d = dalloc<int2>(rows.size(), c.owned);
+        SF_HIP(hipMemcpy(rows_d, rows.data(), rows.size() * sizeof(int2), hipMemcpyHostToDevice));
+        int cpt = nph <= 128 ? 2 : 4;
+        while (nph / cpt > kRowThreadsMax && cpt < 16) cpt *= 2;
+        if (nph < 4 * cpt || nph / cpt > kRowThreadsMax)
+            throw Status{SF_ERR_UNSUPPORTED, "rows of " + std::to_string(nph) + " cells are not supported"};
+        LaunchGeom g = c.geom;
+        g.rows = rows_d;
+        g.nrows = (int)rows.size();
+        g.cpt = cpt;
+        g.row_threads = std::max(64, (nph / cpt + 63) / 64 * 64);
+        g.lds_bytes = 6 * (size_t)(nph / cpt) * esz;
+        if (g.lds_bytes > lds_limit) {
+            throw Status{SF_ERR_UNSUPPORTED,
+                         "line solve of nPhi = " + std::to_string(nph) + " needs " +
+                             std::to_string(g.lds_bytes) + " B of LDS per block, device allows " +
+                             std::to_string(lds_limit) + " (use SF_PREC_MIXED for this size)"};
+        }
+        c.rgeo.push_back(g);
     }
 
     c.partR = dalloc<double>(5 * (size_t)c.stride, c.owned);
@@ -646,7 +680,9 @@ void setup(Base& c, int prec) {
         c.max_outer = c.opt.max_outer > 0 ? c.opt.max_outer : 3;
         c.max_inner = c.opt.max_inner > 0 ? c.opt.max_inner : 1000;
     } else {
-        const double rho = c.opt.inner_rho > 0 ? c.opt.inner_rho : 1e-7;
+        // an FP32 inner solve reduces its residual by about 1e-6 before its
+        // rounding floor; the FP64 refinement takes it from there
+        const double rho = c.opt.inner_rho > 0 ? c.opt.inner_rho : 1e-6;
         c.sp.inner_rho2 = rho * rho;
         c.max_outer = c.opt.max_outer > 0 ? c.opt.max_outer : 4;
         c.max_inner = c.opt.max_inner > 0 ? c.opt.max_inner : 200;

@@ -63,7 +63,7 @@ struct SolveParams {
 
 // Threads of a row block at most: one per C consecutive cells of the widest
 // row, C in {2, 4, 8, 16}.
-constexpr int kRowThreadsMax = 512;
+constexpr int kRowThreadsMax = 1024;
 
 struct LaunchGeom {
     const int4* tiles;   // cell tiles: (bubble, row, first column, width)
