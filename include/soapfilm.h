@@ -103,6 +103,11 @@ typedef struct {
     double reinit_threshold; /* BIMOCQ: reinitialise the maps when their
                                 distortion exceeds this many cells; 0 -> 4
                                 (solver.cu:279-286)                          */
+    int32_t initial_guess; /* first iterate of the Gamma solve: 0 automatic
+                              (2 for the mixed paths, 1 for FP64), 1 Gamma*
+                              as the reference, 2 Gamma* plus the previous
+                              two solves' corrections extrapolated linearly */
+    int32_t pad_;
 } sf_options;
 
 typedef struct {

@@ -62,7 +62,8 @@ class Options(ctypes.Structure):
                 ("max_inner", ctypes.c_int32), ("inner_rho", ctypes.c_double),
                 ("advance_time", ctypes.c_int32), ("inner_method", ctypes.c_int32),
                 ("scalar_advection", ctypes.c_int32), ("multisample", ctypes.c_int32),
-                ("blend", ctypes.c_double), ("reinit_threshold", ctypes.c_double)]
+                ("blend", ctypes.c_double), ("reinit_threshold", ctypes.c_double),
+                ("initial_guess", ctypes.c_int32), ("pad_", ctypes.c_int32)]
 
 
 class BubbleStats(ctypes.Structure):
@@ -236,7 +237,7 @@ class SoapFilm:
                  tol_absolute: bool = False, advance_time: bool = True, max_outer: int = 0,
                  max_inner: int = 0, inner_rho: float = 0.0, inner_method: str = "auto",
                  scalar_advection: str = "plain", blend: float = 0.5, multisample: bool = True,
-                 reinit_threshold: float = 0.0):
+                 reinit_threshold: float = 0.0, initial_guess: str = "auto"):
         self._lib = load_library()
         if isinstance(bubbles, BubbleDesc):
             bubbles = [bubbles]
@@ -246,7 +247,8 @@ class SoapFilm:
                       max_outer, max_inner, inner_rho, int(advance_time),
                       {"auto": 0, "cg": 1, "chebyshev": 2}[inner_method],
                       {"plain": 0, "bimocq": 1}[scalar_advection], int(multisample), blend,
-                      reinit_threshold)
+                      reinit_threshold,
+                      {"auto": 0, "gamma_star": 1, "extrapolated": 2}[initial_guess], 0)
         h = ctypes.c_void_p()
         rc = self._lib.sf_create(ctypes.byref(opt), len(self.bubbles), arr, ctypes.byref(h))
         if rc:

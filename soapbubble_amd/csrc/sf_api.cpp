@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <string>
 #include <vector>
@@ -74,6 +75,12 @@ struct Base {
     BubbleTime* time_d = nullptr;
     SolveCtl* ctl_d = nullptr;
     SolveHist* hist_d = nullptr;
+    SchedStat* sstat_d = nullptr;          // schedule statistics of the captured step
+    SchedStat* sstat_h = nullptr;          // pinned host copies, two per bubble (by chunk parity)
+    hipEvent_t cev[2] = {};                // end of a chunk of replays
+    int chunk_sched[2] = {0, 0};           // first-refinement iterations each chunk ran with
+    hipStream_t cstream = nullptr;         // capture stream (graphs are built while replays run)
+    std::map<std::vector<int>, hipGraphExec_t> graphs;   // one executable graph per schedule
     Tables tab{};
     LaunchGeom geom{};            // cell tiles
     std::vector<LaunchGeom> rgeo; // row blocks, one geometry per row width
@@ -84,6 +91,7 @@ struct Base {
     double* x64[2] = {nullptr, nullptr};
     double* b64 = nullptr;
     int* rsteps = nullptr;
+    double* acc = nullptr;        // per-launch sums of the solve (acc_index layout)
     int nxcd = 1;
     // BIMOCQ scalar advection (allocated only when selected); its state is
     // (re)initialised from the current fields at the first step after an
@@ -94,21 +102,31 @@ struct Base {
     SolveParams sp{};
     int max_outer = 3, max_inner = 500;
     bool calibrated = false;
+    bool recording = false;   // host-driven solves record their inner iterations
+    int calib_steps = 0;
+    // extrapolated initial guess of the Gamma solve; host_hist counts the
+    // steps since the state or dt last changed (the device keeps its own
+    // count per bubble)
+    bool guess = false;
+    int host_hist = 0;
     std::vector<int> calib;   // max inner iterations per refinement seen while calibrating
     std::vector<int> sched;   // inner iterations per refinement of the fixed schedule
     int method = 1;           // inner solver: 1 PCG, 2 Chebyshev
     bool trace_solve = std::getenv("SF_SOLVE_TRACE") != nullptr;
     int steps_done = 0;
     int stage = 0;            // 0 idle, 1 advected, 2 assembled, 3 solved
-    hipGraphExec_t graph = nullptr;
     hipEvent_t ev[6] = {};
     sf_timing timing{};
     std::string err;
     virtual ~Base() {
-        if (graph) (void)hipGraphExecDestroy(graph);
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
-        if (stream) (void)hipStreamSynchronize(stream);
+        for (auto& e : cev)
+            if (e) (void)hipEventDestroy(e);
+        if (sstat_h) (void)hipHostFree(sstat_h);
+        if (cstream) (void)hipStreamDestroy(cstream);
         for (void* p : owned) (void)hipFree(p);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -120,6 +138,17 @@ struct Base {
     virtual void download(int b, int field, double* h) = 0;
     virtual void row_stats() = 0;
     virtual bool fp64_inner() const = 0;
+    // drops the stored solve corrections of bubble b (all when b < 0)
+    void reset_guess(int b) {
+        host_hist = 0;
+        if (!guess) return;
+        std::vector<BubbleTime> t(nb);
+        SF_HIP(hipStreamSynchronize(stream));
+        SF_HIP(hipMemcpy(t.data(), time_d, nb * sizeof(BubbleTime), hipMemcpyDeviceToHost));
+        for (int k = 0; k < nb; ++k)
+            if (b < 0 || b == k) t[k].ghist = t[k].gpend = 0;
+        SF_HIP(hipMemcpy(time_d, t.data(), nb * sizeof(BubbleTime), hipMemcpyHostToDevice));
+    }
 };
 
 template <typename T, typename Ti>
@@ -129,10 +158,11 @@ struct Impl : Base {
     Ti *wN = nullptr, *wW = nullptr, *dg = nullptr, *e = nullptr, *r = nullptr, *z = nullptr,
        *Ap = nullptr;
     Ti* p[2] = {nullptr, nullptr};
+    T *gdel = nullptr, *gnext = nullptr;   // last solve correction, next guess
 
     AirPtrs<T> air() const { return AirPtrs<T>{ua0, va0, au, av}; }
     SysPtrs<Ti> sys(double* x_init) const {
-        return SysPtrs<Ti>{b64, x_init, wN, wW, dg, e, r, z, Ap, rsteps};
+        return SysPtrs<Ti>{b64, x_init, wN, wW, dg, e, r, z, Ap, rsteps, acc};
     }
     bool fp64_inner() const override { return sizeof(Ti) == 8; }
 
@@ -157,6 +187,10 @@ struct Impl : Base {
         Ap = dalloc<Ti>(n, owned);
         p[0] = dalloc<Ti>(n, owned);
         p[1] = dalloc<Ti>(n, owned);
+        if (guess) {
+            gdel = dalloc<T>(n, owned);
+            gnext = dalloc<T>(n, owned);
+        }
         // reference initWithConst (solver.cu:106-110): u = 0, eta = 0.5, Gamma = 1
         std::vector<double> h;
         for (int b = 0; b < nb; ++b) {
@@ -218,52 +252,77 @@ struct Impl : Base {
         // one value for a context whose bubbles share Gamma_m and T
         for (const LaunchGeom& rg : rgeo)
             launch_assemble<T, Ti>(stream, rg, bub_d, time_d, tab, nxt, air(), sys(x64[0]),
-                                   (double)state_gmax(bub_h[0].gamma_max), ctl_d, partR, stride);
+                                   (double)state_gmax(bub_h[0].gamma_max), ctl_d, partR, stride,
+                                   gnext, sp);
     }
 
-    // FP64 residual evaluation k >= 1: x64[k&1] = x64[(k-1)&1] + e.
-    void resid(int k, Ti* ecorr) {
+    // FP64 residual evaluation k >= 1: x64[k&1] = x64[(k-1)&1] + ecorr; sh
+    // (captured steps) collects the schedule statistics.
+    void resid(int k, Ti* ecorr, SchedStat* sh) {
         const double* xo = x64[(k + 1) & 1];
         double* xn = x64[k & 1];
         SysPtrs<Ti> s = sys(x64[0]);
         s.e = ecorr;
         for (const LaunchGeom& rg : rgeo)
-            launch_resid<T, Ti>(stream, rg, bub_d, tab, nxt, ctl_d, s, xo, xn, k, partR, stride);
+            launch_resid<T, Ti>(stream, rg, bub_d, tab, nxt, ctl_d, s, xo, xn, k, partR, stride,
+                                partB, sp, method == 2 ? sh : nullptr);
     }
-    // Chebyshev buffers: e alternates between e and Ap, d lives in p[0].
+    // Chebyshev buffers: e_k lives in Ap for odd k, in e for even k (e_1 is
+    // z); d lives in p[0].
     Ti* cheb_e(int k) { return (k & 1) ? Ap : e; }
-    void cheb(int it) {
+    void cheb(int it, int kr) {
         for (const LaunchGeom& rg : rgeo)
             launch_cheb<Ti>(stream, rg, bub_d, ctl_d, sys(x64[0]), cheb_e(it), cheb_e(it + 1), p[0], it,
-                            sp, partR, partB, stride);
+                            kr, sp, partB, stride);
+    }
+    // one accumulator slot of every bubble, copied to the host
+    std::vector<double> slot_h;
+    void fetch_slot(int slot) {
+        const size_t w = sizeof(double) * kAccVals * kAccSub;
+        slot_h.resize((size_t)nb * kAccVals * kAccSub);
+        SF_HIP(hipMemcpy2DAsync(slot_h.data(), w, acc + acc_index(0, slot, 0, 0),
+                                sizeof(double) * kAccBubble, w, nb, hipMemcpyDeviceToHost, stream));
+        SF_HIP(hipStreamSynchronize(stream));
+    }
+    double slot_at(int b, int v) const {
+        double s = 0.0;
+        for (int q = 0; q < kAccSub; ++q) s += slot_h[((size_t)b * kAccVals + v) * kAccSub + q];
+        return s;
+    }
+    // per bubble: |r|^2, |b|^2 and |r_in|^2 at the start of refinement k,
+    // and whether that residual met the tolerance
+    struct Start { double rr, bb, rin; bool done; };
+    std::vector<Start> refinement_start(int k) {
+        std::vector<Start> st(nb);
+        fetch_slot(acc_slot_res(k));
+        for (int b = 0; b < nb; ++b) {
+            st[b].rr = slot_at(b, 0);
+            st[b].rin = slot_at(b, 3);
+            const double bb = slot_at(b, 1);
+            const double res = sp.absolute ? std::sqrt(st[b].rr)
+                                           : std::sqrt(st[b].rr) / (bb > 0.0 ? std::sqrt(bb) : 1.0);
+            st[b].done = res <= sp.tol;
+        }
+        fetch_slot(acc_slot_res(0));
+        for (int b = 0; b < nb; ++b) st[b].bb = slot_at(b, 1);
+        return st;
     }
     // The inner solve of a refinement is done for a bubble when its inner
     // residual meets the final tolerance (times inner_frac) or has fallen by
-    // the factor inner_rho from the refinement's start, the reduction the
-    // inner precision can deliver; rr0 holds the starting |r_in|^2.
-    bool cheb_done(std::vector<SolveCtl>& c, std::vector<double>& pb,
-                   const std::vector<double>& rr0) {
+    // the factor inner_rho from the refinement's start.
+    bool cheb_done(const std::vector<Start>& st, std::vector<double>& pb) {
         SF_HIP(hipMemcpyAsync(pb.data(), partB + stride, sizeof(double) * stride,
                               hipMemcpyDeviceToHost, stream));
         SF_HIP(hipStreamSynchronize(stream));
         for (int b = 0; b < nb; ++b) {
-            if (c[b].outer_done) continue;
+            if (st[b].done) continue;
             double rr = 0.0;
             for (int j = 0; j < bub_h[b].nth; ++j) rr += pb[bub_h[b].pr0 + j];
             const double tgt = sp.inner_frac * sp.tol;
-            if (!(rr <= tgt * tgt * (sp.absolute ? 1.0 : c[b].bb) || rr <= sp.inner_rho2 * rr0[b]))
+            if (!(rr <= tgt * tgt * (sp.absolute ? 1.0 : st[b].bb) || rr <= sp.inner_rho2 * st[b].rin))
                 return false;
         }
         return true;
-    }
-    std::vector<double> inner_start() {
-        std::vector<double> ps(stride), rr0(nb, 0.0);
-        SF_HIP(hipMemcpyAsync(ps.data(), partR + 3 * stride, sizeof(double) * stride,
-                              hipMemcpyDeviceToHost, stream));
-        SF_HIP(hipStreamSynchronize(stream));
-        for (int b = 0; b < nb; ++b)
-            for (int j = 0; j < bub_h[b].nth; ++j) rr0[b] += ps[bub_h[b].pr0 + j];
-        return rr0;
     }
     void inner(int it) {
         for (const LaunchGeom& rg : rgeo)
@@ -295,29 +354,29 @@ struct Impl : Base {
             // Chebyshev: kernel `it` measures the residual of e_it and
             // produces e_{it+1}; a refinement of n kernels hands e_{n+1} on
             if (host_driven) {
-                std::vector<SolveCtl> c(nb);
                 std::vector<double> pb(stride);
                 for (int k = 0; k < max_outer; ++k) {
-                    cheb(1);
-                    const bool done = all_flags(c, false);
+                    const std::vector<Start> st = refinement_start(k);
+                    bool done = true;
+                    for (int b = 0; b < nb; ++b) done = done && st[b].done;
                     if (trace_solve)
-                        std::fprintf(stderr, "[sf solve] outer %d residual %.3e mu %.4f%s\n", k,
-                                     c[0].rel_res, c[0].mu, done ? " converged" : "");
+                        std::fprintf(stderr, "[sf solve] outer %d residual %.3e%s\n", k,
+                                     std::sqrt(st[0].rr / (st[0].bb > 0 ? st[0].bb : 1.0)),
+                                     done ? " converged" : "");
                     if (done) break;
                     int n = max_inner;
-                    const std::vector<double> rr0 = inner_start();
                     for (int it = 1; it <= max_inner; ++it) {
-                        if (it > 1) cheb(it);
-                        if (cheb_done(c, pb, rr0)) { n = it; break; }
+                        cheb(it, k);
+                        if (cheb_done(st, pb)) { n = it; break; }
                     }
                     used.push_back(n);
                     if (trace_solve) std::fprintf(stderr, "[sf solve]   chebyshev iterations %d\n", n);
-                    resid(k + 1, cheb_e(n + 1));
+                    resid(k + 1, cheb_e(n + 1), nullptr);
                 }
             } else {
                 for (size_t k = 0; k < sched.size(); ++k) {
-                    for (int it = 1; it <= sched[k]; ++it) cheb(it);
-                    resid((int)k + 1, cheb_e(sched[k] + 1));
+                    for (int it = 1; it <= sched[k]; ++it) cheb(it, (int)k);
+                    resid((int)k + 1, cheb_e(sched[k] + 1), sstat_d);
                 }
             }
         } else if (host_driven) {
@@ -346,15 +405,15 @@ struct Impl : Base {
                 used.push_back(mx);
                 if (trace_solve)
                     std::fprintf(stderr, "[sf solve]   inner iterations %d\n", c[0].iters - before[0]);
-                resid(k + 1, e);
+                resid(k + 1, e, nullptr);
             }
         } else {
             for (size_t k = 0; k < sched.size(); ++k) {
                 for (int it = 0; it < sched[k]; ++it) inner(it);
-                resid((int)k + 1, e);
+                resid((int)k + 1, e, nullptr);
             }
         }
-        if (host_driven && !calibrated) {
+        if (host_driven && recording) {
             if (calib.size() < used.size()) calib.resize(used.size(), 0);
             for (size_t k = 0; k < used.size(); ++k) calib[k] = std::max(calib[k], used[k]);
         }
@@ -364,7 +423,8 @@ struct Impl : Base {
     void update(bool finalize) override {
         launch_update<T>(stream, geom, bub_d, time_d, ctl_d, tab, nxt, air(), x64[0], x64[1], cur,
                          finalize ? 1 : 0, hist_d, sp, partR, stride,
-                         bimocq ? bm.chg_e : nullptr, bimocq ? bm.chg_g : nullptr);
+                         bimocq ? bm.chg_e : nullptr, bimocq ? bm.chg_g : nullptr, gdel, gnext,
+                         acc);
         if (bimocq) {
             // the change accumulation of bodyforce (bubble.cu:2486-2502), then
             // estimateDistortion and reInitializeMapping (solver.cu:279-286)
@@ -396,6 +456,7 @@ struct Impl : Base {
         }
         SF_HIP(hipMemcpyAsync(dst + B.off, tmp.data(), N * sizeof(T), hipMemcpyHostToDevice, stream));
         if (field == SF_FIELD_ETA || field == SF_FIELD_GAMMA) bimocq_dirty = true;
+        if (field <= SF_FIELD_GAMMA) reset_guess(b);
         if (nan && field <= SF_FIELD_GAMMA) {
             // the advection samples this bubble with NaN-skipping interpolation
             BubbleTime bt;
@@ -581,7 +642,7 @@ void setup(Base& c, int prec) {
         ct += B.nph;
         pr += B.nth;
         nph_max = std::max(nph_max, B.nph);
-        c.time_h[b] = BubbleTime{0.0, d.dt * 1.0, c.opt.advance_time, 0, 0, 0};
+        c.time_h[b] = BubbleTime{0.0, d.dt * 1.0, c.opt.advance_time, 0, 0, 0, 0, 0};
     }
     c.ncells = off;
     c.nrow_total = pr;
@@ -593,6 +654,11 @@ void setup(Base& c, int prec) {
     SF_HIP(hipMemcpy(c.time_d, c.time_h.data(), nb * sizeof(BubbleTime), hipMemcpyHostToDevice));
     c.ctl_d = dalloc<SolveCtl>(nb, c.owned);
     c.hist_d = dalloc<SolveHist>(nb, c.owned);
+    c.sstat_d = dalloc<SchedStat>(nb, c.owned);
+    SF_HIP(hipHostMalloc((void**)&c.sstat_h, 2 * nb * sizeof(SchedStat)));
+    std::memset(c.sstat_h, 0, 2 * nb * sizeof(SchedStat));
+    for (auto& e : c.cev) SF_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    SF_HIP(hipStreamCreateWithFlags(&c.cstream, hipStreamNonBlocking));
     build_tables(c, c.owned);
 
     // cell tiles of 256 threads: one row chunk of 256, or several short rows
@@ -661,6 +727,7 @@ void setup(Base& c, int prec) {
     c.x64[1] = dalloc<double>(n, c.owned);
     c.b64 = dalloc<double>(n, c.owned);
     c.rsteps = dalloc<int>((size_t)c.stride, c.owned);
+    c.acc = dalloc<double>((size_t)nb * kAccBubble, c.owned);
 
     // solver schedule; the inner method: PCG for an FP64 inner solve (the
     // plain path), Chebyshev for FP32 unless asked otherwise
@@ -675,9 +742,13 @@ void setup(Base& c, int prec) {
     c.bm.blend = c.opt.blend;
     c.bm.threshold = c.opt.reinit_threshold != 0.0 ? c.opt.reinit_threshold : 4.0;
     c.bm.multisample = c.opt.multisample ? 1 : 0;
+    if (c.opt.initial_guess < 0 || c.opt.initial_guess > 2)
+        throw Status{SF_ERR_INVALID, "initial_guess must be 0, 1 or 2"};
+    c.guess = c.opt.initial_guess == 2 || (c.opt.initial_guess == 0 && prec != SF_PREC_FP64);
     if (c.fp64_inner()) {
         c.sp.inner_rho2 = 0.0;
         c.max_outer = c.opt.max_outer > 0 ? c.opt.max_outer : 3;
+        if (c.max_outer > kAccRefine) throw Status{SF_ERR_INVALID, "max_outer is at most 16"};
         c.max_inner = c.opt.max_inner > 0 ? c.opt.max_inner : 1000;
     } else {
         // an FP32 inner solve reduces its residual by about 1e-6 before its
@@ -685,6 +756,7 @@ void setup(Base& c, int prec) {
         const double rho = c.opt.inner_rho > 0 ? c.opt.inner_rho : 1e-6;
         c.sp.inner_rho2 = rho * rho;
         c.max_outer = c.opt.max_outer > 0 ? c.opt.max_outer : 4;
+        if (c.max_outer > kAccRefine) throw Status{SF_ERR_INVALID, "max_outer is at most 16"};
         c.max_inner = c.opt.max_inner > 0 ? c.opt.max_inner : 200;
     }
 
@@ -747,19 +819,58 @@ void eager_step(Base& c, bool host_driven) {
     time_stage(c, 3, 4, c.timing.update_ms);
     time_stage(c, 0, 4, c.timing.step_ms);
     c.timing.steps += 1;
+    c.host_hist += 1;
 }
 
-void build_graph(Base& c) {
+// The executable graph of one step for the current schedule, captured on the
+// capture stream (replays already queued on the context's stream keep
+// running) and kept for reuse.
+hipGraphExec_t step_graph(Base& c) {
+    auto it = c.graphs.find(c.sched);
+    if (it != c.graphs.end()) return it->second;
     hipGraph_t g;
-    SF_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
-    c.advect();
-    c.assemble();
-    c.solve(false, false);
-    c.update(true);
-    SF_HIP(hipStreamEndCapture(c.stream, &g));
-    hipError_t e = hipGraphInstantiate(&c.graph, g, nullptr, nullptr, 0);
+    std::swap(c.stream, c.cstream);
+    hipError_t e = hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        c.advect();
+        c.assemble();
+        c.solve(false, false);
+        c.update(true);
+        e = hipStreamEndCapture(c.stream, &g);
+    }
+    std::swap(c.stream, c.cstream);
+    SF_HIP(e);
+    hipGraphExec_t x = nullptr;
+    e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     SF_HIP(e);
+    c.graphs[c.sched] = x;
+    return x;
+}
+
+constexpr int kAdaptChunk = 8;
+
+// Follows the schedule statistics of a finished chunk of replays that ran
+// with `ran` first-refinement iterations: one more when a step needed a
+// further refinement, one fewer when every step's iterate before the last
+// already met the inner target. Statistics of a chunk that ran with another
+// schedule than the current one only ever raise it.
+void adapt_schedule(Base& c, const SchedStat* st, int ran) {
+    double ratio = 0.0;
+    int extra = 0;
+    for (int b = 0; b < c.nb; ++b) {
+        ratio = std::max(ratio, st[b].tail_ratio);
+        extra += st[b].extra;
+    }
+    int n = c.sched[0];
+    if (extra > 0) n = std::max(n, ran + 1);
+    else if (ran == n && ratio <= 1.0 && n > 1) n -= 1;
+    n = std::min(n, c.max_inner);
+    if (n == c.sched[0]) return;
+    if (c.trace_solve)
+        std::fprintf(stderr, "[sf solve] schedule %d -> %d (tail ratio %.3g, extra %d)\n", c.sched[0], n,
+                     ratio, extra);
+    c.sched[0] = n;
 }
 
 void check_index(sf_ctx* ctx, int b) {
@@ -863,7 +974,13 @@ int sf_set_dt(sf_ctx* ctx, int32_t b, double dt_seconds) {
         for (int k = 0; k < c.nb; ++k) {
             if (b >= 0 && b != k) continue;
             // timeStepGlobal = dt U / radius with the member U = 1 (bubble.cu:2675)
-            c.bub_h[k].dt = dt_seconds * 1.0 / c.der[k].radius;
+            const double dtn = dt_seconds * 1.0 / c.der[k].radius;
+            if (dtn != c.bub_h[k].dt) {
+                // the stored solve corrections belong to the old step
+                c.time_h[k].ghist = c.time_h[k].gpend = 0;
+                c.host_hist = 0;
+            }
+            c.bub_h[k].dt = dtn;
             c.time_h[k].dt_s = dt_seconds;
         }
         SF_HIP(hipMemcpy(c.bub_d, c.bub_h.data(), c.nb * sizeof(BubbleDev), hipMemcpyHostToDevice));
@@ -879,31 +996,73 @@ int sf_step(sf_ctx* ctx, int32_t nsteps) {
         Base& c = *ctx->impl;
         if (c.stage != 0) throw Status{SF_ERR_ORDER, "a stage-level step is in progress"};
         const bool want_graph = c.opt.use_graph && !c.fp64_inner();
+        // the extrapolated initial guess is in full effect from the third
+        // solve after a reset; steps before that run host-driven
+        const int warm = c.guess ? 2 : 0;
         int k = 0;
         if (want_graph && !c.calibrated) {
-            // calibration: two host-driven steps fix the refinement schedule
-            for (; k < nsteps && k < 2; ++k) eager_step(c, true);
-            if (k == 2) {
-                // fixed schedule: the inner iterations of each refinement seen
-                // while calibrating, then one spare refinement of one
-                // iteration (a further factor of about ten) that catches a
-                // step needing more; a bubble that converged skips the
-                // remaining kernels
+            // calibration: one host-driven step with the guess in effect sets
+            // the first schedule
+            for (; k < nsteps && c.calib_steps < 1; ++k) {
+                c.recording = c.host_hist >= warm;
+                eager_step(c, true);
+                if (c.recording) ++c.calib_steps;
+                c.recording = false;
+            }
+            if (c.calib_steps == 1) {
+                // schedule: the inner iterations of each refinement seen while
+                // calibrating, then one spare refinement of one iteration (a
+                // further factor of about ten) that catches a step needing
+                // more; a bubble that converged skips the remaining kernels
                 c.sched.clear();
                 for (int n : c.calib) c.sched.push_back(c.opt.max_inner > 0 ? c.opt.max_inner : std::max(n, 1));
                 if (c.sched.empty()) c.sched.push_back(1);
                 c.sched.push_back(1);
                 if (c.opt.max_outer > 0) c.sched.resize((size_t)c.opt.max_outer, 1);
+                if (c.sched.size() > (size_t)kAccRefine) c.sched.resize(kAccRefine);
                 c.calibrated = true;
-                build_graph(c);
+                if (c.trace_solve) {
+                    std::fprintf(stderr, "[sf solve] graph schedule:");
+                    for (int n : c.sched) std::fprintf(stderr, " %d", n);
+                    std::fprintf(stderr, "\n");
+                }
+                (void)step_graph(c);
             }
         }
-        if (want_graph && c.graph) {
+        if (want_graph && c.calibrated) {
+            for (; k < nsteps && c.host_hist < warm; ++k) eager_step(c, true);
+            // replays in chunks; with the automatic schedule each chunk's
+            // statistics are copied out behind it and read one chunk later,
+            // so the device never waits for the host
+            const bool adapt = c.method == 2 && c.opt.max_inner <= 0;
             if (k < nsteps) {
                 SF_HIP(hipEventRecord(c.ev[0], c.stream));
-                for (; k < nsteps; ++k) SF_HIP(hipGraphLaunch(c.graph, c.stream));
+                int chunk = 0;
+                while (k < nsteps) {
+                    const int n = adapt ? std::min(nsteps - k, kAdaptChunk) : nsteps - k;
+                    hipGraphExec_t g = step_graph(c);
+                    for (int q = 0; q < n; ++q) SF_HIP(hipGraphLaunch(g, c.stream));
+                    k += n;
+                    c.host_hist += n;
+                    if (!adapt) continue;
+                    const int p = chunk & 1;
+                    SF_HIP(hipMemcpyAsync(c.sstat_h + p * c.nb, c.sstat_d, c.nb * sizeof(SchedStat),
+                                          hipMemcpyDeviceToHost, c.stream));
+                    SF_HIP(hipMemsetAsync(c.sstat_d, 0, c.nb * sizeof(SchedStat), c.stream));
+                    SF_HIP(hipEventRecord(c.cev[p], c.stream));
+                    c.chunk_sched[p] = c.sched[0];
+                    if (chunk > 0) {
+                        SF_HIP(hipEventSynchronize(c.cev[p ^ 1]));
+                        adapt_schedule(c, c.sstat_h + (p ^ 1) * c.nb, c.chunk_sched[p ^ 1]);
+                    }
+                    ++chunk;
+                }
                 SF_HIP(hipEventRecord(c.ev[5], c.stream));
                 SF_HIP(hipEventSynchronize(c.ev[5]));
+                if (adapt) {
+                    const int p = (chunk - 1) & 1;
+                    adapt_schedule(c, c.sstat_h + p * c.nb, c.chunk_sched[p]);
+                }
                 float ms = 0.f;
                 SF_HIP(hipEventElapsedTime(&ms, c.ev[0], c.ev[5]));
                 c.timing.step_ms += ms;
@@ -980,6 +1139,7 @@ int sf_update(sf_ctx* ctx) {
         c.update(false);
         c.stage = 0;
         c.steps_done += 1;
+        c.host_hist += 1;
         SF_HIP(hipStreamSynchronize(c.stream));
         return (int)SF_OK;
     });

@@ -33,11 +33,18 @@ struct BubbleTime {
     int advance;
     int pending;
     int nans;   // sticky: set once a NaN has been written into the state fields
+    // initial guess of the Gamma solve: solves whose correction Gamma^{n+1} -
+    // Gamma* is stored (0..2; the host resets it when the state or dt
+    // changes), and the increment the update kernel leaves for the next
+    // advection to apply
+    int ghist;
+    int gpend;
     int pad_;
 };
 
 // Per-bubble control block of the iterative solve. Every field is written by
-// one kernel and read only by later kernels of the same stream.
+// one kernel (the block of a row launch that arrives last, see arrive_last)
+// and read only by later kernels of the same stream.
 struct SolveCtl {
     double rz[2];        // r.z of inner iterations, slot = iteration parity
     double rr_in0;       // |r_in|^2 at the start of the current inner solve
@@ -57,6 +64,15 @@ struct SolveHist {
     int fails;
     int solves;
     double max_res;
+};
+
+// Statistics of the captured step for its inner-iteration schedule, since
+// the host last cleared them: the largest tail ratio of the first
+// refinement and the number of further refinements that ran.
+struct SchedStat {
+    double tail_ratio;
+    int extra;
+    int pad_;
 };
 
 struct Tables {
@@ -705,6 +721,67 @@ __device__ __forceinline__ void sum_partials(const double* __restrict__ part, lo
         for (int k = 0; k < NV; ++k) out[k] += part[k * stride + base + r];
     }
     block_sum<NV>(out, red);
+}
+
+// Per-launch sums of the solve, accumulated without fences: every row block
+// adds its block sums with device-scope atomics into one of kAccSub = 64
+// sub-accumulators (row j uses j mod 64, so at most nth / 64 atomics meet
+// on one address), and kernels of later launches read them (stream order
+// makes them visible); a wave reads one slot with one load per lane. Slots
+// per bubble: the assembly (|r0|^2, |b|^2, r_in.z, |r_in|^2, mu as a
+// maximum) and the FP64 residual of each refinement k = 1..kAccRefine (the
+// same first four). A slot is zeroed by a kernel that neither reads nor adds
+// to it, before the launch that adds to it: the assembly's by the update
+// kernel, refinement 1's by the assembly, refinement k+1's by refinement k.
+constexpr int kAccSub = 64, kAccVals = 5, kAccRefine = 16;
+constexpr int kAccSlots = 1 + kAccRefine;
+constexpr int kAccBubble = kAccSlots * kAccVals * kAccSub;
+__host__ __device__ constexpr int acc_slot_res(int k) { return k; }          // k = 0: assembly
+__host__ __device__ inline size_t acc_index(int b, int slot, int v, int sub) {
+    return (((size_t)b * kAccSlots + slot) * kAccVals + v) * kAccSub + sub;
+}
+// The sum (maximum) of one value of a slot, in every lane of the calling
+// wave; the same in every wave (fixed reduction order).
+__device__ __forceinline__ double acc_sum(const double* __restrict__ acc, int b, int slot, int v) {
+    double x = acc[acc_index(b, slot, v, threadIdx.x & 63)];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    return x;
+}
+__device__ __forceinline__ double acc_max(const double* __restrict__ acc, int b, int slot, int v) {
+    double x = acc[acc_index(b, slot, v, threadIdx.x & 63)];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x = fmax(x, __shfl_xor(x, o, 64));
+    return x;
+}
+__device__ __forceinline__ void acc_add(double* acc, int b, int slot, int v, int sub, double x) {
+    unsafeAtomicAdd(acc + acc_index(b, slot, v, sub & (kAccSub - 1)), x);
+}
+// maximum of non-negative values through their bit patterns
+__device__ __forceinline__ void acc_put_max(double* acc, int b, int slot, int v, int sub, double x) {
+    atomicMax(reinterpret_cast<unsigned long long*>(acc + acc_index(b, slot, v, sub & (kAccSub - 1))),
+              (unsigned long long)__double_as_longlong(x));
+}
+// zeroes one slot; called by every lane of one wave
+__device__ __forceinline__ void acc_zero(double* acc, int b, int slot) {
+#pragma unroll
+    for (int v = 0; v < kAccVals; ++v) acc[acc_index(b, slot, v, threadIdx.x & 63)] = 0.0;
+}
+
+// Flags of SolveCtl that blocks of one launch may set while other blocks of
+// the same launch read them: each flag caches a decision every block makes
+// from the same sums, so a block sees either value and acts alike.
+__device__ __forceinline__ int flag_load(const int* f) {
+    return __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void flag_store(int* f, int v) {
+    __hip_atomic_store(f, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// The FP64 residual norm the tolerance applies to: relative to |b| unless
+// the tolerance is absolute.
+__device__ __forceinline__ double residual_norm(double rr, double bb, bool absolute) {
+    return absolute ? sqrt(rr) : sqrt(rr) / (bb > 0.0 ? sqrt(bb) : 1.0);
 }
 
 // --------------------------------------------- periodic line solve ----

@@ -127,9 +127,15 @@ __global__ __launch_bounds__(256) void k_advect(const int4* __restrict__ tiles,
     const BubbleDev B = bub[tl.x];
     // the previous step's time advance lands here, before any kernel of this
     // step reads the time
-    if (tl.y == 0 && tl.z == 0 && threadIdx.x == 0 && btime[tl.x].pending) {
-        btime[tl.x].t += btime[tl.x].dt_s;
-        btime[tl.x].pending = 0;
+    if (tl.y == 0 && tl.z == 0 && threadIdx.x == 0) {
+        if (btime[tl.x].pending) {
+            btime[tl.x].t += btime[tl.x].dt_s;
+            btime[tl.x].pending = 0;
+        }
+        if (btime[tl.x].gpend) {
+            btime[tl.x].ghist = min(btime[tl.x].ghist + 1, 2);
+            btime[tl.x].gpend = 0;
+        }
     }
     const int j = tl.y + (int)threadIdx.x / tl.w;
     const int i = tl.z + (int)threadIdx.x % tl.w;
@@ -192,10 +198,13 @@ struct RowSys {
     bool brk[C];
 };
 
+// With `guess` (assembly only) the initial iterate is Gamma* + guess instead
+// of Gamma* on cells that are not broken.
 template <typename T, int C, bool RHS>
 __device__ __forceinline__ void row_system(const BubbleDev& B, double t, const Tables& tab,
                                            const StatePtrs<T>& s, const AirPtrs<T>& air, int j,
-                                           int i0, RowSys<C>& q) {
+                                           int i0, RowSys<C>& q,
+                                           const T* __restrict__ guess = nullptr) {
     const int nph = B.nph, m = B.mask, nth = B.nth;
     const size_t row = (size_t)B.off + (size_t)j * nph;
     const double gmax = B.gamma_max;
@@ -212,8 +221,8 @@ __device__ __forceinline__ void row_system(const BubbleDev& B, double t, const T
     ld_row<C>(s.gam + rowN + (hasN ? i0 : iNS), gNv);
     ld_row<C>(s.eta + rowS + (hasS ? i0 : iSS), eSv);
     ld_row<C>(s.gam + rowS + (hasS ? i0 : iSS), gSv);
-    const double eW = (double)s.eta[row + ((i0 - 1) & m)], gW = (double)s.gam[row + ((i0 - 1) & m)];
-    const double eE = (double)s.eta[row + ((i0 + C) & m)], gE = (double)s.gam[row + ((i0 + C) & m)];
+    const double eW = (double)s.eta[row + ((i0 - 1) & m)], eE = (double)s.eta[row + ((i0 + C) & m)];
+    double gW = (double)s.gam[row + ((i0 - 1) & m)], gE = (double)s.gam[row + ((i0 + C) & m)];
 
     const double dt = B.dt, invDt = 1.0 / dt, Cr = B.Cr;
     const double CrDt = Cr * dt, MDt = B.M * dt, s2 = B.invh * B.invh;
@@ -231,6 +240,26 @@ __device__ __forceinline__ void row_system(const BubbleDev& B, double t, const T
         q.brk[k] = is_broken(g[k], e[k], gmax);
         bN[k] = hasN && is_broken((double)gNv[k], (double)eNv[k], gmax);
         bS[k] = hasS && is_broken((double)gSv[k], (double)eSv[k], gmax);
+    }
+    // the initial iterate: Gamma* plus the guess, rounded to the state
+    // precision (its residual is evaluated exactly below, so the rounding
+    // only perturbs the guess)
+    T gxv[C];
+#pragma unroll
+    for (int k = 0; k < C; ++k) gxv[k] = gv[k];
+    if (RHS && guess) {
+        T gc[C];
+        ld_row<C>(guess + row + i0, gc);
+#pragma unroll
+        for (int k = 0; k < C; ++k) gxv[k] = gv[k] + gc[k];
+        ld_row<C>(guess + rowN + (hasN ? i0 : iNS), gc);
+#pragma unroll
+        for (int k = 0; k < C; ++k) gNv[k] = gNv[k] + gc[k];
+        ld_row<C>(guess + rowS + (hasS ? i0 : iSS), gc);
+#pragma unroll
+        for (int k = 0; k < C; ++k) gSv[k] = gSv[k] + gc[k];
+        gW = (double)(T)(gW + (double)guess[row + ((i0 - 1) & m)]);
+        gE = (double)(T)(gE + (double)guess[row + ((i0 + C) & m)]);
     }
     // Every face term of concentrationLinearSystemKernel has the factor
     // 1 / (eta_f + Cr dt): the weight s2 sin M dt / (eta_f + Cr dt), the
@@ -294,7 +323,7 @@ __device__ __forceinline__ void row_system(const BubbleDev& B, double t, const T
         const bool brk = q.brk[k];
         q.wN[k] = (brk || bN[k]) ? 0.0 : wN;
         q.wS[k] = (brk || bS[k]) ? 0.0 : wS;
-        q.x0[k] = brk ? gmax : g[k];
+        q.x0[k] = brk ? gmax : (double)gxv[k];
         q.x0N[k] = bN[k] ? gmax : (double)gNv[k];
         q.x0S[k] = bS[k] ? gmax : (double)gSv[k];
         q.rhs[k] = 0.0;
@@ -440,7 +469,8 @@ template <typename T, typename Ti, int C>
 __global__ void k_assemble(const int2* __restrict__ rows, const BubbleDev* __restrict__ bub,
                            const BubbleTime* __restrict__ btime, Tables tab, StatePtrs<T> s,
                            AirPtrs<T> air, SysPtrs<Ti> sys, T gmax_state,
-                           SolveCtl* __restrict__ ctl, double* __restrict__ part, long stride) {
+                           SolveCtl* __restrict__ ctl, double* __restrict__ part, long stride,
+                           const T* __restrict__ guess, SolveParams sp) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     __shared__ double red[16 * 4];
     __shared__ float redf[16];
@@ -459,7 +489,9 @@ __global__ void k_assemble(const int2* __restrict__ rows, const BubbleDev* __res
     for (int k = 0; k < C; ++k) a[k] = cc[k] = d[k] = rin[k] = Ti(0);
     if (act) {
         RowSys<C> q;
-        row_system<T, C, true>(B, t, tab, s, air, j, i0, q);
+        // the extrapolated correction once a previous solve stored one
+        row_system<T, C, true>(B, t, tab, s, air, j, i0, q,
+                               btime[rw.x].ghist > 0 ? guess : nullptr);
         double xv[C];
         Ti wn[C], ww[C], dg[C];
 #pragma unroll
@@ -519,6 +551,11 @@ __global__ void k_assemble(const int2* __restrict__ rows, const BubbleDev* __res
         for (int q = 0; q < 4; ++q) part[q * stride + B.pr0 + j] = acc[q];
         part[4 * stride + B.pr0 + j] = mu;
         sys.rsteps[B.pr0 + j] = steps;
+    }
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc_add(sys.acc, rw.x, acc_slot_res(0), q, j, acc[q]);
+        acc_put_max(sys.acc, rw.x, acc_slot_res(0), 4, j, mu);
         if (j == 0) {
             SolveCtl z = {};
             z.xbuf = 0;
@@ -526,6 +563,8 @@ __global__ void k_assemble(const int2* __restrict__ rows, const BubbleDev* __res
             ctl[rw.x] = z;
         }
     }
+    // the first refinement's sums start from zero
+    if (j == 0 && threadIdx.x < 64) acc_zero(sys.acc, rw.x, acc_slot_res(1));
 }
 
 // The five-point operator of the inner solve applied to v on this
@@ -580,19 +619,26 @@ __device__ __forceinline__ void ld_stencil(const Ti* __restrict__ v, size_t row,
 }
 
 // ----------------------------------------------------- FP64 residual ----
-// x_new = x_old + e (the inner correction), r = b - A x_new in FP64 with the
-// operator recomputed from eta and Gamma* (bitwise the assembly's),
-// r_in = (Ti) r, z = M^-1 r_in. Partials: |r|^2, |b|^2, r_in.z, |r_in|^2.
+// x_new = x_old + e (the inner correction of refinement k-1), r = b - A x_new
+// in FP64 with the operator recomputed from eta and Gamma* (bitwise the
+// assembly's), r_in = (Ti) r, z = M^-1 r_in. Partials |r|^2, |b|^2, r_in.z,
+// |r_in|^2 go to the per-row slots and the refinement's sums. With `sh`
+// (Chebyshev) the block of row 0 records for the schedule of the inner
+// iterations: after the first refinement, |r_n|^2 of its last inner
+// iteration n over (inner_frac tol)^2 |b|^2 (at most 1: n - 1 iterations
+// would have done), and whether a further refinement ran.
 template <typename T, typename Ti, int C>
 __global__ void k_resid(const int2* __restrict__ rows, const BubbleDev* __restrict__ bub,
                         Tables tab, StatePtrs<T> s, SolveCtl* __restrict__ ctl, SysPtrs<Ti> sys,
                         const double* __restrict__ xo, double* __restrict__ xn, int k,
-                        double* __restrict__ part, long stride) {
+                        double* __restrict__ part, long stride, const double* __restrict__ partB,
+                        SolveParams sp, SchedStat* __restrict__ sh) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     __shared__ double red[16 * 4];
     __shared__ float redf[16];
     const int2 rw = rows[blockIdx.x];
-    if (rw.x < 0 || ctl[rw.x].outer_done) return;
+    if (rw.x < 0 || flag_load(&ctl[rw.x].outer_done)) return;
+    const Ti* __restrict__ ecorr = sys.e;
     const BubbleDev B = bub[rw.x];
     const int j = rw.y, nph = B.nph, m = B.mask, nth = B.nth;
     const bool act = (int)threadIdx.x < nph / C;
@@ -608,7 +654,7 @@ __global__ void k_resid(const int2* __restrict__ rows, const BubbleDev* __restri
         double xc[C], xN[C], xS[C], xW, xE;
         ld_stencil<double, C>(xo, row, i0, m, nph, hasN, hasS, xc, xW, xE, xN, xS);
         Ti ec[C], eN[C], eS[C], eW, eE;
-        ld_stencil<Ti, C>(sys.e, row, i0, m, nph, hasN, hasS, ec, eW, eE, eN, eS);
+        ld_stencil<Ti, C>(ecorr, row, i0, m, nph, hasN, hasS, ec, eW, eE, eN, eS);
         double bv[C];
         ld_row<C>(sys.b64 + row + i0, bv);
         double xnew[C];
@@ -646,9 +692,29 @@ __global__ void k_resid(const int2* __restrict__ rows, const BubbleDev* __restri
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) part[q * stride + B.pr0 + j] = acc[q];
+    }
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc_add(sys.acc, rw.x, acc_slot_res(k), q, j, acc[q]);
         if (j == 0) {
             ctl[rw.x].xbuf = k & 1;
             ctl[rw.x].outers = k + 1;
+        }
+    }
+    // the next refinement's sums start from zero
+    if (j == 0 && threadIdx.x < 64 && k < kAccRefine) acc_zero(sys.acc, rw.x, acc_slot_res(k + 1));
+    if (sh && j == 0 && threadIdx.x < 64) {
+        if (k == 1) {
+            double rr = 0.0;
+            for (int r = threadIdx.x; r < nth; r += 64) rr += partB[stride + B.pr0 + r];
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) rr += __shfl_xor(rr, o, 64);
+            const double tgt = sp.inner_frac * sp.tol;
+            const double bb = acc_sum(sys.acc, rw.x, acc_slot_res(0), 1);
+            const double ratio = rr / (tgt * tgt * (sp.absolute ? 1.0 : bb));
+            if (threadIdx.x == 0 && !(ratio <= sh[rw.x].tail_ratio)) sh[rw.x].tail_ratio = ratio;
+        } else if (threadIdx.x == 0) {
+            sh[rw.x].extra += 1;
         }
     }
 }
@@ -800,40 +866,35 @@ __global__ void k_cgstep(const int2* __restrict__ rows, const BubbleDev* __restr
 // reads e_k (k = 1: e_1 = d_0 = z_0 = M^-1 r_in) and d_{k-1}, forms
 // r_k = r_in - A e_k, z_k = M^-1 r_k, rho_k = 1 / (2 sigma - rho_{k-1}),
 // d_k = rho_k rho_{k-1} d_{k-1} + (2 rho_k / delta) z_k, e_{k+1} = e_k + d_k.
-// Partial |r_k|^2 goes to partB slot 1 (calibration and tracing).
+// Launch 1 of refinement `kr` ends the solve (outer_done) when the FP64
+// residual at the refinement's start met the tolerance; later launches of
+// the step then return at once. Partial |r_k|^2 goes to partB slot 1.
 template <typename Ti, int C>
 __global__ void k_cheb(const int2* __restrict__ rows, const BubbleDev* __restrict__ bub,
                        SolveCtl* __restrict__ ctl, SysPtrs<Ti> sys, const Ti* __restrict__ ek,
-                       Ti* __restrict__ enext, Ti* __restrict__ dv, int it, SolveParams sp,
-                       const double* __restrict__ partR, double* __restrict__ partB, long stride) {
+                       Ti* __restrict__ enext, Ti* __restrict__ dv, int it, int kr,
+                       SolveParams sp, double* __restrict__ partB, long stride) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     __shared__ double red[16 * 4];
     __shared__ float redf[16];
     const int2 rw = rows[blockIdx.x];
     if (rw.x < 0) return;
-    SolveCtl* Cb = ctl + rw.x;
-    if (Cb->outer_done) return;
-    const BubbleDev B = bub[rw.x];
+    const int b = rw.x;
+    SolveCtl* Cb = ctl + b;
+    const int od = flag_load(&Cb->outer_done);
+    const double* __restrict__ acc0 = sys.acc;
+    const double mu = acc_max(acc0, b, acc_slot_res(0), 4);
+    const bool converged =
+        it == 1 && residual_norm(acc_sum(acc0, b, acc_slot_res(kr), 0),
+                                 acc_sum(acc0, b, acc_slot_res(kr), 1), sp.absolute) <= sp.tol;
+    const BubbleDev B = bub[b];
     const int j = rw.y, nph = B.nph, m = B.mask, nth = B.nth;
     const bool leader = (j == 0 && threadIdx.x == 0);
-    double mu;
-    if (it == 1) {
-        double sm[2];
-        sum_partials<2>(partR, stride, B.pr0, nth, sm, red);
-        const double res = sp.absolute ? sqrt(sm[0]) : sqrt(sm[0]) / (sm[1] > 0.0 ? sqrt(sm[1]) : 1.0);
-        if (res <= sp.tol) {
-            if (leader) { Cb->outer_done = 1; Cb->rel_res = res; }
-            return;
-        }
-        mu = max_partials(partR + 4 * stride, B.pr0, nth, red);
-        if (leader) {
-            Cb->mu = mu;
-            Cb->rel_res = res;
-            Cb->bb = sm[1];
-        }
-    } else {
-        mu = Cb->mu;
+    if (od || converged) {
+        if (leader && converged) flag_store(&Cb->outer_done, 1);
+        return;
     }
+    if (leader) Cb->iters += 1;
     const double delta = fmax(mu * 1.02 + 1e-3, 1e-4);
     const double sigma = 1.0 / delta;
     double rho_prev = delta, rho = delta;      // rho_0 = 1 / sigma
@@ -883,10 +944,7 @@ __global__ void k_cheb(const int2* __restrict__ rows, const BubbleDev* __restric
     }
     double a1[1] = {acc};
     block_sum<1>(a1, red);
-    if (threadIdx.x == 0) {
-        partB[stride + B.pr0 + j] = a1[0];
-        if (j == 0) Cb->iters += 1;
-    }
+    if (threadIdx.x == 0) partB[stride + B.pr0 + j] = a1[0];
 }
 
 // ----------------------------------------------------------- finalize ----
@@ -933,7 +991,9 @@ __global__ __launch_bounds__(256) void k_update(const int4* __restrict__ tiles,
                                                 int fin, SolveHist* __restrict__ hist,
                                                 SolveParams sp, const double* __restrict__ partR,
                                                 long stride, double* __restrict__ chg_e,
-                                                double* __restrict__ chg_g) {
+                                                double* __restrict__ chg_g,
+                                                T* __restrict__ gdel, T* __restrict__ gnext,
+                                                double* __restrict__ acc) {
     __shared__ double red[16 * 4];
     const int4 tl = tiles[blockIdx.x];
     if (tl.x < 0) return;
@@ -943,6 +1003,9 @@ __global__ __launch_bounds__(256) void k_update(const int4* __restrict__ tiles,
         // the next step's advection applies
         if (fin) finalize_block(B, ctl + tl.x, hist + tl.x, sp, partR, stride, red);
         if (threadIdx.x == 0 && btime[tl.x].advance) btime[tl.x].pending = 1;
+        if (threadIdx.x == 0 && gdel) btime[tl.x].gpend = 1;
+        // the next assembly adds its sums into a cleared slot
+        if (threadIdx.x < 64) acc_zero(acc, tl.x, acc_slot_res(0));
     }
     const int j = tl.y + (int)threadIdx.x / tl.w;
     const int i = tl.z + (int)threadIdx.x % tl.w;
@@ -965,6 +1028,16 @@ __global__ __launch_bounds__(256) void k_update(const int4* __restrict__ tiles,
     } else {
         gnew = g;
         div = (1.0 - fmin(g, 2 * ga) / ga) / dt;
+    }
+    if (gdel) {
+        // initial guess of the next solve: its correction Gamma^{n+1} - Gamma*
+        // extrapolated linearly from this step's and the previous step's
+        // (this step's alone after a reset); a non-finite value is dropped
+        double dl = g - ga;
+        if (!__builtin_isfinite(dl)) dl = 0.0;
+        const double gn = btime[tl.x].ghist > 0 ? 2.0 * dl - (double)gdel[c] : dl;
+        gdel[c] = (T)dl;
+        gnext[c] = __builtin_isfinite(gn) ? (T)gn : T(0);
     }
     const double eta = (double)st.eta[c];
     bool nan_out;
@@ -1366,10 +1439,12 @@ void launch_advect(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, Bub
 template <typename T, typename Ti>
 void launch_assemble(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub,
                      const BubbleTime* bt, const Tables& tab, StatePtrs<T> st, AirPtrs<T> air,
-                     SysPtrs<Ti> sys, double gmax_state, SolveCtl* ctl, double* part, long stride) {
+                     SysPtrs<Ti> sys, double gmax_state, SolveCtl* ctl, double* part, long stride,
+                     const T* guess, const SolveParams& sp) {
 #define SF_CALL(C)                                                                                \
     hipLaunchKernelGGL((k_assemble<T, Ti, C>), dim3(g.nrows), dim3(g.row_threads), g.lds_bytes, s, \
-                       g.rows, bub, bt, tab, st, air, sys, (T)gmax_state, ctl, part, stride)
+                       g.rows, bub, bt, tab, st, air, sys, (T)gmax_state, ctl, part, stride, guess, \
+                       sp)
     SF_CASES(SF_CALL)
 #undef SF_CALL
 }
@@ -1377,10 +1452,11 @@ void launch_assemble(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub,
 template <typename T, typename Ti>
 void launch_resid(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, const Tables& tab,
                   StatePtrs<T> st, SolveCtl* ctl, SysPtrs<Ti> sys, const double* xo, double* xn,
-                  int k, double* part, long stride) {
+                  int k, double* part, long stride, const double* partB, const SolveParams& sp,
+                  SchedStat* sh) {
 #define SF_CALL(C)                                                                              \
     hipLaunchKernelGGL((k_resid<T, Ti, C>), dim3(g.nrows), dim3(g.row_threads), g.lds_bytes, s, \
-                       g.rows, bub, tab, st, ctl, sys, xo, xn, k, part, stride)
+                       g.rows, bub, tab, st, ctl, sys, xo, xn, k, part, stride, partB, sp, sh)
     SF_CASES(SF_CALL)
 #undef SF_CALL
 }
@@ -1409,11 +1485,11 @@ void launch_cgstep(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, Sol
 
 template <typename Ti>
 void launch_cheb(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, SolveCtl* ctl,
-                 SysPtrs<Ti> sys, const Ti* ek, Ti* enext, Ti* dv, int it, const SolveParams& sp,
-                 const double* partR, double* partB, long stride) {
+                 SysPtrs<Ti> sys, const Ti* ek, Ti* enext, Ti* dv, int it, int kr,
+                 const SolveParams& sp, double* partB, long stride) {
 #define SF_CALL(C)                                                                             \
     hipLaunchKernelGGL((k_cheb<Ti, C>), dim3(g.nrows), dim3(g.row_threads), g.lds_bytes, s,   \
-                       g.rows, bub, ctl, sys, ek, enext, dv, it, sp, partR, partB, stride)
+                       g.rows, bub, ctl, sys, ek, enext, dv, it, kr, sp, partB, stride)
     SF_CASES(SF_CALL)
 #undef SF_CALL
 }
@@ -1428,9 +1504,9 @@ void launch_update(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, Bub
                    SolveCtl* ctl, const Tables& tab, StatePtrs<T> st, AirPtrs<T> air,
                    const double* x0, const double* x1, StatePtrs<T> out, int fin, SolveHist* hist,
                    const SolveParams& sp, const double* partR, long stride, double* chg_e,
-                   double* chg_g) {
+                   double* chg_g, T* gdel, T* gnext, double* acc) {
     hipLaunchKernelGGL(k_update<T>, dim3(g.ntiles), dim3(256), 0, s, g.tiles, bub, bt, ctl, tab, st,
-                       air, x0, x1, out, fin, hist, sp, partR, stride, chg_e, chg_g);
+                       air, x0, x1, out, fin, hist, sp, partR, stride, chg_e, chg_g, gdel, gnext, acc);
 }
 
 template <typename T>
@@ -1449,10 +1525,11 @@ template void launch_advect<double>(hipStream_t, const LaunchGeom&, const Bubble
     template void launch_assemble<T, Ti>(hipStream_t, const LaunchGeom&, const BubbleDev*,        \
                                          const BubbleTime*, const Tables&, StatePtrs<T>,          \
                                          AirPtrs<T>, SysPtrs<Ti>, double, SolveCtl*, double*,     \
-                                         long);                                                   \
+                                         long, const T*, const SolveParams&);                     \
     template void launch_resid<T, Ti>(hipStream_t, const LaunchGeom&, const BubbleDev*,           \
                                       const Tables&, StatePtrs<T>, SolveCtl*, SysPtrs<Ti>,        \
-                                      const double*, double*, int, double*, long);
+                                      const double*, double*, int, double*, long,               \
+                                      const double*, const SolveParams&, SchedStat*);
 SF_INST_SYS(float, float)
 SF_INST_SYS(float, double)
 SF_INST_SYS(double, double)
@@ -1463,20 +1540,20 @@ SF_INST_SYS(double, double)
     template void launch_cgstep<Ti>(hipStream_t, const LaunchGeom&, const BubbleDev*, SolveCtl*,  \
                                     SysPtrs<Ti>, const Ti*, int, const double*, double*, long);   \
     template void launch_cheb<Ti>(hipStream_t, const LaunchGeom&, const BubbleDev*, SolveCtl*,    \
-                                  SysPtrs<Ti>, const Ti*, Ti*, Ti*, int, const SolveParams&,      \
-                                  const double*, double*, long);
+                                  SysPtrs<Ti>, const Ti*, Ti*, Ti*, int, int, const SolveParams&, \
+                                  double*, long);
 SF_INST_TI(float)
 SF_INST_TI(double)
 template void launch_update<float>(hipStream_t, const LaunchGeom&, const BubbleDev*, BubbleTime*,
                                    SolveCtl*, const Tables&, StatePtrs<float>, AirPtrs<float>,
                                    const double*, const double*, StatePtrs<float>, int,
                                    SolveHist*, const SolveParams&, const double*, long, double*,
-                                   double*);
+                                   double*, float*, float*, double*);
 template void launch_update<double>(hipStream_t, const LaunchGeom&, const BubbleDev*, BubbleTime*,
                                     SolveCtl*, const Tables&, StatePtrs<double>, AirPtrs<double>,
                                     const double*, const double*, StatePtrs<double>, int,
                                     SolveHist*, const SolveParams&, const double*, long, double*,
-                                    double*);
+                                    double*, double*, double*, double*);
 template void launch_row_stats<float>(hipStream_t, const LaunchGeom&, const BubbleDev*,
                                       const Tables&, const float*, double*, long);
 template void launch_row_stats<double>(hipStream_t, const LaunchGeom&, const BubbleDev*,

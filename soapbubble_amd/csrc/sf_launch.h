@@ -49,8 +49,9 @@ struct SysPtrs {
     Ti* e;           // inner correction
     Ti* r;           // inner residual
     Ti* z;           // preconditioned residual
-    Ti* Ap;
+    Ti* Ap;          // PCG: A p; Chebyshev: the second inner iterate buffer
     int* rsteps;     // line-solve reductions per row (set by the assembly)
+    double* acc;     // per-launch sums (acc_index layout)
 };
 
 struct SolveParams {
@@ -81,11 +82,13 @@ void launch_advect(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, Bub
 template <typename T, typename Ti>
 void launch_assemble(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub,
                      const BubbleTime* bt, const Tables& tab, StatePtrs<T> st, AirPtrs<T> air,
-                     SysPtrs<Ti> sys, double gmax_state, SolveCtl* ctl, double* part, long stride);
+                     SysPtrs<Ti> sys, double gmax_state, SolveCtl* ctl, double* part, long stride,
+                     const T* guess, const SolveParams& sp);
 template <typename T, typename Ti>
 void launch_resid(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, const Tables& tab,
                   StatePtrs<T> st, SolveCtl* ctl, SysPtrs<Ti> sys, const double* xo, double* xn,
-                  int k, double* part, long stride);
+                  int k, double* part, long stride, const double* partB, const SolveParams& sp,
+                  SchedStat* sh);
 template <typename Ti>
 void launch_spmv(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, SolveCtl* ctl,
                  SysPtrs<Ti> sys, const Ti* pold, Ti* pnew, int it, const SolveParams& sp,
@@ -96,8 +99,8 @@ void launch_cgstep(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, Sol
                    long stride);
 template <typename Ti>
 void launch_cheb(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, SolveCtl* ctl,
-                 SysPtrs<Ti> sys, const Ti* ek, Ti* enext, Ti* dv, int it, const SolveParams& sp,
-                 const double* partR, double* partB, long stride);
+                 SysPtrs<Ti> sys, const Ti* ek, Ti* enext, Ti* dv, int it, int kr,
+                 const SolveParams& sp, double* partB, long stride);
 void launch_finalize(hipStream_t s, int nb, const BubbleDev* bub, SolveCtl* ctl, SolveHist* hist,
                      const SolveParams& sp, const double* partR, long stride);
 template <typename T>
@@ -105,7 +108,7 @@ void launch_update(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, Bub
                    SolveCtl* ctl, const Tables& tab, StatePtrs<T> st, AirPtrs<T> air,
                    const double* x0, const double* x1, StatePtrs<T> out, int fin, SolveHist* hist,
                    const SolveParams& sp, const double* partR, long stride, double* chg_e,
-                   double* chg_g);
+                   double* chg_g, T* gdel, T* gnext, double* acc);
 // BIMOCQ stages (packed over all bubbles)
 template <typename T>
 void launch_bimocq_init(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub,
