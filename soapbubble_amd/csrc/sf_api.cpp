@@ -905,6 +905,11 @@ int sf_create(const sf_options* opt, int32_t nbubbles, const sf_bubble_desc* bub
             return fail(nullptr, SF_ERR_UNSUPPORTED, "bubbles of one context share Gamma_m and T");
     }
     if (opt->precision < 0 || opt->precision > 2) return fail(nullptr, SF_ERR_INVALID, "bad precision");
+    if (opt->initial_guess < 0 || opt->initial_guess > 2)
+        return fail(nullptr, SF_ERR_INVALID, "initial_guess must be 0, 1 or 2");
+    if (opt->scalar_advection < 0 || opt->scalar_advection > 1)
+        return fail(nullptr, SF_ERR_INVALID, "scalar_advection must be 0 (plain) or 1 (BIMOCQ)");
+    if (opt->max_outer > kAccRefine) return fail(nullptr, SF_ERR_INVALID, "max_outer is at most 16");
     auto ctx = std::unique_ptr<sf_ctx>(new sf_ctx());
     ctx->prec = opt->precision;
     ctx->impl = make_impl(opt->precision);

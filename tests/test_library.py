@@ -148,3 +148,19 @@ def test_synth_noise_bounded():
     assert np.abs(d).max() <= 0.05 and np.abs(d).max() > 0.045
     assert abs(d.mean()) < 0.01
     assert np.array_equal(a["gamma"], b["gamma"])
+
+
+@pytest.mark.parametrize("kw,msg", [({"initial_guess": 3}, "initial_guess"),
+                                    ({"max_outer": 17}, "max_outer")])
+def test_create_rejects_bad_options(kw, msg):
+    """Option checks run before any device work (no GPU needed)."""
+    lib = sb.load_library()
+    opt = sb.Options(sb.PRECISIONS["mixed"], 0, 0, 0, 1e-10, 0, 0, 0.0, 1, 0, 0, 1, 0.5, 0.0, 0, 0)
+    for k, v in kw.items():
+        setattr(opt, k, v)
+    arr = (sb.BubbleDesc * 1)(sb.bubble(16, 0.01))
+    out = ctypes.c_void_p()
+    assert lib.sf_create(ctypes.byref(opt), 1, arr, ctypes.byref(out)) == 1
+    assert msg in lib.sf_last_error(None).decode()
+    with pytest.raises(KeyError):
+        sb.SoapFilm(sb.bubble(16, 0.01), "mixed", initial_guess="previous")
