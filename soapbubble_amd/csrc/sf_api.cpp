@@ -267,13 +267,17 @@ struct Impl : Base {
             launch_resid<T, Ti>(stream, rg, bub_d, tab, nxt, ctl_d, s, xo, xn, k, partR, stride,
                                 partB, sp, method == 2 ? sh : nullptr);
     }
-    // Chebyshev buffers: e_k lives in Ap for odd k, in e for even k (e_1 is
-    // z); d lives in p[0].
-    Ti* cheb_e(int k) { return (k & 1) ? Ap : e; }
+    // Chebyshev iterates: e_1 is z; e_k for k >= 2 rotates over e, Ap, p[0]
+    // (k mod 3), so e_{k-1}, e_k and e_{k+1} are distinct.
+    Ti* cheb_e(int k) {
+        if (k <= 1) return z;
+        Ti* const bufs[3] = {e, Ap, p[0]};
+        return bufs[k % 3];
+    }
     void cheb(int it, int kr) {
         for (const LaunchGeom& rg : rgeo)
-            launch_cheb<Ti>(stream, rg, bub_d, ctl_d, sys(x64[0]), cheb_e(it), cheb_e(it + 1), p[0], it,
-                            kr, sp, partB, stride);
+            launch_cheb<Ti>(stream, rg, bub_d, ctl_d, sys(x64[0]), cheb_e(it), cheb_e(it - 1),
+                            cheb_e(it + 1), it, kr, sp, partB, stride);
     }
     // one accumulator slot of every bubble, copied to the host
     std::vector<double> slot_h;

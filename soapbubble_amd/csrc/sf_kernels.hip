@@ -863,7 +863,8 @@ __global__ void k_cgstep(const int2* __restrict__ rows, const BubbleDev* __restr
 // (Saad, Iterative Methods for Sparse Linear Systems, Alg. 12.1), with the
 // spectrum of M^-1 A inside [1 - mu, 1 + mu] (mu from the assembly, widened
 // slightly). No inner products: one kernel per iteration. Iteration k >= 1
-// reads e_k (k = 1: e_1 = d_0 = z_0 = M^-1 r_in) and d_{k-1}, forms
+// reads e_k (k = 1: e_1 = d_0 = z_0 = M^-1 r_in) and e_{k-1} (d_{k-1} =
+// e_k - e_{k-1}, so one vector is written per iteration), forms
 // r_k = r_in - A e_k, z_k = M^-1 r_k, rho_k = 1 / (2 sigma - rho_{k-1}),
 // d_k = rho_k rho_{k-1} d_{k-1} + (2 rho_k / delta) z_k, e_{k+1} = e_k + d_k.
 // Launch 1 of refinement `kr` ends the solve (outer_done) when the FP64
@@ -872,7 +873,7 @@ __global__ void k_cgstep(const int2* __restrict__ rows, const BubbleDev* __restr
 template <typename Ti, int C>
 __global__ void k_cheb(const int2* __restrict__ rows, const BubbleDev* __restrict__ bub,
                        SolveCtl* __restrict__ ctl, SysPtrs<Ti> sys, const Ti* __restrict__ ek,
-                       Ti* __restrict__ enext, Ti* __restrict__ dv, int it, int kr,
+                       const Ti* __restrict__ ekm1, Ti* __restrict__ enext, int it, int kr,
                        SolveParams sp, double* __restrict__ partB, long stride) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     __shared__ double red[16 * 4];
@@ -908,8 +909,8 @@ __global__ void k_cheb(const int2* __restrict__ rows, const BubbleDev* __restric
     const bool hasN = j > 0, hasS = j < nth - 1;
     const bool act = (int)threadIdx.x < nph / C;
     const int i0 = C * threadIdx.x;
+    // e_1 = z_0 and e_0 = 0, so d_0 = e_1 - e_0 = z_0
     const Ti* __restrict__ e = it == 1 ? sys.z : ek;
-    const Ti* __restrict__ dprev = it == 1 ? sys.z : dv;
     Ti ev[C], dvv[C], a[C], cc[C], d[C];
 #pragma unroll
     for (int q = 0; q < C; ++q) ev[q] = dvv[q] = a[q] = cc[q] = d[q] = Ti(0);
@@ -917,7 +918,16 @@ __global__ void k_cheb(const int2* __restrict__ rows, const BubbleDev* __restric
     if (act) {
         Ti eN[C], eS[C], eW, eE;
         ld_stencil<Ti, C>(e, row, i0, m, nph, hasN, hasS, ev, eW, eE, eN, eS);
-        ld_row<C>(dprev + row + i0, dvv);
+        if (it > 1) {
+            // d_{k-1} = e_k - e_{k-1}
+            Ti em[C];
+            ld_row<C>(ekm1 + row + i0, em);
+#pragma unroll
+            for (int q = 0; q < C; ++q) dvv[q] = ev[q] - em[q];
+        } else {
+#pragma unroll
+            for (int q = 0; q < C; ++q) dvv[q] = ev[q];
+        }
         Ti rin[C];
         ld_row<C>(sys.r + row + i0, rin);
         RowOp<Ti, C> op;
@@ -933,13 +943,9 @@ __global__ void k_cheb(const int2* __restrict__ rows, const BubbleDev* __restric
     }
     line_solve<Ti, C>(a, cc, d, nph, steps, reinterpret_cast<Ti*>(lds_raw), redf);
     if (act) {
-        Ti dn[C], en[C];
+        Ti en[C];
 #pragma unroll
-        for (int q = 0; q < C; ++q) {
-            dn[q] = cd * dvv[q] + cz * d[q];
-            en[q] = ev[q] + dn[q];
-        }
-        st_row<C>(dv + row + i0, dn);
+        for (int q = 0; q < C; ++q) en[q] = ev[q] + (cd * dvv[q] + cz * d[q]);
         st_row<C>(enext + row + i0, en);
     }
     double a1[1] = {acc};
@@ -1485,11 +1491,11 @@ void launch_cgstep(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, Sol
 
 template <typename Ti>
 void launch_cheb(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, SolveCtl* ctl,
-                 SysPtrs<Ti> sys, const Ti* ek, Ti* enext, Ti* dv, int it, int kr,
+                 SysPtrs<Ti> sys, const Ti* ek, const Ti* ekm1, Ti* enext, int it, int kr,
                  const SolveParams& sp, double* partB, long stride) {
 #define SF_CALL(C)                                                                             \
     hipLaunchKernelGGL((k_cheb<Ti, C>), dim3(g.nrows), dim3(g.row_threads), g.lds_bytes, s,   \
-                       g.rows, bub, ctl, sys, ek, enext, dv, it, kr, sp, partB, stride)
+                       g.rows, bub, ctl, sys, ek, ekm1, enext, it, kr, sp, partB, stride)
     SF_CASES(SF_CALL)
 #undef SF_CALL
 }
@@ -1540,7 +1546,8 @@ SF_INST_SYS(double, double)
     template void launch_cgstep<Ti>(hipStream_t, const LaunchGeom&, const BubbleDev*, SolveCtl*,  \
                                     SysPtrs<Ti>, const Ti*, int, const double*, double*, long);   \
     template void launch_cheb<Ti>(hipStream_t, const LaunchGeom&, const BubbleDev*, SolveCtl*,    \
-                                  SysPtrs<Ti>, const Ti*, Ti*, Ti*, int, int, const SolveParams&, \
+                                  SysPtrs<Ti>, const Ti*, const Ti*, Ti*, int, int,               \
+                                  const SolveParams&,                                             \
                                   double*, long);
 SF_INST_TI(float)
 SF_INST_TI(double)

@@ -99,7 +99,7 @@ void launch_cgstep(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, Sol
                    long stride);
 template <typename Ti>
 void launch_cheb(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, SolveCtl* ctl,
-                 SysPtrs<Ti> sys, const Ti* ek, Ti* enext, Ti* dv, int it, int kr,
+                 SysPtrs<Ti> sys, const Ti* ek, const Ti* ekm1, Ti* enext, int it, int kr,
                  const SolveParams& sp, double* partB, long stride);
 void launch_finalize(hipStream_t s, int nb, const BubbleDev* bub, SolveCtl* ctl, SolveHist* hist,
                      const SolveParams& sp, const double* partR, long stride);
