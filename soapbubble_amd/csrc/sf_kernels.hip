@@ -678,9 +678,15 @@ __global__ void k_resid(const int2* __restrict__ rows, const BubbleDev* __restri
         }
         st_row<C>(xn + row + i0, xnew);
         st_row<C>(sys.r + row + i0, rin);
-        RowOp<Ti, C> op;
-        op.load(sys, row, i0, m, nph, hasN, hasS);
-        op.line(rin, a, cc, d);
+        // the line system of the inner operator: the assembly stored exactly
+        // these roundings of the same FP64 weights
+#pragma unroll
+        for (int q = 0; q < C; ++q) {
+            const Ti inv = pre_rcp((Ti)w.diag[q]);
+            a[q] = -(Ti)w.wF[q] * inv;
+            cc[q] = -(Ti)w.wF[q + 1] * inv;
+            d[q] = rin[q] * inv;
+        }
     }
     line_solve<Ti, C>(a, cc, d, nph, steps, reinterpret_cast<Ti*>(lds_raw), redf);
     if (act) {
