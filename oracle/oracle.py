@@ -20,11 +20,11 @@ class Params(ctypes.Structure):
     _fields_ = [("n_theta", ctypes.c_int32), ("gravity", ctypes.c_int32),
                 ("evaporation", ctypes.c_int32), ("airflow", ctypes.c_int32),
                 ("floorf_mode", ctypes.c_int32), ("tol_absolute", ctypes.c_int32),
-                ("max_iters", ctypes.c_int32), ("pad_", ctypes.c_int32),
+                ("max_iters", ctypes.c_int32), ("viscous", ctypes.c_int32),
                 ("dt", ctypes.c_double), ("M", ctypes.c_double), ("Cr", ctypes.c_double),
                 ("gs", ctypes.c_double), ("gamma_max", ctypes.c_double),
                 ("eva", ctypes.c_double), ("radius", ctypes.c_double), ("U", ctypes.c_double),
-                ("time_s", ctypes.c_double), ("tol", ctypes.c_double)]
+                ("time_s", ctypes.c_double), ("tol", ctypes.c_double), ("re", ctypes.c_double)]
 
 
 class Stats(ctypes.Structure):
@@ -75,6 +75,8 @@ def lib():
         L.sfo_update.argtypes = [P] + [dp] * 10
         L.sfo_step.restype = ctypes.c_int
         L.sfo_step.argtypes = [P] + [dp] * 6 + [ctypes.POINTER(Stats)]
+        L.sfo_visc_div.argtypes = [P, dp, dp, dp]
+        L.sfo_visc_forces.argtypes = [P] + [dp] * 6
         L.sfo_volume.restype = ctypes.c_double
         L.sfo_volume.argtypes = [P, dp]
         L.sfo_num_threads.restype = ctypes.c_int
@@ -100,7 +102,8 @@ def _p(a):
 
 def params(n_theta: int, dt_seconds: float, *, radius=1.0, H=1e-6, U=1.0, Gamma_m=1e-6,
            T=298.15, gravity=False, evaporation=False, airflow=True, M_scale=1.0, tol=1e-12,
-           time_s=0.0, floorf_mode=False, tol_absolute=False, max_iters=0) -> Params:
+           time_s=0.0, floorf_mode=False, tol_absolute=False, max_iters=0,
+           viscous=False) -> Params:
     p = Params()
     lib().sfo_derive(radius, H, U, Gamma_m, T, dt_seconds, int(gravity), int(evaporation),
                      ctypes.byref(p))
@@ -112,6 +115,7 @@ def params(n_theta: int, dt_seconds: float, *, radius=1.0, H=1e-6, U=1.0, Gamma_
     p.tol = tol
     p.time_s = time_s
     p.M *= M_scale
+    p.viscous = int(viscous)
     return p
 
 
@@ -200,6 +204,22 @@ def step(p: Params, st: dict, uair0, vair0, nsteps: int = 1, dt_seconds: float |
         if advance_time:
             p.time_s += dt_seconds if dt_seconds is not None else p.dt * p.radius
     return out
+
+
+def visc_div(p: Params, uth, uph):
+    """divergenceKernel (bubble.cu:1217-1269): div(u) at the centres."""
+    N, _ = sizes(p.n_theta)
+    d = np.empty(N)
+    lib().sfo_visc_div(ctypes.byref(p), _p(uth), _p(uph), _p(d))
+    return d
+
+
+def visc_forces(p: Params, uth, uph, eta, div):
+    """(fu, fv) of computeforcevelphiKernel_viscous / computeforcevelthetaKernel."""
+    N, Nt = sizes(p.n_theta)
+    fu, fv = np.empty(N), np.empty(Nt)
+    lib().sfo_visc_forces(ctypes.byref(p), _p(uth), _p(uph), _p(eta), _p(div), _p(fu), _p(fv))
+    return fu, fv
 
 
 def volume(p: Params, eta) -> float:

@@ -71,6 +71,7 @@ const double kRho = 1e3;
 const double kG = 9.8;
 const double kRhoA = 1.184;
 const double kNuA = 1.562e-5;
+const double kNu = 1.25e-4;                /* soap solution viscosity */
 const double kEvaporation = -0.00000003; /* header.cuh:95 */
 const double kEps = 1e-7;                /* bubble.cu:28 */
 const double kThresh = 0.0;              /* bubble.cu:27 */
@@ -463,6 +464,7 @@ void sfo_derive(double radius, double H, double U, double Gamma_m, double T,
     out->eva = evaporation ? kEvaporation * radius / (H * 1.0) : 0.0;
     out->gravity = gravity;
     out->evaporation = evaporation;
+    out->re = kNu / (U * radius);   /* re(nu / (U radius)) with the U argument */
 }
 
 double sfo_sample(const sfo_params* p, int kind, const double* field,
@@ -594,12 +596,166 @@ void sfo_col_ind(const sfo_params* p, int32_t* col) {
 
 /* concentrationLinearSystemKernel, bubble.cu:1301-1473 (W = 0 since
  * W1 = W2 = 0, bubble.cu:2633-2638; fu/fv only enter with VISCOUS). */
+
+/* ----------------------------------------------------------- VISCOUS ---- */
+/* helper_math.h's clamp(float, float, float): fmaxf(a, fminf(f, b)), so a
+ * double argument is rounded to float and NaN gives the upper bound. */
+static double clampf_ref(double x, double a, double b) {
+    return (double)std::fmax((float)a, std::fmin((float)x, (float)b));
+}
+
+/* divergenceKernel, bubble.cu:1217-1269 (D1: double sin for sinf). */
+void sfo_visc_div(const sfo_params* p, const double* uth, const double* uph, double* div) {
+    Grid g(p);
+    const int nth = g.nth, nph = g.nph;
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < nth; ++j)
+        for (int i = 0; i < nph; ++i) {
+            const double th = ((double)j + 0.5) * g.h, half = 0.5 * g.h;
+            const double uW = uph[(size_t)j * nph + i], uE = uph[(size_t)j * nph + (i + 1) % nph];
+            const double vN = j != 0 ? uth[(size_t)(j - 1) * nph + i] : 0.0;
+            const double vS = j != nth - 1 ? uth[(size_t)j * nph + i] : 0.0;
+            const double invSin = 1.0 / std::sin(th);
+            const double sinN = std::sin(th - half), sinS = std::sin(th + half);
+            const double factor = invSin * g.invh;
+            const double termTheta = factor * (vS * sinS - vN * sinN);
+            const double termPhi = g.invh * (uE - uW);
+            div[(size_t)j * nph + i] = termTheta + termPhi;
+        }
+}
+
+/* computeforcevelthetaKernel (bubble.cu:1511-1646) into fv[(nth-1) nph] and
+ * computeforcevelphiKernel_viscous (bubble.cu:1649-1816) into fu[nth nph]
+ * (D1: double sin/cos for the phi kernel's sinf/cosf). */
+void sfo_visc_forces(const sfo_params* p, const double* uth, const double* uph,
+                     const double* eta, const double* div, double* fu, double* fv) {
+    Grid g(p);
+    const int nth = g.nth, nph = g.nph, half_n = nph / 2;
+    const double ih = g.invh, re = p->re;
+    auto U = [&](int j, int i) { return uph[(size_t)j * nph + i]; };
+    auto V = [&](int j, int i) { return uth[(size_t)j * nph + i]; };
+    auto E = [&](int j, int i) { return eta[(size_t)j * nph + i]; };
+    auto D = [&](int j, int i) { return div[(size_t)j * nph + i]; };
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < nth - 1; ++j)
+        for (int i = 0; i < nph; ++i) {
+            const double gTheta = ((double)j + 1.0) * g.h;
+            const int iW = (i - 1 + nph) % nph, iE = (i + 1) % nph, opp = (i + half_n) % nph;
+            const double u0 = U(j, i), u1 = U(j, iE), u2 = U(j + 1, i), u3 = U(j + 1, iE);
+            const double divN = D(j, i), divS = D(j + 1, i);
+            const double etaN = E(j, i), etaS = E(j + 1, i);
+            double& out = fv[(size_t)j * nph + i];
+            const double invEta = 2. / (etaN + etaS);
+            if (std::isinf(invEta)) { out = NAN; continue; }
+            const double dv = 0.5 * (divN + divS);
+            const double uPhi = 0.25 * (u0 + u1 + u2 + u3);
+            const double d0 = E(j, iW), d1 = E(j, iE), d2 = E(j + 1, iW), d3 = E(j + 1, iE);
+            const double pDpx = 0.25 * ih * (d1 + d3 - d0 - d2);
+            const double pDpy = ih * (etaS - etaN);
+            const double v1 = V(j, i), v3 = V(j, iW), v4 = V(j, iE);
+            const double v0 = j != 0 ? V(j - 1, i) : 0.5 * (V(j, i) - V(j, opp));
+            const double v2 = j != nth - 2 ? V(j + 1, i) : 0.5 * (V(j, i) - V(j, opp));
+            const double pvpy = 0.5 * ih * (v2 - v0);
+            const double pvpyNorth = ih * (v1 - v0), pvpySouth = ih * (v2 - v1);
+            const double pvpx = 0.5 * ih * (v4 - v3);
+            const double pupy = 0.5 * ih * (u2 + u3 - u0 - u1);
+            const double pupx = 0.5 * ih * (u1 + u3 - u0 - u2);
+            const double pupxy = ih * ih * (u0 + u3 - u1 - u2);
+            const double pvpxx = ih * ih * (v3 + v4 - 2 * v1);
+            const double sinT = std::sin(gTheta), cscT = 1. / sinT, cosT = std::cos(gTheta);
+            const double cotT = cosT * cscT;
+            const double s11N = 0. + 2. * (pvpyNorth + divN);
+            const double s11S = 0. + 2. * (pvpySouth + divS);
+            const double s22 = -2. * (pvpy - 2. * dv);
+            const double s12 = cscT * pvpx + pupy - uPhi * cotT;
+            const double halfStep = 0.5 * g.h;
+            const double sinN = std::sin(gTheta - halfStep), sinS = std::sin(gTheta + halfStep);
+            const double psspy = ih * (s11S * sinS - s11N * sinN);
+            const double pspx = cscT * pvpxx + pupxy - cotT * pupx;
+            const double f2 = re * cscT * invEta * pDpx * s12;
+            const double f4 = re * invEta * pDpy * 2 * (dv + pvpy);
+            const double f5 = re * cscT * (psspy + pspx - cosT * s22);
+            const double f7 = p->gs * sinT;
+            out = clampf_ref(f2 + f5 + f4, -0.1, 0.1) + f7;
+            if (sinT < 0.1) out = f7;
+        }
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < nth; ++j)
+        for (int i = 0; i < nph; ++i) {
+            const double gTheta = ((double)j + 0.5) * g.h;
+            const int iW = (i - 1 + nph) % nph;
+            const int oW = (iW + half_n) % nph, oC = (i + half_n) % nph;
+            const double divW = D(j, iW), divE = D(j, i);
+            const double etaW = E(j, iW), etaE = E(j, i);
+            double v0, v1, v2, v3;
+            if (j != 0) { v0 = V(j - 1, iW); v1 = V(j - 1, i); }
+            else { v0 = 0.5 * (V(j, iW) - V(j, oW)); v1 = 0.5 * (V(j, i) - V(j, oC)); }
+            if (j != nth - 1) { v2 = V(j, iW); v3 = V(j, i); }
+            else { v2 = 0.5 * (V(j - 1, iW) - V(j - 1, oW)); v3 = 0.5 * (V(j - 1, i) - V(j - 1, oC)); }
+            double& out = fu[(size_t)j * nph + i];
+            const double invEta = 2. / (etaW + etaE);
+            if (std::isinf(invEta)) { out = NAN; continue; }
+            const double dv = 0.5 * (divW + divE);
+            const double pvpx = 0.5 * ih * (v1 + v3 - v0 - v2);
+            const double pvpyWest = ih * (v2 - v0), pvpyEast = ih * (v3 - v1);
+            const double pvpy = 0.5 * ih * (v2 + v3 - v0 - v1);
+            double pupyNorth = 0.0, pupySouth = 0.0;
+            const double u1 = U(j, i);
+            double u3, u4;
+            if (j != 0) { u3 = U(j - 1, i); pupyNorth = ih * (u1 - u3); }
+            else u3 = -U(j, oC);
+            if (j != nth - 1) { u4 = U(j + 1, i); pupySouth = ih * (u4 - u1); }
+            else u4 = -U(j, oC);
+            const double pupy = 0.5 * ih * (u4 - u3);
+            const double sinT = std::sin(gTheta), cscT = 1. / sinT, cosT = std::cos(gTheta);
+            const double cotT = cosT * cscT;
+            const double s12 = cscT * pvpx + pupy - u1 * cotT;
+            const double pDpx = ih * (etaE - etaW);
+            double d0, d1, d2, d3;
+            if (j != 0) { d0 = E(j - 1, iW); d2 = E(j - 1, i); }
+            else { d0 = E(j, oW); d2 = E(j, oC); }
+            if (j != nth - 1) { d1 = E(j + 1, iW); d3 = E(j + 1, i); }
+            else { d1 = E(j, oW); d3 = E(j, oC); }
+            const double pDpy = 0.25 * ih * (d1 + d3 - d0 - d2);
+            const double halfStep = 0.5 * g.h;
+            const double tS = gTheta + halfStep, tN = gTheta - halfStep;
+            const double sinN = std::sin(tN), sinS = std::sin(tS);
+            const double cosN = std::cos(tN), cosS = std::cos(tS);
+            double uNorth = 0.5 * (u3 + u1), uSouth = 0.5 * (u1 + u4);
+            if (j == 0) uNorth = 0.5 * (u1 - u3);
+            if (j == nth - 1) uSouth = 0.5 * (u1 - u4);
+            const double psspy = ih * (ih * (v0 + v3 - v1 - v2) + sinS * pupySouth -
+                                       sinN * pupyNorth - cosS * uSouth + cosN * uNorth);
+            const double s22W = 2 * (2 * divW - pvpyWest), s22E = 2 * (2 * divE - pvpyEast);
+            const double pspx = ih * (s22E - s22W);
+            const double f2 = re * invEta * pDpy * s12;
+            const double f4 = re * invEta * cscT * pDpx * 2 * (2 * dv - pvpy);
+            const double f5 = re * cscT * (psspy + pspx + cosT * s12);
+            out = clampf_ref(f2 + f4 + f5, -0.1, 0.1);
+            if (sinT < 0.1) out = 0.;
+        }
+}
+
+/* fu, fv of the VISCOUS stages from the (advected) velocity and thickness */
+static void visc_forces(const sfo_params* p, const double* uth, const double* uph,
+                        const double* eta, std::vector<double>& fu, std::vector<double>& fv) {
+    const size_t nth = p->n_theta, nph = 2 * nth, N = nth * nph;
+    std::vector<double> div(N);
+    fu.assign(N, 0.0);
+    fv.assign(N - nph, 0.0);
+    sfo_visc_div(p, uth, uph, div.data());
+    sfo_visc_forces(p, uth, uph, eta, div.data(), fu.data(), fv.data());
+}
+
 void sfo_assemble(const sfo_params* p, const double* uth, const double* uph,
                   double* gam, const double* eta, const double* uair,
                   const double* vair, double* val, double* rhs) {
     Grid g(p);
     const int nth = g.nth, nph = g.nph;
     const double dt = p->dt, t = p->time_s;
+    std::vector<double> fu, fv;
+    const bool visc = p->viscous && p->gravity;
+    if (visc) visc_forces(p, uth, uph, eta, fu, fv);
 #pragma omp parallel for schedule(static)
     for (int j = 0; j < nth; ++j) {
         for (int i = 0; i < nph; ++i) {
@@ -673,7 +829,16 @@ void sfo_assemble(const sfo_params* p, const double* uth, const double* uph,
                                  vAirS * sinS / (etaDown / p->Cr * invDt + 1.0) -
                                  vAirN * sinN / (etaUp / p->Cr * invDt + 1.0));
             }
-            if (p->gravity) {
+            if (visc) {
+                /* `gravity` with `VISCOUS`, bubble.cu:1458-1464 */
+                const double fuW = fu[(size_t)j * nph + i], fuE = fu[(size_t)j * nph + iE];
+                const double fvN = j != 0 ? fv[(size_t)(j - 1) * nph + i] : 0.0;
+                const double fvS = j != nth - 1 ? fv[(size_t)j * nph + i] : 0.0;
+                div += g.invh * (sinS * fvS * etaDown / (invDt * etaDown + p->Cr) -
+                                 sinN * fvN * etaUp / (invDt * etaUp + p->Cr) +
+                                 fuE * etaRight / (invDt * etaRight + p->Cr) -
+                                 fuW * etaLeft / (invDt * etaLeft + p->Cr));
+            } else if (p->gravity) {
                 div += p->gs * g.invh *
                        (sinS * sinS * etaDown / (invDt * etaDown + p->Cr) -
                         sinN * sinN * etaUp / (invDt * etaUp + p->Cr));
@@ -824,6 +989,47 @@ void sfo_update(const sfo_params* p, const double* uth, const double* uph,
             div[k] = (1.0 - gamma / gamma_a) / dt;
         }
     }
+    if (p->viscous) {
+        /* applyforcevelthetaKernel_viscous and applyforcevelphiKernel_viscous
+         * (bubble.cu:1938-2038): the viscous forces, the t/12 mirror blend of
+         * the air (D3 index), no separate gravity term (it is in fv) */
+        std::vector<double> fu, fv;
+        visc_forces(p, uth, uph, eta, fu, fv);
+        const double t = p->time_s;
+#pragma omp parallel for schedule(static)
+        for (int j = 0; j < nth - 1; ++j)
+            for (int i = 0; i < nph; ++i) {
+                size_t k = (size_t)j * nph + i, ks = k + nph;
+                double invEta = 2. / (eta[k] + eta[ks]);
+                if (std::isinf(invEta)) { uth_out[k] = NAN; continue; }
+                double v1 = uth[k];
+                double pGpy = g.invh * (gam_next[ks] - gam_next[k]);
+                double f3 = -p->M * invEta * pGpy;
+                int opp = (nph - i) % nph;   /* D3 */
+                double vAir = 0.5 * (vair[(size_t)j * nph + i] * t / 12.0 -
+                                     vair[(size_t)j * nph + opp] * (1 - t / 12.0));
+                double f6 = p->Cr * invEta * vAir;
+                uth_out[k] = (v1 / dt + f3 + f6 + fv[k]) / (1. / dt + p->Cr * invEta);
+            }
+#pragma omp parallel for schedule(static)
+        for (int j = 0; j < nth; ++j) {
+            double gTheta = ((double)j + 0.5) * g.h;
+            double cscTheta = 1. / std::sin(gTheta);
+            for (int i = 0; i < nph; ++i) {
+                size_t k = (size_t)j * nph + i, kw = (size_t)j * nph + (i - 1 + nph) % nph;
+                double invEta = 2. / (eta[kw] + eta[k]);
+                if (std::isinf(invEta)) { uph_out[k] = NAN; continue; }
+                double u1 = uph[k];
+                double pGpx = g.invh * (gam_next[k] - gam_next[kw]);
+                double f3 = -p->M * invEta * cscTheta * pGpx;
+                int opp = (nph - i) % nph;   /* D3 */
+                double uAir = 0.5 * (uair[(size_t)j * nph + i] * t / 12.0 +
+                                     uair[(size_t)j * nph + opp] * (1 - t / 12.0));
+                double f6 = p->Cr * invEta * uAir;
+                uph_out[k] = (u1 / dt + f3 + f6 + fu[k]) / (1. / dt + p->Cr * invEta);
+            }
+        }
+    } else {
     /* applyforcevelthetaKernel, bubble.cu:1819-1876 */
 #pragma omp parallel for schedule(static)
     for (int j = 0; j < nth - 1; ++j) {
@@ -857,6 +1063,7 @@ void sfo_update(const sfo_params* p, const double* uth, const double* uph,
             double f3 = 0.0, f4 = 0.0;
             uph_out[k] = (u1 / dt + (f1 + f2 + f3 + f4)) / (1. / dt + p->Cr * invEta);
         }
+    }
     }
     /* applyforceThickness, bubble.cu:2041-2063 */
 #pragma omp parallel for schedule(static)

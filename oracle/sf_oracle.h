@@ -31,11 +31,12 @@ typedef struct {
     int32_t tol_absolute;  /* 0: stop on ||r||/||b|| <= tol; 1: on ||r|| <= tol
                               (AmgX "convergence": "ABSOLUTE", PCGF_F.json)    */
     int32_t max_iters;     /* solver iteration cap                             */
-    int32_t pad_;
+    int32_t viscous;       /* header.cuh:99 `VISCOUS` (off in the reference)   */
     double dt;             /* non-dimensional step, bubble.cu:2578             */
     double M, Cr, gs, gamma_max, eva, radius, U;
     double time_s;         /* currentTimeGlobal: air-flow time in seconds      */
     double tol;            /* solver tolerance                                  */
+    double re;             /* Re^-1 = nu / (U radius), bubble.cu:2575          */
 } sfo_params;
 
 typedef struct {
@@ -74,6 +75,15 @@ void sfo_col_ind(const sfo_params* p, int32_t* col);
 void sfo_assemble(const sfo_params* p, const double* uth, const double* uph,
                   double* gam, const double* eta, const double* uair,
                   const double* vair, double* val, double* rhs);
+/* The VISCOUS stages: divergenceKernel (bubble.cu:1217-1269), div[N] at the
+ * centres from u; computeforcevelthetaKernel (bubble.cu:1511-1646), fv on
+ * the u_theta faces, and computeforcevelphiKernel_viscous (bubble.cu:
+ * 1649-1816), fu on the u_phi faces, from u, eta and that div. With
+ * p->viscous, sfo_assemble (with gravity) and sfo_update compute them from
+ * their velocity and thickness inputs. */
+void sfo_visc_div(const sfo_params* p, const double* uth, const double* uph, double* div);
+void sfo_visc_forces(const sfo_params* p, const double* uth, const double* uph,
+                     const double* eta, const double* div, double* fu, double* fv);
 /* Solves the assembled system (replacing AmgX PCGF, bubble.cu:1479-1508).
  * x holds x0 on entry and the solution on exit. Returns 0 if converged. */
 int sfo_solve(const sfo_params* p, const double* val, const double* rhs,
