@@ -83,6 +83,7 @@ typedef struct {
 /* Derived non-dimensional parameters (bubble.cu:2573-2580, 2627-2638). */
 typedef struct {
     double M, Cr, gs, gamma_max, eva, dt, radius, U, h;
+    double re;            /* Re^-1 = nu / (U radius) (bubble.cu:2575), VISCOUS only */
 } sf_derived;
 
 typedef struct {
@@ -107,7 +108,11 @@ typedef struct {
                               (2 for the mixed paths, 1 for FP64), 1 Gamma*
                               as the reference, 2 Gamma* plus the previous
                               two solves' corrections extrapolated linearly */
-    int32_t pad_;
+    int32_t viscous;      /* 1: the VISCOUS stress path (header.cuh:99, off
+                             in the reference): divergenceKernel and the
+                             viscous forces enter the assembly (with
+                             gravity) and the velocity updates
+                             (bubble.cu:1217-1269, 1458-1464, 1511-2038)  */
 } sf_options;
 
 typedef struct {

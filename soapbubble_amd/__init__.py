@@ -52,7 +52,7 @@ class BubbleDesc(ctypes.Structure):
 
 class Derived(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in ("M", "Cr", "gs", "gamma_max", "eva", "dt",
-                                                "radius", "U", "h")]
+                                                "radius", "U", "h", "re")]
 
 
 class Options(ctypes.Structure):
@@ -63,7 +63,7 @@ class Options(ctypes.Structure):
                 ("advance_time", ctypes.c_int32), ("inner_method", ctypes.c_int32),
                 ("scalar_advection", ctypes.c_int32), ("multisample", ctypes.c_int32),
                 ("blend", ctypes.c_double), ("reinit_threshold", ctypes.c_double),
-                ("initial_guess", ctypes.c_int32), ("pad_", ctypes.c_int32)]
+                ("initial_guess", ctypes.c_int32), ("viscous", ctypes.c_int32)]
 
 
 class BubbleStats(ctypes.Structure):
@@ -237,7 +237,8 @@ class SoapFilm:
                  tol_absolute: bool = False, advance_time: bool = True, max_outer: int = 0,
                  max_inner: int = 0, inner_rho: float = 0.0, inner_method: str = "auto",
                  scalar_advection: str = "plain", blend: float = 0.5, multisample: bool = True,
-                 reinit_threshold: float = 0.0, initial_guess: str = "auto"):
+                 reinit_threshold: float = 0.0, initial_guess: str = "auto",
+                 viscous: bool = False):
         self._lib = load_library()
         if isinstance(bubbles, BubbleDesc):
             bubbles = [bubbles]
@@ -248,7 +249,8 @@ class SoapFilm:
                       {"auto": 0, "cg": 1, "chebyshev": 2}[inner_method],
                       {"plain": 0, "bimocq": 1}[scalar_advection], int(multisample), blend,
                       reinit_threshold,
-                      {"auto": 0, "gamma_star": 1, "extrapolated": 2}[initial_guess], 0)
+                      {"auto": 0, "gamma_star": 1, "extrapolated": 2}[initial_guess],
+                      int(viscous))
         h = ctypes.c_void_p()
         rc = self._lib.sf_create(ctypes.byref(opt), len(self.bubbles), arr, ctypes.byref(h))
         if rc:

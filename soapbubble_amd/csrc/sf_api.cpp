@@ -160,7 +160,8 @@ struct Impl : Base {
     Ti* p[2] = {nullptr, nullptr};
     T *gdel = nullptr, *gnext = nullptr;   // last solve correction, next guess
 
-    AirPtrs<T> air() const { return AirPtrs<T>{ua0, va0, au, av}; }
+    double *vdiv = nullptr, *vfu = nullptr, *vfv = nullptr;   // VISCOUS div, forces
+    AirPtrs<T> air() const { return AirPtrs<T>{ua0, va0, au, av, vfu, vfv}; }
     SysPtrs<Ti> sys(double* x_init) const {
         return SysPtrs<Ti>{b64, x_init, wN, wW, dg, e, r, z, Ap, rsteps, acc};
     }
@@ -190,6 +191,11 @@ struct Impl : Base {
         if (guess) {
             gdel = dalloc<T>(n, owned);
             gnext = dalloc<T>(n, owned);
+        }
+        if (opt.viscous) {
+            vdiv = dalloc<double>(n, owned);
+            vfu = dalloc<double>(n, owned);
+            vfv = dalloc<double>(n, owned);
         }
         // reference initWithConst (solver.cu:106-110): u = 0, eta = 0.5, Gamma = 1
         std::vector<double> h;
@@ -248,6 +254,9 @@ struct Impl : Base {
     // Assembly fused with the first FP64 residual and line solve; x0 lands
     // in x64[0].
     void assemble() override {
+        // VISCOUS: div(u*) and the viscous forces first (Solver::bodyforce,
+        // bubble.cu:2405-2423)
+        if (vfu) launch_viscous<T>(stream, geom, bub_d, tab, nxt, vdiv, vfu, vfv);
         // the broken-cell value of Gamma* is the state-precision gamma_max,
         // one value for a context whose bubbles share Gamma_m and T
         for (const LaunchGeom& rg : rgeo)
@@ -641,6 +650,8 @@ void setup(Base& c, int prec) {
         B.gravity = d.gravity;
         B.evaporation = d.evaporation;
         B.airflow = d.airflow;
+        B.viscous = c.opt.viscous ? 1 : 0;
+        B.re = c.der[b].re;
         off += (long long)B.nth * B.nph;
         rt += B.nth + 1;
         ct += B.nph;
@@ -914,6 +925,7 @@ int sf_create(const sf_options* opt, int32_t nbubbles, const sf_bubble_desc* bub
     if (opt->scalar_advection < 0 || opt->scalar_advection > 1)
         return fail(nullptr, SF_ERR_INVALID, "scalar_advection must be 0 (plain) or 1 (BIMOCQ)");
     if (opt->max_outer > kAccRefine) return fail(nullptr, SF_ERR_INVALID, "max_outer is at most 16");
+    if (opt->viscous < 0 || opt->viscous > 1) return fail(nullptr, SF_ERR_INVALID, "viscous must be 0 or 1");
     auto ctx = std::unique_ptr<sf_ctx>(new sf_ctx());
     ctx->prec = opt->precision;
     ctx->impl = make_impl(opt->precision);

@@ -20,7 +20,8 @@ struct BubbleDev {
     long long off;               // first cell of this bubble in every cell field
     double h, invh, dt;          // grid spacing, 1/h, non-dimensional step
     double M, Cr, gs, gamma_max, eva;
-    int gravity, evaporation, airflow, pad_;
+    double re;                   // Re^-1 of the VISCOUS path
+    int gravity, evaporation, airflow, viscous;
 };
 
 // Air-flow time of one bubble (seconds) and its step in seconds; kernels

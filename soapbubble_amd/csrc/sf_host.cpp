@@ -18,6 +18,7 @@ const double kRho = 1e3;
 const double kG = 9.8;
 const double kRhoA = 1.184;
 const double kNuA = 1.562e-5;
+const double kNu = 1.25e-4;   // soap solution kinematic viscosity, header.cuh:83
 const double kEvaporation = -0.00000003;  // header.cuh:95
 const double kPi = 3.14159265358979323846;
 const double kHalfPi = 1.57079632679489661923;
@@ -156,6 +157,7 @@ void derive_params(const sf_bubble_desc& d, sf_derived& o) {
     o.U = 1.0;
     o.eva = d.evaporation ? kEvaporation * d.radius / (d.H * 1.0) : 0.0;
     o.h = kPi / d.n_theta;
+    o.re = kNu / (d.U * d.radius);   // re(nu / (U radius)) with the U argument
 }
 
 void air_shapes(int nth, double radius, double U, double* su, double* sv) {

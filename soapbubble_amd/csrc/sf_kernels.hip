@@ -157,6 +157,163 @@ __device__ __forceinline__ double air_at(const T* __restrict__ a0, const T* __re
     return __builtin_isnan(v) ? 0.0 : v;
 }
 
+
+// ----------------------------------------------------------- VISCOUS ----
+// The VISCOUS stress path (header.cuh:99; off in the reference): div(u) at
+// the centres, then the viscous forces on the u_phi and u_theta faces from
+// the advected velocity and thickness; they enter the assembly (with
+// gravity) and the velocity updates. FP64 arithmetic in every precision
+// mode (the metric terms csc, cot grow at the poles).
+
+// helper_math.h's clamp(float, float, float) = fmaxf(a, fminf(f, b)): the
+// value is rounded to float and NaN gives the upper bound.
+__device__ __forceinline__ double clamp_ref(double x, float a, float b) {
+    return (double)fmaxf(a, fminf((float)x, b));
+}
+
+// divergenceKernel (bubble.cu:1217-1269) of cell (j, i).
+template <typename T>
+__global__ __launch_bounds__(256) void k_visc_div(const int4* __restrict__ tiles,
+                                                  const BubbleDev* __restrict__ bub, Tables tab,
+                                                  StatePtrs<T> st, double* __restrict__ div) {
+    const int4 tl = tiles[blockIdx.x];
+    if (tl.x < 0) return;
+    const BubbleDev B = bub[tl.x];
+    const int j = tl.y + (int)threadIdx.x / tl.w;
+    const int i = tl.z + (int)threadIdx.x % tl.w;
+    if (j >= B.nth || i >= B.nph) return;
+    const int nph = B.nph, m = B.mask;
+    const size_t row = (size_t)B.off + (size_t)j * nph;
+    const double uW = (double)st.uph[row + i], uE = (double)st.uph[row + ((i + 1) & m)];
+    const double vN = j != 0 ? (double)st.uth[row - nph + i] : 0.0;
+    const double vS = j != B.nth - 1 ? (double)st.uth[row + i] : 0.0;
+    const double sinN = tab.sinFace[B.rt0 + j], sinS = tab.sinFace[B.rt0 + j + 1];
+    const double factor = tab.cscTc[B.rt0 + j] * B.invh;
+    const double termTheta = factor * (vS * sinS - vN * sinN);
+    const double termPhi = B.invh * (uE - uW);
+    div[row + i] = termTheta + termPhi;
+}
+
+// computeforcevelthetaKernel (bubble.cu:1511-1646) on the u_theta face below
+// cell (j, i) and computeforcevelphiKernel_viscous (bubble.cu:1649-1816) on
+// its u_phi face, from the advected state and div.
+template <typename T>
+__global__ __launch_bounds__(256) void k_visc_force(const int4* __restrict__ tiles,
+                                                    const BubbleDev* __restrict__ bub, Tables tab,
+                                                    StatePtrs<T> st, const double* __restrict__ dv,
+                                                    double* __restrict__ fu,
+                                                    double* __restrict__ fv) {
+    const int4 tl = tiles[blockIdx.x];
+    if (tl.x < 0) return;
+    const BubbleDev B = bub[tl.x];
+    const int j = tl.y + (int)threadIdx.x / tl.w;
+    const int i = tl.z + (int)threadIdx.x % tl.w;
+    if (j >= B.nth || i >= B.nph) return;
+    const int nph = B.nph, m = B.mask, nth = B.nth, hn = nph / 2;
+    const size_t base = (size_t)B.off;
+    const double ih = B.invh, re = B.re, halfh = 0.5 * B.h;
+    auto U = [&](int jj, int ii) { return (double)st.uph[base + (size_t)jj * nph + ii]; };
+    auto V = [&](int jj, int ii) { return (double)st.uth[base + (size_t)jj * nph + ii]; };
+    auto E = [&](int jj, int ii) { return (double)st.eta[base + (size_t)jj * nph + ii]; };
+    auto D = [&](int jj, int ii) { return dv[base + (size_t)jj * nph + ii]; };
+    const int iW = (i - 1) & m, iE = (i + 1) & m;
+    if (j < nth - 1) {
+        const int opp = (i + hn) & m;
+        const double u0 = U(j, i), u1 = U(j, iE), u2 = U(j + 1, i), u3 = U(j + 1, iE);
+        const double divN = D(j, i), divS = D(j + 1, i);
+        const double etaN = E(j, i), etaS = E(j + 1, i);
+        double out;
+        const double invEta = 2. / (etaN + etaS);
+        if (__builtin_isinf(invEta)) {
+            out = __builtin_nan("");
+        } else {
+            const double dvm = 0.5 * (divN + divS);
+            const double uPhi = 0.25 * (u0 + u1 + u2 + u3);
+            const double d0 = E(j, iW), d1 = E(j, iE), d2 = E(j + 1, iW), d3 = E(j + 1, iE);
+            const double pDpx = 0.25 * ih * (d1 + d3 - d0 - d2);
+            const double pDpy = ih * (etaS - etaN);
+            const double v1 = V(j, i), v3 = V(j, iW), v4 = V(j, iE);
+            const double v0 = j != 0 ? V(j - 1, i) : 0.5 * (V(j, i) - V(j, opp));
+            const double v2 = j != nth - 2 ? V(j + 1, i) : 0.5 * (V(j, i) - V(j, opp));
+            const double pvpy = 0.5 * ih * (v2 - v0);
+            const double pvpyNorth = ih * (v1 - v0), pvpySouth = ih * (v2 - v1);
+            const double pvpx = 0.5 * ih * (v4 - v3);
+            const double pupy = 0.5 * ih * (u2 + u3 - u0 - u1);
+            const double pupx = 0.5 * ih * (u1 + u3 - u0 - u2);
+            const double pupxy = ih * ih * (u0 + u3 - u1 - u2);
+            const double pvpxx = ih * ih * (v3 + v4 - 2 * v1);
+            const double sinT = tab.sinTf[B.rt0 + j + 1], cosT = tab.cosTf[B.rt0 + j + 1];
+            const double cscT = 1. / sinT, cotT = cosT * cscT;
+            const double s11N = 0. + 2. * (pvpyNorth + divN);
+            const double s11S = 0. + 2. * (pvpySouth + divS);
+            const double s22 = -2. * (pvpy - 2. * dvm);
+            const double s12 = cscT * pvpx + pupy - uPhi * cotT;
+            // sin(theta -+ h/2) at the neighbouring centres
+            const double sinN = tab.sinTc[B.rt0 + j], sinS = tab.sinTc[B.rt0 + j + 1];
+            const double psspy = ih * (s11S * sinS - s11N * sinN);
+            const double pspx = cscT * pvpxx + pupxy - cotT * pupx;
+            const double f2 = re * cscT * invEta * pDpx * s12;
+            const double f4 = re * invEta * pDpy * 2 * (dvm + pvpy);
+            const double f5 = re * cscT * (psspy + pspx - cosT * s22);
+            const double f7 = B.gs * sinT;
+            out = sinT < 0.1 ? f7 : clamp_ref(f2 + f5 + f4, -0.1f, 0.1f) + f7;
+        }
+        fv[base + (size_t)j * nph + i] = out;
+    }
+    {
+        const int oW = (iW + hn) & m, oC = (i + hn) & m;
+        const double divW = D(j, iW), divE = D(j, i);
+        const double etaW = E(j, iW), etaE = E(j, i);
+        double out;
+        const double invEta = 2. / (etaW + etaE);
+        if (__builtin_isinf(invEta)) {
+            out = __builtin_nan("");
+        } else {
+            double v0, v1, v2, v3;
+            if (j != 0) { v0 = V(j - 1, iW); v1 = V(j - 1, i); }
+            else { v0 = 0.5 * (V(j, iW) - V(j, oW)); v1 = 0.5 * (V(j, i) - V(j, oC)); }
+            if (j != nth - 1) { v2 = V(j, iW); v3 = V(j, i); }
+            else { v2 = 0.5 * (V(j - 1, iW) - V(j - 1, oW)); v3 = 0.5 * (V(j - 1, i) - V(j - 1, oC)); }
+            const double dvm = 0.5 * (divW + divE);
+            const double pvpx = 0.5 * ih * (v1 + v3 - v0 - v2);
+            const double pvpyWest = ih * (v2 - v0), pvpyEast = ih * (v3 - v1);
+            const double pvpy = 0.5 * ih * (v2 + v3 - v0 - v1);
+            double pupyNorth = 0.0, pupySouth = 0.0, u3, u4;
+            const double u1 = U(j, i);
+            if (j != 0) { u3 = U(j - 1, i); pupyNorth = ih * (u1 - u3); }
+            else u3 = -U(j, oC);
+            if (j != nth - 1) { u4 = U(j + 1, i); pupySouth = ih * (u4 - u1); }
+            else u4 = -U(j, oC);
+            const double pupy = 0.5 * ih * (u4 - u3);
+            const double sinT = tab.sinTc[B.rt0 + j], cosT = tab.cosTc[B.rt0 + j];
+            const double cscT = 1. / sinT, cotT = cosT * cscT;
+            const double s12 = cscT * pvpx + pupy - u1 * cotT;
+            const double pDpx = ih * (etaE - etaW);
+            double d0, d1, d2, d3;
+            if (j != 0) { d0 = E(j - 1, iW); d2 = E(j - 1, i); }
+            else { d0 = E(j, oW); d2 = E(j, oC); }
+            if (j != nth - 1) { d1 = E(j + 1, iW); d3 = E(j + 1, i); }
+            else { d1 = E(j, oW); d3 = E(j, oC); }
+            const double pDpy = 0.25 * ih * (d1 + d3 - d0 - d2);
+            // sin, cos at theta -+ h/2: the faces j and j + 1
+            const double sinN = tab.sinTf[B.rt0 + j], sinS = tab.sinTf[B.rt0 + j + 1];
+            const double cosN = tab.cosTf[B.rt0 + j], cosS = tab.cosTf[B.rt0 + j + 1];
+            double uNorth = 0.5 * (u3 + u1), uSouth = 0.5 * (u1 + u4);
+            if (j == 0) uNorth = 0.5 * (u1 - u3);
+            if (j == nth - 1) uSouth = 0.5 * (u1 - u4);
+            const double psspy = ih * (ih * (v0 + v3 - v1 - v2) + sinS * pupySouth -
+                                       sinN * pupyNorth - cosS * uSouth + cosN * uNorth);
+            const double s22W = 2 * (2 * divW - pvpyWest), s22E = 2 * (2 * divE - pvpyEast);
+            const double pspx = ih * (s22E - s22W);
+            const double f2 = re * invEta * pDpy * s12;
+            const double f4 = re * invEta * cscT * pDpx * 2 * (2 * dvm - pvpy);
+            const double f5 = re * cscT * (psspy + pspx + cosT * s12);
+            out = sinT < 0.1 ? 0. : clamp_ref(f2 + f4 + f5, -0.1f, 0.1f);
+        }
+        fu[base + (size_t)j * nph + i] = out;
+    }
+}
+
 __device__ __forceinline__ bool is_broken(double gamma, double eta, double gmax) {
     return gamma > gmax - kEps || eta == kThresh;
 }
@@ -200,7 +357,7 @@ struct RowSys {
 
 // With `guess` (assembly only) the initial iterate is Gamma* + guess instead
 // of Gamma* on cells that are not broken.
-template <typename T, int C, bool RHS>
+template <typename T, int C, bool RHS, bool VISC = false>
 __device__ __forceinline__ void row_system(const BubbleDev& B, double t, const Tables& tab,
                                            const StatePtrs<T>& s, const AirPtrs<T>& air, int j,
                                            int i0, RowSys<C>& q,
@@ -357,7 +514,17 @@ __device__ __forceinline__ void row_system(const BubbleDev& B, double t, const T
             div += B.invh * CrDt *
                    (uAirF[k + 1] * invR - uAirF[k] * invL + vAirS * sinS * invD - vAirN * sinN * invU);
         }
-        if (B.gravity) {
+        if (VISC && B.gravity) {
+            // `gravity` with `VISCOUS` (bubble.cu:1458-1464): the viscous
+            // forces in place of the gravity term
+            const size_t cc = row + i;
+            const double fuW = air.fu[cc], fuE = air.fu[row + ((i + 1) & m)];
+            const double fvN = hasN ? air.fv[cc - nph] : 0.0;
+            const double fvS = hasS ? air.fv[cc] : 0.0;
+            div += B.invh * dt *
+                   (sinS * fvS * etaDown * invD - sinN * fvN * etaUp * invU +
+                    fuE * etaRight * invR - fuW * etaLeft * invL);
+        } else if (B.gravity) {
             div += B.gs * B.invh * dt *
                    (sinS * sinS * etaDown * invD - sinN * sinN * etaUp * invU);
         }
@@ -465,7 +632,7 @@ __device__ __forceinline__ int line_solve(Ti (&a)[C], Ti (&c)[C], Ti (&d)[C], in
 // matrix. Partials as k_resid. The block of row 0 resets the bubble's
 // solver control block; the broken-cell override of Gamma* is written back
 // to the state as the reference does.
-template <typename T, typename Ti, int C>
+template <typename T, typename Ti, int C, bool VISC>
 __global__ void k_assemble(const int2* __restrict__ rows, const BubbleDev* __restrict__ bub,
                            const BubbleTime* __restrict__ btime, Tables tab, StatePtrs<T> s,
                            AirPtrs<T> air, SysPtrs<Ti> sys, T gmax_state,
@@ -490,8 +657,8 @@ __global__ void k_assemble(const int2* __restrict__ rows, const BubbleDev* __res
     if (act) {
         RowSys<C> q;
         // the extrapolated correction once a previous solve stored one
-        row_system<T, C, true>(B, t, tab, s, air, j, i0, q,
-                               btime[rw.x].ghist > 0 ? guess : nullptr);
+        row_system<T, C, true, VISC>(B, t, tab, s, air, j, i0, q,
+                                     btime[rw.x].ghist > 0 ? guess : nullptr);
         double xv[C];
         Ti wn[C], ww[C], dg[C];
 #pragma unroll
@@ -1027,8 +1194,14 @@ __global__ __launch_bounds__(256) void k_update(const int4* __restrict__ tiles,
     const size_t c = (size_t)B.off + (size_t)j * nph + i;
     const size_t cW = (size_t)B.off + (size_t)j * nph + ((i - 1) & m);
     const double gmax = B.gamma_max, dt = B.dt, invh = B.invh, Cr = B.Cr, M = B.M;
+    const double tnow = btime[tl.x].t;
     double kt = 0.0;
-    if (B.airflow) { const double t = btime[tl.x].t; kt = 3.0 - fabs(t - 3.0); }
+    if (B.airflow) kt = 3.0 - fabs(tnow - 3.0);
+    // VISCOUS (applyforcevel{theta,phi}Kernel_viscous, bubble.cu:1938-2038):
+    // the viscous force, the t/12 mirror blend of the air and no separate
+    // gravity term
+    const bool visc = air.fu != nullptr;
+    const int opp = (nph - i) & m;
 
     // divergenceKernel_fromGamma (bubble.cu:1276-1295)
     const double ga = (double)st.gam[c];
@@ -1075,6 +1248,18 @@ __global__ __launch_bounds__(256) void k_update(const int4* __restrict__ tiles,
         double res;
         if (__builtin_isinf(invEta)) {
             res = __builtin_nan("");
+        } else if (visc) {
+            const double gW = clamp_gamma(x[cW], gmax);
+            const double pGpx = invh * (gnew - gW);
+            const double f3 = -M * invEta * tab.cscTc[B.rt0 + j] * pGpx;
+            double uAir = 0.0;
+            if (B.airflow) {
+                const size_t rowc = (size_t)B.off + (size_t)j * nph;
+                uAir = 0.5 * (air_at(air.ua0, air.au, c, kt) * tnow / 12.0 +
+                              air_at(air.ua0, air.au, rowc + opp, kt) * (1 - tnow / 12.0));
+            }
+            const double f6 = Cr * invEta * uAir;
+            res = (u1 / dt + f3 + f6 + air.fu[c]) / (1. / dt + Cr * invEta);
         } else {
             const double gW = clamp_gamma(x[cW], gmax);
             const double pGpx = invh * (gnew - gW) * tab.cscTc[B.rt0 + j];
@@ -1094,6 +1279,18 @@ __global__ __launch_bounds__(256) void k_update(const int4* __restrict__ tiles,
         double res;
         if (__builtin_isinf(invEta)) {
             res = __builtin_nan("");
+        } else if (visc) {
+            const double gS = clamp_gamma(x[c + nph], gmax);
+            const double pGpy = invh * (gS - gnew);
+            const double f3 = -M * invEta * pGpy;
+            double vAir = 0.0;
+            if (B.airflow) {
+                const size_t rowc = (size_t)B.off + (size_t)j * nph;
+                vAir = 0.5 * (air_at(air.va0, air.av, c, kt) * tnow / 12.0 -
+                              air_at(air.va0, air.av, rowc + opp, kt) * (1 - tnow / 12.0));
+            }
+            const double f6 = Cr * invEta * vAir;
+            res = (v1 / dt + f3 + f6 + air.fv[c]) / (1. / dt + Cr * invEta);
         } else {
             const double gS = clamp_gamma(x[c + nph], gmax);
             const double pGpy = invh * (gS - gnew);
@@ -1453,10 +1650,17 @@ void launch_assemble(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub,
                      const BubbleTime* bt, const Tables& tab, StatePtrs<T> st, AirPtrs<T> air,
                      SysPtrs<Ti> sys, double gmax_state, SolveCtl* ctl, double* part, long stride,
                      const T* guess, const SolveParams& sp) {
-#define SF_CALL(C)                                                                                \
-    hipLaunchKernelGGL((k_assemble<T, Ti, C>), dim3(g.nrows), dim3(g.row_threads), g.lds_bytes, s, \
-                       g.rows, bub, bt, tab, st, air, sys, (T)gmax_state, ctl, part, stride, guess, \
-                       sp)
+#define SF_CALL(C)                                                                                 \
+    do {                                                                                           \
+        if (air.fu)                                                                                \
+            hipLaunchKernelGGL((k_assemble<T, Ti, C, true>), dim3(g.nrows), dim3(g.row_threads),   \
+                               g.lds_bytes, s, g.rows, bub, bt, tab, st, air, sys, (T)gmax_state,  \
+                               ctl, part, stride, guess, sp);                                      \
+        else                                                                                       \
+            hipLaunchKernelGGL((k_assemble<T, Ti, C, false>), dim3(g.nrows), dim3(g.row_threads),  \
+                               g.lds_bytes, s, g.rows, bub, bt, tab, st, air, sys, (T)gmax_state,  \
+                               ctl, part, stride, guess, sp);                                      \
+    } while (0)
     SF_CASES(SF_CALL)
 #undef SF_CALL
 }
@@ -1505,6 +1709,18 @@ void launch_cheb(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, Solve
     SF_CASES(SF_CALL)
 #undef SF_CALL
 }
+
+template <typename T>
+void launch_viscous(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, const Tables& tab,
+                    StatePtrs<T> st, double* div, double* fu, double* fv) {
+    hipLaunchKernelGGL(k_visc_div<T>, dim3(g.ntiles), dim3(256), 0, s, g.tiles, bub, tab, st, div);
+    hipLaunchKernelGGL(k_visc_force<T>, dim3(g.ntiles), dim3(256), 0, s, g.tiles, bub, tab, st, div,
+                       fu, fv);
+}
+template void launch_viscous<float>(hipStream_t, const LaunchGeom&, const BubbleDev*,
+                                    const Tables&, StatePtrs<float>, double*, double*, double*);
+template void launch_viscous<double>(hipStream_t, const LaunchGeom&, const BubbleDev*,
+                                     const Tables&, StatePtrs<double>, double*, double*, double*);
 
 void launch_finalize(hipStream_t s, int nb, const BubbleDev* bub, SolveCtl* ctl, SolveHist* hist,
                      const SolveParams& sp, const double* partR, long stride) {

@@ -37,6 +37,10 @@ struct AirPtrs {
     const T* va0;  // u_theta-face air velocity at t = 0 (vair_init)
     const T* au;   // corridor shape projected on e_phi (per unit time factor)
     const T* av;   // corridor shape projected on e_theta
+    // VISCOUS forces (null without): fu on the u_phi faces, fv on the
+    // u_theta faces, in the cell layout (FP64)
+    const double* fu;
+    const double* fv;
 };
 
 template <typename Ti>
@@ -101,6 +105,10 @@ template <typename Ti>
 void launch_cheb(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, SolveCtl* ctl,
                  SysPtrs<Ti> sys, const Ti* ek, const Ti* ekm1, Ti* enext, int it, int kr,
                  const SolveParams& sp, double* partB, long stride);
+// VISCOUS: div(u) at the centres, then fu, fv (cell layout)
+template <typename T>
+void launch_viscous(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, const Tables& tab,
+                    StatePtrs<T> st, double* div, double* fu, double* fv);
 void launch_finalize(hipStream_t s, int nb, const BubbleDev* bub, SolveCtl* ctl, SolveHist* hist,
                      const SolveParams& sp, const double* partR, long stride);
 template <typename T>
