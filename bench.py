@@ -129,6 +129,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-bimocq", action="store_true")
+    ap.add_argument("--no-viscous", action="store_true")
+    ap.add_argument("--viscous-steps", type=int, default=20)
     ap.add_argument("--no-ensembles", action="store_true")
     ap.add_argument("--bimocq-steps", type=int, default=10)
     ap.add_argument("--profile", choices=("graph", "eager"), default=None,
@@ -258,6 +260,31 @@ def main():
                   "err_vs_fp64_gpu": {k: rel_l2(gb[k], rb[k]) for k in ("eta", "gamma",
                                                                         "u_theta", "u_phi")}}
 
+    # the VISCOUS option (gravity on, so the viscous forces enter the
+    # assembly): mixed with a graph, and the FP64 path for the error
+    viscous = None
+    if not args.no_viscous:
+        nv_steps = args.viscous_steps
+        out = {}
+        for prec, graph in (("mixed", True), ("fp64", False)):
+            ctx = sb.SoapFilm(sb.bubble(n, dt, gravity=True), prec, use_graph=graph,
+                              viscous=True)
+            ctx.load_state(0, st0)
+            ctx.upload(0, "uair0", ua)
+            ctx.upload(0, "vair0", va)
+            el = timed_run(ctx, args.warmup, nv_steps)
+            out[prec] = (el, ctx.state(0), ctx.stats(0))
+            ctx.close()
+        el, gv, sv = out["mixed"]
+        rv = out["fp64"][1]
+        viscous = {"value": cells * nv_steps / el, "unit": "cell-steps/s",
+                   "ms_per_step": el * 1e3 / nv_steps, "steps": nv_steps,
+                   "warmup": args.warmup, "gravity": True,
+                   "fp64_ms_per_step": out["fp64"][0] * 1e3 / nv_steps,
+                   "failed_solves": sv["failed_solves"], "max_residual": sv["max_residual"],
+                   "err_vs_fp64_gpu": {k: rel_l2(gv[k], rv[k]) for k in ("eta", "gamma",
+                                                                         "u_theta", "u_phi")}}
+
     # packed ensembles of uneven resolution (batched form), mixed, graph
     ensembles = None
     if not args.no_ensembles:
@@ -305,6 +332,7 @@ def main():
         "err_vs_fp64_gpu": err_fp64,
         "err_vs_oracle": err_oracle,
         "bimocq": bimocq,
+        "viscous": viscous,
         "ensembles": ensembles,
     }
     print(json.dumps(out))

@@ -174,7 +174,7 @@ void Solver::check(int rc) {
 
 Solver::Solver(size_t nPhi, size_t nTheta, double radius, double dt, double H, int device,
                const std::string& AMGconfig, sf_precision precision, double Gamma_m, double T,
-               bool bimocq, double blend)
+               bool bimocq, double blend, bool viscous)
     : nPhi_(nPhi), nTheta_(nTheta), radius_(radius), dt_(dt) {
     if (nPhi != 2 * nTheta) throw Error(SF_ERR_UNSUPPORTED, "nPhi must be 2 nTheta");
     SolverSettings st;
@@ -209,6 +209,7 @@ Solver::Solver(size_t nPhi, size_t nTheta, double radius, double dt, double H, i
     o.multisample = 1;
     o.blend = blend;
     o.reinit_threshold = 4.0;   // solver.cu:284
+    o.viscous = viscous ? 1 : 0;
     check(sf_create(&o, 1, &d, &ctx_));
 }
 
@@ -301,7 +302,7 @@ Bubble::Bubble(const Config& c)
 int Bubble::run() {
     if (!outputDir_.empty()) std::filesystem::create_directories(outputDir_);
     Solver solver(2 * nTheta_, nTheta_, radius_, dt_, H_, device_, AMGconfig_, precision, Gamma_m_,
-                  T_, bimocq, blend_);
+                  T_, bimocq, blend_, viscous);
     solver.initThicknessfromPic(thicknessImage_);
     solver.initAirflowfromPic(airImage);
     solver.write_thickness_img(outputDir_, 0);

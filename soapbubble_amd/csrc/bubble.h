@@ -52,9 +52,12 @@ public:
     // The scalar advection is BIMOCQ with five samples per point, as
     // header.cuh:97-98 builds the reference, with blendCoeff `blend`;
     // bimocq = false selects the plain advectionAllCentered.
+    // viscous = true selects the VISCOUS stress path (header.cuh:99, off in
+    // the reference).
     Solver(size_t nPhi, size_t nTheta, double radius, double dt, double H, int device,
            const std::string& AMGconfig, sf_precision precision = SF_PREC_FP64,
-           double Gamma_m = 1e-6, double T = 298.15, bool bimocq = true, double blend = 0.5);
+           double Gamma_m = 1e-6, double T = 298.15, bool bimocq = true, double blend = 0.5,
+           bool viscous = false);
     ~Solver();
     Solver(const Solver&) = delete;
     Solver& operator=(const Solver&) = delete;
@@ -93,6 +96,7 @@ public:
     std::string airImage = "../init/velocityfield_512.exr";
     sf_precision precision = SF_PREC_FP64;
     bool bimocq = true;   // header.cuh:97 defines BIMOCQ
+    bool viscous = false; // header.cuh:99 leaves VISCOUS undefined
     bool verbose = true;
     // Bubble::run (bubble.cu:2609-2704): frames of DT seconds, each made of
     // steps of dt and a partial last step; the air-flow time is the frame

@@ -1,7 +1,7 @@
 // soapBubble: the reference's command-line driver (kernel/main.cu:4-56) over
 // libsoapfilm.so.
 //   soapBubble config.txt [--precision fp64|mixed] [--advection bimocq|plain]
-//                         [--air airflow.exr]
+//                         [--air airflow.exr] [--viscous on|off]
 #include <cstring>
 #include <iostream>
 #include <string>
@@ -11,7 +11,7 @@
 static int usage() {
     std::cout << "Please provide the path to config.txt as an argument." << std::endl;
     std::cout << "Usage example: ./soapBubble ../config.txt [--precision fp64|mixed] "
-                 "[--advection bimocq|plain] [--air airflow.exr]"
+                 "[--advection bimocq|plain] [--air airflow.exr] [--viscous on|off]"
               << std::endl;
     std::cout << "Configuration file was missing, exiting." << std::endl;
     return -1;
@@ -35,9 +35,12 @@ int main(int argc, char** argv) {
                 bubble.bimocq = val == "bimocq";
             } else if (key == "--air") {
                 bubble.airImage = val;
+            } else if (key == "--viscous" && (val == "on" || val == "off")) {
+                bubble.viscous = val == "on";
             } else {
                 std::cerr << "unknown option " << key << " " << val
-                          << " (--precision fp64|mixed, --advection bimocq|plain, --air path)"
+                          << " (--precision fp64|mixed, --advection bimocq|plain, --air path, "
+                             "--viscous on|off)"
                           << std::endl;
                 return 2;
             }
