@@ -863,7 +863,7 @@ hipGraphExec_t step_graph(Base& c) {
     return x;
 }
 
-constexpr int kAdaptChunk = 8;
+constexpr int kAdaptChunk = 4;
 
 // Follows the schedule statistics of a finished chunk of replays that ran
 // with `ran` first-refinement iterations: one more when a step needed a
@@ -1031,12 +1031,17 @@ int sf_step(sf_ctx* ctx, int32_t nsteps) {
                 c.recording = false;
             }
             if (c.calib_steps == 1) {
-                // schedule: the inner iterations of each refinement seen while
-                // calibrating, then one spare refinement of one iteration (a
-                // further factor of about ten) that catches a step needing
-                // more; a bubble that converged skips the remaining kernels
+                // schedule: per refinement one inner iteration fewer than the
+                // host-driven step ran (it stopped at the first n whose
+                // residual met the target, and launch n - 1 already produces
+                // that iterate e_n), then one spare refinement of one
+                // iteration (a further factor of about ten) that catches a
+                // step needing more; a bubble that converged skips the
+                // remaining kernels
                 c.sched.clear();
-                for (int n : c.calib) c.sched.push_back(c.opt.max_inner > 0 ? c.opt.max_inner : std::max(n, 1));
+                for (int n : c.calib)
+                    c.sched.push_back(c.opt.max_inner > 0 ? c.opt.max_inner
+                                                          : std::max(c.method == 2 ? n - 1 : n, 1));
                 if (c.sched.empty()) c.sched.push_back(1);
                 c.sched.push_back(1);
                 if (c.opt.max_outer > 0) c.sched.resize((size_t)c.opt.max_outer, 1);

@@ -605,14 +605,32 @@ __device__ __forceinline__ P2d index_of_unit(double invh, const U3& r) {
 // lerpCoords on unit vectors: the point a fraction alpha along the great
 // circle from f to t (Rodrigues rotation about k = f x t / |f x t|); f when
 // the two coincide (k undefined), as the reference returns `from`.
+// Map values at neighbouring grid points are a few cells apart, so the angle
+// between f and t is small: there theta = asin(|f x t|) by its series and
+// sin, cos of alpha theta by Taylor polynomials (every truncation below
+// 1e-20 relative for |f x t| < 0.02) replace acos and sincos; larger angles
+// take the library functions.
 __device__ __forceinline__ U3 slerp_u(const U3& f, const U3& t, double alpha) {
     double kx = f.y * t.z - f.z * t.y, ky = f.z * t.x - f.x * t.z, kz = f.x * t.y - f.y * t.x;
-    const double kin = 1.0 / sqrt(kx * kx + ky * ky + kz * kz);
+    const double k2 = kx * kx + ky * ky + kz * kz;
+    const double kin = 1.0 / sqrt(k2);
     kx *= kin; ky *= kin; kz *= kin;
     if (__builtin_isnan(kx)) return f;
-    const double span = safe_acos_d(f.x * t.x + f.y * t.y + f.z * t.z) * alpha;
+    const double cth = f.x * t.x + f.y * t.y + f.z * t.z;
     double sn, cs;
-    sincos(span, &sn, &cs);
+    if (k2 < 4e-4 && cth > 0.0) {
+        const double sth = k2 * kin;   // |f x t| = sin(theta)
+        // asin(s) = s (1 + s^2/6 + 3 s^4/40 + 5 s^6/112 + 35 s^8/1152 + 63 s^10/2816)
+        const double th =
+            sth * (1.0 + k2 * (1.0 / 6 + k2 * (3.0 / 40 + k2 * (5.0 / 112 +
+                                                                 k2 * (35.0 / 1152 + k2 * (63.0 / 2816))))));
+        const double a = th * alpha, a2 = a * a;
+        sn = a * (1.0 - a2 / 6 * (1.0 - a2 / 20 * (1.0 - a2 / 42 * (1.0 - a2 / 72))));
+        cs = 1.0 - a2 / 2 * (1.0 - a2 / 12 * (1.0 - a2 / 30 * (1.0 - a2 / 56)));
+    } else {
+        const double span = safe_acos_d(cth) * alpha;
+        sincos(span, &sn, &cs);
+    }
     const double kf = kx * f.x + ky * f.y + kz * f.z;
     const double cx = ky * f.z - kz * f.y, cy = kz * f.x - kx * f.z, cz = kx * f.y - ky * f.x;
     return U3{f.x * cs + cx * sn + kx * kf * (1 - cs), f.y * cs + cy * sn + ky * kf * (1 - cs),
