@@ -67,7 +67,7 @@ typedef enum {
  * main.cu:20-36) that reach the step, plus the switches of header.cuh:86-98
  * as run-time flags. */
 typedef struct {
-    int32_t n_theta;      /* numberOfGridsInTheta; nPhi = 2 n_theta (power of two) */
+    int32_t n_theta;      /* numberOfGridsInTheta; nPhi = 2 n_theta; a power of two, 4..8192 */
     int32_t gravity;      /* header.cuh:94 `gravity`                            */
     int32_t evaporation;  /* header.cuh:95 `evaporation`                        */
     int32_t airflow;      /* header.cuh:96 `airflow`                            */

@@ -910,9 +910,12 @@ int sf_create(const sf_options* opt, int32_t nbubbles, const sf_bubble_desc* bub
     if (!opt || !bubbles || nbubbles <= 0) return fail(nullptr, SF_ERR_INVALID, "no bubbles");
     for (int b = 0; b < nbubbles; ++b) {
         const sf_bubble_desc& d = bubbles[b];
-        if (d.n_theta < 4 || !is_pow2(d.n_theta))
+        // a row of 2 n_theta cells is one block of at most kRowThreadsMax
+        // threads holding at most 16 cells each
+        if (d.n_theta < 4 || !is_pow2(d.n_theta) || 2 * d.n_theta > 16 * kRowThreadsMax)
             return fail(nullptr, SF_ERR_UNSUPPORTED,
-                        "n_theta must be a power of two >= 4 (bubble " + std::to_string(b) + ")");
+                        "n_theta must be a power of two from 4 to " + std::to_string(8 * kRowThreadsMax) +
+                            " (bubble " + std::to_string(b) + ")");
         if (!(d.radius > 0) || !(d.H > 0) || !(d.U > 0) || !(d.Gamma_m > 0) || !(d.T > 0) ||
             !(d.dt > 0) || !(d.M_scale > 0))
             return fail(nullptr, SF_ERR_INVALID, "non-positive physical parameter");

@@ -150,6 +150,19 @@ def test_synth_noise_bounded():
     assert np.array_equal(a["gamma"], b["gamma"])
 
 
+@pytest.mark.parametrize("n_theta", [2, 12, 16384])
+def test_create_rejects_unsupported_sizes(n_theta):
+    """n_theta is a power of two from 4 (rows of 8 cells) to 8192 (rows of
+    16384 cells, one block of 1024 threads with 16 cells each); the check
+    runs before any device work."""
+    lib = sb.load_library()
+    opt = sb.Options(sb.PRECISIONS["mixed"], 0, 0, 0, 1e-10, 0, 0, 0.0, 1, 0, 0, 1, 0.5, 0.0, 0, 0)
+    arr = (sb.BubbleDesc * 1)(sb.bubble(n_theta, 0.01))
+    out = ctypes.c_void_p()
+    assert lib.sf_create(ctypes.byref(opt), 1, arr, ctypes.byref(out)) == 3
+    assert "from 4 to 8192" in lib.sf_last_error(None).decode()
+
+
 @pytest.mark.parametrize("kw,msg", [({"initial_guess": 3}, "initial_guess"),
                                     ({"max_outer": 17}, "max_outer")])
 def test_create_rejects_bad_options(kw, msg):
