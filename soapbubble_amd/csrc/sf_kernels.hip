@@ -57,7 +57,12 @@ __device__ __forceinline__ void advect_cell(const BubbleDev& B, const Tables& ta
     {
         const T gx = (T)j + T(0.5), gy = (T)i;
         T guT, guP;
-        if (interior) {
+        if (interior && !NANS) {
+            // the same samples without their zero-weight terms (which add
+            // 0 x finite: nothing, in fields that hold no NaN)
+            guT = lerp_s<T, false>(v.uth.at(j - 1, i), v.uth.at(j, i), T(0.5));
+            guP = v.uph.template belt<false>(j, i, T(0.5));
+        } else if (interior) {
             guT = lerp_s<T, NANS>(v.uth.template belt<NANS>(j - 1, i, T(0)),
                                   v.uth.template belt<NANS>(j, i, T(0)), T(0.5));
             guP = lerp_s<T, NANS>(v.uph.template belt<NANS>(j, i, T(0.5)),
@@ -1155,6 +1160,17 @@ __global__ void k_finalize(const BubbleDev* __restrict__ bub, SolveCtl* __restri
 }
 
 // ------------------------------------------------------------- update ----
+// a / y through the refined reciprocal (within an ulp or two of the
+// quotient, a quarter of the division's cost); a divisor of magnitude
+// outside [1e-300, 1e300] (zero, denormal, huge, infinite or NaN) takes the
+// division itself, so that 2 / 0 = inf and 0 / 0 = NaN come out as the
+// reference's.
+__device__ __forceinline__ double qdiv(double a, double y) {
+    const double ay = fabs(y);
+    if (ay >= 1e-300 && ay <= 1e300) return a * rcp64(y);
+    return a / y;
+}
+
 __device__ __forceinline__ double clamp_gamma(double g, double gmax) {
     return (g <= 0.0 || g > gmax - kEps) ? gmax : g;
 }
@@ -1194,6 +1210,8 @@ __global__ __launch_bounds__(256) void k_update(const int4* __restrict__ tiles,
     const size_t c = (size_t)B.off + (size_t)j * nph + i;
     const size_t cW = (size_t)B.off + (size_t)j * nph + ((i - 1) & m);
     const double gmax = B.gamma_max, dt = B.dt, invh = B.invh, Cr = B.Cr, M = B.M;
+    // the reference's divisions by dt as products with 1 / dt (an ulp)
+    const double invDt = 1.0 / dt;
     const double tnow = btime[tl.x].t;
     double kt = 0.0;
     if (B.airflow) kt = 3.0 - fabs(tnow - 3.0);
@@ -1209,10 +1227,10 @@ __global__ __launch_bounds__(256) void k_update(const int4* __restrict__ tiles,
     double gnew, div;
     if (g <= 0.0 || g > gmax - kEps) {
         gnew = gmax;
-        div = 1.0 / dt;
+        div = invDt;
     } else {
         gnew = g;
-        div = (1.0 - fmin(g, 2 * ga) / ga) / dt;
+        div = (1.0 - qdiv(fmin(g, 2 * ga), ga)) * invDt;
     }
     if (gdel) {
         // initial guess of the next solve: its correction Gamma^{n+1} - Gamma*
@@ -1244,7 +1262,7 @@ __global__ __launch_bounds__(256) void k_update(const int4* __restrict__ tiles,
     {
         const double etaW = (double)st.eta[cW];
         const double u1 = (double)st.uph[c];
-        const double invEta = 2. / (etaW + eta);
+        const double invEta = qdiv(2., etaW + eta);
         double res;
         if (__builtin_isinf(invEta)) {
             res = __builtin_nan("");
@@ -1259,14 +1277,14 @@ __global__ __launch_bounds__(256) void k_update(const int4* __restrict__ tiles,
                               air_at(air.ua0, air.au, rowc + opp, kt) * (1 - tnow / 12.0));
             }
             const double f6 = Cr * invEta * uAir;
-            res = (u1 / dt + f3 + f6 + air.fu[c]) / (1. / dt + Cr * invEta);
+            res = qdiv(u1 * invDt + f3 + f6 + air.fu[c], invDt + Cr * invEta);
         } else {
             const double gW = clamp_gamma(x[cW], gmax);
             const double pGpx = invh * (gnew - gW) * tab.cscTc[B.rt0 + j];
             const double f1 = -M * invEta * pGpx;
             const double uAir = B.airflow ? air_at(air.ua0, air.au, c, kt) : 0.0;
             const double f2 = Cr * invEta * uAir;
-            res = (u1 / dt + (f1 + f2 + 0.0 + 0.0)) / (1. / dt + Cr * invEta);
+            res = qdiv(u1 * invDt + (f1 + f2 + 0.0 + 0.0), invDt + Cr * invEta);
         }
         out.uph[c] = (T)res;
         nan_out |= __builtin_isnan(res);
@@ -1275,7 +1293,7 @@ __global__ __launch_bounds__(256) void k_update(const int4* __restrict__ tiles,
     if (j < nth - 1) {
         const double etaS = (double)st.eta[c + nph];
         const double v1 = (double)st.uth[c];
-        const double invEta = 2. / (eta + etaS);
+        const double invEta = qdiv(2., eta + etaS);
         double res;
         if (__builtin_isinf(invEta)) {
             res = __builtin_nan("");
@@ -1290,7 +1308,7 @@ __global__ __launch_bounds__(256) void k_update(const int4* __restrict__ tiles,
                               air_at(air.va0, air.av, rowc + opp, kt) * (1 - tnow / 12.0));
             }
             const double f6 = Cr * invEta * vAir;
-            res = (v1 / dt + f3 + f6 + air.fv[c]) / (1. / dt + Cr * invEta);
+            res = qdiv(v1 * invDt + f3 + f6 + air.fv[c], invDt + Cr * invEta);
         } else {
             const double gS = clamp_gamma(x[c + nph], gmax);
             const double pGpy = invh * (gS - gnew);
@@ -1298,7 +1316,7 @@ __global__ __launch_bounds__(256) void k_update(const int4* __restrict__ tiles,
             const double vAir = B.airflow ? air_at(air.va0, air.av, c, kt) : 0.0;
             const double f2 = Cr * invEta * vAir;
             const double f3 = B.gravity ? B.gs * tab.sinTf[B.rt0 + j + 1] : 0.0;
-            res = (v1 / dt + f1 + f2 + f3 + 0.0) / (1. / dt + Cr * invEta);
+            res = qdiv(v1 * invDt + f1 + f2 + f3 + 0.0, invDt + Cr * invEta);
         }
         out.uth[c] = (T)res;
         nan_out |= __builtin_isnan(res);
