@@ -216,6 +216,14 @@ __device__ __forceinline__ bool validate_coord(T& x, T& y, int nth, int nph) {
     return ret;
 }
 
+// Cell (j, i) of thread threadIdx.x in a packed tile {bubble, first row,
+// first column, width}; tile widths are powers of two.
+__device__ __forceinline__ void tile_cell(const int4& tl, int& j, int& i) {
+    const int sh = __builtin_ctz((unsigned)tl.w);
+    j = tl.y + ((int)threadIdx.x >> sh);
+    i = tl.z + ((int)threadIdx.x & (tl.w - 1));
+}
+
 template <typename T>
 struct Field {
     const T* __restrict__ p;   // bubble base
@@ -258,18 +266,17 @@ __device__ __forceinline__ float bload(__amdgpu_buffer_rsrc_t r, int off, int so
     return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, soff, 0));
 }
 
-// x, y: sampler-relative index coordinates, x in [0, rows - 1)
+// x, y: sampler-relative index coordinates, x in [0, rows - 1). The phi
+// wrap acts on the integer column (nph is a power of two), so the weight
+// y - floor(y) keeps every bit of y, also for y < 0.
 __device__ __forceinline__ FastIdx fast_index(float x, float y, int nph, int mask) {
-    const float tp = (float)nph;
-    y = y < 0.0f ? y + tp : y;
-    y = y >= tp ? y - tp : y;
     const float fx = floorf(x), fy = floorf(y);
     FastIdx q;
     q.aT = x - fx;
     q.aP = y - fy;
-    const int tI = (int)fx, pI = (int)fy;
-    q.off = (tI * nph + pI) * 4;
-    q.off1 = pI == mask ? q.off - mask * 4 : q.off + 4;
+    const int row = (int)fx * nph, pI = (int)fy;
+    q.off = (row + (pI & mask)) * 4;
+    q.off1 = (row + ((pI + 1) & mask)) * 4;
     return q;
 }
 
@@ -454,47 +461,57 @@ struct VelCtx {
     T invh, dt;
 };
 
-// atan2(n, d) for the small angle between a traced point and its start
-// point: the odd series t - t^3/3 + t^5/5 of atan(t), t = n/d, below
-// |t| = 1/10 (truncation under 1.5e-8 rad), the general atan2 otherwise
-// (the phi change of a trace that passes close to a pole).
-__device__ __forceinline__ float small_atan2(float n, float d) {
-    if (d > 0.0f && fabsf(n) < 0.1f * d) {
-        const float t = n * __builtin_amdgcn_rcpf(d), t2 = t * t;
-        return t * (1.0f + t2 * (-1.0f / 3.0f + t2 * 0.2f));
+// The small angle a = atan2(n, d) between a traced point and its start
+// point, given (n, d) = k (sin a, cos a) and k_inv = 1 / k: with s = n k_inv
+// = sin a, the odd series s + s^3/6 + 3 s^5/40 of asin(s) below |s| = 1/10
+// (truncation under 5e-9 rad; no division, the scale k is 1 or the 1/rho
+// that the tangent frame needs anyway), the general atan2 otherwise (a
+// large step, or the phi change of a trace that passes close to a pole).
+__device__ __forceinline__ float small_angle(float n, float d, float k_inv) {
+    const float sn = n * k_inv, cs = d * k_inv;
+    if (cs > 0.0f && fabsf(sn) < 0.1f) {
+        const float s2 = sn * sn;
+        return sn * (1.0f + s2 * (1.0f / 6.0f + s2 * (3.0f / 40.0f)));
     }
     return tatan2(n, d);
 }
 
-// Index-space coordinates of a traced point (x, y, z) and rho = sin(theta).
-// FP64: theta = atan2(rho, z), phi = atan2(y, x). FP32: theta and phi as the
-// start point's plus the angles turned from it, theta - theta0 =
-// atan2(rho z0 - z rho0, z z0 + rho rho0) in the meridian half-plane and
-// phi - phi0 = atan2(y x0 - x y0, x x0 + y y0) (x0, y0 the start point's
-// cos phi0, sin phi0): the same angles, with the grid-exact start index
+// Index-space coordinates of a traced point (x, y, z), rho = sin(theta) and
+// ir = 1 / rho (used where rho > 0). FP64: theta = atan2(rho, z), phi =
+// atan2(y, x). FP32: theta and phi as the start point's plus the angles
+// turned from it, theta - theta0 = atan2(rho z0 - z rho0, z z0 + rho rho0) in
+// the meridian half-plane (a unit vector: scale 1) and phi - phi0 =
+// atan2(y x0 - x y0, x x0 + y y0) (x0, y0 the start point's cos phi0,
+// sin phi0; scale rho): the same angles, with the grid-exact start index
 // carrying the magnitude so FP32 keeps the small increment's precision.
 template <typename T>
 __device__ __forceinline__ void point_index(const Circle<T>& o, T x, T y, T z, T invh, T& ti,
-                                            T& pi_, T& rho);
+                                            T& pi_, T& rho, T& ir);
 template <>
 __device__ __forceinline__ void point_index(const Circle<double>& o, double x, double y, double z,
-                                            double invh, double& ti, double& pi_, double& rho) {
+                                            double invh, double& ti, double& pi_, double& rho,
+                                            double& ir) {
     rho = sqrt(x * x + y * y);
+    ir = 1.0 / rho;
     ti = atan2(rho, z) * invh;
     pi_ = atan2(y, x) * invh;
 }
 template <>
 __device__ __forceinline__ void point_index(const Circle<float>& o, float x, float y, float z,
-                                            float invh, float& ti, float& pi_, float& rho) {
-    rho = tsqrt(x * x + y * y);
-    ti = o.gx + small_atan2(rho * o.cT - z * o.sT, z * o.cT + rho * o.sT) * invh;
-    pi_ = o.gy + small_atan2(y * o.cP - x * o.sP, x * o.cP + y * o.sP) * invh;
+                                            float invh, float& ti, float& pi_, float& rho,
+                                            float& ir) {
+    const float r2 = x * x + y * y;
+    ir = __builtin_amdgcn_rsqf(r2);
+    rho = r2 > 0.0f ? r2 * ir : 0.0f;
+    ti = o.gx + small_angle(rho * o.cT - z * o.sT, z * o.cT + rho * o.sT, 1.0f) * invh;
+    pi_ = o.gy + small_angle(y * o.cP - x * o.sP, x * o.cP + y * o.sP, ir) * invh;
 }
 
 template <typename T>
-__device__ __forceinline__ void tangent_vec(T x, T y, T z, T rho, T vT, T vP, T& o0, T& o1, T& o2) {
+__device__ __forceinline__ void tangent_vec(T x, T y, T z, T rho, T ir, T vT, T vP, T& o0, T& o1,
+                                            T& o2) {
     T cp = T(1), sp = T(0);
-    if (rho > T(0)) { T ir = trcp(rho); cp = x * ir; sp = y * ir; }
+    if (rho > T(0)) { cp = x * ir; sp = y * ir; }
     o0 = vT * z * cp - vP * sp;
     o1 = vT * z * sp + vP * cp;
     o2 = -vT * rho;
@@ -515,12 +532,12 @@ __device__ __forceinline__ bool circle_mid(const VelCtx<T>& c, T gx, T gy, T sin
     T s, co;
     sincos_step(T(-0.5) * o.unorm * c.dt, &s, &co);
     T mx = o.u0 * s + o.w0 * co, my = o.u1 * s + o.w1 * co, mz = o.u2 * s + o.w2 * co;
-    T mti, mpi, rho;
-    point_index(o, mx, my, mz, c.invh, mti, mpi, rho);
+    T mti, mpi, rho, ir;
+    point_index(o, mx, my, mz, c.invh, mti, mpi, rho, ir);
     T muT = sample_vtheta<T, NANS>(c.uth, c.nth, mti, mpi);
     T muP = sample_vphi<T, NANS>(c.uph, c.nth, mti, mpi);
     T m0, m1, m2;
-    tangent_vec(mx, my, mz, rho, muT, muP, m0, m1, m2);
+    tangent_vec(mx, my, mz, rho, ir, muT, muP, m0, m1, m2);
     T uc0 = o.u0 * co - o.w0 * s, uc1 = o.u1 * co - o.w1 * s, uc2 = o.u2 * co - o.w2 * s;
     T vc0 = my * uc2 - mz * uc1, vc1 = mz * uc0 - mx * uc2, vc2 = mx * uc1 - my * uc0;
     T mguT = m0 * vc0 + m1 * vc1 + m2 * vc2;
@@ -543,12 +560,12 @@ __device__ __forceinline__ bool circle_mid(const VelCtx<T>& c, T gx, T gy, T sin
 // the Cartesian point (bubble.cu:498-505, 734-743).
 template <typename T>
 __device__ __forceinline__ void circle_depart(const VelCtx<T>& c, const Circle<T>& o, T& s, T& co,
-                                              T& px, T& py, T& pz, T& ti, T& pi_, T& rho) {
+                                              T& px, T& py, T& pz, T& ti, T& pi_, T& rho, T& ir) {
     sincos_step(-o.unorm * c.dt, &s, &co);
     px = o.u0 * s + o.w0 * co;
     py = o.u1 * s + o.w1 * co;
     pz = o.u2 * s + o.w2 * co;
-    point_index(o, px, py, pz, c.invh, ti, pi_, rho);
+    point_index(o, px, py, pz, c.invh, ti, pi_, rho, ir);
 }
 
 // Velocity at the departure point transported back along the circle into
@@ -556,12 +573,12 @@ __device__ __forceinline__ void circle_depart(const VelCtx<T>& c, const Circle<T
 template <typename T, bool NANS = true>
 __device__ __forceinline__ void transport_back(const VelCtx<T>& c, const Circle<T>& o, T& r0, T& r1,
                                                T& r2) {
-    T s, co, px, py, pz, ti, pi_, rho;
-    circle_depart(c, o, s, co, px, py, pz, ti, pi_, rho);
+    T s, co, px, py, pz, ti, pi_, rho, ir;
+    circle_depart(c, o, s, co, px, py, pz, ti, pi_, rho, ir);
     T puT = sample_vtheta<T, NANS>(c.uth, c.nth, ti, pi_);
     T puP = sample_vphi<T, NANS>(c.uph, c.nth, ti, pi_);
     T p0, p1, p2;
-    tangent_vec(px, py, pz, rho, puT, puP, p0, p1, p2);
+    tangent_vec(px, py, pz, rho, ir, puT, puP, p0, p1, p2);
     T uc0 = o.u0 * co - o.w0 * s, uc1 = o.u1 * co - o.w1 * s, uc2 = o.u2 * co - o.w2 * s;
     T vc0 = py * uc2 - pz * uc1, vc1 = pz * uc0 - px * uc2, vc2 = px * uc1 - py * uc0;
     T a = p0 * vc0 + p1 * vc1 + p2 * vc2;

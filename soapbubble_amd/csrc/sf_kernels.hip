@@ -74,8 +74,8 @@ __device__ __forceinline__ void advect_cell(const BubbleDev& B, const Tables& ta
         Circle<T> o;
         T tx = gx, ty = gy;
         if (circle_mid<T, NANS>(v, gx, gy, sinTc, cosTc, sinPc, cosPc, guT, guP, o)) {
-            T s, co, px, py, pz, rho;
-            circle_depart(v, o, s, co, px, py, pz, tx, ty, rho);
+            T s, co, px, py, pz, rho, ir;
+            circle_depart(v, o, s, co, px, py, pz, tx, ty, rho, ir);
         }
         T en, gn;
         sample_centered2<T, NANS>(feta, fgam, B.nth, tx, ty, en, gn);
@@ -142,8 +142,8 @@ __global__ __launch_bounds__(256) void k_advect(const int4* __restrict__ tiles,
             btime[tl.x].gpend = 0;
         }
     }
-    const int j = tl.y + (int)threadIdx.x / tl.w;
-    const int i = tl.z + (int)threadIdx.x % tl.w;
+    int j, i;
+    tile_cell(tl, j, i);
     if (j >= B.nth || i >= B.nph) return;
     if (btime[tl.x].nans)
         advect_cell<T, true>(B, tab, in, out, j, i);
@@ -184,8 +184,8 @@ __global__ __launch_bounds__(256) void k_visc_div(const int4* __restrict__ tiles
     const int4 tl = tiles[blockIdx.x];
     if (tl.x < 0) return;
     const BubbleDev B = bub[tl.x];
-    const int j = tl.y + (int)threadIdx.x / tl.w;
-    const int i = tl.z + (int)threadIdx.x % tl.w;
+    int j, i;
+    tile_cell(tl, j, i);
     if (j >= B.nth || i >= B.nph) return;
     const int nph = B.nph, m = B.mask;
     const size_t row = (size_t)B.off + (size_t)j * nph;
@@ -211,8 +211,8 @@ __global__ __launch_bounds__(256) void k_visc_force(const int4* __restrict__ til
     const int4 tl = tiles[blockIdx.x];
     if (tl.x < 0) return;
     const BubbleDev B = bub[tl.x];
-    const int j = tl.y + (int)threadIdx.x / tl.w;
-    const int i = tl.z + (int)threadIdx.x % tl.w;
+    int j, i;
+    tile_cell(tl, j, i);
     if (j >= B.nth || i >= B.nph) return;
     const int nph = B.nph, m = B.mask, nth = B.nth, hn = nph / 2;
     const size_t base = (size_t)B.off;
@@ -1202,8 +1202,8 @@ __global__ __launch_bounds__(256) void k_update(const int4* __restrict__ tiles,
         // the next assembly adds its sums into a cleared slot
         if (threadIdx.x < 64) acc_zero(acc, tl.x, acc_slot_res(0));
     }
-    const int j = tl.y + (int)threadIdx.x / tl.w;
-    const int i = tl.z + (int)threadIdx.x % tl.w;
+    int j, i;
+    tile_cell(tl, j, i);
     if (j >= B.nth || i >= B.nph) return;
     const int nph = B.nph, m = B.mask, nth = B.nth;
     const double* __restrict__ x = ctl[tl.x].xbuf ? x1 : x0;
@@ -1373,7 +1373,8 @@ __global__ __launch_bounds__(256) void k_bm_init(const int4* __restrict__ tiles,
     const int4 tl = tiles[blockIdx.x];
     if (tl.x < 0) return;
     const BubbleDev B = bub[tl.x];
-    const int j = tl.y + (int)threadIdx.x / tl.w, i = tl.z + (int)threadIdx.x % tl.w;
+    int j, i;
+    tile_cell(tl, j, i);
     if (j >= B.nth || i >= B.nph) return;
     const size_t c = (size_t)B.off + (size_t)j * B.nph + i;
     set_identity(P.fwd, c, B.h, j, i);
@@ -1397,8 +1398,8 @@ __device__ __forceinline__ void trace_centre(const VelCtx<T>& v, const Tables& t
     tx = gx;
     ty = gy;
     if (circle_mid<T, NANS>(v, gx, gy, sinTc, cosTc, sinPc, cosPc, guT, guP, o)) {
-        T s, co, px, py, pz, rho;
-        circle_depart(v, o, s, co, px, py, pz, tx, ty, rho);
+        T s, co, px, py, pz, rho, ir;
+        circle_depart(v, o, s, co, px, py, pz, tx, ty, rho, ir);
     }
 }
 
@@ -1435,7 +1436,8 @@ __global__ __launch_bounds__(256) void k_bm_maps(const int4* __restrict__ tiles,
     const int4 tl = tiles[blockIdx.x];
     if (tl.x < 0) return;
     const BubbleDev B = bub[tl.x];
-    const int j = tl.y + (int)threadIdx.x / tl.w, i = tl.z + (int)threadIdx.x % tl.w;
+    int j, i;
+    tile_cell(tl, j, i);
     if (j >= B.nth || i >= B.nph) return;
     if (btime[tl.x].nans)
         bm_maps_cell<T, true>(B, tab, cur, P, j, i);
@@ -1452,7 +1454,8 @@ __global__ __launch_bounds__(256) void k_bm_eta(const int4* __restrict__ tiles,
     const int4 tl = tiles[blockIdx.x];
     if (tl.x < 0) return;
     const BubbleDev B = bub[tl.x];
-    const int j = tl.y + (int)threadIdx.x / tl.w, i = tl.z + (int)threadIdx.x % tl.w;
+    int j, i;
+    tile_cell(tl, j, i);
     if (j >= B.nth || i >= B.nph) return;
     const size_t o = (size_t)B.off, c = o + (size_t)j * B.nph + i;
     if ((double)cur.eta[c] == kThresh) {
@@ -1492,7 +1495,8 @@ __global__ __launch_bounds__(256) void k_bm_accum(const int4* __restrict__ tiles
     const int4 tl = tiles[blockIdx.x];
     if (tl.x < 0) return;
     const BubbleDev B = bub[tl.x];
-    const int j = tl.y + (int)threadIdx.x / tl.w, i = tl.z + (int)threadIdx.x % tl.w;
+    int j, i;
+    tile_cell(tl, j, i);
     float dmax = 0.0f;
     if (j < B.nth && i < B.nph) {
         const size_t o = (size_t)B.off, c = o + (size_t)j * B.nph + i;
@@ -1555,7 +1559,8 @@ __global__ __launch_bounds__(256) void k_bm_correct1(const int4* __restrict__ ti
     const int4 tl = tiles[blockIdx.x];
     if (tl.x < 0 || !P.flag[tl.x]) return;
     const BubbleDev B = bub[tl.x];
-    const int j = tl.y + (int)threadIdx.x / tl.w, i = tl.z + (int)threadIdx.x % tl.w;
+    int j, i;
+    tile_cell(tl, j, i);
     if (j >= B.nth || i >= B.nph) return;
     const size_t o = (size_t)B.off, c = o + (size_t)j * B.nph + i;
     const Field<T> fe{cur.eta + o, B.nph, B.mask, B.nth - 1};
@@ -1601,7 +1606,8 @@ __global__ __launch_bounds__(256) void k_bm_correct2(const int4* __restrict__ ti
     const int4 tl = tiles[blockIdx.x];
     if (tl.x < 0 || !P.flag[tl.x]) return;
     const BubbleDev B = bub[tl.x];
-    const int j = tl.y + (int)threadIdx.x / tl.w, i = tl.z + (int)threadIdx.x % tl.w;
+    int j, i;
+    tile_cell(tl, j, i);
     if (j >= B.nth || i >= B.nph) return;
     const size_t o = (size_t)B.off, c = o + (size_t)j * B.nph + i;
     const Field<double> ee{P.err_e + o, B.nph, B.mask, B.nth - 1};
@@ -1630,7 +1636,8 @@ __global__ __launch_bounds__(256) void k_bm_finish(const int4* __restrict__ tile
     const int4 tl = tiles[blockIdx.x];
     if (tl.x < 0 || !P.flag[tl.x]) return;
     const BubbleDev B = bub[tl.x];
-    const int j = tl.y + (int)threadIdx.x / tl.w, i = tl.z + (int)threadIdx.x % tl.w;
+    int j, i;
+    tile_cell(tl, j, i);
     if (j >= B.nth || i >= B.nph) return;
     const size_t c = (size_t)B.off + (size_t)j * B.nph + i;
     P.e_initlast[c] = P.e_init[c];
