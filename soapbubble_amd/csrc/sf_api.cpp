@@ -266,15 +266,18 @@ struct Impl : Base {
     }
 
     // FP64 residual evaluation k >= 1: x64[k&1] = x64[(k-1)&1] + ecorr; sh
-    // (captured steps) collects the schedule statistics.
-    void resid(int k, Ti* ecorr, SchedStat* sh) {
+    // (captured steps) collects the schedule statistics. `last`: no refinement
+    // follows (no line solve of the new residual); `check_prev`: refinement
+    // k - 1 ran no inner iteration, so this launch tests its start residual.
+    void resid(int k, Ti* ecorr, SchedStat* sh, bool last = false, bool check_prev = false) {
         const double* xo = x64[(k + 1) & 1];
         double* xn = x64[k & 1];
         SysPtrs<Ti> s = sys(x64[0]);
         s.e = ecorr;
+        if (last) s.z = nullptr;
         for (const LaunchGeom& rg : rgeo)
             launch_resid<T, Ti>(stream, rg, bub_d, tab, nxt, ctl_d, s, xo, xn, k, partR, stride,
-                                partB, sp, method == 2 ? sh : nullptr);
+                                partB, sp, method == 2 ? sh : nullptr, check_prev ? 1 : 0);
     }
     // Chebyshev iterates: e_1 is z; e_k for k >= 2 rotates over e, Ap, p[0]
     // (k mod 3), so e_{k-1}, e_k and e_{k+1} are distinct.
@@ -387,9 +390,13 @@ struct Impl : Base {
                     resid(k + 1, cheb_e(n + 1), nullptr);
                 }
             } else {
+                // a refinement of no inner iteration (only ever the last one:
+                // its correction z is read by its residual launch, which then
+                // writes no z of its own) takes e_1 = z of the residual before
                 for (size_t k = 0; k < sched.size(); ++k) {
                     for (int it = 1; it <= sched[k]; ++it) cheb(it, (int)k);
-                    resid((int)k + 1, cheb_e(sched[k] + 1), sstat_d);
+                    resid((int)k + 1, cheb_e(sched[k] + 1), sstat_d, k + 1 == sched.size(),
+                          sched[k] == 0 && k > 0);
                 }
             }
         } else if (host_driven) {
@@ -1049,6 +1056,12 @@ int sf_step(sf_ctx* ctx, int32_t nsteps) {
                 c.sched.push_back(1);
                 if (c.opt.max_outer > 0) c.sched.resize((size_t)c.opt.max_outer, 1);
                 if (c.sched.size() > (size_t)kAccRefine) c.sched.resize(kAccRefine);
+                // Chebyshev: the last refinement (run only when those before
+                // missed the tolerance) takes the line solve z = M^-1 r that
+                // the residual before it produced as its correction, with no
+                // inner launch (the start-residual test moves into its
+                // residual launch): one skipped launch per step fewer
+                if (c.method == 2 && c.sched.size() >= 2) c.sched.back() = 0;
                 c.calibrated = true;
                 if (c.trace_solve) {
                     std::fprintf(stderr, "[sf solve] graph schedule:");

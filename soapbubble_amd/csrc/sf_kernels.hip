@@ -798,18 +798,31 @@ __device__ __forceinline__ void ld_stencil(const Ti* __restrict__ v, size_t row,
 // (Chebyshev) the block of row 0 records for the schedule of the inner
 // iterations: after the first refinement, |r_n|^2 of its last inner
 // iteration n over (inner_frac tol)^2 |b|^2 (at most 1: n - 1 iterations
-// would have done), and whether a further refinement ran.
+// would have done), and whether a further refinement ran. Without sys.z (the
+// last refinement of a step) no line solve follows. With `check_prev` (the
+// refinement had no inner iteration: its correction is the z of the residual
+// before) the start-of-refinement test of the Chebyshev launches is done
+// here: a start residual (slot k - 1) that met the tolerance ends the solve.
 template <typename T, typename Ti, int C>
 __global__ void k_resid(const int2* __restrict__ rows, const BubbleDev* __restrict__ bub,
                         Tables tab, StatePtrs<T> s, SolveCtl* __restrict__ ctl, SysPtrs<Ti> sys,
                         const double* __restrict__ xo, double* __restrict__ xn, int k,
                         double* __restrict__ part, long stride, const double* __restrict__ partB,
-                        SolveParams sp, SchedStat* __restrict__ sh) {
+                        SolveParams sp, SchedStat* __restrict__ sh, int check_prev) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     __shared__ double red[16 * 4];
     __shared__ float redf[16];
     const int2 rw = rows[blockIdx.x];
     if (rw.x < 0 || flag_load(&ctl[rw.x].outer_done)) return;
+    if (check_prev) {
+        const double rr = acc_sum(sys.acc, rw.x, acc_slot_res(k - 1), 0);
+        const double bb = acc_sum(sys.acc, rw.x, acc_slot_res(k - 1), 1);
+        if (residual_norm(rr, bb, sp.absolute) <= sp.tol) {
+            if (rw.y == 0 && threadIdx.x == 0) flag_store(&ctl[rw.x].outer_done, 1);
+            return;
+        }
+    }
+    const bool next = sys.z != nullptr;
     const Ti* __restrict__ ecorr = sys.e;
     const BubbleDev B = bub[rw.x];
     const int j = rw.y, nph = B.nph, m = B.mask, nth = B.nth;
@@ -860,11 +873,13 @@ __global__ void k_resid(const int2* __restrict__ rows, const BubbleDev* __restri
             d[q] = rin[q] * inv;
         }
     }
-    line_solve<Ti, C>(a, cc, d, nph, steps, reinterpret_cast<Ti*>(lds_raw), redf);
-    if (act) {
-        st_row<C>(sys.z + row + i0, d);
+    if (next) {
+        line_solve<Ti, C>(a, cc, d, nph, steps, reinterpret_cast<Ti*>(lds_raw), redf);
+        if (act) {
+            st_row<C>(sys.z + row + i0, d);
 #pragma unroll
-        for (int q = 0; q < C; ++q) acc[2] += (double)rin[q] * (double)d[q];
+            for (int q = 0; q < C; ++q) acc[2] += (double)rin[q] * (double)d[q];
+        }
     }
     block_sum<4>(acc, red);
     if (threadIdx.x == 0) {
@@ -1694,10 +1709,11 @@ template <typename T, typename Ti>
 void launch_resid(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, const Tables& tab,
                   StatePtrs<T> st, SolveCtl* ctl, SysPtrs<Ti> sys, const double* xo, double* xn,
                   int k, double* part, long stride, const double* partB, const SolveParams& sp,
-                  SchedStat* sh) {
+                  SchedStat* sh, int check_prev) {
 #define SF_CALL(C)                                                                              \
     hipLaunchKernelGGL((k_resid<T, Ti, C>), dim3(g.nrows), dim3(g.row_threads), g.lds_bytes, s, \
-                       g.rows, bub, tab, st, ctl, sys, xo, xn, k, part, stride, partB, sp, sh)
+                       g.rows, bub, tab, st, ctl, sys, xo, xn, k, part, stride, partB, sp, sh,  \
+                       check_prev)
     SF_CASES(SF_CALL)
 #undef SF_CALL
 }
@@ -1782,7 +1798,7 @@ template void launch_advect<double>(hipStream_t, const LaunchGeom&, const Bubble
     template void launch_resid<T, Ti>(hipStream_t, const LaunchGeom&, const BubbleDev*,           \
                                       const Tables&, StatePtrs<T>, SolveCtl*, SysPtrs<Ti>,        \
                                       const double*, double*, int, double*, long,               \
-                                      const double*, const SolveParams&, SchedStat*);
+                                      const double*, const SolveParams&, SchedStat*, int);
 SF_INST_SYS(float, float)
 SF_INST_SYS(float, double)
 SF_INST_SYS(double, double)

@@ -85,3 +85,33 @@ def test_adaptive_schedule_ensemble():
             assert s["failed_solves"] == 0, b
             assert s["max_residual"] <= 1e-10, b
             assert np.isfinite(ctx.state(b)["gamma"]).all()
+
+
+def test_last_refinement_applies_the_line_solve():
+    """The captured step's last refinement runs no inner iteration: when the
+    refinements before it miss the tolerance it corrects x by the line solve
+    z = M^-1 r of the residual before it. From Gamma* (no extrapolated
+    guess) one inner iteration a refinement cannot reach 1e-10, so the second
+    refinement runs in every step and must lower the residual the step
+    reports (by about the line-Jacobi bound mu of the grid, below 1/2),
+    against the same step with one refinement. `outer` counts the FP64
+    residual evaluations of the last solve, the assembly's included."""
+    n, dt, steps = 64, 0.005, 8
+    _say(f"last refinement {n}x{2 * n}")
+    st, ua, va = make_case(n, 13, l_max=6, u_scale=0.5)
+    res = {}
+    for outer in (1, 2):
+        with sb.SoapFilm(sb.bubble(n, dt), "mixed", use_graph=True, max_inner=1,
+                         max_outer=outer, initial_guess="gamma_star") as ctx:
+            load(ctx, 0, st, ua, va)
+            ctx.step(steps)
+            ctx.synchronize()
+            s = ctx.stats(0)
+            assert np.isfinite(ctx.state(0)["gamma"]).all()
+        _say(f"  max_outer {outer}: residual {s['residual']:.3e}, outer {s['outer']}, "
+             f"failed {s['failed_solves']}")
+        assert s["outer"] == outer + 1
+        res[outer] = (s["residual"], s["failed_solves"])
+    assert res[1][1] == steps               # one refinement of one iteration: every solve misses
+    assert 0.0 < res[2][0] < 0.5 * res[1][0]
+    assert res[2][1] <= res[1][1]
