@@ -167,6 +167,16 @@ __device__ __forceinline__ double rcp64(double y) {
     e = fma(-y, r, 1.0);
     return fma(r, e, r);
 }
+// a / y through the refined reciprocal (within an ulp or two of the
+// quotient, a quarter of the division's cost); a divisor of magnitude
+// outside [1e-300, 1e300] (zero, denormal, huge, infinite or NaN) takes the
+// division itself, so that 2 / 0 = inf and 0 / 0 = NaN come out as the
+// reference's.
+__device__ __forceinline__ double qdiv(double a, double y) {
+    const double ay = fabs(y);
+    if (ay >= 1e-300 && ay <= 1e300) return a * rcp64(y);
+    return a / y;
+}
 __device__ __forceinline__ float trcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ double trcp(double x) { return 1.0 / x; }
 

@@ -155,12 +155,25 @@ __global__ __launch_bounds__(256) void k_advect(const int4* __restrict__ tiles,
 // u_air(t) = u_air_init + (3 - |t - 3|) * corridor, NaN -> 0 (airFlowU/V,
 // bubble.cu:2301-2380). The corridor shape is time-independent and is
 // precomputed on the host.
+// FP32 fields (the mixed paths) evaluate it in FP32, as the reference's
+// float kernels do; FP64 fields in FP64.
+template <typename T>
+__device__ __forceinline__ double air_val(T a0, T shape, double kt) {
+    if constexpr (sizeof(T) == 4) {
+        float v = fmaf(shape, (float)kt, a0);
+        v = __builtin_isnan(v) ? 0.0f : v;
+        return (double)v;
+    } else {
+        double v = (double)a0 + (double)shape * kt;
+        return __builtin_isnan(v) ? 0.0 : v;
+    }
+}
 template <typename T>
 __device__ __forceinline__ double air_at(const T* __restrict__ a0, const T* __restrict__ shape,
                                          size_t k, double kt) {
-    double v = (double)a0[k] + (double)shape[k] * kt;
-    return __builtin_isnan(v) ? 0.0 : v;
+    return air_val(a0[k], shape[k], kt);
 }
+
 
 
 // ----------------------------------------------------------- VISCOUS ----
@@ -216,7 +229,7 @@ __global__ __launch_bounds__(256) void k_visc_force(const int4* __restrict__ til
     if (j >= B.nth || i >= B.nph) return;
     const int nph = B.nph, m = B.mask, nth = B.nth, hn = nph / 2;
     const size_t base = (size_t)B.off;
-    const double ih = B.invh, re = B.re, halfh = 0.5 * B.h;
+    const double ih = B.invh, re = B.re;
     auto U = [&](int jj, int ii) { return (double)st.uph[base + (size_t)jj * nph + ii]; };
     auto V = [&](int jj, int ii) { return (double)st.uth[base + (size_t)jj * nph + ii]; };
     auto E = [&](int jj, int ii) { return (double)st.eta[base + (size_t)jj * nph + ii]; };
@@ -347,6 +360,30 @@ __device__ __forceinline__ void st_row(X* __restrict__ p, const X (&v)[C]) {
     for (int k = 0; k < C; ++k) q[k] = v[k];
 }
 
+// The t/12 blend of the air with its phi mirror (bubble.cu:1364-1379) at the
+// N consecutive columns i0 + k of a row (N = C: cells, N = C + 1: the phi
+// faces of C cells, the last one wrapping): 0.5 (a(i) bl +- a(nph - i) (1 -
+// bl)), + for u_phi, - for u_theta. The mirror columns nph - i0 - k, k >= 1,
+// are the C-aligned block [nph - i0 - C, nph - i0) read backwards, so both
+// sides come as row vectors; column (nph - i0) mod nph is the one extra.
+template <typename T, int C, int N, bool MINUS>
+__device__ __forceinline__ void air_blend(const T* __restrict__ a0, const T* __restrict__ shape,
+                                          size_t row, int i0, int nph, int m, double kt, double bl,
+                                          double (&out)[N]) {
+    T a[C], sh[C], am[C], sm[C];
+    ld_row<C>(a0 + row + i0, a);
+    ld_row<C>(shape + row + i0, sh);
+    const size_t mb = row + (size_t)(nph - i0 - C), cm = row + ((nph - i0) & m);
+    ld_row<C>(a0 + mb, am);
+    ld_row<C>(shape + mb, sm);
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        const double own = k < C ? air_val(a[k], sh[k], kt) : air_at(a0, shape, row + ((i0 + C) & m), kt);
+        const double mir = k == 0 ? air_at(a0, shape, cm, kt) : air_val(am[C - k], sm[C - k], kt);
+        out[k] = MINUS ? 0.5 * (own * bl - mir * (1 - bl)) : 0.5 * (own * bl + mir * (1 - bl));
+    }
+}
+
 // Operator (and optionally right-hand side) of the C consecutive cells
 // [i0, i0 + C) of row j, computed from row-vector loads. Every phi face is
 // evaluated once and shared by its two cells; each expression is the one of
@@ -442,7 +479,9 @@ __device__ __forceinline__ void row_system(const BubbleDev& B, double t, const T
         const bool br = k == C ? bE : q.brk[k];
         q.wF[k] = (bl || br) ? 0.0 : wFr[k];
     }
-    double uv[C + 1], vNv[C], vSv[C], uAirF[C + 1];
+    double uv[C + 1], vNv[C], vSv[C], uAirF[C + 1], vAirNv[C], vAirSv[C];
+#pragma unroll
+    for (int k = 0; k < C; ++k) vAirNv[k] = vAirSv[k] = 0.0;
     const double kt = 3.0 - fabs(t - 3.0);
     const double bl = t / 12.0;
     if (RHS) {
@@ -462,13 +501,21 @@ __device__ __forceinline__ void row_system(const BubbleDev& B, double t, const T
             for (int k = 0; k < C; ++k) vSv[k] = (double)vs[k];
         }
         if (B.airflow) {
-            // the t/12 blend with the phi mirror (bubble.cu:1364-1379), per
-            // phi face, shared by the two cells of the face
+            // per phi face, shared by the two cells of the face; v at the
+            // north and south faces of the cells. FP32 fields: row vectors;
+            // FP64 fields (twice the registers) per column.
+            if constexpr (sizeof(T) == 4) {
+                air_blend<T, C, C + 1, false>(air.ua0, air.au, row, i0, nph, m, kt, bl, uAirF);
+                if (hasN)
+                    air_blend<T, C, C, true>(air.va0, air.av, row - nph, i0, nph, m, kt, bl, vAirNv);
+                if (hasS) air_blend<T, C, C, true>(air.va0, air.av, row, i0, nph, m, kt, bl, vAirSv);
+            } else {
 #pragma unroll
-            for (int k = 0; k <= C; ++k) {
-                const int i = (i0 + k) & m, opp = (nph - i) & m;
-                uAirF[k] = 0.5 * (air_at(air.ua0, air.au, row + i, kt) * bl +
-                                  air_at(air.ua0, air.au, row + opp, kt) * (1 - bl));
+                for (int k = 0; k <= C; ++k) {
+                    const int i = (i0 + k) & m, opp = (nph - i) & m;
+                    uAirF[k] = 0.5 * (air_at(air.ua0, air.au, row + i, kt) * bl +
+                                      air_at(air.ua0, air.au, row + opp, kt) * (1 - bl));
+                }
             }
         }
     }
@@ -494,7 +541,7 @@ __device__ __forceinline__ void row_system(const BubbleDev& B, double t, const T
             q.rhs[k] = gmax;
             continue;
         }
-        q.diag[k] = sinTheta * invDt / g[k] - (((-wN) + (-wW)) + (-wE) + (-wS));
+        q.diag[k] = qdiv(sinTheta * invDt, g[k]) - (((-wN) + (-wW)) + (-wE) + (-wS));
         if (!RHS) continue;
         const int i = i0 + k;
         double uW = uv[k], uE = uv[k + 1], vN = vNv[k], vS = vSv[k];
@@ -505,16 +552,18 @@ __device__ __forceinline__ void row_system(const BubbleDev& B, double t, const T
         double div = B.invh * (uE * etaRight * invR - uW * etaLeft * invL +
                                (vS * sinS * etaDown * invD - vN * sinN * etaUp * invU));
         if (B.airflow) {
-            const int opp = (nph - i) & m;
-            double vAirN = 0.0, vAirS = 0.0;
-            if (hasN) {
-                const size_t rn = row - nph;
-                vAirN = 0.5 * (air_at(air.va0, air.av, rn + i, kt) * bl -
-                               air_at(air.va0, air.av, rn + opp, kt) * (1 - bl));
-            }
-            if (hasS) {
-                vAirS = 0.5 * (air_at(air.va0, air.av, row + i, kt) * bl -
-                               air_at(air.va0, air.av, row + opp, kt) * (1 - bl));
+            double vAirN = vAirNv[k], vAirS = vAirSv[k];
+            if constexpr (sizeof(T) != 4) {
+                const int opp = (nph - i) & m;
+                if (hasN) {
+                    const size_t rn = row - nph;
+                    vAirN = 0.5 * (air_at(air.va0, air.av, rn + i, kt) * bl -
+                                   air_at(air.va0, air.av, rn + opp, kt) * (1 - bl));
+                }
+                if (hasS) {
+                    vAirS = 0.5 * (air_at(air.va0, air.av, row + i, kt) * bl -
+                                   air_at(air.va0, air.av, row + opp, kt) * (1 - bl));
+                }
             }
             div += B.invh * CrDt *
                    (uAirF[k + 1] * invR - uAirF[k] * invL + vAirS * sinS * invD - vAirN * sinN * invU);
@@ -656,6 +705,7 @@ __global__ void k_assemble(const int2* __restrict__ rows, const BubbleDev* __res
     const size_t row = (size_t)B.off + (size_t)j * nph;
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     double mu = 0.0;
+    float muf = 0.0f;
     Ti a[C], cc[C], d[C], rin[C];
 #pragma unroll
     for (int k = 0; k < C; ++k) a[k] = cc[k] = d[k] = rin[k] = Ti(0);
@@ -680,8 +730,8 @@ __global__ void k_assemble(const int2* __restrict__ rows, const BubbleDev* __res
             // largest ratio below
             // (a bound with a 2 % margin downstream: FP32 is ample)
             if (!q.brk[k])
-                mu = fmax(mu, (double)((float)(q.wN[k] + q.wS[k]) *
-                                       __builtin_amdgcn_rcpf((float)(q.diag[k] - wW - wE))));
+                muf = fmaxf(muf, (float)(q.wN[k] + q.wS[k]) *
+                                     __builtin_amdgcn_rcpf((float)(q.diag[k] - wW - wE)));
             rin[k] = (Ti)r0;
             xv[k] = q.x0[k];
             wn[k] = (Ti)q.wN[k];
@@ -717,7 +767,7 @@ __global__ void k_assemble(const int2* __restrict__ rows, const BubbleDev* __res
         for (int k = 0; k < C; ++k) acc[2] += (double)rin[k] * (double)d[k];
     }
     block_sum<4>(acc, red);
-    mu = block_max(mu, red);
+    mu = block_max((double)muf, red);
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) part[q * stride + B.pr0 + j] = acc[q];
@@ -1175,17 +1225,6 @@ __global__ void k_finalize(const BubbleDev* __restrict__ bub, SolveCtl* __restri
 }
 
 // ------------------------------------------------------------- update ----
-// a / y through the refined reciprocal (within an ulp or two of the
-// quotient, a quarter of the division's cost); a divisor of magnitude
-// outside [1e-300, 1e300] (zero, denormal, huge, infinite or NaN) takes the
-// division itself, so that 2 / 0 = inf and 0 / 0 = NaN come out as the
-// reference's.
-__device__ __forceinline__ double qdiv(double a, double y) {
-    const double ay = fabs(y);
-    if (ay >= 1e-300 && ay <= 1e300) return a * rcp64(y);
-    return a / y;
-}
-
 __device__ __forceinline__ double clamp_gamma(double g, double gmax) {
     return (g <= 0.0 || g > gmax - kEps) ? gmax : g;
 }
