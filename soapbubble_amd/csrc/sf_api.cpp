@@ -251,8 +251,8 @@ struct Impl : Base {
         if (bimocq) launch_bimocq_eta<T>(stream, geom, bub_d, cur, nxt, bm);
     }
 
-    // Assembly fused with the first FP64 residual and line solve; x0 lands
-    // in x64[0].
+    // Assembly fused with the first FP64 residual and line solve; x0 is not
+    // stored (the first residual launch forms it again).
     void assemble() override {
         // VISCOUS: div(u*) and the viscous forces first (Solver::bodyforce,
         // bubble.cu:2405-2423)
@@ -265,7 +265,8 @@ struct Impl : Base {
                                    gnext, sp);
     }
 
-    // FP64 residual evaluation k >= 1: x64[k&1] = x64[(k-1)&1] + ecorr; sh
+    // FP64 residual evaluation k >= 1: x64[k&1] = x64[(k-1)&1] + ecorr (k = 1:
+    // x_0 formed again from Gamma* and the guess, as the assembly formed it); sh
     // (captured steps) collects the schedule statistics. `last`: no refinement
     // follows (no line solve of the new residual); `check_prev`: refinement
     // k - 1 ran no inner iteration, so this launch tests its start residual.
@@ -276,8 +277,9 @@ struct Impl : Base {
         s.e = ecorr;
         if (last) s.z = nullptr;
         for (const LaunchGeom& rg : rgeo)
-            launch_resid<T, Ti>(stream, rg, bub_d, tab, nxt, ctl_d, s, xo, xn, k, partR, stride,
-                                partB, sp, method == 2 ? sh : nullptr, check_prev ? 1 : 0);
+            launch_resid<T, Ti>(stream, rg, bub_d, time_d, tab, nxt, ctl_d, s, gnext, xo, xn, k,
+                                partR, stride, partB, sp, method == 2 ? sh : nullptr,
+                                check_prev ? 1 : 0);
     }
     // Chebyshev iterates: e_1 is z; e_k for k >= 2 rotates over e, Ap, p[0]
     // (k mod 3), so e_{k-1}, e_k and e_{k+1} are distinct.

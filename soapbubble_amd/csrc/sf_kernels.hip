@@ -446,7 +446,7 @@ __device__ __forceinline__ void row_system(const BubbleDev& B, double t, const T
     T gxv[C];
 #pragma unroll
     for (int k = 0; k < C; ++k) gxv[k] = gv[k];
-    if (RHS && guess) {
+    if (guess) {
         T gc[C];
         ld_row<C>(guess + row + i0, gc);
 #pragma unroll
@@ -714,7 +714,6 @@ __global__ void k_assemble(const int2* __restrict__ rows, const BubbleDev* __res
         // the extrapolated correction once a previous solve stored one
         row_system<T, C, true, VISC>(B, t, tab, s, air, j, i0, q,
                                      btime[rw.x].ghist > 0 ? guess : nullptr);
-        double xv[C];
         Ti wn[C], ww[C], dg[C];
 #pragma unroll
         for (int k = 0; k < C; ++k) {
@@ -733,7 +732,6 @@ __global__ void k_assemble(const int2* __restrict__ rows, const BubbleDev* __res
                 muf = fmaxf(muf, (float)(q.wN[k] + q.wS[k]) *
                                      __builtin_amdgcn_rcpf((float)(q.diag[k] - wW - wE)));
             rin[k] = (Ti)r0;
-            xv[k] = q.x0[k];
             wn[k] = (Ti)q.wN[k];
             ww[k] = (Ti)wW;
             dg[k] = (Ti)q.diag[k];
@@ -746,7 +744,6 @@ __global__ void k_assemble(const int2* __restrict__ rows, const BubbleDev* __res
             acc[3] += (double)rin[k] * (double)rin[k];
         }
         st_row<C>(sys.b64 + row + i0, q.rhs);
-        st_row<C>(sys.x64 + row + i0, xv);
         st_row<C>(sys.wN + row + i0, wn);
         st_row<C>(sys.wW + row + i0, ww);
         st_row<C>(sys.diag + row + i0, dg);
@@ -853,9 +850,13 @@ __device__ __forceinline__ void ld_stencil(const Ti* __restrict__ v, size_t row,
 // refinement had no inner iteration: its correction is the z of the residual
 // before) the start-of-refinement test of the Chebyshev launches is done
 // here: a start residual (slot k - 1) that met the tolerance ends the solve.
-template <typename T, typename Ti, int C>
+// FIRST (k = 1): x_0 is formed again here, bitwise as the assembly formed it
+// (Gamma* plus the stored guess, gamma_max on broken cells), instead of being
+// stored by the assembly and read back.
+template <typename T, typename Ti, int C, bool FIRST>
 __global__ void k_resid(const int2* __restrict__ rows, const BubbleDev* __restrict__ bub,
-                        Tables tab, StatePtrs<T> s, SolveCtl* __restrict__ ctl, SysPtrs<Ti> sys,
+                        const BubbleTime* __restrict__ btime, Tables tab, StatePtrs<T> s,
+                        SolveCtl* __restrict__ ctl, SysPtrs<Ti> sys, const T* __restrict__ guess,
                         const double* __restrict__ xo, double* __restrict__ xn, int k,
                         double* __restrict__ part, long stride, const double* __restrict__ partB,
                         SolveParams sp, SchedStat* __restrict__ sh, int check_prev) {
@@ -887,7 +888,22 @@ __global__ void k_resid(const int2* __restrict__ rows, const BubbleDev* __restri
     for (int q = 0; q < C; ++q) a[q] = cc[q] = d[q] = rin[q] = Ti(0);
     if (act) {
         double xc[C], xN[C], xS[C], xW, xE;
-        ld_stencil<double, C>(xo, row, i0, m, nph, hasN, hasS, xc, xW, xE, xN, xS);
+        RowSys<C> w;
+        if constexpr (FIRST) {
+            row_system<T, C, false>(B, 0.0, tab, s, AirPtrs<T>{}, j, i0, w,
+                                    btime[rw.x].ghist > 0 ? guess : nullptr);
+#pragma unroll
+            for (int q = 0; q < C; ++q) {
+                xc[q] = w.x0[q];
+                xN[q] = w.x0N[q];
+                xS[q] = w.x0S[q];
+            }
+            xW = w.x0W;
+            xE = w.x0E;
+        } else {
+            ld_stencil<double, C>(xo, row, i0, m, nph, hasN, hasS, xc, xW, xE, xN, xS);
+            row_system<T, C, false>(B, 0.0, tab, s, AirPtrs<T>{}, j, i0, w);
+        }
         Ti ec[C], eN[C], eS[C], eW, eE;
         ld_stencil<Ti, C>(ecorr, row, i0, m, nph, hasN, hasS, ec, eW, eE, eN, eS);
         double bv[C];
@@ -896,8 +912,6 @@ __global__ void k_resid(const int2* __restrict__ rows, const BubbleDev* __restri
 #pragma unroll
         for (int q = 0; q < C; ++q) xnew[q] = xc[q] + (double)ec[q];
         const double xWn = xW + (double)eW, xEn = xE + (double)eE;
-        RowSys<C> w;
-        row_system<T, C, false>(B, 0.0, tab, s, AirPtrs<T>{}, j, i0, w);
 #pragma unroll
         for (int q = 0; q < C; ++q) {
             const double l = q == 0 ? xWn : xnew[q - 1];
@@ -1745,14 +1759,21 @@ void launch_assemble(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub,
 }
 
 template <typename T, typename Ti>
-void launch_resid(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, const Tables& tab,
-                  StatePtrs<T> st, SolveCtl* ctl, SysPtrs<Ti> sys, const double* xo, double* xn,
-                  int k, double* part, long stride, const double* partB, const SolveParams& sp,
-                  SchedStat* sh, int check_prev) {
-#define SF_CALL(C)                                                                              \
-    hipLaunchKernelGGL((k_resid<T, Ti, C>), dim3(g.nrows), dim3(g.row_threads), g.lds_bytes, s, \
-                       g.rows, bub, tab, st, ctl, sys, xo, xn, k, part, stride, partB, sp, sh,  \
-                       check_prev)
+void launch_resid(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, const BubbleTime* bt,
+                  const Tables& tab, StatePtrs<T> st, SolveCtl* ctl, SysPtrs<Ti> sys,
+                  const T* guess, const double* xo, double* xn, int k, double* part, long stride,
+                  const double* partB, const SolveParams& sp, SchedStat* sh, int check_prev) {
+#define SF_CALL(C)                                                                                \
+    do {                                                                                          \
+        if (k == 1)                                                                               \
+            hipLaunchKernelGGL((k_resid<T, Ti, C, true>), dim3(g.nrows), dim3(g.row_threads),     \
+                               g.lds_bytes, s, g.rows, bub, bt, tab, st, ctl, sys, guess, xo, xn, \
+                               k, part, stride, partB, sp, sh, check_prev);                       \
+        else                                                                                      \
+            hipLaunchKernelGGL((k_resid<T, Ti, C, false>), dim3(g.nrows), dim3(g.row_threads),    \
+                               g.lds_bytes, s, g.rows, bub, bt, tab, st, ctl, sys, guess, xo, xn, \
+                               k, part, stride, partB, sp, sh, check_prev);                       \
+    } while (0)
     SF_CASES(SF_CALL)
 #undef SF_CALL
 }
@@ -1835,9 +1856,10 @@ template void launch_advect<double>(hipStream_t, const LaunchGeom&, const Bubble
                                          AirPtrs<T>, SysPtrs<Ti>, double, SolveCtl*, double*,     \
                                          long, const T*, const SolveParams&);                     \
     template void launch_resid<T, Ti>(hipStream_t, const LaunchGeom&, const BubbleDev*,           \
-                                      const Tables&, StatePtrs<T>, SolveCtl*, SysPtrs<Ti>,        \
-                                      const double*, double*, int, double*, long,               \
-                                      const double*, const SolveParams&, SchedStat*, int);
+                                      const BubbleTime*, const Tables&, StatePtrs<T>, SolveCtl*,  \
+                                      SysPtrs<Ti>, const T*, const double*, double*, int,         \
+                                      double*, long, const double*, const SolveParams&,           \
+                                      SchedStat*, int);
 SF_INST_SYS(float, float)
 SF_INST_SYS(float, double)
 SF_INST_SYS(double, double)

@@ -89,10 +89,10 @@ void launch_assemble(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub,
                      SysPtrs<Ti> sys, double gmax_state, SolveCtl* ctl, double* part, long stride,
                      const T* guess, const SolveParams& sp);
 template <typename T, typename Ti>
-void launch_resid(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, const Tables& tab,
-                  StatePtrs<T> st, SolveCtl* ctl, SysPtrs<Ti> sys, const double* xo, double* xn,
-                  int k, double* part, long stride, const double* partB, const SolveParams& sp,
-                  SchedStat* sh, int check_prev);
+void launch_resid(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, const BubbleTime* bt,
+                  const Tables& tab, StatePtrs<T> st, SolveCtl* ctl, SysPtrs<Ti> sys,
+                  const T* guess, const double* xo, double* xn, int k, double* part, long stride,
+                  const double* partB, const SolveParams& sp, SchedStat* sh, int check_prev);
 template <typename Ti>
 void launch_spmv(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, SolveCtl* ctl,
                  SysPtrs<Ti> sys, const Ti* pold, Ti* pnew, int it, const SolveParams& sp,
