@@ -366,21 +366,28 @@ __device__ __forceinline__ void st_row(X* __restrict__ p, const X (&v)[C]) {
 // bl)), + for u_phi, - for u_theta. The mirror columns nph - i0 - k, k >= 1,
 // are the C-aligned block [nph - i0 - C, nph - i0) read backwards, so both
 // sides come as row vectors; column (nph - i0) mod nph is the one extra.
-template <typename T, int C, int N, bool MINUS>
-__device__ __forceinline__ void air_blend(const T* __restrict__ a0, const T* __restrict__ shape,
-                                          size_t row, int i0, int nph, int m, double kt, double bl,
-                                          double (&out)[N]) {
-    T a[C], sh[C], am[C], sm[C];
+// FP32 fields only; the blend in FP32 too (the reference's float
+// arithmetic), which keeps the 3 C + 1 results in single registers.
+template <int C, int N, bool MINUS>
+__device__ __forceinline__ void air_blend(const float* __restrict__ a0,
+                                          const float* __restrict__ shape, size_t row, int i0,
+                                          int nph, int m, float kt, float bl, float (&out)[N]) {
+    float a[C], sh[C], am[C], sm[C];
     ld_row<C>(a0 + row + i0, a);
     ld_row<C>(shape + row + i0, sh);
     const size_t mb = row + (size_t)(nph - i0 - C), cm = row + ((nph - i0) & m);
     ld_row<C>(a0 + mb, am);
     ld_row<C>(shape + mb, sm);
+    auto val = [&](float x, float y) {
+        const float v = fmaf(y, kt, x);
+        return __builtin_isnan(v) ? 0.0f : v;
+    };
+    const size_t ce = row + ((i0 + C) & m);
 #pragma unroll
     for (int k = 0; k < N; ++k) {
-        const double own = k < C ? air_val(a[k], sh[k], kt) : air_at(a0, shape, row + ((i0 + C) & m), kt);
-        const double mir = k == 0 ? air_at(a0, shape, cm, kt) : air_val(am[C - k], sm[C - k], kt);
-        out[k] = MINUS ? 0.5 * (own * bl - mir * (1 - bl)) : 0.5 * (own * bl + mir * (1 - bl));
+        const float own = k < C ? val(a[k], sh[k]) : val(a0[ce], shape[ce]);
+        const float mir = k == 0 ? val(a0[cm], shape[cm]) : val(am[C - k], sm[C - k]);
+        out[k] = MINUS ? 0.5f * (own * bl - mir * (1.0f - bl)) : 0.5f * (own * bl + mir * (1.0f - bl));
     }
 }
 
@@ -479,9 +486,10 @@ __device__ __forceinline__ void row_system(const BubbleDev& B, double t, const T
         const bool br = k == C ? bE : q.brk[k];
         q.wF[k] = (bl || br) ? 0.0 : wFr[k];
     }
-    double uv[C + 1], vNv[C], vSv[C], uAirF[C + 1], vAirNv[C], vAirSv[C];
+    double uv[C + 1], vNv[C], vSv[C];
+    T uAirF[C + 1], vAirNv[C], vAirSv[C];   // the blended air (state precision)
 #pragma unroll
-    for (int k = 0; k < C; ++k) vAirNv[k] = vAirSv[k] = 0.0;
+    for (int k = 0; k < C; ++k) vAirNv[k] = vAirSv[k] = T(0);
     const double kt = 3.0 - fabs(t - 3.0);
     const double bl = t / 12.0;
     if (RHS) {
@@ -505,10 +513,11 @@ __device__ __forceinline__ void row_system(const BubbleDev& B, double t, const T
             // north and south faces of the cells. FP32 fields: row vectors;
             // FP64 fields (twice the registers) per column.
             if constexpr (sizeof(T) == 4) {
-                air_blend<T, C, C + 1, false>(air.ua0, air.au, row, i0, nph, m, kt, bl, uAirF);
+                const float ktf = (float)kt, blf = (float)bl;
+                air_blend<C, C + 1, false>(air.ua0, air.au, row, i0, nph, m, ktf, blf, uAirF);
                 if (hasN)
-                    air_blend<T, C, C, true>(air.va0, air.av, row - nph, i0, nph, m, kt, bl, vAirNv);
-                if (hasS) air_blend<T, C, C, true>(air.va0, air.av, row, i0, nph, m, kt, bl, vAirSv);
+                    air_blend<C, C, true>(air.va0, air.av, row - nph, i0, nph, m, ktf, blf, vAirNv);
+                if (hasS) air_blend<C, C, true>(air.va0, air.av, row, i0, nph, m, ktf, blf, vAirSv);
             } else {
 #pragma unroll
                 for (int k = 0; k <= C; ++k) {
@@ -552,7 +561,7 @@ __device__ __forceinline__ void row_system(const BubbleDev& B, double t, const T
         double div = B.invh * (uE * etaRight * invR - uW * etaLeft * invL +
                                (vS * sinS * etaDown * invD - vN * sinN * etaUp * invU));
         if (B.airflow) {
-            double vAirN = vAirNv[k], vAirS = vAirSv[k];
+            double vAirN = (double)vAirNv[k], vAirS = (double)vAirSv[k];
             if constexpr (sizeof(T) != 4) {
                 const int opp = (nph - i) & m;
                 if (hasN) {
@@ -566,7 +575,8 @@ __device__ __forceinline__ void row_system(const BubbleDev& B, double t, const T
                 }
             }
             div += B.invh * CrDt *
-                   (uAirF[k + 1] * invR - uAirF[k] * invL + vAirS * sinS * invD - vAirN * sinN * invU);
+                   ((double)uAirF[k + 1] * invR - (double)uAirF[k] * invL + vAirS * sinS * invD -
+                    vAirN * sinN * invU);
         }
         if (VISC && B.gravity) {
             // `gravity` with `VISCOUS` (bubble.cu:1458-1464): the viscous
