@@ -84,6 +84,7 @@ struct Base {
     Tables tab{};
     LaunchGeom geom{};            // cell tiles
     std::vector<LaunchGeom> rgeo; // row blocks, one geometry per row width
+    std::vector<LaunchGeom> rgeo_inner;   // the same rows for the Chebyshev launches
     double* partR = nullptr;
     double* partA = nullptr;
     double* partB = nullptr;
@@ -289,7 +290,7 @@ struct Impl : Base {
         return bufs[k % 3];
     }
     void cheb(int it, int kr) {
-        for (const LaunchGeom& rg : rgeo)
+        for (const LaunchGeom& rg : rgeo_inner)
             launch_cheb<Ti>(stream, rg, bub_d, ctl_d, sys(x64[0]), cheb_e(it), cheb_e(it - 1),
                             cheb_e(it + 1), it, kr, sp, partB, stride);
     }
@@ -715,6 +716,7 @@ void setup(Base& c, int prec) {
     std::sort(widths.begin(), widths.end());
     widths.erase(std::unique(widths.begin(), widths.end()), widths.end());
     c.rgeo.clear();
+    c.rgeo_inner.clear();
     for (int nph : widths) {
         std::vector<int2> rows;
         for (int b = 0; b < nb; ++b)
@@ -740,6 +742,18 @@ void setup(Base& c, int prec) {
                              std::to_string(lds_limit) + " (use SF_PREC_MIXED for this size)"};
         }
         c.rgeo.push_back(g);
+        // The Chebyshev iterations (FP32 inner solve, rows of at least 2048
+        // cells) hold 8 cells a thread: half the threads and barriers of
+        // the line solve per row, and their 86 registers still fit every
+        // row block of a CU at once. The reduction depth recorded at 4 cells
+        // a thread is one level deeper than 8 cells a thread need.
+        LaunchGeom gi = g;
+        if (!c.fp64_inner() && cpt == 4 && nph / 8 >= 256) {
+            gi.cpt = 8;
+            gi.row_threads = nph / 8;
+            gi.lds_bytes = 6 * (size_t)(nph / 8) * esz;
+        }
+        c.rgeo_inner.push_back(gi);
     }
 
     c.partR = dalloc<double>(5 * (size_t)c.stride, c.owned);

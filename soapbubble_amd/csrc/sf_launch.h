@@ -76,7 +76,7 @@ struct LaunchGeom {
     const int2* rows;    // row blocks: (bubble, row)
     int nrows;
     int row_threads;     // threads of a row block
-    int cpt;             // consecutive cells per thread of a row block (2, 4 or 8)
+    int cpt;             // consecutive cells per thread of a row block (2, 4, 8 or 16)
     size_t lds_bytes;    // dynamic LDS of a row block (the reduced line system)
 };
 
