@@ -227,6 +227,7 @@ struct Impl : Base {
                              &bm.g_initlast, &bm.g_delta, &bm.g_deltalast, &bm.chg_e, &bm.chg_g,
                              &bm.err_e, &bm.err_g, &bm.cpy_e, &bm.cpy_g};
         for (double** a : arrays) *a = dalloc<double>(n, owned);
+        bm.tab = tab;
         bm.dist_bits = dalloc<unsigned>((size_t)nb, owned);
         bm.flag = dalloc<int>((size_t)nb, owned);
         bm.dist_last = dalloc<double>((size_t)nb, owned);

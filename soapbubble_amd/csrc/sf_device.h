@@ -628,19 +628,69 @@ struct P2d {
     double x, y;   // (theta, phi) index coordinates
 };
 
-__device__ __forceinline__ P2d index_of_unit(double invh, const U3& r) {
-    return P2d{safe_acos_d(r.z) * invh, atan2(r.y, r.x) * invh};
+// Index coordinates (acos z, atan2(y, x)) / h of a unit vector a few cells
+// from the centre of cell (j, i): the
+// centre's index coordinates (j + 1/2, i) plus the angles turned from it,
+// each the asin of a sine that the (sin, cos) pair of the meridian half-plane
+// (a unit vector) or the equatorial plane (scale rho) gives, by its series
+// below 0.02 (truncation under 1e-22 rad); farther points take acos and
+// atan2. The same angles up to rounding, phi in the centre's period rather
+// than atan2's (-pi, pi] (the samplers wrap phi either way).
+struct CellRef {
+    double gx, gy, sT, cT, sP, cP;
+};
+__device__ __forceinline__ CellRef cell_ref(const Tables& tab, const BubbleDev& B, int j, int i) {
+    return CellRef{j + 0.5, (double)i, tab.sinTc[B.rt0 + j], tab.cosTc[B.rt0 + j],
+                   tab.sinPc[B.ct0 + i], tab.cosPc[B.ct0 + i]};
+}
+__device__ __forceinline__ double asin_small(double s) {
+    const double s2 = s * s;
+    return s * (1.0 + s2 * (1.0 / 6 + s2 * (3.0 / 40 + s2 * (5.0 / 112 + s2 * (35.0 / 1152 +
+                                                                              s2 * (63.0 / 2816))))));
+}
+__device__ __forceinline__ P2d index_near(double invh, const U3& r, const CellRef& o) {
+    const double rho = sqrt(r.x * r.x + r.y * r.y);
+    const double nt = rho * o.cT - r.z * o.sT, ct = r.z * o.cT + rho * o.sT;
+    const double np = r.y * o.cP - r.x * o.sP, cp = r.x * o.cP + r.y * o.sP;
+    P2d q;
+    if (ct > 0.0 && fabs(nt) < 0.02) {
+        q.x = o.gx + asin_small(nt) * invh;
+    } else {
+        q.x = safe_acos_d(r.z) * invh;
+    }
+    if (cp > 0.0 && fabs(np) < 0.02 * rho) {
+        q.y = o.gy + asin_small(np * rcp64(rho)) * invh;
+    } else {
+        q.y = atan2(r.y, r.x) * invh;
+    }
+    return q;
 }
 
 // lerpCoords on unit vectors: the point a fraction alpha along the great
 // circle from f to t (Rodrigues rotation about k = f x t / |f x t|); f when
 // the two coincide (k undefined), as the reference returns `from`.
 // Map values at neighbouring grid points are a few cells apart, so the angle
-// between f and t is small: there theta = asin(|f x t|) by its series and
-// sin, cos of alpha theta by Taylor polynomials (every truncation below
-// 1e-20 relative for |f x t| < 0.02) replace acos and sincos; larger angles
-// take the library functions.
+// theta between f and t is small: below |t - f| = 0.02 the rotation is the
+// same point written as a f + b t with a = sin((1 - alpha) theta) / sin theta,
+// b = sin(alpha theta) / sin theta, both by their series in theta^2, and
+// theta^2 = (2 asin(|t - f| / 2))^2 by its series in |t - f|^2 (every
+// truncation below 1e-16 relative there): no square root, division or
+// normalisation. Larger angles rotate with acos and sincos.
+__device__ __forceinline__ double slerp_coef(double beta, double u) {
+    const double b2 = beta * beta;
+    return beta * (1.0 + u * (1.0 - b2) *
+                             (1.0 / 6 + u * ((7.0 - 3.0 * b2) / 360 +
+                                             u * (31.0 + b2 * (-18.0 + 3.0 * b2)) / 15120)));
+}
 __device__ __forceinline__ U3 slerp_u(const U3& f, const U3& t, double alpha) {
+    const double dx = t.x - f.x, dy = t.y - f.y, dz = t.z - f.z;
+    const double d2 = dx * dx + dy * dy + dz * dz;
+    if (d2 < 4e-4) {
+        if (d2 == 0.0) return f;
+        const double u = d2 * (1.0 + d2 * (1.0 / 12 + d2 * (1.0 / 90 + d2 * (1.0 / 560))));
+        const double a = slerp_coef(1.0 - alpha, u), b = slerp_coef(alpha, u);
+        return U3{a * f.x + b * t.x, a * f.y + b * t.y, a * f.z + b * t.z};
+    }
     double kx = f.y * t.z - f.z * t.y, ky = f.z * t.x - f.x * t.z, kz = f.x * t.y - f.y * t.x;
     const double k2 = kx * kx + ky * ky + kz * kz;
     const double kin = 1.0 / sqrt(k2);
@@ -648,19 +698,8 @@ __device__ __forceinline__ U3 slerp_u(const U3& f, const U3& t, double alpha) {
     if (__builtin_isnan(kx)) return f;
     const double cth = f.x * t.x + f.y * t.y + f.z * t.z;
     double sn, cs;
-    if (k2 < 4e-4 && cth > 0.0) {
-        const double sth = k2 * kin;   // |f x t| = sin(theta)
-        // asin(s) = s (1 + s^2/6 + 3 s^4/40 + 5 s^6/112 + 35 s^8/1152 + 63 s^10/2816)
-        const double th =
-            sth * (1.0 + k2 * (1.0 / 6 + k2 * (3.0 / 40 + k2 * (5.0 / 112 +
-                                                                 k2 * (35.0 / 1152 + k2 * (63.0 / 2816))))));
-        const double a = th * alpha, a2 = a * a;
-        sn = a * (1.0 - a2 / 6 * (1.0 - a2 / 20 * (1.0 - a2 / 42 * (1.0 - a2 / 72))));
-        cs = 1.0 - a2 / 2 * (1.0 - a2 / 12 * (1.0 - a2 / 30 * (1.0 - a2 / 56)));
-    } else {
-        const double span = safe_acos_d(cth) * alpha;
-        sincos(span, &sn, &cs);
-    }
+    const double span = safe_acos_d(cth) * alpha;
+    sincos(span, &sn, &cs);
     const double kf = kx * f.x + ky * f.y + kz * f.z;
     const double cx = ky * f.z - kz * f.y, cy = kz * f.x - kx * f.z, cz = kx * f.y - ky * f.x;
     return U3{f.x * cs + cx * sn + kx * kf * (1 - cs), f.y * cs + cy * sn + ky * kf * (1 - cs),

@@ -1545,13 +1545,14 @@ __global__ __launch_bounds__(256) void k_bm_eta(const int4* __restrict__ tiles,
                         fdl = F(P.e_deltalast);
     const int ns = P.multisample ? 5 : 1;
     const double bl = P.blend;
+    const CellRef cr = cell_ref(P.tab, B, j, i);
     double a = 0.0;
     for (int s = 0; s < ns; ++s) {
         const int q = ns == 5 ? s : 4;
         const double w = ns == 5 ? kMsW[q] : 1.0;
-        const P2d ip = index_of_unit(B.invh, sample_map_u(P.nbwd, o, B.nth, B.nph,
-                                                          j + 0.5 + kMsDx[q], i + kMsDy[q]));
-        const P2d lp = index_of_unit(B.invh, sample_map_u(P.prev, o, B.nth, B.nph, ip.x, ip.y));
+        const P2d ip = index_near(B.invh, sample_map_u(P.nbwd, o, B.nth, B.nph,
+                                                        j + 0.5 + kMsDx[q], i + kMsDy[q]), cr);
+        const P2d lp = index_near(B.invh, sample_map_u(P.prev, o, B.nth, B.nph, ip.x, ip.y), cr);
         a += (1.0 - bl) * w *
              (sample_centered<double, true>(fil, B.nth, lp.x, lp.y) +
               sample_centered<double, true>(fd, B.nth, ip.x, ip.y) +
@@ -1581,12 +1582,13 @@ __global__ __launch_bounds__(256) void k_bm_accum(const int4* __restrict__ tiles
         const Field<double> ce{P.chg_e + o, B.nph, B.mask, B.nth - 1};
         const Field<double> cg{P.chg_g + o, B.nph, B.mask, B.nth - 1};
         const int ns = P.multisample ? 5 : 1;
+        const CellRef cr = cell_ref(P.tab, B, j, i);
         double de = 0.0, dg = 0.0;
         for (int s = 0; s < ns; ++s) {
             const int q = ns == 5 ? s : 4;
             const double w = ns == 5 ? kMsW[q] : 1.0;
-            const P2d ip = index_of_unit(B.invh, sample_map_u(P.nfwd, o, B.nth, B.nph,
-                                                              j + 0.5 + kMsDx[q], i + kMsDy[q]));
+            const P2d ip = index_near(B.invh, sample_map_u(P.nfwd, o, B.nth, B.nph,
+                                                            j + 0.5 + kMsDx[q], i + kMsDy[q]), cr);
             de += w * sample_centered<double, true>(ce, B.nth, ip.x, ip.y);
             dg += w * sample_centered<double, true>(cg, B.nth, ip.x, ip.y);
         }
@@ -1594,7 +1596,7 @@ __global__ __launch_bounds__(256) void k_bm_accum(const int4* __restrict__ tiles
         P.g_delta[c] += dg;
         const U3 f{P.nfwd.x[c], P.nfwd.y[c], P.nfwd.z[c]};
         const U3 b{P.nbwd.x[c], P.nbwd.y[c], P.nbwd.z[c]};
-        const P2d fi = index_of_unit(B.invh, f), bi = index_of_unit(B.invh, b);
+        const P2d fi = index_near(B.invh, f, cr), bi = index_near(B.invh, b, cr);
         const U3 x = unit_of_index(B.h, j + 0.5, (double)i);
         const U3 a = sample_map_u(P.nbwd, o, B.nth, B.nph, fi.x, fi.y);
         const U3 e = sample_map_u(P.nfwd, o, B.nth, B.nph, bi.x, bi.y);
@@ -1644,9 +1646,10 @@ __global__ __launch_bounds__(256) void k_bm_correct1(const int4* __restrict__ ti
     const Field<T> fe{cur.eta + o, B.nph, B.mask, B.nth - 1};
     const Field<T> fg{cur.gam + o, B.nph, B.mask, B.nth - 1};
     double te = 0.0, tg = 0.0;
+    const CellRef cr = cell_ref(P.tab, B, j, i);
     for (int q = 0; q < 5; ++q) {
-        const P2d ip = index_of_unit(B.invh, sample_map_u(P.fwd, o, B.nth, B.nph,
-                                                          j + 0.5 + kMsDx[q], i + kMsDy[q]));
+        const P2d ip = index_near(B.invh, sample_map_u(P.fwd, o, B.nth, B.nph,
+                                                        j + 0.5 + kMsDx[q], i + kMsDy[q]), cr);
         te += kMsW[q] * (double)sample_centered<T, true>(fe, B.nth, (T)ip.x, (T)ip.y);
         tg += kMsW[q] * (double)sample_centered<T, true>(fg, B.nth, (T)ip.x, (T)ip.y);
     }
@@ -1691,9 +1694,10 @@ __global__ __launch_bounds__(256) void k_bm_correct2(const int4* __restrict__ ti
     const Field<double> ee{P.err_e + o, B.nph, B.mask, B.nth - 1};
     const Field<double> eg{P.err_g + o, B.nph, B.mask, B.nth - 1};
     double ve = P.cpy_e[c], vg = P.cpy_g[c];
+    const CellRef cr = cell_ref(P.tab, B, j, i);
     for (int q = 0; q < 5; ++q) {
-        const P2d ip = index_of_unit(B.invh, sample_map_u(P.bwd, o, B.nth, B.nph,
-                                                          j + 0.5 + kMsDx[q], i + kMsDy[q]));
+        const P2d ip = index_near(B.invh, sample_map_u(P.bwd, o, B.nth, B.nph,
+                                                        j + 0.5 + kMsDx[q], i + kMsDy[q]), cr);
         ve -= kMsW[q] * sample_centered<double, true>(ee, B.nth, ip.x, ip.y);
         vg -= kMsW[q] * sample_centered<double, true>(eg, B.nth, ip.x, ip.y);
     }

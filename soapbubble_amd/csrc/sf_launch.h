@@ -21,6 +21,7 @@ struct BimocqPtrs {
     int* reinits;          // per bubble: reinitialisations so far
     double blend, threshold;
     int multisample;
+    Tables tab;            // the grid's sin/cos tables (cell-relative map positions)
 };
 
 template <typename T>
