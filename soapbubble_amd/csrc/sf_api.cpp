@@ -228,6 +228,7 @@ struct Impl : Base {
                              &bm.err_e, &bm.err_g, &bm.cpy_e, &bm.cpy_g};
         for (double** a : arrays) *a = dalloc<double>(n, owned);
         bm.tab = tab;
+        bm.bt = time_d;
         bm.dist_bits = dalloc<unsigned>((size_t)nb, owned);
         bm.flag = dalloc<int>((size_t)nb, owned);
         bm.dist_last = dalloc<double>((size_t)nb, owned);

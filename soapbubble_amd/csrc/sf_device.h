@@ -327,6 +327,22 @@ __device__ T sample_centered(const Field<T>& f, int nth, T rx, T ry) {
             return fast_bilerp(field_rsrc(f.p, nth, f.nph), fast_index(rx - 0.5f, ry, f.nph, f.mask),
                                f.nph * 4);
     }
+    if constexpr (sizeof(T) == 8 && !NANS) {
+        // FP64 fields without NaNs, away from the polar rows: the bilinear
+        // interpolation of rows floor(x) and floor(x) + 1 that the general
+        // branch below reduces to, the phi wrap on the integer column
+        if (rx >= 0.5 && rx < (double)nth - 0.5) {
+            const T x = rx - T(0.5), fx = tfloor(x), fy = tfloor(ry);
+            const T aT = x - fx, aP = ry - fy;
+            const int pI = (int)fy;
+            const T* r0 = f.p + (size_t)(int)fx * f.nph;
+            const T* r1 = r0 + f.nph;
+            const int c0 = pI & f.mask, c1 = (pI + 1) & f.mask;
+            const T lower = (T(1) - aP) * r0[c0] + aP * r0[c1];
+            const T higher = (T(1) - aP) * r1[c0] + aP * r1[c1];
+            return (T(1) - aT) * lower + aT * higher;
+        }
+    }
     T x = rx - T(0.5), y = ry;
     const int nph = f.nph;
     bool fl = validate_coord(x, y, nth, nph);

@@ -1525,16 +1525,9 @@ __global__ __launch_bounds__(256) void k_bm_maps(const int4* __restrict__ tiles,
 
 // advectionCenteredBimocq (bubble.cu:967-1013) through the new backward map
 // and the previous interval's backward map; overwrites the plain eta*.
-template <typename T>
-__global__ __launch_bounds__(256) void k_bm_eta(const int4* __restrict__ tiles,
-                                                const BubbleDev* __restrict__ bub,
-                                                StatePtrs<T> cur, StatePtrs<T> out, BimocqPtrs P) {
-    const int4 tl = tiles[blockIdx.x];
-    if (tl.x < 0) return;
-    const BubbleDev B = bub[tl.x];
-    int j, i;
-    tile_cell(tl, j, i);
-    if (j >= B.nth || i >= B.nph) return;
+template <typename T, bool NANS>
+__device__ __forceinline__ void bm_eta_cell(const BubbleDev& B, StatePtrs<T> cur, StatePtrs<T> out,
+                                            const BimocqPtrs& P, int j, int i) {
     const size_t o = (size_t)B.off, c = o + (size_t)j * B.nph + i;
     if ((double)cur.eta[c] == kThresh) {
         out.eta[c] = (T)kThresh;
@@ -1554,13 +1547,31 @@ __global__ __launch_bounds__(256) void k_bm_eta(const int4* __restrict__ tiles,
                                                         j + 0.5 + kMsDx[q], i + kMsDy[q]), cr);
         const P2d lp = index_near(B.invh, sample_map_u(P.prev, o, B.nth, B.nph, ip.x, ip.y), cr);
         a += (1.0 - bl) * w *
-             (sample_centered<double, true>(fil, B.nth, lp.x, lp.y) +
-              sample_centered<double, true>(fd, B.nth, ip.x, ip.y) +
-              sample_centered<double, true>(fdl, B.nth, lp.x, lp.y));
-        a += bl * w * (sample_centered<double, true>(fin, B.nth, ip.x, ip.y) +
-                       sample_centered<double, true>(fd, B.nth, ip.x, ip.y));
+             (sample_centered<double, NANS>(fil, B.nth, lp.x, lp.y) +
+              sample_centered<double, NANS>(fd, B.nth, ip.x, ip.y) +
+              sample_centered<double, NANS>(fdl, B.nth, lp.x, lp.y));
+        a += bl * w * (sample_centered<double, NANS>(fin, B.nth, ip.x, ip.y) +
+                       sample_centered<double, NANS>(fd, B.nth, ip.x, ip.y));
     }
     out.eta[c] = (T)fmax(kThresh, a);
+}
+
+// The NaN-skipping samplers only once a NaN exists in the bubble (its
+// sticky flag; the BIMOCQ fields derive from eta, which holds none before).
+template <typename T>
+__global__ __launch_bounds__(256) void k_bm_eta(const int4* __restrict__ tiles,
+                                                const BubbleDev* __restrict__ bub,
+                                                StatePtrs<T> cur, StatePtrs<T> out, BimocqPtrs P) {
+    const int4 tl = tiles[blockIdx.x];
+    if (tl.x < 0) return;
+    const BubbleDev B = bub[tl.x];
+    int j, i;
+    tile_cell(tl, j, i);
+    if (j >= B.nth || i >= B.nph) return;
+    if (P.bt[tl.x].nans)
+        bm_eta_cell<T, true>(B, cur, out, P, j, i);
+    else
+        bm_eta_cell<T, false>(B, cur, out, P, j, i);
 }
 
 // accumulateChangesCentered (bubble.cu:2081-2114) for eta and Gamma through

@@ -22,6 +22,7 @@ struct BimocqPtrs {
     double blend, threshold;
     int multisample;
     Tables tab;            // the grid's sin/cos tables (cell-relative map positions)
+    const BubbleTime* bt;  // per bubble: the NaN flag selects the NaN-skipping samplers
 };
 
 template <typename T>
