@@ -1594,21 +1594,30 @@ __global__ __launch_bounds__(256) void k_bm_accum(const int4* __restrict__ tiles
         const Field<double> cg{P.chg_g + o, B.nph, B.mask, B.nth - 1};
         const int ns = P.multisample ? 5 : 1;
         const CellRef cr = cell_ref(P.tab, B, j, i);
+        // the changes derive from eta and Gamma: NaN-free while the bubble's
+        // flag is clear (k_update sets it before this launch)
+        const bool nans = P.bt[tl.x].nans != 0;
         double de = 0.0, dg = 0.0;
         for (int s = 0; s < ns; ++s) {
             const int q = ns == 5 ? s : 4;
             const double w = ns == 5 ? kMsW[q] : 1.0;
             const P2d ip = index_near(B.invh, sample_map_u(P.nfwd, o, B.nth, B.nph,
                                                             j + 0.5 + kMsDx[q], i + kMsDy[q]), cr);
-            de += w * sample_centered<double, true>(ce, B.nth, ip.x, ip.y);
-            dg += w * sample_centered<double, true>(cg, B.nth, ip.x, ip.y);
+            if (nans) {
+                de += w * sample_centered<double, true>(ce, B.nth, ip.x, ip.y);
+                dg += w * sample_centered<double, true>(cg, B.nth, ip.x, ip.y);
+            } else {
+                de += w * sample_centered<double, false>(ce, B.nth, ip.x, ip.y);
+                dg += w * sample_centered<double, false>(cg, B.nth, ip.x, ip.y);
+            }
         }
         P.e_delta[c] += de;
         P.g_delta[c] += dg;
         const U3 f{P.nfwd.x[c], P.nfwd.y[c], P.nfwd.z[c]};
         const U3 b{P.nbwd.x[c], P.nbwd.y[c], P.nbwd.z[c]};
         const P2d fi = index_near(B.invh, f, cr), bi = index_near(B.invh, b, cr);
-        const U3 x = unit_of_index(B.h, j + 0.5, (double)i);
+        // the cell centre from the sin/cos tables
+        const U3 x{cr.cP * cr.sT, cr.sP * cr.sT, cr.cT};
         const U3 a = sample_map_u(P.nbwd, o, B.nth, B.nph, fi.x, fi.y);
         const U3 e = sample_map_u(P.nfwd, o, B.nth, B.nph, bi.x, bi.y);
         // dist (bubble.cu:60-79): great-circle distance in cells
