@@ -91,3 +91,46 @@ def test_bimocq_ensemble_and_graph(oracle):
             for k in ("u_theta", "u_phi", "eta", "gamma"):
                 e = rel_l2(got[k], ref[k])
                 assert e < 1e-4, (b, k, e)
+
+
+# At 256 x 512 (h = 0.012 rad) map values lie well within 0.02 rad of their
+# cell and of each other, so the map arithmetic runs its series branches
+# (the spherical interpolation as a f + b t, cell-relative index angles);
+# the coarse grids above mostly take the acos/atan2 branches. Threshold 1e-4
+# cells forces reinitialisations at this resolution.
+@pytest.mark.parametrize("threshold", [4.0, 1e-4])
+def test_bimocq_fine_grid_fp64(oracle, threshold):
+    _say(f"BIMOCQ fp64 256x512, 3 steps, reinit threshold {threshold}")
+    ref, got, s, bm, _ = _run_pair(oracle, 256, 0.002, 3, "fp64", threshold, seed=21)
+    assert s["failed_solves"] == 0
+    assert s["reinits"] == bm.reinit_count
+    if threshold < 1:
+        assert bm.reinit_count >= 1
+    assert abs(s["distortion"] - bm.distortion) <= 1e-6 * max(1.0, bm.distortion)
+    for k in ("u_theta", "u_phi", "eta", "gamma"):
+        e = rel_l2(got[k], ref[k])
+        _say(f"  {k}: rel L2 {e:.3e}")
+        assert e < 1e-9, (k, e)
+
+
+def test_bimocq_fine_grid_mixed(oracle):
+    _say("BIMOCQ mixed 256x512, 4 steps")
+    ref, got, s, bm, _ = _run_pair(oracle, 256, 0.002, 4, "mixed", 4.0, seed=22)
+    assert s["failed_solves"] == 0
+    for k in ("u_theta", "u_phi", "eta", "gamma"):
+        e = rel_l2(got[k], ref[k])
+        _say(f"  {k}: rel L2 {e:.3e}")
+        assert e < 1e-4, (k, e)
+
+
+@pytest.mark.parametrize("threshold", [4.0, 0.01])
+def test_bimocq_smallest_grid_fp64(oracle, threshold):
+    """4 x 8: every row is a polar row; maps cross the poles."""
+    _say(f"BIMOCQ fp64 4x8, 6 steps, reinit threshold {threshold}")
+    ref, got, s, bm, _ = _run_pair(oracle, 4, 0.01, 6, "fp64", threshold, seed=23)
+    assert s["failed_solves"] == 0
+    assert s["reinits"] == bm.reinit_count
+    for k in ("u_theta", "u_phi", "eta", "gamma"):
+        e = rel_l2(got[k], ref[k])
+        _say(f"  {k}: rel L2 {e:.3e}")
+        assert e < 1e-9, (k, e)
