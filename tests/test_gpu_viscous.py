@@ -55,3 +55,26 @@ def test_viscous_changes_the_step(oracle):
             ctx.step(steps)
             out[visc] = ctx.state(0)
     assert rel_l2(out[True]["u_theta"], out[False]["u_theta"]) > 1e-6
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-9), ("mixed", 5e-5)])
+def test_viscous_fine_grid(oracle, prec, tol):
+    """256 x 512 (rows of 4 cells a thread in the row kernels), graph replay
+    for the mixed path."""
+    n, dt, steps = 256, 0.002, 4
+    _say(f"viscous {prec} {n}x{2 * n} {steps} steps")
+    st, ua, va = make_case(n, 33, l_max=8, u_scale=0.5)
+    p = oracle.params(n, dt, gravity=True, viscous=True)
+    ref = copy_state(st)
+    oracle.step(p, ref, ua, va, steps)
+    with sb.SoapFilm(sb.bubble(n, dt, gravity=True), prec, viscous=True,
+                     use_graph=prec == "mixed", tol=1e-12 if prec == "fp64" else 1e-10) as ctx:
+        load(ctx, 0, st, ua, va)
+        ctx.step(steps)
+        got = ctx.state(0)
+        s = ctx.stats(0)
+        assert s["failed_solves"] == 0
+    for k in ("u_theta", "u_phi", "eta", "gamma"):
+        e = rel_l2(got[k], ref[k])
+        _say(f"  {k}: rel L2 {e:.3e}")
+        assert e < tol, (k, e)
