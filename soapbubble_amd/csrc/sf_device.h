@@ -725,6 +725,19 @@ __device__ __forceinline__ U3 slerp_u(const U3& f, const U3& t, double alpha) {
 // sampleMapping (bubble.cu:539-603) on a unit-vector map: the corner
 // selection of the reference, three spherical interpolations.
 __device__ U3 sample_map_u(const UMap& M, size_t off, int nth, int nph, double rx, double ry) {
+    if (rx >= 0.5 && rx < (double)nth - 0.5) {
+        // away from the polar rows: the corners of rows floor(x), floor(x) + 1
+        // that the general branch below selects there, the phi wrap on the
+        // integer column
+        const double x = rx - 0.5, fx = floor(x), fy = floor(ry);
+        const double aT = x - fx, aP = ry - fy;
+        const int m = nph - 1, pI = (int)fy;
+        const size_t r0 = off + (size_t)(int)fx * nph, r1 = r0 + nph;
+        const size_t c0 = (size_t)(pI & m), c1 = (size_t)((pI + 1) & m);
+        auto at = [&](size_t c) { return U3{M.x[c], M.y[c], M.z[c]}; };
+        return slerp_u(slerp_u(at(r0 + c0), at(r0 + c1), aP), slerp_u(at(r1 + c0), at(r1 + c1), aP),
+                       aT);
+    }
     double x = rx - 0.5, y = ry;
     const bool fl = validate_coord(x, y, nth, nph);
     const int m = nph - 1;
