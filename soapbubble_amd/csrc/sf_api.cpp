@@ -889,6 +889,16 @@ hipGraphExec_t step_graph(Base& c) {
 }
 
 constexpr int kAdaptChunk = 4;
+// A schedule longer than this many inner iterations in a refinement (a solve
+// far from converging, NaN input) runs host-driven instead of as a graph of
+// hundreds of mostly skipped launches; the next sf_step calibrates again.
+constexpr int kGraphMaxInner = 48;
+
+void drop_calibration(Base& c) {
+    c.calibrated = false;
+    c.calib_steps = 0;
+    c.calib.clear();
+}
 
 // Follows the schedule statistics of a finished chunk of replays that ran
 // with `ran` first-refinement iterations: one more when a step needed a
@@ -1081,12 +1091,13 @@ int sf_step(sf_ctx* ctx, int32_t nsteps) {
                 // residual launch): one skipped launch per step fewer
                 if (c.method == 2 && c.sched.size() >= 2) c.sched.back() = 0;
                 c.calibrated = true;
+                if (c.sched[0] > kGraphMaxInner) drop_calibration(c);
                 if (c.trace_solve) {
                     std::fprintf(stderr, "[sf solve] graph schedule:");
                     for (int n : c.sched) std::fprintf(stderr, " %d", n);
                     std::fprintf(stderr, "\n");
                 }
-                (void)step_graph(c);
+                if (c.calibrated) (void)step_graph(c);
             }
         }
         if (want_graph && c.calibrated) {
@@ -1098,7 +1109,7 @@ int sf_step(sf_ctx* ctx, int32_t nsteps) {
             if (k < nsteps) {
                 SF_HIP(hipEventRecord(c.ev[0], c.stream));
                 int chunk = 0;
-                while (k < nsteps) {
+                while (k < nsteps && c.sched[0] <= kGraphMaxInner) {
                     const int n = adapt ? std::min(nsteps - k, kAdaptChunk) : nsteps - k;
                     hipGraphExec_t g = step_graph(c);
                     for (int q = 0; q < n; ++q) SF_HIP(hipGraphLaunch(g, c.stream));
@@ -1119,7 +1130,7 @@ int sf_step(sf_ctx* ctx, int32_t nsteps) {
                 }
                 SF_HIP(hipEventRecord(c.ev[5], c.stream));
                 SF_HIP(hipEventSynchronize(c.ev[5]));
-                if (adapt) {
+                if (adapt && chunk > 0) {
                     const int p = (chunk - 1) & 1;
                     adapt_schedule(c, c.sstat_h + p * c.nb, c.chunk_sched[p]);
                 }
@@ -1127,6 +1138,7 @@ int sf_step(sf_ctx* ctx, int32_t nsteps) {
                 SF_HIP(hipEventElapsedTime(&ms, c.ev[0], c.ev[5]));
                 c.timing.step_ms += ms;
             }
+            if (c.sched[0] > kGraphMaxInner) drop_calibration(c);
         }
         for (; k < nsteps; ++k) eager_step(c, true);
         c.steps_done += nsteps;

@@ -115,3 +115,26 @@ def test_last_refinement_applies_the_line_solve():
     assert res[1][1] == steps               # one refinement of one iteration: every solve misses
     assert 0.0 < res[2][0] < 0.5 * res[1][0]
     assert res[2][1] <= res[1][1]
+
+
+def test_unreachable_tolerance_runs_host_driven():
+    """A tolerance the refinement cannot reach: every solve spends its whole
+    budget and is counted as failed, the state stays finite, and the captured
+    path, whose calibrated schedule would run hundreds of inner launches a
+    refinement, runs the steps host-driven instead (and calibrates again on
+    the next call)."""
+    n, dt = 32, 0.008
+    _say(f"unreachable tolerance {n}x{2 * n}")
+    st, ua, va = make_case(n, 17, l_max=4, u_scale=0.5)
+    with sb.SoapFilm(sb.bubble(n, dt), "mixed", use_graph=True, tol=1e-20) as ctx:
+        load(ctx, 0, st, ua, va)
+        ctx.step(3)
+        s = ctx.stats(0)
+        _say(f"  residual {s['residual']:.3e}, iterations {s['iterations']}, failed {s['failed_solves']}")
+        assert s["failed_solves"] == 3
+        assert np.isfinite(s["residual"]) and 1e-20 < s["residual"] < 1e-9
+        ctx.step(2)
+        s = ctx.stats(0)
+        assert s["failed_solves"] == 5
+        for k in FIELDS:
+            assert np.isfinite(ctx.state(0)[k]).all(), k
