@@ -318,7 +318,8 @@ def main():
         "data": "synthetic (seeded spherical harmonics + noise)",
         "config": {"workload": f"one bubble {n}x{nph}, dt={dt}, mixed precision, hipGraph step",
                    "n_theta": n, "n_phi": nph, "cells": cells, "dt": dt, "seed": seed,
-                   "precision": "fp32 fields/advection/stencils; fp32 PCG inside fp64 "
+                   "precision": "fp32 fields and advection; system assembled in fp64, "
+                                "fp32 Chebyshev-accelerated phi-line Jacobi inside fp64 "
                                 "iterative refinement to rel. residual 1e-10"},
         "roofline": {"bound": "hbm", "kernel": "k_advect<float>", "achieved": adv_gbs,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": adv_gbs / HBM_PEAK_GBS,
