@@ -80,7 +80,8 @@ struct Base {
     hipEvent_t cev[2] = {};                // end of a chunk of replays
     int chunk_sched[2] = {0, 0};           // first-refinement iterations each chunk ran with
     hipStream_t cstream = nullptr;         // capture stream (graphs are built while replays run)
-    std::map<std::vector<int>, hipGraphExec_t> graphs;   // one executable graph per schedule
+    // executable graphs by (schedule, steps per launch, statistics slot)
+    std::map<std::vector<int>, hipGraphExec_t> graphs;
     Tables tab{};
     LaunchGeom geom{};            // cell tiles
     std::vector<LaunchGeom> rgeo; // row blocks, one geometry per row width
@@ -862,20 +863,28 @@ void eager_step(Base& c, bool host_driven) {
     c.host_hist += 1;
 }
 
-// The executable graph of one step for the current schedule, captured on the
-// capture stream (replays already queued on the context's stream keep
-// running) and kept for reuse.
-hipGraphExec_t step_graph(Base& c) {
-    auto it = c.graphs.find(c.sched);
+// The executable graph of `nsteps` steps for the current schedule, followed
+// (slot >= 0) by the hand-off of the chunk's schedule statistics to pinned
+// slot `slot`; captured on the capture stream (replays already queued on the
+// context's stream keep running) and kept for reuse. One graph launch per
+// chunk: a launch costs about as much as a short kernel.
+hipGraphExec_t step_graph(Base& c, int nsteps, int slot) {
+    std::vector<int> key = c.sched;
+    key.push_back(nsteps);
+    key.push_back(slot);
+    auto it = c.graphs.find(key);
     if (it != c.graphs.end()) return it->second;
     hipGraph_t g;
     std::swap(c.stream, c.cstream);
     hipError_t e = hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
-        c.advect();
-        c.assemble();
-        c.solve(false, false);
-        c.update(true);
+        for (int q = 0; q < nsteps; ++q) {
+            c.advect();
+            c.assemble();
+            c.solve(false, false);
+            c.update(true);
+        }
+        if (slot >= 0) launch_sched_stats(c.stream, c.sstat_d, c.sstat_h + slot * c.nb, c.nb);
         e = hipStreamEndCapture(c.stream, &g);
     }
     std::swap(c.stream, c.cstream);
@@ -884,7 +893,7 @@ hipGraphExec_t step_graph(Base& c) {
     e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     SF_HIP(e);
-    c.graphs[c.sched] = x;
+    c.graphs[key] = x;
     return x;
 }
 
@@ -1097,29 +1106,30 @@ int sf_step(sf_ctx* ctx, int32_t nsteps) {
                     for (int n : c.sched) std::fprintf(stderr, " %d", n);
                     std::fprintf(stderr, "\n");
                 }
-                if (c.calibrated) (void)step_graph(c);
+                if (c.calibrated) {
+                    const bool adapt = c.method == 2 && c.opt.max_inner <= 0;
+                    (void)step_graph(c, kAdaptChunk, adapt ? 0 : -1);
+                    if (adapt) (void)step_graph(c, kAdaptChunk, 1);
+                }
             }
         }
         if (want_graph && c.calibrated) {
             for (; k < nsteps && c.host_hist < warm; ++k) eager_step(c, true);
-            // replays in chunks; with the automatic schedule each chunk's
-            // statistics are copied out behind it and read one chunk later,
+            // replays in chunks of kAdaptChunk steps, one graph launch each;
+            // with the automatic schedule each chunk's graph ends by handing
+            // its statistics to the host, which reads them one chunk later,
             // so the device never waits for the host
             const bool adapt = c.method == 2 && c.opt.max_inner <= 0;
             if (k < nsteps) {
                 SF_HIP(hipEventRecord(c.ev[0], c.stream));
                 int chunk = 0;
                 while (k < nsteps && c.sched[0] <= kGraphMaxInner) {
-                    const int n = adapt ? std::min(nsteps - k, kAdaptChunk) : nsteps - k;
-                    hipGraphExec_t g = step_graph(c);
-                    for (int q = 0; q < n; ++q) SF_HIP(hipGraphLaunch(g, c.stream));
+                    const int n = std::min(nsteps - k, kAdaptChunk);
+                    const int p = chunk & 1;
+                    SF_HIP(hipGraphLaunch(step_graph(c, n, adapt ? p : -1), c.stream));
                     k += n;
                     c.host_hist += n;
                     if (!adapt) continue;
-                    const int p = chunk & 1;
-                    SF_HIP(hipMemcpyAsync(c.sstat_h + p * c.nb, c.sstat_d, c.nb * sizeof(SchedStat),
-                                          hipMemcpyDeviceToHost, c.stream));
-                    SF_HIP(hipMemsetAsync(c.sstat_d, 0, c.nb * sizeof(SchedStat), c.stream));
                     SF_HIP(hipEventRecord(c.cev[p], c.stream));
                     c.chunk_sched[p] = c.sched[0];
                     if (chunk > 0) {

@@ -1862,6 +1862,20 @@ void launch_finalize(hipStream_t s, int nb, const BubbleDev* bub, SolveCtl* ctl,
     hipLaunchKernelGGL(k_finalize, dim3(nb), dim3(256), 0, s, bub, ctl, hist, sp, partR, stride);
 }
 
+// The schedule statistics of a chunk of captured steps handed to the host:
+// copied into the host's pinned slot and cleared for the next chunk (a
+// kernel, so that the chunk's graph holds it).
+__global__ __launch_bounds__(64) void k_sched_stats(SchedStat* __restrict__ d,
+                                                    SchedStat* __restrict__ h, int nb) {
+    for (int b = threadIdx.x; b < nb; b += 64) {
+        h[b] = d[b];
+        d[b] = SchedStat{0.0, 0, 0};
+    }
+}
+void launch_sched_stats(hipStream_t s, SchedStat* d, SchedStat* h, int nb) {
+    hipLaunchKernelGGL(k_sched_stats, dim3(1), dim3(64), 0, s, d, h, nb);
+}
+
 template <typename T>
 void launch_update(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, BubbleTime* bt,
                    SolveCtl* ctl, const Tables& tab, StatePtrs<T> st, AirPtrs<T> air,

@@ -113,6 +113,7 @@ void launch_viscous(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, co
                     StatePtrs<T> st, double* div, double* fu, double* fv);
 void launch_finalize(hipStream_t s, int nb, const BubbleDev* bub, SolveCtl* ctl, SolveHist* hist,
                      const SolveParams& sp, const double* partR, long stride);
+void launch_sched_stats(hipStream_t s, SchedStat* d, SchedStat* h, int nb);
 template <typename T>
 void launch_update(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, BubbleTime* bt,
                    SolveCtl* ctl, const Tables& tab, StatePtrs<T> st, AirPtrs<T> air,
