@@ -898,6 +898,9 @@ hipGraphExec_t step_graph(Base& c, int nsteps, int slot) {
 }
 
 constexpr int kAdaptChunk = 4;
+// Steps captured in one graph launch (a whole chunk: measured 1, 2 and 4
+// steps per launch, 4 the fastest).
+constexpr int kGraphSteps = 4;
 // A schedule longer than this many inner iterations in a refinement (a solve
 // far from converging, NaN input) runs host-driven instead of as a graph of
 // hundreds of mostly skipped launches; the next sf_step calibrates again.
@@ -1108,8 +1111,11 @@ int sf_step(sf_ctx* ctx, int32_t nsteps) {
                 }
                 if (c.calibrated) {
                     const bool adapt = c.method == 2 && c.opt.max_inner <= 0;
-                    (void)step_graph(c, kAdaptChunk, adapt ? 0 : -1);
-                    if (adapt) (void)step_graph(c, kAdaptChunk, 1);
+                    (void)step_graph(c, kGraphSteps, -1);
+                    if (adapt) {
+                        (void)step_graph(c, kGraphSteps, 0);
+                        (void)step_graph(c, kGraphSteps, 1);
+                    }
                 }
             }
         }
@@ -1126,7 +1132,13 @@ int sf_step(sf_ctx* ctx, int32_t nsteps) {
                 while (k < nsteps && c.sched[0] <= kGraphMaxInner) {
                     const int n = std::min(nsteps - k, kAdaptChunk);
                     const int p = chunk & 1;
-                    SF_HIP(hipGraphLaunch(step_graph(c, n, adapt ? p : -1), c.stream));
+                    // kGraphSteps steps per launch, the chunk's last launch
+                    // ending with the statistics hand-off
+                    for (int q = 0; q < n; q += kGraphSteps) {
+                        const int m = std::min(kGraphSteps, n - q);
+                        const bool last = q + m == n;
+                        SF_HIP(hipGraphLaunch(step_graph(c, m, adapt && last ? p : -1), c.stream));
+                    }
                     k += n;
                     c.host_hist += n;
                     if (!adapt) continue;
