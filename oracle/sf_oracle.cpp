@@ -46,10 +46,12 @@
  *  update       per-cell closed forms (friction, div, broken, evaporation)
  *  derive       closed forms and the published parameter values
  *  volume       midpoint-rule area of the sphere, O(h^2)
- * Parity unpinned (reference text only): the phi-mirror half of the t/12
- * air blend (only its own-index half is pinned, at t = 12); the u_theta
- * sample exactly at the north pole; floorf_mode (D2) and the float
- * sinf/cosf of D1, which the oracle does not reproduce.
+ *  air blend    the t/12 mirror blend of the assembly by its symmetries:
+ *               mirror-symmetric u with antisymmetric v is t-independent,
+ *               antisymmetric u with symmetric v scales by t/6 - 1
+ * Parity unpinned (reference text only): the u_theta sample exactly at the
+ * north pole; floorf_mode (D2) and the float sinf/cosf of D1, which the
+ * oracle does not reproduce.
  */
 #include "sf_oracle.h"
 

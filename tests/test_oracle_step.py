@@ -478,3 +478,45 @@ def test_volume_quadrature(oracle):
         v = oracle.volume(p, np.ones(2 * n * n))
         errs.append(abs(v - 4 * math.pi))
     assert errs[-1] < 2e-3 and errs[0] / errs[1] > 3.9 and errs[1] / errs[2] > 3.9
+
+
+def _mirror_cols(a, rows, nph):
+    """Column i -> (nPhi - i) mod nPhi, the index mirror of the t/12 air blend."""
+    return a.reshape(rows, nph)[:, (nph - np.arange(nph)) % nph].ravel()
+
+
+def test_air_mirror_blend_invariants(oracle):
+    """The t/12 blend of the assembly's air term with its phi mirror
+    (bubble.cu:1364-1392): u_air enters as 0.5 (a(i) t/12 + a(opp) (1 - t/12)),
+    v_air as 0.5 (a(i) t/12 - a(opp) (1 - t/12)), opp = (nPhi - i) mod nPhi.
+    A mirror-symmetric u with a mirror-antisymmetric v makes the blend
+    0.5 a(i) at every t; an antisymmetric u with a symmetric v scales the air
+    term by t/6 - 1 (zero at t = 6, sign change about it). A wrong mirror
+    column, weight or sign breaks one of the three."""
+    n, dt = 16, 0.01
+    nph = 2 * n
+    st, ua, va = make_case(n, 41, u_scale=0.5)
+    u_sym = 0.5 * (ua + _mirror_cols(ua, n, nph))
+    u_anti = 0.5 * (ua - _mirror_cols(ua, n, nph))
+    v_sym = 0.5 * (va + _mirror_cols(va, n - 1, nph))
+    v_anti = 0.5 * (va - _mirror_cols(va, n - 1, nph))
+
+    def rhs(t, u, v):
+        p = oracle.params(n, dt)
+        p.time_s = t
+        return oracle.assemble(p, st, u, v)[1]
+
+    r0 = rhs(3.0, np.zeros_like(ua), np.zeros_like(va))
+    scale = np.max(np.abs(r0))
+    a, b = rhs(0.7, u_sym, v_anti), rhs(9.1, u_sym, v_anti)
+    assert np.max(np.abs(a - r0)) > 1e-6 * scale          # the air term is there
+    assert np.max(np.abs(a - b)) <= 1e-13 * scale
+    c0 = rhs(0.0, u_anti, v_sym) - r0
+    c3 = rhs(3.0, u_anti, v_sym) - r0
+    c6 = rhs(6.0, u_anti, v_sym) - r0
+    c12 = rhs(12.0, u_anti, v_sym) - r0
+    m = np.max(np.abs(c0))
+    assert m > 1e-6 * scale
+    assert np.max(np.abs(c6)) <= 1e-13 * scale
+    assert np.max(np.abs(c12 + c0)) <= 1e-10 * m
+    assert np.max(np.abs(c3 - 0.5 * c0)) <= 1e-10 * m
