@@ -1253,8 +1253,154 @@ __device__ __forceinline__ double clamp_gamma(double g, double gmax) {
     return (g <= 0.0 || g > gmax - kEps) ? gmax : g;
 }
 
+// The main-path loads of one cell of k_update (the neighbours west and
+// south as the reference reads them).
 template <typename T>
-__global__ __launch_bounds__(256) void k_update(const int4* __restrict__ tiles,
+struct UpdIn {
+    double ga, g, gW, gS, eta, etaW, etaS, u1, v1;
+    T ua0, uas, va0, vas, gd;
+};
+
+// One cell of the post-solve updates; writes u_phi, u_theta, eta, Gamma,
+// the guess pair and the BIMOCQ changes of cell c; returns whether a NaN
+// was written.
+template <typename T>
+__device__ __forceinline__ bool update_cell(const BubbleDev& B, int ghist,
+                                            const Tables& tab, const AirPtrs<T>& air,
+                                            const double* __restrict__ x, int j, int i, size_t c,
+                                            const UpdIn<T>& in, double kt, double tnow,
+                                            double invDt, T& o_eta, T& o_gam, T& o_uph,
+                                            T& o_uth, T& o_gdel, T& o_gnext, bool guess,
+                                            double* __restrict__ chg_e,
+                                            double* __restrict__ chg_g) {
+    const int nph = B.nph, m = B.mask, nth = B.nth;
+    const double gmax = B.gamma_max, dt = B.dt, invh = B.invh, Cr = B.Cr, M = B.M;
+    // VISCOUS (applyforcevel{theta,phi}Kernel_viscous, bubble.cu:1938-2038):
+    // the viscous force, the t/12 mirror blend of the air and no separate
+    // gravity term
+    const bool visc = air.fu != nullptr;
+    const int opp = (nph - i) & m;
+
+    // divergenceKernel_fromGamma (bubble.cu:1276-1295)
+    const double ga = in.ga;
+    const double g = in.g;
+    double gnew, div;
+    if (g <= 0.0 || g > gmax - kEps) {
+        gnew = gmax;
+        div = invDt;
+    } else {
+        gnew = g;
+        div = (1.0 - qdiv(fmin(g, 2 * ga), ga)) * invDt;
+    }
+    if (guess) {
+        // initial guess of the next solve: its correction Gamma^{n+1} - Gamma*
+        // extrapolated linearly from this step's and the previous step's
+        // (this step's alone after a reset); a non-finite value is dropped
+        double dl = g - ga;
+        if (!__builtin_isfinite(dl)) dl = 0.0;
+        const double gn = ghist > 0 ? 2.0 * dl - (double)in.gd : dl;
+        o_gdel = (T)dl;
+        o_gnext = __builtin_isfinite(gn) ? (T)gn : T(0);
+    }
+    const double eta = in.eta;
+    bool nan_out;
+    // applyforceThickness (bubble.cu:2041-2063)
+    {
+        double v = eta * (1 - dt * div);
+        if (B.evaporation) v += B.eva * dt;
+        o_eta = (T)fmax(kThresh, v);
+        o_gam = (T)gnew;
+        nan_out = __builtin_isnan(gnew);
+        if (chg_e) {
+            // the changes of this step for BIMOCQ (bubble.cu:2062, 2486-2490),
+            // from the stored values
+            chg_e[c] = (double)(T)fmax(kThresh, v) - eta;
+            chg_g[c] = (double)(T)gnew - ga;
+        }
+    }
+    // applyforcevelphiKernel (bubble.cu:1879-1935)
+    {
+        const double etaW = in.etaW;
+        const double u1 = in.u1;
+        const double invEta = qdiv(2., etaW + eta);
+        double res;
+        if (__builtin_isinf(invEta)) {
+            res = __builtin_nan("");
+        } else if (visc) {
+            const double gW = clamp_gamma(in.gW, gmax);
+            const double pGpx = invh * (gnew - gW);
+            const double f3 = -M * invEta * tab.cscTc[B.rt0 + j] * pGpx;
+            double uAir = 0.0;
+            if (B.airflow) {
+                const size_t rowc = (size_t)B.off + (size_t)j * nph;
+                uAir = 0.5 * (air_at(air.ua0, air.au, c, kt) * tnow / 12.0 +
+                              air_at(air.ua0, air.au, rowc + opp, kt) * (1 - tnow / 12.0));
+            }
+            const double f6 = Cr * invEta * uAir;
+            res = qdiv(u1 * invDt + f3 + f6 + air.fu[c], invDt + Cr * invEta);
+        } else {
+            const double gW = clamp_gamma(in.gW, gmax);
+            const double pGpx = invh * (gnew - gW) * tab.cscTc[B.rt0 + j];
+            const double f1 = -M * invEta * pGpx;
+            const double uAir = B.airflow ? air_val(in.ua0, in.uas, kt) : 0.0;
+            const double f2 = Cr * invEta * uAir;
+            res = qdiv(u1 * invDt + (f1 + f2 + 0.0 + 0.0), invDt + Cr * invEta);
+        }
+        o_uph = (T)res;
+        nan_out |= __builtin_isnan(res);
+    }
+    // applyforcevelthetaKernel (bubble.cu:1819-1876)
+    if (j < nth - 1) {
+        const double etaS = in.etaS;
+        const double v1 = in.v1;
+        const double invEta = qdiv(2., eta + etaS);
+        double res;
+        if (__builtin_isinf(invEta)) {
+            res = __builtin_nan("");
+        } else if (visc) {
+            const double gS = clamp_gamma(in.gS, gmax);
+            const double pGpy = invh * (gS - gnew);
+            const double f3 = -M * invEta * pGpy;
+            double vAir = 0.0;
+            if (B.airflow) {
+                const size_t rowc = (size_t)B.off + (size_t)j * nph;
+                vAir = 0.5 * (air_at(air.va0, air.av, c, kt) * tnow / 12.0 -
+                              air_at(air.va0, air.av, rowc + opp, kt) * (1 - tnow / 12.0));
+            }
+            const double f6 = Cr * invEta * vAir;
+            res = qdiv(v1 * invDt + f3 + f6 + air.fv[c], invDt + Cr * invEta);
+        } else {
+            const double gS = clamp_gamma(in.gS, gmax);
+            const double pGpy = invh * (gS - gnew);
+            const double f1 = -M * invEta * pGpy;
+            const double vAir = B.airflow ? air_val(in.va0, in.vas, kt) : 0.0;
+            const double f2 = Cr * invEta * vAir;
+            const double f3 = B.gravity ? B.gs * tab.sinTf[B.rt0 + j + 1] : 0.0;
+            res = qdiv(v1 * invDt + f1 + f2 + f3 + 0.0, invDt + Cr * invEta);
+        }
+        o_uth = (T)res;
+        nan_out |= __builtin_isnan(res);
+    }
+    return nan_out;
+}
+
+// Two consecutive cells of a row per thread (cells 2t, 2t + 1 of the tile;
+// tile widths are even), so that each field is one 2-vector load and store.
+template <typename T>
+struct alignas(2 * sizeof(T)) Pair2 {
+    T a, b;
+};
+template <typename T>
+__device__ __forceinline__ Pair2<T> ld2(const T* __restrict__ p) {
+    return *reinterpret_cast<const Pair2<T>*>(p);
+}
+template <typename T>
+__device__ __forceinline__ void st2(T* __restrict__ p, T a, T b) {
+    *reinterpret_cast<Pair2<T>*>(p) = Pair2<T>{a, b};
+}
+
+template <typename T>
+__global__ __launch_bounds__(128) void k_update(const int4* __restrict__ tiles,
                                                 const BubbleDev* __restrict__ bub,
                                                 BubbleTime* __restrict__ btime,
                                                 SolveCtl* __restrict__ ctl, Tables tab,
@@ -1280,124 +1426,62 @@ __global__ __launch_bounds__(256) void k_update(const int4* __restrict__ tiles,
         // the next assembly adds its sums into a cleared slot
         if (threadIdx.x < 64) acc_zero(acc, tl.x, acc_slot_res(0));
     }
-    int j, i;
-    tile_cell(tl, j, i);
+    const int sh = __builtin_ctz((unsigned)tl.w) - 1;   // tile width / 2 = 2^sh
+    const int j = tl.y + ((int)threadIdx.x >> sh);
+    const int i = tl.z + 2 * ((int)threadIdx.x & ((tl.w >> 1) - 1));
     if (j >= B.nth || i >= B.nph) return;
     const int nph = B.nph, m = B.mask, nth = B.nth;
+    const bool south = j < nth - 1;
     const double* __restrict__ x = ctl[tl.x].xbuf ? x1 : x0;
-    const size_t c = (size_t)B.off + (size_t)j * nph + i;
-    const size_t cW = (size_t)B.off + (size_t)j * nph + ((i - 1) & m);
-    const double gmax = B.gamma_max, dt = B.dt, invh = B.invh, Cr = B.Cr, M = B.M;
+    const size_t row = (size_t)B.off + (size_t)j * nph;
+    const size_t c = row + i, cW = row + ((i - 1) & m);
     // the reference's divisions by dt as products with 1 / dt (an ulp)
-    const double invDt = 1.0 / dt;
+    const double invDt = 1.0 / B.dt;
     const double tnow = btime[tl.x].t;
-    double kt = 0.0;
-    if (B.airflow) kt = 3.0 - fabs(tnow - 3.0);
-    // VISCOUS (applyforcevel{theta,phi}Kernel_viscous, bubble.cu:1938-2038):
-    // the viscous force, the t/12 mirror blend of the air and no separate
-    // gravity term
-    const bool visc = air.fu != nullptr;
-    const int opp = (nph - i) & m;
+    const int ghist = btime[tl.x].ghist;
+    const double kt = B.airflow ? 3.0 - fabs(tnow - 3.0) : 0.0;
 
-    // divergenceKernel_fromGamma (bubble.cu:1276-1295)
-    const double ga = (double)st.gam[c];
-    const double g = x[c];
-    double gnew, div;
-    if (g <= 0.0 || g > gmax - kEps) {
-        gnew = gmax;
-        div = invDt;
-    } else {
-        gnew = g;
-        div = (1.0 - qdiv(fmin(g, 2 * ga), ga)) * invDt;
+    const Pair2<T> gam = ld2(st.gam + c), eta = ld2(st.eta + c), uph = ld2(st.uph + c);
+    const Pair2<double> xc = ld2(x + c);
+    UpdIn<T> in0{}, in1{};
+    in0.ga = (double)gam.a; in1.ga = (double)gam.b;
+    in0.g = xc.a; in1.g = xc.b;
+    in0.gW = x[cW]; in1.gW = xc.a;
+    in0.eta = (double)eta.a; in1.eta = (double)eta.b;
+    in0.etaW = (double)st.eta[cW]; in1.etaW = (double)eta.a;
+    in0.u1 = (double)uph.a; in1.u1 = (double)uph.b;
+    if (south) {
+        const Pair2<T> etaS = ld2(st.eta + c + nph), uth = ld2(st.uth + c);
+        const Pair2<double> xS = ld2(x + c + nph);
+        in0.gS = xS.a; in1.gS = xS.b;
+        in0.etaS = (double)etaS.a; in1.etaS = (double)etaS.b;
+        in0.v1 = (double)uth.a; in1.v1 = (double)uth.b;
     }
-    if (gdel) {
-        // initial guess of the next solve: its correction Gamma^{n+1} - Gamma*
-        // extrapolated linearly from this step's and the previous step's
-        // (this step's alone after a reset); a non-finite value is dropped
-        double dl = g - ga;
-        if (!__builtin_isfinite(dl)) dl = 0.0;
-        const double gn = btime[tl.x].ghist > 0 ? 2.0 * dl - (double)gdel[c] : dl;
-        gdel[c] = (T)dl;
-        gnext[c] = __builtin_isfinite(gn) ? (T)gn : T(0);
-    }
-    const double eta = (double)st.eta[c];
-    bool nan_out;
-    // applyforceThickness (bubble.cu:2041-2063)
-    {
-        double v = eta * (1 - dt * div);
-        if (B.evaporation) v += B.eva * dt;
-        out.eta[c] = (T)fmax(kThresh, v);
-        out.gam[c] = (T)gnew;
-        nan_out = __builtin_isnan(gnew);
-        if (chg_e) {
-            // the changes of this step for BIMOCQ (bubble.cu:2062, 2486-2490),
-            // from the stored values
-            chg_e[c] = (double)(T)fmax(kThresh, v) - eta;
-            chg_g[c] = (double)(T)gnew - ga;
+    if (B.airflow) {
+        const Pair2<T> ua = ld2(air.ua0 + c), us = ld2(air.au + c);
+        in0.ua0 = ua.a; in1.ua0 = ua.b; in0.uas = us.a; in1.uas = us.b;
+        if (south) {
+            const Pair2<T> va = ld2(air.va0 + c), vs = ld2(air.av + c);
+            in0.va0 = va.a; in1.va0 = va.b; in0.vas = vs.a; in1.vas = vs.b;
         }
     }
-    // applyforcevelphiKernel (bubble.cu:1879-1935)
-    {
-        const double etaW = (double)st.eta[cW];
-        const double u1 = (double)st.uph[c];
-        const double invEta = qdiv(2., etaW + eta);
-        double res;
-        if (__builtin_isinf(invEta)) {
-            res = __builtin_nan("");
-        } else if (visc) {
-            const double gW = clamp_gamma(x[cW], gmax);
-            const double pGpx = invh * (gnew - gW);
-            const double f3 = -M * invEta * tab.cscTc[B.rt0 + j] * pGpx;
-            double uAir = 0.0;
-            if (B.airflow) {
-                const size_t rowc = (size_t)B.off + (size_t)j * nph;
-                uAir = 0.5 * (air_at(air.ua0, air.au, c, kt) * tnow / 12.0 +
-                              air_at(air.ua0, air.au, rowc + opp, kt) * (1 - tnow / 12.0));
-            }
-            const double f6 = Cr * invEta * uAir;
-            res = qdiv(u1 * invDt + f3 + f6 + air.fu[c], invDt + Cr * invEta);
-        } else {
-            const double gW = clamp_gamma(x[cW], gmax);
-            const double pGpx = invh * (gnew - gW) * tab.cscTc[B.rt0 + j];
-            const double f1 = -M * invEta * pGpx;
-            const double uAir = B.airflow ? air_at(air.ua0, air.au, c, kt) : 0.0;
-            const double f2 = Cr * invEta * uAir;
-            res = qdiv(u1 * invDt + (f1 + f2 + 0.0 + 0.0), invDt + Cr * invEta);
-        }
-        out.uph[c] = (T)res;
-        nan_out |= __builtin_isnan(res);
+    const bool guess = gdel != nullptr;
+    if (guess) {
+        const Pair2<T> gd = ld2(gdel + c);
+        in0.gd = gd.a; in1.gd = gd.b;
     }
-    // applyforcevelthetaKernel (bubble.cu:1819-1876)
-    if (j < nth - 1) {
-        const double etaS = (double)st.eta[c + nph];
-        const double v1 = (double)st.uth[c];
-        const double invEta = qdiv(2., eta + etaS);
-        double res;
-        if (__builtin_isinf(invEta)) {
-            res = __builtin_nan("");
-        } else if (visc) {
-            const double gS = clamp_gamma(x[c + nph], gmax);
-            const double pGpy = invh * (gS - gnew);
-            const double f3 = -M * invEta * pGpy;
-            double vAir = 0.0;
-            if (B.airflow) {
-                const size_t rowc = (size_t)B.off + (size_t)j * nph;
-                vAir = 0.5 * (air_at(air.va0, air.av, c, kt) * tnow / 12.0 -
-                              air_at(air.va0, air.av, rowc + opp, kt) * (1 - tnow / 12.0));
-            }
-            const double f6 = Cr * invEta * vAir;
-            res = qdiv(v1 * invDt + f3 + f6 + air.fv[c], invDt + Cr * invEta);
-        } else {
-            const double gS = clamp_gamma(x[c + nph], gmax);
-            const double pGpy = invh * (gS - gnew);
-            const double f1 = -M * invEta * pGpy;
-            const double vAir = B.airflow ? air_at(air.va0, air.av, c, kt) : 0.0;
-            const double f2 = Cr * invEta * vAir;
-            const double f3 = B.gravity ? B.gs * tab.sinTf[B.rt0 + j + 1] : 0.0;
-            res = qdiv(v1 * invDt + f1 + f2 + f3 + 0.0, invDt + Cr * invEta);
-        }
-        out.uth[c] = (T)res;
-        nan_out |= __builtin_isnan(res);
+    T e0, e1, g0, g1, u0, u1, v0 = T(0), v1 = T(0), d0 = T(0), d1 = T(0), n0 = T(0), n1 = T(0);
+    bool nan_out = update_cell(B, ghist, tab, air, x, j, i, c, in0, kt, tnow, invDt, e0, g0, u0,
+                               v0, d0, n0, guess, chg_e, chg_g);
+    nan_out |= update_cell(B, ghist, tab, air, x, j, i + 1, c + 1, in1, kt, tnow, invDt, e1, g1,
+                           u1, v1, d1, n1, guess, chg_e, chg_g);
+    st2(out.eta + c, e0, e1);
+    st2(out.gam + c, g0, g1);
+    st2(out.uph + c, u0, u1);
+    if (south) st2(out.uth + c, v0, v1);
+    if (guess) {
+        st2(gdel + c, d0, d1);
+        st2(gnext + c, n0, n1);
     }
     // the advection of the next step takes its NaN-skipping path from here on
     if (nan_out) btime[tl.x].nans = 1;
@@ -1882,7 +1966,7 @@ void launch_update(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, Bub
                    const double* x0, const double* x1, StatePtrs<T> out, int fin, SolveHist* hist,
                    const SolveParams& sp, const double* partR, long stride, double* chg_e,
                    double* chg_g, T* gdel, T* gnext, double* acc) {
-    hipLaunchKernelGGL(k_update<T>, dim3(g.ntiles), dim3(256), 0, s, g.tiles, bub, bt, ctl, tab, st,
+    hipLaunchKernelGGL(k_update<T>, dim3(g.ntiles), dim3(128), 0, s, g.tiles, bub, bt, ctl, tab, st,
                        air, x0, x1, out, fin, hist, sp, partR, stride, chg_e, chg_g, gdel, gnext, acc);
 }
 
