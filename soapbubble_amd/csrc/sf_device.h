@@ -930,39 +930,41 @@ __device__ __forceinline__ float wave_max(float v) {
 // steps < 0: reduce adaptively (a block-wide max per reduction) and return
 // the number of reductions done; steps >= 0: do exactly that many (the
 // count returned by the adaptive solve of the same matrix).
-// On exit sd holds x. CPT bounds the cells per thread.
+// The reductions alternate between the arrays (sa, sc, sd) and the free
+// arrays (ta, tc, td), so one barrier per reduction orders them; on exit
+// *x points to the array that holds the solution. CPT bounds the cells per
+// thread.
 template <typename Ti, int CPT>
-__device__ __forceinline__ int pcr_periodic(Ti* sa, Ti* sc, Ti* sd, int n, int steps,
-                                            float* red) {
+__device__ __forceinline__ int pcr_periodic(Ti* sa, Ti* sc, Ti* sd, Ti* ta, Ti* tc, Ti* td, int n,
+                                            int steps, float* red, Ti** x) {
     const int tid = threadIdx.x, nt = blockDim.x, m = n - 1;
     const int full = 31 - __clz(n) - 1;     // log2(n) - 1
+    Ti *a = sa, *c = sc, *d = sd, *na = ta, *nc = tc, *nd = td;
     int done = 0;
     for (int s = 1; s < n / 2; s <<= 1) {
         if (steps >= 0 && done >= steps) break;
-        Ti na[CPT], nc[CPT], nd[CPT];
         float mx = 0.0f;
 #pragma unroll
         for (int k = 0; k < CPT; ++k) {
             int i = tid + k * nt;
             if (i < n) {
                 int im = (i - s) & m, ip = (i + s) & m;
-                Ti ai = sa[i], ci = sc[i], di = sd[i];
-                Ti am = sa[im], cm = sc[im], dm = sd[im];
-                Ti ap = sa[ip], cp = sc[ip], dp = sd[ip];
+                Ti ai = a[i], ci = c[i], di = d[i];
+                Ti am = a[im], cm = c[im], dm = d[im];
+                Ti ap = a[ip], cp = c[ip], dp = d[ip];
                 Ti iD = pre_rcp(Ti(1) - ai * cm - ci * ap);
-                na[k] = -ai * am * iD;
-                nc[k] = -ci * cp * iD;
-                nd[k] = (di - ai * dm - ci * dp) * iD;
-                mx = fmaxf(mx, (float)(fabs(na[k]) + fabs(nc[k])));
+                const Ti a2 = -ai * am * iD, c2 = -ci * cp * iD;
+                na[i] = a2;
+                nc[i] = c2;
+                nd[i] = (di - ai * dm - ci * dp) * iD;
+                mx = fmaxf(mx, (float)(fabs(a2) + fabs(c2)));
             }
         }
-        __syncthreads();
-#pragma unroll
-        for (int k = 0; k < CPT; ++k) {
-            int i = tid + k * nt;
-            if (i < n) { sa[i] = na[k]; sc[i] = nc[k]; sd[i] = nd[k]; }
-        }
         ++done;
+        Ti* t;
+        t = a; a = na; na = t;
+        t = c; c = nc; nc = t;
+        t = d; d = nd; nd = t;
         if (steps < 0) {
             mx = wave_max(mx);
             if ((tid & 63) == 0) red[tid >> 6] = mx;
@@ -970,29 +972,29 @@ __device__ __forceinline__ int pcr_periodic(Ti* sa, Ti* sc, Ti* sd, int n, int s
             float bm = 0.0f;
             for (int w = 0; w < (nt + 63) >> 6; ++w) bm = fmaxf(bm, red[w]);
             __syncthreads();
-            if (bm < (float)PcrTol<Ti>::v) return done;   // sd already holds x
+            if (bm < (float)PcrTol<Ti>::v) {   // d already holds x
+                *x = d;
+                return done;
+            }
         } else {
             __syncthreads();
         }
     }
-    if (done < full) return done;                    // truncated: x_i = d_i
-    Ti xs[CPT];
+    if (done < full) {   // truncated: x_i = d_i
+        *x = d;
+        return done;
+    }
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
         int i = tid + k * nt;
         if (i < n) {
             int j = (i + n / 2) & m;
-            Ti ei = sa[i] + sc[i], ej = sa[j] + sc[j];
-            xs[k] = (sd[i] - ei * sd[j]) * pre_rcp(Ti(1) - ei * ej);
+            Ti ei = a[i] + c[i], ej = a[j] + c[j];
+            nd[i] = (d[i] - ei * d[j]) * pre_rcp(Ti(1) - ei * ej);
         }
     }
     __syncthreads();
-#pragma unroll
-    for (int k = 0; k < CPT; ++k) {
-        int i = tid + k * nt;
-        if (i < n) sd[i] = xs[k];
-    }
-    __syncthreads();
+    *x = nd;
     return done;
 }
 

@@ -678,9 +678,12 @@ __device__ __forceinline__ int line_solve(Ti (&a)[C], Ti (&c)[C], Ti (&d)[C], in
         rd[tid] = D * ib;
     }
     __syncthreads();
-    const int done = pcr_periodic<Ti, 1>(ra, rc, rd, R, steps, redf);
+    // the interface unknowns' arrays xa, xb, xg are free from here on: the
+    // second buffer of the reduction
+    Ti* xs;
+    const int done = pcr_periodic<Ti, 1>(ra, rc, rd, xa, xb, xg, R, steps, redf, &xs);
     if (act) {
-        const Ti Xt = rd[tid], Xp = rd[(tid - 1) & (R - 1)];
+        const Ti Xt = xs[tid], Xp = xs[(tid - 1) & (R - 1)];
 #pragma unroll
         for (int k = 0; k < C - 1; ++k) d[k] = al[k] + be[k] * Xp + ga[k] * Xt;
         d[C - 1] = Xt;
