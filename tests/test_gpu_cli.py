@@ -26,9 +26,11 @@ def _smooth(n, seed):
     return f / np.abs(f).max()
 
 
-@pytest.mark.parametrize("advection", ["plain", "bimocq"])
-def test_cli_frames_match_oracle(tmp_path, oracle, advection):
-    print(f"soapBubble 32x64 {advection}, 3 frames of 0.012 s at dt 0.005 s", flush=True)
+@pytest.mark.parametrize("advection,viscous", [("plain", False), ("bimocq", False),
+                                               ("plain", True)])
+def test_cli_frames_match_oracle(tmp_path, oracle, advection, viscous):
+    print(f"soapBubble 32x64 {advection} viscous={viscous}, 3 frames of 0.012 s at dt 0.005 s",
+          flush=True)
     n, radius, H, dt, DT, frames = 32, 0.05, 1e-6, 0.005, 0.012, 3
     thick = str(tmp_path / "thickness.exr")
     air = str(tmp_path / "air.exr")
@@ -37,7 +39,8 @@ def test_cli_frames_match_oracle(tmp_path, oracle, advection):
     _amg(tmp_path, tol="1e-12", conv="RELATIVE")
     cfg = _config(tmp_path, n_theta=n, radius=radius, H=H, timePerStep=dt, timePerFrame=DT,
                   frameNumbers=frames, thicknessInit=thick)
-    r = subprocess.run([CLI, cfg, "--precision", "fp64", "--advection", advection, "--air", air],
+    r = subprocess.run([CLI, cfg, "--precision", "fp64", "--advection", advection, "--air", air,
+                        "--viscous", "on" if viscous else "off"],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     out = tmp_path / "out"
@@ -46,7 +49,7 @@ def test_cli_frames_match_oracle(tmp_path, oracle, advection):
     f = sb.texture_fields(n, thick, air, radius)
     st = {"u_theta": np.zeros((n - 1) * 2 * n), "u_phi": np.zeros(2 * n * n),
           "eta": f["eta"].copy(), "gamma": np.ones(2 * n * n)}
-    kw = dict(radius=radius, H=H, gravity=True, evaporation=True, tol=1e-12)
+    kw = dict(radius=radius, H=H, gravity=True, evaporation=True, tol=1e-12, viscous=viscous)
     bm = None
     if advection == "bimocq":
         # blendCoeff 0.5 of the config, five samples, threshold 4 cells
