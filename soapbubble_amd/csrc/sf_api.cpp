@@ -1044,12 +1044,20 @@ int sf_set_dt(sf_ctx* ctx, int32_t b, double dt_seconds) {
         SF_HIP(hipMemcpy(c.time_h.data(), c.time_d, c.nb * sizeof(BubbleTime), hipMemcpyDeviceToHost));
         for (int k = 0; k < c.nb; ++k) {
             if (b >= 0 && b != k) continue;
+            // the last step's time advance, still pending for the next
+            // advection, is by the step size it was taken with
+            if (c.time_h[k].pending) {
+                c.time_h[k].t += c.time_h[k].dt_s;
+                c.time_h[k].pending = 0;
+            }
             // timeStepGlobal = dt U / radius with the member U = 1 (bubble.cu:2675)
             const double dtn = dt_seconds * 1.0 / c.der[k].radius;
             if (dtn != c.bub_h[k].dt) {
-                // the stored solve corrections belong to the old step
+                // the stored solve corrections and the inner schedule belong
+                // to the old step
                 c.time_h[k].ghist = c.time_h[k].gpend = 0;
                 c.host_hist = 0;
+                drop_calibration(c);
             }
             c.bub_h[k].dt = dtn;
             c.time_h[k].dt_s = dt_seconds;

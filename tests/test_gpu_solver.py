@@ -118,23 +118,62 @@ def test_last_refinement_applies_the_line_solve():
 
 
 def test_unreachable_tolerance_runs_host_driven():
-    """A tolerance the refinement cannot reach: every solve spends its whole
-    budget and is counted as failed, the state stays finite, and the captured
-    path, whose calibrated schedule would run hundreds of inner launches a
-    refinement, runs the steps host-driven instead (and calibrates again on
-    the next call)."""
+    """A tolerance the refinement cannot reach, with no early stop of the
+    inner solve (inner_rho 1e-30): every solve spends its whole budget (four
+    refinements of 200 inner iterations) and is counted as failed, the state
+    stays finite, and the captured path, whose calibrated schedule would run
+    hundreds of inner launches a refinement, runs the steps host-driven
+    instead (and calibrates again on the next call)."""
     n, dt = 32, 0.008
     _say(f"unreachable tolerance {n}x{2 * n}")
     st, ua, va = make_case(n, 17, l_max=4, u_scale=0.5)
-    with sb.SoapFilm(sb.bubble(n, dt), "mixed", use_graph=True, tol=1e-20) as ctx:
+    with sb.SoapFilm(sb.bubble(n, dt), "mixed", use_graph=True, tol=1e-20,
+                     inner_rho=1e-30) as ctx:
         load(ctx, 0, st, ua, va)
         ctx.step(3)
         s = ctx.stats(0)
         _say(f"  residual {s['residual']:.3e}, iterations {s['iterations']}, failed {s['failed_solves']}")
         assert s["failed_solves"] == 3
         assert np.isfinite(s["residual"]) and 1e-20 < s["residual"] < 1e-9
+        # the last solve ran host-driven: more inner launches than a captured
+        # schedule of at most 48 a refinement holds
+        assert s["iterations"] > 4 * 48
         ctx.step(2)
         s = ctx.stats(0)
         assert s["failed_solves"] == 5
         for k in FIELDS:
             assert np.isfinite(ctx.state(0)[k]).all(), k
+
+
+def test_set_dt_under_graph_replay(oracle):
+    """A step size change between captured runs (the frame loop's partial
+    steps): the stored guess and the inner schedule are dropped and set up
+    again, every solve still meets the tolerance, and the states follow the
+    host-driven run and the oracle with the same step sequence."""
+    n, dt1, dt2 = 64, 0.004, 0.008
+    _say(f"set_dt under graph replay {n}x{2 * n}")
+    st, ua, va = make_case(n, 19, l_max=6, u_scale=0.5)
+    out = []
+    for use_graph in (False, True):
+        with sb.SoapFilm(sb.bubble(n, dt1), "mixed", use_graph=use_graph) as ctx:
+            load(ctx, 0, st, ua, va)
+            ctx.step(12)
+            ctx.set_dt(dt2)
+            ctx.step(12)
+            ctx.set_dt(dt1)
+            ctx.step(6)
+            ctx.synchronize()
+            s = ctx.stats(0)
+            assert s["failed_solves"] == 0 and s["max_residual"] <= 1e-10, use_graph
+            out.append(ctx.state(0))
+    ref = {k: v.copy() for k, v in st.items()}
+    t = 0.0
+    for dt, steps in ((dt1, 12), (dt2, 12), (dt1, 6)):
+        p = oracle.params(n, dt, time_s=t)
+        oracle.step(p, ref, ua, va, steps)
+        t = p.time_s
+    for k in FIELDS:
+        assert rel_l2(out[1][k], out[0][k]) < 2e-6, k
+        e = rel_l2(out[1][k], ref[k])
+        _say(f"  {k}: rel L2 vs oracle {e:.3e}")
+        assert e < 5e-5, k
