@@ -178,6 +178,11 @@ int sf_synchronize(sf_ctx* ctx);
  * recompute the advected fields from the unchanged state; call at the start
  * of a step). A measurement aid with no counterpart in the reference. */
 int sf_time_advection(sf_ctx* ctx, int32_t nlaunch, double* avg_ms);
+/* Average device time of one inner (Chebyshev) iteration launch over
+ * nlaunch back-to-back launches between two events on the context's stream.
+ * Call between sf_assemble and sf_solve; the solve that follows is
+ * unchanged. A measurement aid with no counterpart in the reference. */
+int sf_time_inner(sf_ctx* ctx, int32_t nlaunch, double* avg_ms);
 /* Solver statistics and Solver::isBroken (solver.cu:296-298, 568-574). */
 int sf_get_stats(sf_ctx* ctx, int32_t bubble, sf_bubble_stats* out);
 /* Device time per stage, accumulated since creation or the last reset. */

@@ -36,7 +36,7 @@ STATUS = {0: "SF_OK", 1: "SF_ERR_INVALID", 2: "SF_ERR_HIP", 3: "SF_ERR_UNSUPPORT
 EXPORTS = (
     "sf_create", "sf_destroy", "sf_upload_field", "sf_download_field", "sf_set_time",
     "sf_set_dt", "sf_step", "sf_advect", "sf_assemble", "sf_solve", "sf_update",
-    "sf_synchronize", "sf_time_advection", "sf_get_stats", "sf_get_timing", "sf_reset_timing", "sf_last_error",
+    "sf_synchronize", "sf_time_advection", "sf_time_inner", "sf_get_stats", "sf_get_timing", "sf_reset_timing", "sf_last_error",
     "sf_derive_params", "sf_synth_fields", "sf_texture_fields", "sf_real_ylm",
     "sf_write_thickness_exr", "sf_write_image_exr",
 )
@@ -113,6 +113,7 @@ def load_library(path: str = LIB_PATH):
         "sf_update": (ctypes.c_int, [vp]),
         "sf_synchronize": (ctypes.c_int, [vp]),
         "sf_time_advection": (ctypes.c_int, [vp, ctypes.c_int32, P(ctypes.c_double)]),
+        "sf_time_inner": (ctypes.c_int, [vp, ctypes.c_int32, P(ctypes.c_double)]),
         "sf_get_stats": (ctypes.c_int, [vp, ctypes.c_int32, P(BubbleStats)]),
         "sf_get_timing": (ctypes.c_int, [vp, P(Timing)]),
         "sf_reset_timing": (ctypes.c_int, [vp]),
@@ -313,6 +314,13 @@ class SoapFilm:
         """Average device time (ms) of one advection launch."""
         out = ctypes.c_double()
         self._check(self._lib.sf_time_advection(self._h, nlaunch, ctypes.byref(out)))
+        return out.value
+
+    def time_inner(self, nlaunch: int = 20) -> float:
+        """Average device time (ms) of one inner (Chebyshev) iteration launch;
+        call between assemble() and solve()."""
+        out = ctypes.c_double()
+        self._check(self._lib.sf_time_inner(self._h, nlaunch, ctypes.byref(out)))
         return out.value
 
     def synchronize(self):

@@ -139,6 +139,8 @@ struct Base {
     virtual void upload(int b, int field, const double* h) = 0;
     virtual void download(int b, int field, double* h) = 0;
     virtual void row_stats() = 0;
+    // one inner Chebyshev launch (iteration `it` of refinement `kr`)
+    virtual void inner_launch(int it, int kr) = 0;
     virtual bool fp64_inner() const = 0;
     // drops the stored solve corrections of bubble b (all when b < 0)
     void reset_guess(int b) {
@@ -297,6 +299,7 @@ struct Impl : Base {
             launch_cheb<Ti>(stream, rg, bub_d, ctl_d, sys(x64[0]), cheb_e(it), cheb_e(it - 1),
                             cheb_e(it + 1), it, kr, sp, partB, stride);
     }
+    void inner_launch(int it, int kr) override { cheb(it, kr); }
     // one accumulator slot of every bubble, copied to the host
     std::vector<double> slot_h;
     void fetch_slot(int slot) {
@@ -1190,6 +1193,33 @@ int sf_time_advection(sf_ctx* ctx, int32_t nlaunch, double* avg_ms) {
         SF_HIP(hipEventSynchronize(c.ev[5]));
         float ms = 0.f;
         SF_HIP(hipEventElapsedTime(&ms, c.ev[0], c.ev[5]));
+        *avg_ms = (double)ms / nlaunch;
+        return (int)SF_OK;
+    });
+}
+
+int sf_time_inner(sf_ctx* ctx, int32_t nlaunch, double* avg_ms) {
+    return guarded(ctx, [&] {
+        if (!ctx || !ctx->impl || !avg_ms) throw Status{SF_ERR_INVALID, "null argument"};
+        if (nlaunch <= 0) throw Status{SF_ERR_INVALID, "launch count must be positive"};
+        Base& c = *ctx->impl;
+        if (c.stage != 2) throw Status{SF_ERR_ORDER, "sf_time_inner must follow sf_assemble"};
+        if (c.method != 2) throw Status{SF_ERR_INVALID, "the inner method is not Chebyshev"};
+        // the launches are the second inner iteration of the first
+        // refinement (no start test; e_1 = z from the assembly, e_2 whatever
+        // its buffer holds): they write e_3, which the solve writes before
+        // it reads it, so the solve that follows is unchanged; the control
+        // blocks (their iteration counts) are restored
+        std::vector<SolveCtl> keep(c.nb);
+        SF_HIP(hipMemcpy(keep.data(), c.ctl_d, c.nb * sizeof(SolveCtl), hipMemcpyDeviceToHost));
+        c.inner_launch(2, 0);
+        SF_HIP(hipEventRecord(c.ev[0], c.stream));
+        for (int k = 0; k < nlaunch; ++k) c.inner_launch(2, 0);
+        SF_HIP(hipEventRecord(c.ev[5], c.stream));
+        SF_HIP(hipEventSynchronize(c.ev[5]));
+        float ms = 0.f;
+        SF_HIP(hipEventElapsedTime(&ms, c.ev[0], c.ev[5]));
+        SF_HIP(hipMemcpy(c.ctl_d, keep.data(), c.nb * sizeof(SolveCtl), hipMemcpyHostToDevice));
         *avg_ms = (double)ms / nlaunch;
         return (int)SF_OK;
     });

@@ -177,3 +177,59 @@ def test_set_dt_under_graph_replay(oracle):
         e = rel_l2(out[1][k], ref[k])
         _say(f"  {k}: rel L2 vs oracle {e:.3e}")
         assert e < 5e-5, k
+
+
+@pytest.mark.parametrize("n,prec", [(64, "mixed"), (64, "fp64"), (1024, "mixed")])
+def test_inner_timing_leaves_the_step_unchanged(n, prec):
+    """sf_time_inner between the assembly and the solve (the measurement of
+    the inner launch) changes nothing the step computes: the state after the
+    step is bitwise the one of the same step without it, and so are the
+    solve statistics. The solve's convergence tests read sums accumulated by
+    atomics in no fixed order, but no test lies within rounding of its
+    bound here, so the decisions and hence every value are the same."""
+    dt = 0.004 if n == 64 else 0.001
+    _say(f"inner timing {prec} {n}x{2 * n}")
+    st, ua, va = make_case(n, 23, l_max=8, u_scale=0.5)
+    out = []
+    for timed in (False, True):
+        with sb.SoapFilm(sb.bubble(n, dt), prec, inner_method="chebyshev") as ctx:
+            load(ctx, 0, st, ua, va)
+            ctx.step(2)
+            ctx.advect()
+            ctx.assemble()
+            if timed:
+                ms = ctx.time_inner(5)
+                _say(f"  inner launch {ms * 1e3:.2f} us")
+                assert ms > 0
+            ctx.solve()
+            ctx.update()
+            ctx.step(1)
+            ctx.synchronize()
+            out.append((ctx.state(0), ctx.stats(0)))
+    for k in FIELDS:
+        assert np.array_equal(out[0][0][k], out[1][0][k]), k
+    for k in ("iterations", "outer", "failed_solves"):
+        assert out[0][1][k] == out[1][1][k], k
+
+
+def test_inner_timing_checks_its_place():
+    """sf_time_inner outside its place in the step (SF_ERR_ORDER), or with
+    the PCG inner method (SF_ERR_INVALID), is refused with a status and
+    leaves the context usable."""
+    n = 32
+    st, ua, va = make_case(n, 3, l_max=4, u_scale=0.5)
+    with sb.SoapFilm(sb.bubble(n, 0.008), "mixed") as ctx:
+        load(ctx, 0, st, ua, va)
+        with pytest.raises(sb.SoapFilmError) as e:
+            ctx.time_inner(3)
+        assert e.value.code == 6
+        ctx.step(1)
+    with sb.SoapFilm(sb.bubble(n, 0.008), "mixed", inner_method="cg") as ctx:
+        load(ctx, 0, st, ua, va)
+        ctx.advect()
+        ctx.assemble()
+        with pytest.raises(sb.SoapFilmError) as e:
+            ctx.time_inner(3)
+        assert e.value.code == 1
+        ctx.solve()
+        ctx.update()
