@@ -49,6 +49,13 @@
  *  air blend    the t/12 mirror blend of the assembly by its symmetries:
  *               mirror-symmetric u with antisymmetric v is t-independent,
  *               antisymmetric u with symmetric v scales by t/6 - 1
+ *  BIMOCQ       (tests/test_oracle_bimocq.py) lerpCoords closed forms,
+ *               identity maps, zero-velocity map updates, the still film
+ *               equal to the plain step, reinitialisation bounds; on a
+ *               meridian flow the closed forms bwd = x - k s, fwd = x + k s
+ *               of k map updates, the change accumulated through the
+ *               forward map and the BIMOCQ eta through both backward maps
+ *               (linear fields: exact values)
  * Parity unpinned (reference text only): the u_theta sample exactly at the
  * north pole; floorf_mode (D2) and the float sinf/cosf of D1, which the
  * oracle does not reproduce.

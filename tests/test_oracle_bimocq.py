@@ -167,3 +167,96 @@ def test_multisample_is_a_bounded_average(oracle):
     c = np.full_like(st["eta"], 0.8)
     bc = oracle.BimocqState(p, c, st["gamma"])
     assert np.abs(oracle.advect_bimocq(p, c, bc) - 0.8).max() <= 1e-14
+
+
+# A meridian flow (u_theta = omega, u_phi = 0): every trace runs along its
+# meridian, a great circle, by s = omega dt / h cells (the trace pin of
+# tests/test_oracle_samplers.py), and a map lookup between two points of one
+# meridian is exact along it (equal angles), with no phi fraction. So k map
+# updates from the identity give bwd(x) = x - k s (updates with +dt) and
+# fwd(x) = x + k s (-dt) exactly, on the rows the polar rows' cross-pole
+# velocity samples have not reached (one row more per update).
+_MER = dict(n=32, dt=0.01, omega=1.3)
+
+
+def _meridian_maps(oracle, k):
+    n, dt, omega = _MER["n"], _MER["dt"], _MER["omega"]
+    nph = 2 * n
+    p = oracle.params(n, dt)
+    uth, uph = np.full((n - 1) * nph, omega), np.zeros(n * nph)
+    mt, mp = _identity(n)
+    maps = []
+    bt, bp, ft, fp = mt, mp, mt, mp
+    for _ in range(k):
+        bt, bp = oracle.map_update(p, uth, uph, dt, bt, bp)
+        ft, fp = oracle.map_update(p, uth, uph, -dt, ft, fp)
+        maps.append((bt, bp, ft, fp))
+    return p, omega * dt / (math.pi / n), maps
+
+
+def test_maps_follow_a_meridian_flow(oracle):
+    """updateMappingKernel on a moving flow against its closed form: a dt of
+    the wrong sign, or the map sampled at x instead of the traced point,
+    fails it."""
+    n = _MER["n"]
+    nph = 2 * n
+    p, s, maps = _meridian_maps(oracle, 3)
+    mt, mp = _identity(n)
+    for k, (bt, bp, ft, fp) in enumerate(maps, start=1):
+        rows = slice((k + 1) * nph, (n - 1 - k) * nph)
+        assert np.abs(bt[rows] - (mt[rows] - k * s)).max() <= 1e-11, k
+        assert np.abs(ft[rows] - (mt[rows] + k * s)).max() <= 1e-11, k
+        for q in (bp, fp):
+            assert np.abs((q[rows] - mp[rows] + n) % nph - n).max() <= 1e-10, k
+    # and the two maps are inverse there: the distortion measure's terms vanish
+    bt, bp, ft, fp = maps[-1]
+    for j in range(6, n - 6):
+        back = oracle.sample_mapping(p, bt, bp, ft[j * nph + 5], fp[j * nph + 5])
+        assert abs(back[0] - (j + 0.5)) <= 1e-11
+
+
+def test_accumulate_through_a_shifted_forward_map(oracle):
+    """accumulateChangesCentered pulls the step's change back through the
+    forward map: with fwd(x) = x + 2 s and a change linear in theta,
+    c_j = a j + b, each interior row gains exactly a (j + 2 s) + b (the
+    backward map in its place gives a (j - 2 s) + b)."""
+    n = _MER["n"]
+    nph = 2 * n
+    p, s, maps = _meridian_maps(oracle, 2)
+    ft, fp = maps[1][2], maps[1][3]
+    a, b = 0.03, 0.2
+    change = np.repeat(a * np.arange(n) + b, nph)
+    delta = np.full(n * nph, 0.5)
+    oracle.lib().sfo_accumulate(oracle.ctypes.byref(p), oracle._p(ft), oracle._p(fp),
+                                oracle._p(change), oracle._p(delta), 0)
+    d = delta.reshape(n, nph)
+    for j in range(3, n - 3):
+        assert np.abs(d[j] - (0.5 + a * (j + 2 * s) + b)).max() <= 1e-12, j
+
+
+def test_bimocq_eta_through_shifted_maps(oracle):
+    """advectionCenteredBimocq through the backward maps of a meridian flow
+    (current interval bwd(x) = x - 2 s, previous interval x - s) with the
+    four fields linear in theta and distinct: the single-sample eta is
+    (1 - bl)(init_last(x - 3 s) + delta(x - 2 s) + delta_last(x - 3 s)) +
+    bl (init(x - 2 s) + delta(x - 2 s)) exactly, so a swapped map, field or
+    blend weight fails it."""
+    n = _MER["n"]
+    nph = 2 * n
+    p, s, maps = _meridian_maps(oracle, 2)
+    bl = 0.3
+    ones = np.ones(n * nph)
+    bm = oracle.BimocqState(p, ones, ones, blend=bl, multisample=False)
+    lin = {"eta_init": (0.02, 1.0), "eta_initlast": (-0.01, 0.8),
+           "eta_delta": (0.005, 0.1), "eta_deltalast": (0.003, -0.05)}
+    for k, (a, b) in lin.items():
+        bm.view(k)[:] = np.repeat(a * np.arange(n) + b, nph)
+    bm.view("bwd_t")[:], bm.view("bwd_p")[:] = maps[1][0], maps[1][1]
+    bm.view("bwdprev_t")[:], bm.view("bwdprev_p")[:] = maps[0][0], maps[0][1]
+    out = oracle.advect_bimocq(p, ones, bm).reshape(n, nph)
+    f = lambda k, x: lin[k][0] * x + lin[k][1]     # at row coordinate x
+    for j in range(4, n - 4):
+        ip, lp = j - 2 * s, j - 3 * s
+        want = (1 - bl) * (f("eta_initlast", lp) + f("eta_delta", ip) + f("eta_deltalast", lp)) \
+            + bl * (f("eta_init", ip) + f("eta_delta", ip))
+        assert np.abs(out[j] - want).max() <= 1e-12, (j, out[j][0], want)
