@@ -79,6 +79,66 @@ def inner_bytes(n_theta, esz):
     return esz * 7 * n_theta * 2 * n_theta
 
 
+def frame_loop(n_theta, advection, frames=(3, 13), steps_per_frame=10):
+    """The soapBubble driver end to end (config.txt, init textures, the frame
+    loop of Bubble::run with its per-step broken-film check, one EXR frame a
+    frame), mixed precision, the reference's AMG settings (PCGF_F.json:
+    ABSOLUTE 1e-7). Two runs of different frame counts; their difference per
+    frame leaves out the process start (HIP initialisation, textures, context
+    set-up). Textures: smooth synthetic images written as the reference's
+    HALF B, G, R EXR."""
+    import shutil
+    import subprocess
+    import tempfile
+
+    import soapbubble_amd as sb
+
+    exe = os.path.join(ROOT, "soapbubble_amd", "soapBubble")
+    if not os.path.exists(exe):
+        return None
+    d = tempfile.mkdtemp(prefix="sf_frames_")
+    try:
+        h, w = 512, 1024
+        y, x = np.meshgrid(np.linspace(0, 1, h), np.linspace(0, 1, w), indexing="ij")
+        th = 0.5 + 0.2 * np.sin(2 * np.pi * x) * np.sin(np.pi * y) + 0.05 * np.cos(6 * np.pi * x)
+        sb.write_image_exr(os.path.join(d, "thick.exr"), th, th, th)
+        sb.write_image_exr(os.path.join(d, "air.exr"), 0.5 * np.cos(2 * np.pi * y),
+                           0.5 * np.sin(2 * np.pi * x) * np.sin(np.pi * y), 0 * x)
+        with open(os.path.join(d, "amg.json"), "w") as f:
+            f.write('{"config_version": 2, "solver": {"preconditioner": {"solver": "AMG", '
+                    '"max_iters": 1}, "solver": "PCGF", "max_iters": 100, "convergence": '
+                    '"ABSOLUTE", "tolerance": 1e-07, "norm": "L2"}}')
+        radius = 0.05
+        dt_s = 0.001 * radius            # a non-dimensional step of 0.001
+        el = {}
+        for fr in frames:
+            vals = dict(radius=radius, H=1e-6, U=1, cm=0.5, Gammam=1e-6, T=298.15, Ds=0.0,
+                        rm=5e-9, numberOfGridsInTheta=n_theta, timePerStep=dt_s,
+                        timePerFrame=steps_per_frame * dt_s, frameNumbers=fr,
+                        outputDir=os.path.join(d, f"out{fr}"),
+                        thicknessInit=os.path.join(d, "thick.exr"),
+                        AMGconfig=os.path.join(d, "amg.json"), blendCoeff=0.5, gpu=0)
+            cfg = os.path.join(d, f"config{fr}.txt")
+            with open(cfg, "w") as f:
+                f.write("".join(f"{k} {v}\n" for k, v in vals.items()))
+            t0 = time.perf_counter()
+            r = subprocess.run([exe, cfg, "--precision", "mixed", "--advection", advection,
+                                "--air", os.path.join(d, "air.exr")],
+                               capture_output=True, text=True, timeout=300)
+            el[fr] = time.perf_counter() - t0
+            if r.returncode != 0 or f"Frames completed: {fr}" not in r.stdout:
+                raise RuntimeError(f"soapBubble failed ({r.returncode}): {r.stdout} {r.stderr}")
+        a, b = frames
+        per = (el[b] - el[a]) / (b - a)
+        return {"workload": f"soapBubble {n_theta}x{2 * n_theta}, {advection} advection, "
+                            f"{steps_per_frame} steps of dt 0.001 a frame, EXR frames",
+                "frames": list(frames), "elapsed_s": [el[a], el[b]],
+                "ms_per_frame": per * 1e3, "ms_per_step": per * 1e3 / steps_per_frame,
+                "cell_steps_per_s": n_theta * 2 * n_theta * steps_per_frame / per}
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
 def ensemble(name, groups, steps, warmup, l_max, seed0):
     """A packed ensemble (one context, one launch per stage for all bubbles,
     graph replay): groups of (count, n_theta, dt). Per-bubble seed
@@ -145,6 +205,7 @@ def main():
     ap.add_argument("--no-viscous", action="store_true")
     ap.add_argument("--viscous-steps", type=int, default=20)
     ap.add_argument("--no-ensembles", action="store_true")
+    ap.add_argument("--no-frames", action="store_true")
     ap.add_argument("--bimocq-steps", type=int, default=10)
     ap.add_argument("--profile", choices=("graph", "eager", "inner"), default=None,
                     help="run only the headline steps (for rocprofv3), graph or eager launches; "
@@ -335,6 +396,12 @@ def main():
                      [(4, 2048, 0.0005), (64, 128, 0.004)], 10, args.warmup, 128, 4000),
         ]
 
+    # the soapBubble driver's frame loop, plain and BIMOCQ (the reference's
+    # default scalar advection)
+    frames = None
+    if not args.no_frames:
+        frames = [frame_loop(n, "plain"), frame_loop(n, "bimocq")]
+
     # HBM bytes per launch measured by rocprofv3 for this workload
     traffic = adv_traffic = None
     tpath = os.path.join(ROOT, "profiles", "hbm_traffic_1024x2048_mixed.json")
@@ -380,6 +447,7 @@ def main():
         "bimocq": bimocq,
         "viscous": viscous,
         "ensembles": ensembles,
+        "frame_loop": frames,
     }
     print(json.dumps(out))
 

@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <thread>
 
 namespace sfh {
 
@@ -362,21 +363,47 @@ bool write_image_exr(const std::string& path, int w, int hgt, const float* r, co
     putf(fw, (float)w);   // the image width, as the reference's writer stores it
     attr(hdr, "screenWindowWidth", "float", fw);
     hdr += '\0';
+    // offset table, then per scanline: y, byte count, the B, G, R halves
     const uint64_t block = 8 + (uint64_t)w * 3 * 2;
-    std::string out = hdr;
-    uint64_t pos = hdr.size() + 8ull * hgt;
+    const size_t head = hdr.size() + 8ull * hgt;
+    std::vector<char> out(head + block * (uint64_t)hgt);
+    std::memcpy(out.data(), hdr.data(), hdr.size());
     for (int y = 0; y < hgt; ++y) {
-        out.append(reinterpret_cast<const char*>(&pos), 8);
-        pos += block;
+        const uint64_t pos = head + block * (uint64_t)y;
+        std::memcpy(out.data() + hdr.size() + 8ull * y, &pos, 8);
     }
-    std::vector<uint16_t> line(w);
-    for (int y = 0; y < hgt; ++y) {
-        put32(out, y);
-        put32(out, w * 3 * 2);
-        for (const float* plane : {b, g, r}) {
-            for (int i = 0; i < w; ++i) line[i] = float_to_half(plane[(size_t)y * w + i]);
-            out.append(reinterpret_cast<const char*>(line.data()), 2 * (size_t)w);
+    // one plane given for all three channels (a gray image) is converted once
+    const bool gray = r == g && g == b;
+    auto rows = [&](int y0, int y1) {
+        for (int y = y0; y < y1; ++y) {
+            char* q = out.data() + head + block * (uint64_t)y;
+            const int32_t yy = y, nbytes = w * 3 * 2;
+            std::memcpy(q, &yy, 4);
+            std::memcpy(q + 4, &nbytes, 4);
+            uint16_t* hb = reinterpret_cast<uint16_t*>(q + 8);
+            const float* planes[3] = {b, g, r};
+            for (int k = 0; k < 3; ++k) {
+                uint16_t* dst = hb + (size_t)k * w;
+                if (gray && k > 0) {
+                    std::memcpy(dst, hb, 2 * (size_t)w);
+                    continue;
+                }
+                const float* src = planes[k] + (size_t)y * w;
+                for (int i = 0; i < w; ++i) dst[i] = float_to_half(src[i]);
+            }
         }
+    };
+    // large images: scanlines split over a few host threads
+    const int nt = (int)std::min<long>(
+        std::max(1u, std::min(8u, std::thread::hardware_concurrency())),
+        std::max<long>(1, (long)w * hgt / (1L << 18)));
+    if (nt <= 1) {
+        rows(0, hgt);
+    } else {
+        std::vector<std::thread> th;
+        for (int t = 0; t < nt; ++t)
+            th.emplace_back(rows, (int)((long)hgt * t / nt), (int)((long)hgt * (t + 1) / nt));
+        for (auto& x : th) x.join();
     }
     std::ofstream f(path, std::ios::binary);
     if (!f) return false;

@@ -212,3 +212,29 @@ def test_write_image_half_rounding(tmp_path):
     assert np.array_equal(got, _half_away(ties)), (got, _half_away(ties))
     # ties round away from zero where round-to-even would not
     assert got[0] == 0x3C01 and np.float16(ties[0]).view(np.uint16) == 0x3C00
+
+
+@pytest.mark.parametrize("gray", [True, False])
+def test_write_image_large_scanlines(tmp_path, gray):
+    """An image large enough that its scanlines are encoded by several host
+    threads (1201 rows: an uneven split), one plane for all three channels
+    (converted once) or three planes: every channel of every scanline holds
+    the ties-away halves of its plane, and the offset table points at each
+    scanline."""
+    h, w = 1201, 1024
+    rng = np.random.default_rng(7)
+    planes = [rng.uniform(-3, 3, (h, w)).astype(np.float32) for _ in range(3)]
+    if gray:
+        planes = [planes[0]] * 3
+    r, g, b = planes
+    p = str(tmp_path / "big.exr")
+    sb.write_image_exr(p, r, g, b)
+    for k, plane in enumerate((b, g, r)):
+        got = _raw_half_plane(p, w, h, k)
+        assert np.array_equal(got, _half_away(plane.ravel()).reshape(h, w)), k
+    data = open(p, "rb").read()
+    hdr_end = len(data) - h * (8 + 6 * w) - 8 * h
+    offs = np.frombuffer(data, "<u8", h, hdr_end)
+    assert np.array_equal(offs, hdr_end + 8 * h + np.arange(h) * (8 + 6 * w))
+    ys = [int(np.frombuffer(data, "<i4", 1, int(o))[0]) for o in offs[::97]]
+    assert ys == list(range(0, h, 97))
