@@ -36,7 +36,14 @@
  *  trace        zero velocity, uniform flow along the equator and a
  *               meridian (exact), O(h^2) convergence for solid-body rotation
  *  advect       zero-velocity identity, constants, the two grid reflections,
- *               a geodesic flow along a grid meridian (exact)
+ *               a geodesic flow along a grid meridian (exact); the
+ *               velocity transport (tests/test_oracle_transport.py): a
+ *               restatement through rotation matrices on random and
+ *               tilted-rotation fields (1e-12), the departure point of a
+ *               tilted solid-body rotation at third order in dt, and the
+ *               advected velocity against the sampled field transported
+ *               along the great circle from the exact departure point,
+ *               third order (a flipped frame cross product gives 2 or 1)
  *  airflow      closed form: 0.5 (3 - |t - 3|) taper(beta) (n x r), t = 0
  *  col_ind      the neighbour pattern
  *  assemble     symmetry, row sums, Cholesky, known answer, O(h^2) Y_lm
@@ -54,8 +61,10 @@
  *               equal to the plain step, reinitialisation bounds; on a
  *               meridian flow the closed forms bwd = x - k s, fwd = x + k s
  *               of k map updates, the change accumulated through the
- *               forward map and the BIMOCQ eta through both backward maps
- *               (linear fields: exact values)
+ *               forward map, the BIMOCQ eta through both backward maps,
+ *               and the error correction of the reinitialisation
+ *               (correctCentered1/2) through both maps (linear fields:
+ *               exact values)
  * Parity unpinned (reference text only): the u_theta sample exactly at the
  * north pole; floorf_mode (D2) and the float sinf/cosf of D1, which the
  * oracle does not reproduce.

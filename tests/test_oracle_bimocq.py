@@ -260,3 +260,70 @@ def test_bimocq_eta_through_shifted_maps(oracle):
         want = (1 - bl) * (f("eta_initlast", lp) + f("eta_delta", ip) + f("eta_deltalast", lp)) \
             + bl * (f("eta_init", ip) + f("eta_delta", ip))
         assert np.abs(out[j] - want).max() <= 1e-12, (j, out[j][0], want)
+
+
+def test_reinit_error_correction_through_shifted_maps(oracle):
+    """correctCentered1/2 (bubble.cu:2180-2280) through the maps of a
+    meridian flow (fwd(x) = x + 2 s, bwd(x) = x - 2 s): with the current
+    field c, the accumulated change delta and the initial field init linear
+    in theta, every sample is exact, so the error estimate is the closed form
+    e = (sum_q w_q c(fwd(x + d_q)) - delta - init) / 2 and the corrected
+    field is c - sum_q w_q e(bwd(x + d_q)) (the five points d_q: quarter-cell
+    diagonals of weight 1/8 and the centre, 1/2). The map lookups at the
+    quarter-cell points bow toward the pole by a few thousandths of a cell
+    (a great-circle arc between two points of one latitude); their values
+    come from sampleMapping, pinned by its own closed forms. init is chosen
+    as c(x + 2 s) - delta - 2 eps with eps of one sign and below a third of
+    c's row step,
+    so e is about eps, non-zero, and the correction stays inside the 3 x 3
+    clamp: an error estimate of the wrong sign (or a map swapped for the
+    other) moves the result by about 2 eps."""
+    n = _MER["n"]
+    nph = 2 * n
+    p, s, maps = _meridian_maps(oracle, 2)
+    bt, bp, ft, fp = maps[1]
+    rows = np.arange(n, dtype=float)
+    W = (0.125, 0.125, 0.125, 0.125, 0.5)
+    DX = (-0.25, 0.25, -0.25, 0.25, 0.0)
+    DY = (-0.25, -0.25, 0.25, 0.25, 0.0)
+    spec = {"eta": (0.02, 1.0, (0.004, 0.05), (0.003, 0.0001)),
+            "gamma": (-0.015, 1.2, (-0.002, 0.03), (-0.002, -0.00005))}
+    st = {}
+    ones = np.ones(n * nph)
+    bm = oracle.BimocqState(p, ones, ones)
+    bm.view("fwd_t")[:], bm.view("fwd_p")[:] = ft, fp
+    bm.view("bwd_t")[:], bm.view("bwd_p")[:] = bt, bp
+    for k, (a, b, (da, db), (ea, eb)) in spec.items():
+        c = a * rows + b
+        delta = da * rows + db
+        eps = ea + eb * rows
+        init = a * (rows + 2 * s) + b - delta - 2 * eps
+        st[k] = np.repeat(c, nph)
+        pre = "eta" if k == "eta" else "gam"
+        bm.view(pre + "_delta")[:] = np.repeat(delta, nph)
+        bm.view(pre + "_init")[:] = np.repeat(init, nph)
+    col = 5
+    fwd_rows = np.array([[oracle.sample_mapping(p, ft, fp, j + 0.5 + DX[q], col + DY[q])[0]
+                          for q in range(5)] for j in range(n)])
+    bwd_rows = np.array([[oracle.sample_mapping(p, bt, bp, j + 0.5 + DX[q], col + DY[q])[0]
+                          for q in range(5)] for j in range(n)])
+    want = {}
+    for k, (a, b, (da, db), (ea, eb)) in spec.items():
+        c = a * rows + b
+        delta = da * rows + db
+        init = a * (rows + 2 * s) + b - delta - 2 * (ea + eb * rows)
+        te = (np.asarray(W) * (a * (fwd_rows - 0.5) + b)).sum(axis=1)
+        e = 0.5 * (te - delta - init)
+        assert np.abs(e[4:n - 4]).min() > 1e-4          # the correction is not zero
+        corr = (np.asarray(W) * np.interp(bwd_rows - 0.5, rows, e)).sum(axis=1)
+        want[k] = c - corr
+        inner = slice(6, n - 6)
+        assert np.all(np.abs(corr[inner]) < 0.5 * abs(a))   # inside the 3 x 3 clamp
+    full = {"u_theta": np.zeros((n - 1) * nph), "u_phi": np.zeros(n * nph),
+            "eta": st["eta"], "gamma": st["gamma"]}
+    oracle.bimocq_reinit(p, full, bm)
+    for k in ("eta", "gamma"):
+        got = full[k].reshape(n, nph)
+        for j in range(6, n - 6):
+            # the reference clamps in float: the result is a float
+            assert np.abs(got[j] - want[k][j]).max() <= 2e-7 * abs(want[k][j]), (k, j)
