@@ -10,11 +10,14 @@ inner solve inside FP64 iterative refinement to a relative residual of
 One "step" = advection of u_theta, u_phi, eta, Gamma + assembly + implicit
 solve + updates. The metric is cell-steps/s over the timed steps.
 
-Also reported on the same line: the plain FP64 GPU path on the same input,
-the error of the mixed path against the FP64 path and against the oracle,
-the fraction of HBM peak of the inner-iteration kernel, the largest share of
-the step (roofline), and of the advection kernel (roofline_advect), and the
-FP64 CPU oracle's throughput on a bounded sample (cpu_baseline).
+Also reported on the same line: the plain FP64 GPU path on the same input
+(timed after the same warm-up), the error of the mixed path against the
+FP64 path and against the oracle after 100 steps, the fraction of HBM peak
+of the inner-iteration kernel, the largest share of the step (roofline),
+and of the advection kernel (roofline_advect), the FP64 CPU oracle's
+throughput over those 100 steps (cpu_baseline), and the packed ensembles of
+96 and 68 bubbles (the second with its error against the oracle per
+bubble).
 """
 import argparse
 import json
@@ -28,6 +31,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    """Progress of the benchmark's legs on stderr (the JSON line is stdout)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
 
 
 def rel_l2(a, b):
@@ -139,38 +147,89 @@ def frame_loop(n_theta, advection, frames=(3, 13), steps_per_frame=10):
         shutil.rmtree(d, ignore_errors=True)
 
 
-def ensemble(name, groups, steps, warmup, l_max, seed0):
-    """A packed ensemble (one context, one launch per stage for all bubbles,
-    graph replay): groups of (count, n_theta, dt). Per-bubble seed
-    seed0 + 16 b, air amplitude U[0.2, 1.0), Marangoni number M times
-    U[0.5, 2.0)."""
+def ensemble_specs(groups, l_max, seed0):
+    """Per-bubble descriptors and fields of a packed ensemble: groups of
+    (count, n_theta, dt); bubble b has seed seed0 + 16 b, air amplitude
+    U[0.2, 1.0) and Marangoni number M times U[0.5, 2.0)."""
     import soapbubble_amd as sb
 
     rng = np.random.default_rng(seed0)
-    descs, fields = [], []
+    specs = []
     b = 0
     for count, n, dt in groups:
         for _ in range(count):
             amp = rng.uniform(0.2, 1.0)
             ms = rng.uniform(0.5, 2.0)
             f = sb.synth_fields(n, seed0 + 16 * b, l_max, 12, 0.0, amp)
-            descs.append(sb.bubble(n, dt, M_scale=ms))
-            fields.append((n, f))
+            specs.append({"n": n, "dt": dt, "M_scale": ms, "fields": f})
             b += 1
-    cells = sum(n * 2 * n for n, _ in fields)
-    ctx = sb.SoapFilm(descs, "mixed", use_graph=True)
-    for k, (n, f) in enumerate(fields):
+    return specs
+
+
+def ensemble_ctx(specs):
+    import soapbubble_amd as sb
+
+    ctx = sb.SoapFilm([sb.bubble(q["n"], q["dt"], M_scale=q["M_scale"]) for q in specs], "mixed",
+                      use_graph=True)
+    for k, q in enumerate(specs):
+        n, f = q["n"], q["fields"]
         nph = 2 * n
         ctx.load_state(k, {"u_theta": np.zeros((n - 1) * nph), "u_phi": np.zeros(n * nph),
                            "eta": f["eta"], "gamma": f["gamma"]})
         ctx.upload(k, "uair0", f["uair0"])
         ctx.upload(k, "vair0", f["vair0"])
+    return ctx
+
+
+def ensemble(name, groups, steps, warmup, l_max, seed0, err_steps=0):
+    """A packed ensemble (one context, one launch per stage for all bubbles,
+    graph replay), timed over `steps` steps after `warmup`. With err_steps,
+    a second run of err_steps steps from the same state is compared bubble
+    by bubble with the FP64 oracle (relative L2 of every field)."""
+    specs = ensemble_specs(groups, l_max, seed0)
+    cells = sum(q["n"] * 2 * q["n"] for q in specs)
+    ctx = ensemble_ctx(specs)
     el = timed_run(ctx, warmup, steps)
-    fails = sum(ctx.stats(k)["failed_solves"] for k in range(len(descs)))
+    fails = sum(ctx.stats(k)["failed_solves"] for k in range(len(specs)))
     ctx.close()
-    return {"workload": name, "bubbles": len(descs), "cells": cells, "steps": steps,
-            "value": cells * steps / el, "unit": "cell-steps/s", "ms_per_step": el * 1e3 / steps,
-            "failed_solves": fails}
+    out = {"workload": name, "bubbles": len(specs), "cells": cells, "steps": steps,
+           "warmup": warmup, "value": cells * steps / el, "unit": "cell-steps/s",
+           "ms_per_step": el * 1e3 / steps, "failed_solves": fails}
+    if err_steps:
+        from oracle import oracle as O
+
+        ctx = ensemble_ctx(specs)
+        ctx.step(err_steps)
+        per = []
+        for k, q in enumerate(specs):
+            g = ctx.state(k)
+            f, n = q["fields"], q["n"]
+            nph = 2 * n
+            st = {"u_theta": np.zeros((n - 1) * nph), "u_phi": np.zeros(n * nph),
+                  "eta": f["eta"].copy(), "gamma": f["gamma"].copy()}
+            O.step(O.params(n, q["dt"], M_scale=q["M_scale"]), st, f["uair0"], f["vair0"],
+                   err_steps)
+            per.append({kk: rel_l2(g[kk], st[kk]) for kk in ("eta", "gamma", "u_theta", "u_phi")})
+        ctx.close()
+        worst = {kk: max(e[kk] for e in per) for kk in per[0]}
+        out["err_vs_oracle"] = {
+            "steps": err_steps, "worst": worst,
+            "per_bubble": [{"n_theta": q["n"], "eta": float(f"{e['eta']:.3g}"),
+                            "gamma": float(f"{e['gamma']:.3g}"),
+                            "u": float(f"{max(e['u_theta'], e['u_phi']):.3g}")}
+                           for q, e in zip(specs, per)]}
+    return out
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def copy_calibration(mib):
@@ -198,7 +257,10 @@ def main():
     ap.add_argument("--n-theta", type=int, default=1024)
     ap.add_argument("--dt", type=float, default=0.001)
     ap.add_argument("--fp64-steps", type=int, default=20)
-    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--oracle-steps", type=int, default=100,
+                    help="steps of the FP64 oracle (CPU baseline and the headline's error)")
+    ap.add_argument("--c4-err-steps", type=int, default=5,
+                    help="steps of the 68-bubble ensemble compared with the oracle per bubble")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-bimocq", action="store_true")
@@ -250,6 +312,7 @@ def main():
                           "ms_per_step": el * 1e3 / args.steps, "failed_solves": s["failed_solves"]}))
         return
 
+    log("headline: mixed, graph")
     # headline: mixed precision, hipGraph per step
     ctx = new_ctx("mixed", n, dt, st0, ua, va, True)
     elapsed = timed_run(ctx, args.warmup, args.steps)
@@ -260,6 +323,7 @@ def main():
     steps_total = args.warmup + args.steps
     ctx.close()
 
+    log("stage times, advection and inner launch timings")
     # per-stage device times of the same kernels, eager launches, from HIP
     # events on the library's stream
     ctx = new_ctx("mixed", n, dt, st0, ua, va, False)
@@ -290,22 +354,30 @@ def main():
     stage_ms = {k: round(tm[k] / nst, 4) for k in ("advect_ms", "assemble_ms", "solve_ms",
                                                      "update_ms", "step_ms")}
 
+    log("plain FP64 path")
     # plain FP64 path on the same input
     fp64 = None
     err_fp64 = None
     if not args.no_fp64:
+        # the same warm-up as the mixed leg, so both are timed with the
+        # GPU's clocks ramped
         ctx = new_ctx("fp64", n, dt, st0, ua, va, False)
-        k64 = min(args.fp64_steps, steps_total)
-        el64 = timed_run(ctx, 0, k64)
+        w64 = min(args.warmup, steps_total)
+        k64 = min(args.fp64_steps, steps_total - w64)
+        el64 = timed_run(ctx, w64, k64)
         fp64 = cells * k64 / el64
-        if k64 < steps_total:
-            ctx.step(steps_total - k64)
+        if w64 + k64 < steps_total:
+            ctx.step(steps_total - w64 - k64)
         ref_state = ctx.state(0)
         ctx.close()
         err_fp64 = {k: rel_l2(mixed_state[k], ref_state[k]) for k in ("eta", "gamma",
                                                                      "u_theta", "u_phi")}
 
-    # FP64 CPU oracle on a bounded sample of the same workload
+    log("oracle (CPU baseline) and the headline path after the same steps")
+    # FP64 CPU oracle over the workload's first `oracle_steps` steps (100:
+    # the acceptance point of the mixed path's error) from the same state:
+    # its time is the CPU baseline, its state the reference of the headline
+    # path (mixed, graph replay) after the same steps
     cpu = None
     err_oracle = None
     if not args.no_cpu:
@@ -313,25 +385,26 @@ def main():
 
         p = O.params(n, dt)
         st = {k: v.copy() for k, v in st0.items()}
+        k = args.oracle_steps
         t0 = time.perf_counter()
-        k = 0
-        while True:
-            O.step(p, st, ua, va, 1)
-            k += 1
-            if time.perf_counter() - t0 > args.cpu_seconds or k >= steps_total:
-                break
+        O.step(p, st, ua, va, k)
         el = time.perf_counter() - t0
         cpu = {"value": cells * k / el, "unit": "cell-steps/s", "cores": O.num_threads(),
-               "kind": "port",
-               "sample": f"{k} steps of the same {n}x{nph} workload, FP64 OpenMP oracle"}
-        ctx = new_ctx("mixed", n, dt, st0, ua, va, False)
+               "cpu_model": cpu_model(), "kind": "port",
+               "sample": f"the first {k} steps of the same {n}x{nph} workload, FP64 OpenMP oracle"}
+        ctx = new_ctx("mixed", n, dt, st0, ua, va, True)
         ctx.step(k)
         g = ctx.state(0)
+        gs = ctx.stats(0)
         ctx.close()
-        err_oracle = {"steps": k}
+        v0 = O.volume(p, st0["eta"])
+        err_oracle = {"steps": k, "path": "mixed, hipGraph replay"}
         err_oracle.update({kk: rel_l2(g[kk], st[kk]) for kk in ("eta", "gamma", "u_theta",
                                                                  "u_phi")})
+        err_oracle["volume_drift"] = {"gpu": gs["volume"] / v0 - 1.0,
+                                      "oracle": O.volume(p, st["eta"]) / v0 - 1.0}
 
+    log("BIMOCQ")
     # the reference's default scalar advection (BIMOCQ, five samples per
     # point, blendCoeff 0.5) on the same input: mixed with a graph, and the
     # FP64 path for the error
@@ -360,6 +433,7 @@ def main():
                   "err_vs_fp64_gpu": {k: rel_l2(gb[k], rb[k]) for k in ("eta", "gamma",
                                                                         "u_theta", "u_phi")}}
 
+    log("VISCOUS")
     # the VISCOUS option (gravity on, so the viscous forces enter the
     # assembly): mixed with a graph, and the FP64 path for the error
     viscous = None
@@ -385,17 +459,20 @@ def main():
                    "err_vs_fp64_gpu": {k: rel_l2(gv[k], rv[k]) for k in ("eta", "gamma",
                                                                          "u_theta", "u_phi")}}
 
+    log("ensembles")
     # packed ensembles of uneven resolution (batched form), mixed, graph
     ensembles = None
     if not args.no_ensembles:
         ensembles = [
             ensemble("32 x 64x128 (dt 0.005) + 32 x 128x256 (0.004) + 32 x 256x512 (0.002)",
-                     [(32, 64, 0.005), (32, 128, 0.004), (32, 256, 0.002)], 20, args.warmup,
+                     [(32, 64, 0.005), (32, 128, 0.004), (32, 256, 0.002)], 200, args.warmup,
                      16, 3000),
             ensemble("4 x 2048x4096 (dt 0.0005) + 64 x 128x256 (0.004)",
-                     [(4, 2048, 0.0005), (64, 128, 0.004)], 10, args.warmup, 128, 4000),
+                     [(4, 2048, 0.0005), (64, 128, 0.004)], 50, args.warmup, 128, 4000,
+                     err_steps=0 if args.no_cpu else args.c4_err_steps),
         ]
 
+    log("frame loop")
     # the soapBubble driver's frame loop, plain and BIMOCQ (the reference's
     # default scalar advection)
     frames = None

@@ -1124,8 +1124,18 @@ int sf_step(sf_ctx* ctx, int32_t nsteps) {
                     const bool adapt = c.method == 2 && c.opt.max_inner <= 0;
                     (void)step_graph(c, kGraphSteps, -1);
                     if (adapt) {
-                        (void)step_graph(c, kGraphSteps, 0);
-                        (void)step_graph(c, kGraphSteps, 1);
+                        // the graphs of the calibrated schedule and of one
+                        // inner iteration fewer (where the schedule usually
+                        // settles as the extrapolated guess improves), so
+                        // that no instantiation falls between replays
+                        const int n0 = c.sched[0];
+                        for (int n : {n0, n0 - 1}) {
+                            if (n < 1) continue;
+                            c.sched[0] = n;
+                            (void)step_graph(c, kGraphSteps, 0);
+                            (void)step_graph(c, kGraphSteps, 1);
+                        }
+                        c.sched[0] = n0;
                     }
                 }
             }

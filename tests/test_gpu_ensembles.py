@@ -66,3 +66,15 @@ def test_large_uneven_ensemble_per_bubble():
     _say("ensemble 1024 + 16 x 128")
     worst = _run(oracle, [(1, 1024, 0.001), (16, 128, 0.004)], 6, 128, 4000)
     assert worst < 5e-5
+
+
+def test_c4_batch_per_bubble():
+    """The large uneven batch itself: four 2048x4096 bubbles (l <= 128 modes,
+    dt 0.0005) and sixty-four 128x256 ones (dt 0.004) in one packed context,
+    3 steps (the oracle needs about a second a step for each large bubble on
+    16 host threads, so the batch's 50 steps are cut to 3 here); every
+    bubble's every field against its own oracle run."""
+    from oracle import oracle
+    _say("c4 batch 4 x 2048 + 64 x 128")
+    worst = _run(oracle, [(4, 2048, 0.0005), (64, 128, 0.004)], 3, 128, 4000)
+    assert worst < 5e-5
