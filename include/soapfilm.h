@@ -173,6 +173,34 @@ int sf_solve(sf_ctx* ctx);
 int sf_update(sf_ctx* ctx);
 
 int sf_synchronize(sf_ctx* ctx);
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under stream capture, tight serving loops, harnesses that
+ * forbid hipMalloc): nsteps steps of Solver::stepForward (plain scalar
+ * advection) of one bubble whose state already lives on the device, like
+ * sf_step on a one-bubble SF_PREC_MIXED context, without a context.
+ * Allocates nothing: every work array of the step is carved from a fixed
+ * 48 MiB device array inside the library (held by each GPU the library runs
+ * on until the process ends), so the bubble has at most 2^18 cells
+ * (n_theta <= 256 for the power-of-two grids); larger ones return
+ * SF_ERR_UNSUPPORTED. Fields are device arrays of float in the layout above
+ * (u_theta (nTheta-1) x nPhi, the others nTheta x nPhi; uair0 / vair0 the
+ * air flow at t = 0, as SF_FIELD_UAIR0 / SF_FIELD_VAIR0); outputs may alias
+ * the inputs. opt->precision must be SF_PREC_MIXED and scalar_advection 0;
+ * the solve runs a fixed schedule of opt->max_outer refinements (0: 2) of
+ * opt->max_inner inner iterations (0: 16) plus one spare refinement, and a
+ * bubble that has converged skips the rest; solves that miss the tolerance
+ * are counted in stats->failed_solves (stats may be NULL). t_seconds is the
+ * air-flow time at the first step. Work is queued on `stream` (a
+ * hipStream_t; NULL: the default stream) after what the caller queued there;
+ * the call returns when it has finished. The array is shared by every call
+ * on a GPU and keeps the last call's contents: calls are serialised by a
+ * lock inside the library, and nothing else may use the array. */
+int sf_step_noalloc(const sf_options* opt, const sf_bubble_desc* bubble, double t_seconds,
+                    int32_t nsteps, void* stream, const float* u_theta, const float* u_phi,
+                    const float* eta, const float* gamma, const float* uair0, const float* vair0,
+                    float* u_theta_out, float* u_phi_out, float* eta_out, float* gamma_out,
+                    sf_bubble_stats* stats);
+
 /* Average device time of one advection launch over nlaunch back-to-back
  * launches between two events on the context's stream (the launches
  * recompute the advected fields from the unchanged state; call at the start

@@ -38,7 +38,7 @@ EXPORTS = (
     "sf_set_dt", "sf_step", "sf_advect", "sf_assemble", "sf_solve", "sf_update",
     "sf_synchronize", "sf_time_advection", "sf_time_inner", "sf_get_stats", "sf_get_timing", "sf_reset_timing", "sf_last_error",
     "sf_derive_params", "sf_synth_fields", "sf_texture_fields", "sf_real_ylm",
-    "sf_write_thickness_exr", "sf_write_image_exr",
+    "sf_write_thickness_exr", "sf_write_image_exr", "sf_step_noalloc",
 )
 
 
@@ -130,6 +130,8 @@ def load_library(path: str = LIB_PATH):
         "sf_write_image_exr": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, ctypes.c_int32,
                                               P(ctypes.c_float), P(ctypes.c_float),
                                               P(ctypes.c_float)]),
+        "sf_step_noalloc": (ctypes.c_int, [P(Options), P(BubbleDesc), ctypes.c_double,
+                                           ctypes.c_int32, vp] + [vp] * 10 + [P(BubbleStats)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -230,6 +232,40 @@ class FieldShape:
         return self.n_theta * nph
 
 
+def options(precision: str = "fp64", *, device: int = 0, use_graph: bool = False,
+            tol: float = 1e-10, tol_absolute: bool = False, advance_time: bool = True,
+            max_outer: int = 0, max_inner: int = 0, inner_rho: float = 0.0,
+            inner_method: str = "auto", scalar_advection: str = "plain", blend: float = 0.5,
+            multisample: bool = True, reinit_threshold: float = 0.0,
+            initial_guess: str = "auto", viscous: bool = False) -> Options:
+    """The sf_options struct of include/soapfilm.h."""
+    return Options(PRECISIONS[precision], device, int(use_graph), int(tol_absolute), tol,
+                   max_outer, max_inner, inner_rho, int(advance_time),
+                   {"auto": 0, "cg": 1, "chebyshev": 2}[inner_method],
+                   {"plain": 0, "bimocq": 1}[scalar_advection], int(multisample), blend,
+                   reinit_threshold,
+                   {"auto": 0, "gamma_star": 1, "extrapolated": 2}[initial_guess],
+                   int(viscous))
+
+
+def step_noalloc(bubble: BubbleDesc, opt: Options, t_seconds: float, nsteps: int,
+                 fields_in: Sequence[int], air: Sequence[int], fields_out: Sequence[int],
+                 stream: int | None = None) -> dict:
+    """sf_step_noalloc: nsteps steps of one bubble on device-resident float
+    arrays, given as device addresses (u_theta, u_phi, eta, Gamma in and out;
+    uair0, vair0). The library allocates nothing. Returns the statistics."""
+    lib = load_library()
+    st = BubbleStats()
+    vp = ctypes.c_void_p
+    rc = lib.sf_step_noalloc(ctypes.byref(opt), ctypes.byref(bubble), float(t_seconds), nsteps,
+                             vp(stream) if stream else None, *[vp(int(a)) for a in fields_in],
+                             *[vp(int(a)) for a in air], *[vp(int(a)) for a in fields_out],
+                             ctypes.byref(st))
+    if rc:
+        raise SoapFilmError(rc, lib.sf_last_error(None).decode())
+    return {k: getattr(st, k) for k, _ in BubbleStats._fields_ if k != "pad_"}
+
+
 class SoapFilm:
     """A context: the packed state of one or more bubbles on one GPU."""
 
@@ -245,13 +281,12 @@ class SoapFilm:
             bubbles = [bubbles]
         self.bubbles = list(bubbles)
         arr = (BubbleDesc * len(self.bubbles))(*self.bubbles)
-        opt = Options(PRECISIONS[precision], device, int(use_graph), int(tol_absolute), tol,
-                      max_outer, max_inner, inner_rho, int(advance_time),
-                      {"auto": 0, "cg": 1, "chebyshev": 2}[inner_method],
-                      {"plain": 0, "bimocq": 1}[scalar_advection], int(multisample), blend,
-                      reinit_threshold,
-                      {"auto": 0, "gamma_star": 1, "extrapolated": 2}[initial_guess],
-                      int(viscous))
+        opt = options(precision, device=device, use_graph=use_graph, tol=tol,
+                      tol_absolute=tol_absolute, advance_time=advance_time, max_outer=max_outer,
+                      max_inner=max_inner, inner_rho=inner_rho, inner_method=inner_method,
+                      scalar_advection=scalar_advection, blend=blend, multisample=multisample,
+                      reinit_threshold=reinit_threshold, initial_guess=initial_guess,
+                      viscous=viscous)
         h = ctypes.c_void_p()
         rc = self._lib.sf_create(ctypes.byref(opt), len(self.bubbles), arr, ctypes.byref(h))
         if rc:

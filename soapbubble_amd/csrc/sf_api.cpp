@@ -21,6 +21,7 @@
 #include "../../include/soapfilm.h"
 #include "sf_launch.h"
 #include "sf_host.h"
+#include "sf_internal.h"
 
 using namespace sfk;
 
@@ -46,10 +47,38 @@ struct Status {
 
 constexpr double kPi = 3.14159265358979323846;
 
+// A caller-provided device region that the set-up carves its arrays from
+// instead of allocating them (the entry point without allocation): a bump
+// pointer with 256-byte alignment, each array cleared on the caller's stream.
+struct Arena {
+    char* base = nullptr;
+    size_t size = 0, used = 0;
+    hipStream_t stream = nullptr;
+};
+thread_local Arena* g_arena = nullptr;
+
+struct ArenaScope {
+    explicit ArenaScope(Arena* a) { g_arena = a; }
+    ~ArenaScope() { g_arena = nullptr; }
+};
+
 template <typename X>
 X* dalloc(size_t n, std::vector<void*>& owned) {
     void* p = nullptr;
     if (n == 0) n = 1;
+    if (g_arena) {
+        const size_t at = (g_arena->used + 255) & ~(size_t)255, bytes = n * sizeof(X);
+        if (at + bytes > g_arena->size)
+            throw Status{SF_ERR_UNSUPPORTED, "the work arrays need more than the " +
+                                                 std::to_string(g_arena->size >> 20) +
+                                                 " MiB of the fixed device array"};
+        p = g_arena->base + at;
+        g_arena->used = at + bytes;
+        // cleared and finished before the set-up's synchronous copies write into it
+        SF_HIP(hipMemsetAsync(p, 0, bytes, g_arena->stream));
+        SF_HIP(hipStreamSynchronize(g_arena->stream));
+        return static_cast<X*>(p);
+    }
     hipError_t e = hipMalloc(&p, n * sizeof(X));
     if (e != hipSuccess) throw Status{SF_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e)};
     owned.push_back(p);
@@ -120,6 +149,9 @@ struct Base {
     hipEvent_t ev[6] = {};
     sf_timing timing{};
     std::string err;
+    // set up inside a caller's device array on the caller's stream: no
+    // allocation, no stream, event, pinned buffer or graph of its own
+    bool noalloc = false;
     virtual ~Base() {
         if (stream) (void)hipStreamSynchronize(stream);
         for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
@@ -130,7 +162,7 @@ struct Base {
         if (sstat_h) (void)hipHostFree(sstat_h);
         if (cstream) (void)hipStreamDestroy(cstream);
         for (void* p : owned) (void)hipFree(p);
-        if (stream) (void)hipStreamDestroy(stream);
+        if (stream && !noalloc) (void)hipStreamDestroy(stream);
     }
     virtual void advect() = 0;
     virtual void assemble() = 0;
@@ -202,8 +234,9 @@ struct Impl : Base {
             vfv = dalloc<double>(n, owned);
         }
         // reference initWithConst (solver.cu:106-110): u = 0, eta = 0.5, Gamma = 1
+        // (the entry point without allocation copies the caller's state in)
         std::vector<double> h;
-        for (int b = 0; b < nb; ++b) {
+        for (int b = 0; b < nb && !noalloc; ++b) {
             const size_t N = (size_t)bub_h[b].nth * bub_h[b].nph;
             h.assign(N, 0.5);
             upload(b, SF_FIELD_ETA, h.data());
@@ -633,8 +666,10 @@ void finish_setup(Impl<T, Ti>& c) {
 void setup(Base& c, int prec) {
     const int nb = c.nb;
     SF_HIP(hipSetDevice(c.device));
-    SF_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
-    for (auto& e : c.ev) SF_HIP(hipEventCreate(&e));
+    if (!c.noalloc) {
+        SF_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+        for (auto& e : c.ev) SF_HIP(hipEventCreate(&e));
+    }
     hipDeviceProp_t prop;
     SF_HIP(hipGetDeviceProperties(&prop, c.device));
 
@@ -686,10 +721,12 @@ void setup(Base& c, int prec) {
     c.ctl_d = dalloc<SolveCtl>(nb, c.owned);
     c.hist_d = dalloc<SolveHist>(nb, c.owned);
     c.sstat_d = dalloc<SchedStat>(nb, c.owned);
-    SF_HIP(hipHostMalloc((void**)&c.sstat_h, 2 * nb * sizeof(SchedStat)));
-    std::memset(c.sstat_h, 0, 2 * nb * sizeof(SchedStat));
-    for (auto& e : c.cev) SF_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    SF_HIP(hipStreamCreateWithFlags(&c.cstream, hipStreamNonBlocking));
+    if (!c.noalloc) {
+        SF_HIP(hipHostMalloc((void**)&c.sstat_h, 2 * nb * sizeof(SchedStat)));
+        std::memset(c.sstat_h, 0, 2 * nb * sizeof(SchedStat));
+        for (auto& e : c.cev) SF_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        SF_HIP(hipStreamCreateWithFlags(&c.cstream, hipStreamNonBlocking));
+    }
     build_tables(c, c.owned);
 
     // cell tiles of 256 threads: one row chunk of 256, or several short rows
@@ -943,20 +980,50 @@ void check_index(sf_ctx* ctx, int b) {
     if (b < 0 || b >= ctx->impl->nb) throw Status{SF_ERR_INVALID, "bubble index out of range"};
 }
 
-}  // namespace
-
-extern "C" {
-
-int sf_derive_params(const sf_bubble_desc* desc, sf_derived* out) {
-    if (!desc || !out) return fail(nullptr, SF_ERR_INVALID, "null argument");
-    sfh::derive_params(*desc, *out);
-    return SF_OK;
+// Solver statistics of bubble b (sf_get_stats): the row statistics launch,
+// then the control blocks and partial sums read back.
+void fill_stats(Base& c, int b, sf_bubble_stats* out) {
+    c.row_stats();
+    const BubbleDev& B = c.bub_h[b];
+    std::vector<double> ps(2 * (size_t)c.stride);
+    SF_HIP(hipMemcpyAsync(ps.data(), c.partS, ps.size() * sizeof(double), hipMemcpyDeviceToHost,
+                          c.stream));
+    SolveCtl sc;
+    SF_HIP(hipMemcpyAsync(&sc, c.ctl_d + b, sizeof(sc), hipMemcpyDeviceToHost, c.stream));
+    SolveHist hs;
+    SF_HIP(hipMemcpyAsync(&hs, c.hist_d + b, sizeof(hs), hipMemcpyDeviceToHost, c.stream));
+    BubbleTime bt;
+    SF_HIP(hipMemcpyAsync(&bt, c.time_d + b, sizeof(bt), hipMemcpyDeviceToHost, c.stream));
+    SF_HIP(hipStreamSynchronize(c.stream));
+    double vol = 0.0, pos = 0.0;
+    for (int j = 0; j < B.nth; ++j) {
+        vol += ps[B.pr0 + j];
+        pos += ps[c.stride + B.pr0 + j];
+    }
+    std::memset(out, 0, sizeof(*out));
+    out->iterations = sc.iters;
+    out->outer = sc.outers;
+    out->converged = sc.converged;
+    out->residual = sc.rel_res;
+    out->volume = vol * B.h * B.h;
+    out->broken = (pos == 0.0) || (c.steps_done > 0 && !sc.converged);
+    out->steps = c.steps_done;
+    out->time = bt.t + (bt.pending ? bt.dt_s : 0.0);
+    out->failed_solves = hs.fails;
+    out->max_residual = hs.max_res;
+    if (c.bimocq) {
+        double dl = 0.0;
+        int ri = 0;
+        SF_HIP(hipMemcpy(&dl, c.bm.dist_last + b, sizeof(double), hipMemcpyDeviceToHost));
+        SF_HIP(hipMemcpy(&ri, c.bm.reinits + b, sizeof(int), hipMemcpyDeviceToHost));
+        out->distortion = dl;
+        out->reinits = ri;
+    }
 }
 
-int sf_create(const sf_options* opt, int32_t nbubbles, const sf_bubble_desc* bubbles,
-              sf_ctx** out) {
-    if (!out) return fail(nullptr, SF_ERR_INVALID, "null output pointer");
-    *out = nullptr;
+// The argument checks of sf_create (a bubble's size and physical
+// parameters, the options), shared by the entry point without allocation.
+int validate_create(const sf_options* opt, int32_t nbubbles, const sf_bubble_desc* bubbles) {
     if (!opt || !bubbles || nbubbles <= 0) return fail(nullptr, SF_ERR_INVALID, "no bubbles");
     for (int b = 0; b < nbubbles; ++b) {
         const sf_bubble_desc& d = bubbles[b];
@@ -979,6 +1046,123 @@ int sf_create(const sf_options* opt, int32_t nbubbles, const sf_bubble_desc* bub
         return fail(nullptr, SF_ERR_INVALID, "scalar_advection must be 0 (plain) or 1 (BIMOCQ)");
     if (opt->max_outer > kAccRefine) return fail(nullptr, SF_ERR_INVALID, "max_outer is at most 16");
     if (opt->viscous < 0 || opt->viscous > 1) return fail(nullptr, SF_ERR_INVALID, "viscous must be 0 or 1");
+    return SF_OK;
+}
+
+}  // namespace
+
+namespace sfapi {
+
+int record_error(int code, const char* msg) { return fail(nullptr, code, msg); }
+
+// Bubbles up to this many cells: the work arrays of one at 2^18 cells take
+// about 38 MiB of the fixed array (mixed precision, VISCOUS on).
+constexpr long long kNoAllocMaxCells = 1LL << 18;
+
+int check_noalloc(const sf_options* opt, const sf_bubble_desc* bubble, int nsteps,
+                  const float* const in[4], const float* uair0, const float* vair0,
+                  float* const out[4]) {
+    const int rc0 = validate_create(opt, 1, bubble);
+    if (rc0 != SF_OK) return rc0;
+    return guarded(nullptr, [&] {
+        if (opt->precision != SF_PREC_MIXED)
+            throw Status{SF_ERR_UNSUPPORTED, "sf_step_noalloc takes SF_PREC_MIXED (FP32 fields)"};
+        if (opt->scalar_advection != 0)
+            throw Status{SF_ERR_UNSUPPORTED, "sf_step_noalloc runs the plain scalar advection"};
+        if (nsteps < 0) throw Status{SF_ERR_INVALID, "negative step count"};
+        for (int k = 0; k < 4; ++k)
+            if (!in[k] || !out[k]) throw Status{SF_ERR_INVALID, "null field pointer"};
+        if (!uair0 || !vair0) throw Status{SF_ERR_INVALID, "null air-flow pointer"};
+        const long long cells = 2LL * bubble->n_theta * bubble->n_theta;
+        if (cells > kNoAllocMaxCells)
+            throw Status{SF_ERR_UNSUPPORTED,
+                         "sf_step_noalloc takes bubbles of at most " + std::to_string(kNoAllocMaxCells) +
+                             " cells (n_theta " + std::to_string(bubble->n_theta) + " has " +
+                             std::to_string(cells) + ")"};
+        return (int)SF_OK;
+    });
+}
+
+int step_in_scratch(void* scratch, size_t bytes, const sf_options* opt,
+                    const sf_bubble_desc* bubble, double t_seconds, int nsteps,
+                    hipStream_t stream, const float* const in[4], const float* uair0,
+                    const float* vair0, float* const out[4], sf_bubble_stats* stats) {
+    const int rc0 = check_noalloc(opt, bubble, nsteps, in, uair0, vair0, out);
+    if (rc0 != SF_OK) return rc0;
+    return guarded(nullptr, [&] {
+        std::unique_ptr<Impl<float, float>> impl(new Impl<float, float>());
+        Impl<float, float>& c = *impl;
+        c.opt = *opt;
+        c.device = opt->device;
+        c.nb = 1;
+        c.desc.assign(bubble, bubble + 1);
+        c.noalloc = true;
+        c.stream = stream;
+        {
+            Arena a;
+            a.base = static_cast<char*>(scratch);
+            a.size = bytes;
+            a.stream = stream;
+            ArenaScope scope(&a);
+            setup(c, SF_PREC_MIXED);
+        }
+        const BubbleDev& B = c.bub_h[0];
+        const size_t N = (size_t)B.nth * B.nph, Nt = N - B.nph;
+        c.time_h[0].t = t_seconds;
+        SF_HIP(hipMemcpy(c.time_d, c.time_h.data(), sizeof(BubbleTime), hipMemcpyHostToDevice));
+        // the caller's state and air flow into the work arrays (the u_theta
+        // array keeps its zero padding row)
+        SF_HIP(hipMemcpyAsync(c.cur.uth, in[0], Nt * sizeof(float), hipMemcpyDeviceToDevice, stream));
+        SF_HIP(hipMemcpyAsync(c.cur.uph, in[1], N * sizeof(float), hipMemcpyDeviceToDevice, stream));
+        SF_HIP(hipMemcpyAsync(c.cur.eta, in[2], N * sizeof(float), hipMemcpyDeviceToDevice, stream));
+        SF_HIP(hipMemcpyAsync(c.cur.gam, in[3], N * sizeof(float), hipMemcpyDeviceToDevice, stream));
+        SF_HIP(hipMemcpyAsync(c.ua0, uair0, N * sizeof(float), hipMemcpyDeviceToDevice, stream));
+        SF_HIP(hipMemcpyAsync(c.va0, vair0, Nt * sizeof(float), hipMemcpyDeviceToDevice, stream));
+        launch_nan_scan<float>(stream, c.geom, c.bub_d, c.time_d, c.cur);
+        // a fixed schedule (no host round trip inside the steps): max_outer
+        // refinements of max_inner inner iterations, then the spare
+        // refinement whose correction is the last line solve; a bubble that
+        // has converged skips the remaining kernels
+        const int K = opt->max_inner > 0 ? opt->max_inner : 16;
+        const int R = opt->max_outer > 0 ? std::min(opt->max_outer, kAccRefine - 1) : 2;
+        c.sched.assign((size_t)R, K);
+        c.sched.push_back(0);
+        for (int k = 0; k < nsteps; ++k) {
+            c.advect();
+            c.assemble();
+            c.solve(false, false);
+            c.update(true);
+        }
+        c.steps_done = nsteps;
+        SF_HIP(hipMemcpyAsync(out[0], c.cur.uth, Nt * sizeof(float), hipMemcpyDeviceToDevice, stream));
+        SF_HIP(hipMemcpyAsync(out[1], c.cur.uph, N * sizeof(float), hipMemcpyDeviceToDevice, stream));
+        SF_HIP(hipMemcpyAsync(out[2], c.cur.eta, N * sizeof(float), hipMemcpyDeviceToDevice, stream));
+        SF_HIP(hipMemcpyAsync(out[3], c.cur.gam, N * sizeof(float), hipMemcpyDeviceToDevice, stream));
+        if (stats) fill_stats(c, 0, stats);
+        SF_HIP(hipStreamSynchronize(stream));
+        return (int)SF_OK;
+    });
+}
+
+}  // namespace sfapi
+
+namespace {
+}  // namespace
+
+extern "C" {
+
+int sf_derive_params(const sf_bubble_desc* desc, sf_derived* out) {
+    if (!desc || !out) return fail(nullptr, SF_ERR_INVALID, "null argument");
+    sfh::derive_params(*desc, *out);
+    return SF_OK;
+}
+
+int sf_create(const sf_options* opt, int32_t nbubbles, const sf_bubble_desc* bubbles,
+              sf_ctx** out) {
+    if (!out) return fail(nullptr, SF_ERR_INVALID, "null output pointer");
+    *out = nullptr;
+    const int rc0 = validate_create(opt, nbubbles, bubbles);
+    if (rc0 != SF_OK) return rc0;
     auto ctx = std::unique_ptr<sf_ctx>(new sf_ctx());
     ctx->prec = opt->precision;
     ctx->impl = make_impl(opt->precision);
@@ -1299,43 +1483,7 @@ int sf_get_stats(sf_ctx* ctx, int32_t b, sf_bubble_stats* out) {
     return guarded(ctx, [&] {
         check_index(ctx, b);
         if (!out) throw Status{SF_ERR_INVALID, "null output"};
-        Base& c = *ctx->impl;
-        c.row_stats();
-        const BubbleDev& B = c.bub_h[b];
-        std::vector<double> ps(2 * (size_t)c.stride);
-        SF_HIP(hipMemcpyAsync(ps.data(), c.partS, ps.size() * sizeof(double), hipMemcpyDeviceToHost,
-                              c.stream));
-        SolveCtl sc;
-        SF_HIP(hipMemcpyAsync(&sc, c.ctl_d + b, sizeof(sc), hipMemcpyDeviceToHost, c.stream));
-        SolveHist hs;
-        SF_HIP(hipMemcpyAsync(&hs, c.hist_d + b, sizeof(hs), hipMemcpyDeviceToHost, c.stream));
-        BubbleTime bt;
-        SF_HIP(hipMemcpyAsync(&bt, c.time_d + b, sizeof(bt), hipMemcpyDeviceToHost, c.stream));
-        SF_HIP(hipStreamSynchronize(c.stream));
-        double vol = 0.0, pos = 0.0;
-        for (int j = 0; j < B.nth; ++j) {
-            vol += ps[B.pr0 + j];
-            pos += ps[c.stride + B.pr0 + j];
-        }
-        std::memset(out, 0, sizeof(*out));
-        out->iterations = sc.iters;
-        out->outer = sc.outers;
-        out->converged = sc.converged;
-        out->residual = sc.rel_res;
-        out->volume = vol * B.h * B.h;
-        out->broken = (pos == 0.0) || (c.steps_done > 0 && !sc.converged);
-        out->steps = c.steps_done;
-        out->time = bt.t + (bt.pending ? bt.dt_s : 0.0);
-        out->failed_solves = hs.fails;
-        out->max_residual = hs.max_res;
-        if (c.bimocq) {
-            double dl = 0.0;
-            int ri = 0;
-            SF_HIP(hipMemcpy(&dl, c.bm.dist_last + b, sizeof(double), hipMemcpyDeviceToHost));
-            SF_HIP(hipMemcpy(&ri, c.bm.reinits + b, sizeof(int), hipMemcpyDeviceToHost));
-            out->distortion = dl;
-            out->reinits = ri;
-        }
+        fill_stats(*ctx->impl, b, out);
         return (int)SF_OK;
     });
 }

@@ -1490,6 +1490,25 @@ __global__ __launch_bounds__(128) void k_update(const int4* __restrict__ tiles,
     if (nan_out) btime[tl.x].nans = 1;
 }
 
+// Marks a bubble whose state fields hold a NaN (the sticky flag that selects
+// the NaN-skipping samplers), for state that reached the device without
+// passing through the host upload's scan.
+template <typename T>
+__global__ __launch_bounds__(256) void k_nan_scan(const int4* __restrict__ tiles,
+                                                  const BubbleDev* __restrict__ bub,
+                                                  BubbleTime* __restrict__ btime, StatePtrs<T> st) {
+    const int4 tl = tiles[blockIdx.x];
+    if (tl.x < 0) return;
+    const BubbleDev B = bub[tl.x];
+    int j, i;
+    tile_cell(tl, j, i);
+    if (j >= B.nth || i >= B.nph) return;
+    const size_t c = (size_t)B.off + (size_t)j * B.nph + i;
+    bool nan = tisnan(st.uph[c]) || tisnan(st.eta[c]) || tisnan(st.gam[c]);
+    if (j < B.nth - 1) nan = nan || tisnan(st.uth[c]);
+    if (nan) btime[tl.x].nans = 1;
+}
+
 // Per-row film volume (sum eta sin(theta)) and count of cells with eta > 0.
 template <typename T>
 __global__ void k_row_stats(const int2* __restrict__ rows, const BubbleDev* __restrict__ bub,
@@ -1974,6 +1993,12 @@ void launch_update(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, Bub
 }
 
 template <typename T>
+void launch_nan_scan(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, BubbleTime* bt,
+                     StatePtrs<T> st) {
+    hipLaunchKernelGGL(k_nan_scan<T>, dim3(g.ntiles), dim3(256), 0, s, g.tiles, bub, bt, st);
+}
+
+template <typename T>
 void launch_row_stats(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, const Tables& tab,
                       const T* eta, double* out, long stride) {
     hipLaunchKernelGGL(k_row_stats<T>, dim3(g.nrows), dim3(256), 0, s, g.rows, bub, tab, eta, out,
@@ -2020,6 +2045,10 @@ template void launch_update<double>(hipStream_t, const LaunchGeom&, const Bubble
                                     const double*, const double*, StatePtrs<double>, int,
                                     SolveHist*, const SolveParams&, const double*, long, double*,
                                     double*, double*, double*, double*);
+template void launch_nan_scan<float>(hipStream_t, const LaunchGeom&, const BubbleDev*,
+                                     BubbleTime*, StatePtrs<float>);
+template void launch_nan_scan<double>(hipStream_t, const LaunchGeom&, const BubbleDev*,
+                                      BubbleTime*, StatePtrs<double>);
 template void launch_row_stats<float>(hipStream_t, const LaunchGeom&, const BubbleDev*,
                                       const Tables&, const float*, double*, long);
 template void launch_row_stats<double>(hipStream_t, const LaunchGeom&, const BubbleDev*,

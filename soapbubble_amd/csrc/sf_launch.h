@@ -135,6 +135,10 @@ void launch_bimocq_accumulate(hipStream_t s, const LaunchGeom& g, const BubbleDe
 template <typename T>
 void launch_bimocq_reinit(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub,
                           StatePtrs<T> cur, const BimocqPtrs& P);
+// sets the per-bubble NaN flag of bubbles whose state holds a NaN
+template <typename T>
+void launch_nan_scan(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, BubbleTime* bt,
+                     StatePtrs<T> st);
 template <typename T>
 void launch_row_stats(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, const Tables& tab,
                       const T* eta, double* out, long stride);

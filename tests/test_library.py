@@ -177,3 +177,29 @@ def test_create_rejects_bad_options(kw, msg):
     assert msg in lib.sf_last_error(None).decode()
     with pytest.raises(KeyError):
         sb.SoapFilm(sb.bubble(16, 0.01), "mixed", initial_guess="previous")
+
+
+@pytest.mark.parametrize("n_theta,kw,code,msg", [
+    (512, {}, 3, "at most 262144 cells"),
+    (64, {"precision": "fp64"}, 3, "SF_PREC_MIXED"),
+    (64, {"scalar_advection": "bimocq"}, 3, "plain scalar advection"),
+    (12, {}, 3, "power of two"),
+])
+def test_step_noalloc_rejects_before_device_work(n_theta, kw, code, msg):
+    """The entry point without allocation checks its sizes and options before
+    it touches a device (no GPU needed): bubbles above 2^18 cells, other
+    precisions than mixed, BIMOCQ."""
+    opt = sb.options(kw.get("precision", "mixed"),
+                     scalar_advection=kw.get("scalar_advection", "plain"))
+    with pytest.raises(sb.SoapFilmError) as e:
+        sb.step_noalloc(sb.bubble(n_theta, 0.01), opt, 0.0, 1, [1, 1, 1, 1], [1, 1],
+                        [1, 1, 1, 1])
+    assert e.value.args[0].startswith(sb.STATUS[code]), e.value.args
+    assert msg in e.value.args[0]
+
+
+def test_step_noalloc_rejects_null_fields():
+    with pytest.raises(sb.SoapFilmError) as e:
+        sb.step_noalloc(sb.bubble(16, 0.01), sb.options("mixed"), 0.0, 1, [1, 0, 1, 1], [1, 1],
+                        [1, 1, 1, 1])
+    assert "null field pointer" in e.value.args[0]
