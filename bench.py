@@ -232,6 +232,54 @@ def cpu_model():
     return None
 
 
+def stiff_leg(warmup, steps, err_steps, with_oracle):
+    """The reference's own config.txt regime: radius 0.05 m, 256 x 512,
+    timePerStep 0.005 s (a non-dimensional step of 0.1), where the
+    line-Jacobi preconditioned operator has a spectral bound of 0.991; mixed
+    precision with the geometric-multigrid preconditioned inner PCG (graph
+    replay) and with the default Chebyshev inner solve (whose schedule is too
+    long to capture: host-driven steps), ms per step of both, and the
+    multigrid path's error against the oracle after err_steps steps."""
+    import soapbubble_amd as sb
+
+    n, dt, radius = 256, 0.005, 0.05
+    f = sb.synth_fields(n, 2100, 16, 12, 0.0, 1.0)
+    nph = 2 * n
+    st = {"u_theta": np.zeros((n - 1) * nph), "u_phi": np.zeros(n * nph),
+          "eta": f["eta"], "gamma": f["gamma"]}
+    out = {"workload": f"one bubble {n}x{nph}, radius {radius} m, dt {dt} s (config.txt), mixed",
+           "steps": steps, "warmup": warmup}
+    for name, method in (("multigrid", "mg"), ("chebyshev", "auto")):
+        ctx = sb.SoapFilm(sb.bubble(n, dt, radius=radius), "mixed", use_graph=True,
+                          inner_method=method)
+        ctx.load_state(0, st)
+        ctx.upload(0, "uair0", f["uair0"])
+        ctx.upload(0, "vair0", f["vair0"])
+        el = timed_run(ctx, warmup, steps)
+        s = ctx.stats(0)
+        ctx.close()
+        out[name] = {"ms_per_step": el * 1e3 / steps, "cell_steps_per_s": n * nph * steps / el,
+                     "last_inner_iterations": s["iterations"], "last_refinements": s["outer"],
+                     "failed_solves": s["failed_solves"], "max_residual": s["max_residual"]}
+    if with_oracle and err_steps:
+        from oracle import oracle as O
+
+        ctx = sb.SoapFilm(sb.bubble(n, dt, radius=radius), "mixed", use_graph=True,
+                          inner_method="mg")
+        ctx.load_state(0, st)
+        ctx.upload(0, "uair0", f["uair0"])
+        ctx.upload(0, "vair0", f["vair0"])
+        ctx.step(err_steps)
+        g = ctx.state(0)
+        ctx.close()
+        ref = {k: v.copy() for k, v in st.items()}
+        O.step(O.params(n, dt, radius=radius), ref, f["uair0"], f["vair0"], err_steps)
+        out["multigrid"]["err_vs_oracle"] = {"steps": err_steps}
+        out["multigrid"]["err_vs_oracle"].update(
+            {k: rel_l2(g[k], ref[k]) for k in ("eta", "gamma", "u_theta", "u_phi")})
+    return out
+
+
 def copy_calibration(mib):
     """A plain device copy of known size: its FETCH_SIZE / WRITE_SIZE counter
     readings against the bytes it must move give the counters' scale."""
@@ -268,6 +316,7 @@ def main():
     ap.add_argument("--viscous-steps", type=int, default=20)
     ap.add_argument("--no-ensembles", action="store_true")
     ap.add_argument("--no-frames", action="store_true")
+    ap.add_argument("--no-stiff", action="store_true")
     ap.add_argument("--bimocq-steps", type=int, default=10)
     ap.add_argument("--profile", choices=("graph", "eager", "inner"), default=None,
                     help="run only the headline steps (for rocprofv3), graph or eager launches; "
@@ -473,6 +522,12 @@ def main():
         ]
 
     log("frame loop")
+    # the stiff regime of the reference's config.txt: multigrid and Chebyshev
+    stiff = None
+    if not args.no_stiff:
+        log("stiff config.txt regime: multigrid and Chebyshev")
+        stiff = stiff_leg(10, 20, 5, not args.no_cpu)
+
     # the soapBubble driver's frame loop, plain and BIMOCQ (the reference's
     # default scalar advection)
     frames = None
@@ -524,6 +579,7 @@ def main():
         "bimocq": bimocq,
         "viscous": viscous,
         "ensembles": ensembles,
+        "stiff": stiff,
         "frame_loop": frames,
     }
     print(json.dumps(out))

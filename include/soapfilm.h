@@ -67,7 +67,9 @@ typedef enum {
  * main.cu:20-36) that reach the step, plus the switches of header.cuh:86-98
  * as run-time flags. */
 typedef struct {
-    int32_t n_theta;      /* numberOfGridsInTheta; nPhi = 2 n_theta; a power of two, 4..8192 */
+    int32_t n_theta;      /* numberOfGridsInTheta; nPhi = 2 n_theta; 4..8192, the row
+                             splitting into at most 1024 equal parts of 2, 4, 8
+                             or 16 cells (every n_theta up to 1024)          */
     int32_t gravity;      /* header.cuh:94 `gravity`                            */
     int32_t evaporation;  /* header.cuh:95 `evaporation`                        */
     int32_t airflow;      /* header.cuh:96 `airflow`                            */
@@ -96,7 +98,10 @@ typedef struct {
     int32_t max_inner;    /* inner PCG iterations per refinement; 0 -> automatic  */
     double inner_rho;     /* inner residual reduction; 0 -> automatic             */
     int32_t advance_time; /* 1: air-flow time advances by dt every step           */
-    int32_t inner_method; /* 0 automatic, 1 PCG, 2 Chebyshev (inner solve)        */
+    int32_t inner_method; /* inner solve: 0 automatic, 1 PCG with the phi-line
+                             Jacobi preconditioner, 2 Chebyshev, 3 PCG with the
+                             geometric-multigrid preconditioner (phi-line
+                             zebra relaxation, semi-coarsening in theta)     */
     int32_t scalar_advection; /* 0 plain (advectionAllCentered), 1 BIMOCQ for eta
                                  (header.cuh:97 BIMOCQ, bubble.cu:1188-1205)      */
     int32_t multisample;  /* BIMOCQ: 1 five samples per point (header.cuh:98)     */

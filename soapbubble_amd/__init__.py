@@ -241,7 +241,7 @@ def options(precision: str = "fp64", *, device: int = 0, use_graph: bool = False
     """The sf_options struct of include/soapfilm.h."""
     return Options(PRECISIONS[precision], device, int(use_graph), int(tol_absolute), tol,
                    max_outer, max_inner, inner_rho, int(advance_time),
-                   {"auto": 0, "cg": 1, "chebyshev": 2}[inner_method],
+                   {"auto": 0, "cg": 1, "chebyshev": 2, "mg": 3}[inner_method],
                    {"plain": 0, "bimocq": 1}[scalar_advection], int(multisample), blend,
                    reinit_threshold,
                    {"auto": 0, "gamma_star": 1, "extrapolated": 2}[initial_guess],

@@ -86,6 +86,10 @@ X* dalloc(size_t n, std::vector<void*>& owned) {
     return static_cast<X*>(p);
 }
 
+// The XCD-aware order of the work items of a launch (defined below).
+template <typename X>
+std::vector<X> xcd_order(const std::vector<X>& work, X pad, int nxcd);
+
 // Everything that does not depend on the precision.
 struct Base {
     sf_options opt{};
@@ -115,6 +119,7 @@ struct Base {
     LaunchGeom geom{};            // cell tiles
     std::vector<LaunchGeom> rgeo; // row blocks, one geometry per row width
     std::vector<LaunchGeom> rgeo_inner;   // the same rows for the Chebyshev launches
+    std::vector<int> rgeo_nph;            // the row width of each entry of rgeo
     double* partR = nullptr;
     double* partA = nullptr;
     double* partB = nullptr;
@@ -143,6 +148,7 @@ struct Base {
     std::vector<int> calib;   // max inner iterations per refinement seen while calibrating
     std::vector<int> sched;   // inner iterations per refinement of the fixed schedule
     int method = 1;           // inner solver: 1 PCG, 2 Chebyshev
+    bool mg = false;          // PCG preconditioned by the multigrid V-cycle
     bool trace_solve = std::getenv("SF_SOLVE_TRACE") != nullptr;
     int steps_done = 0;
     int stage = 0;            // 0 idle, 1 advected, 2 assembled, 3 solved
@@ -197,6 +203,18 @@ struct Impl : Base {
     T *gdel = nullptr, *gnext = nullptr;   // last solve correction, next guess
 
     double *vdiv = nullptr, *vfu = nullptr, *vfv = nullptr;   // VISCOUS div, forces
+
+    // multigrid preconditioner: the arrays of every level (level 0 is the
+    // assembled operator itself, its iterate z) and, per level and row
+    // width, the row lists of the half-sweeps: even / odd rows of bubbles
+    // with a coarser level, even / odd rows of bubbles whose coarsest level
+    // this is, all rows (coarse operator)
+    struct MgLaunch {
+        LaunchGeom even{}, odd{}, ceven{}, codd{}, all{};
+    };
+    std::vector<std::vector<MgLaunch>> mgl;
+    std::vector<MgPtrs<Ti>> mgp;
+    int mg_levels = 0;
     AirPtrs<T> air() const { return AirPtrs<T>{ua0, va0, au, av, vfu, vfv}; }
     SysPtrs<Ti> sys(double* x_init) const {
         return SysPtrs<Ti>{b64, x_init, wN, wW, dg, e, r, z, Ap, rsteps, acc};
@@ -243,6 +261,123 @@ struct Impl : Base {
             h.assign(N, 1.0);
             upload(b, SF_FIELD_GAMMA, h.data());
         }
+    }
+
+    void alloc_mg() {
+        // rows per level of each bubble: halved while the count is even
+        std::vector<std::vector<int>> nl(nb);
+        for (int b = 0; b < nb; ++b) {
+            int n = bub_h[b].nth;
+            nl[b].push_back(n);
+            while (n % 2 == 0 && n > 1) {
+                n /= 2;
+                nl[b].push_back(n);
+            }
+            mg_levels = std::max(mg_levels, (int)nl[b].size());
+        }
+        mgp.assign(mg_levels, MgPtrs<Ti>{});
+        mgl.assign(mg_levels, std::vector<MgLaunch>(rgeo.size()));
+        for (int l = 0; l < mg_levels; ++l) {
+            std::vector<MgLevelDev> lv(nb, MgLevelDev{0, 0, 0});
+            long long tot = 0;
+            for (int b = 0; b < nb; ++b) {
+                if (l >= (int)nl[b].size()) continue;
+                lv[b].n = nl[b][l];
+                lv[b].off = l == 0 ? bub_h[b].off : tot;
+                tot += ((long long)nl[b][l] * bub_h[b].nph + 63) / 64 * 64;
+            }
+            MgLevelDev* lv_d = dalloc<MgLevelDev>(nb, owned);
+            SF_HIP(hipMemcpy(lv_d, lv.data(), nb * sizeof(MgLevelDev), hipMemcpyHostToDevice));
+            MgPtrs<Ti>& P = mgp[l];
+            P.lv = lv_d;
+            if (l == 0) {
+                P.wN = wN; P.wW = wW; P.dg = dg; P.x = z;
+            } else {
+                const size_t n = (size_t)tot;
+                P.wN = dalloc<Ti>(n, owned);
+                P.wW = dalloc<Ti>(n, owned);
+                P.dg = dalloc<Ti>(n, owned);
+                P.x = dalloc<Ti>(n, owned);
+                P.U = dalloc<Ti>(n, owned);
+                P.D = dalloc<Ti>(n, owned);
+            }
+            for (size_t w = 0; w < rgeo.size(); ++w) {
+                std::vector<int2> lists[5];
+                for (int b = 0; b < nb; ++b) {
+                    if (bub_h[b].nph != rgeo_nph[w] || l >= (int)nl[b].size()) continue;
+                    const bool coarsest = l == (int)nl[b].size() - 1;
+                    for (int j = 0; j < nl[b][l]; ++j) {
+                        lists[(coarsest ? 2 : 0) + (j & 1)].push_back(make_int2(b, j));
+                        lists[4].push_back(make_int2(b, j));
+                    }
+                }
+                LaunchGeom* gs[5] = {&mgl[l][w].even, &mgl[l][w].odd, &mgl[l][w].ceven,
+                                     &mgl[l][w].codd, &mgl[l][w].all};
+                for (int k = 0; k < 5; ++k) {
+                    *gs[k] = rgeo[w];
+                    gs[k]->nrows = 0;
+                    gs[k]->rows = nullptr;
+                    if (lists[k].empty() || (k == 4 && l == 0)) continue;
+                    std::vector<int2> rows = xcd_order(lists[k], make_int2(-1, 0), nxcd);
+                    int2* rows_d = dalloc<int2>(rows.size(), owned);
+                    SF_HIP(hipMemcpy(rows_d, rows.data(), rows.size() * sizeof(int2),
+                                     hipMemcpyHostToDevice));
+                    gs[k]->rows = rows_d;
+                    gs[k]->nrows = (int)rows.size();
+                }
+            }
+        }
+    }
+
+    // the coarse operators of levels 1.. from the assembled one
+    void mg_coarsen() {
+        for (int l = 1; l < mg_levels; ++l)
+            for (const MgLaunch& q : mgl[l])
+                if (q.all.nrows) launch_mg_coarsen<Ti>(stream, q.all, bub_d, mgp[l - 1], mgp[l]);
+    }
+
+    // z <- one V(1,1) cycle applied to r (the even rows of z already hold
+    // the line-Jacobi solve of r, the cycle's first half-sweep). `start`: at
+    // the start of a refinement (its first launch ends a solve whose FP64
+    // residual already meets the tolerance); else inside the inner PCG
+    // (launches return once the inner solve is done). The r.z partials of
+    // level 0 go to rz_out.
+    void mg_vcycle(bool start, double* rz_out) {
+        const int nw = (int)rgeo.size();
+        std::vector<char> first((size_t)nw, 1);
+        auto line = [&](int l, int w, const LaunchGeom& g, int flags) {
+            if (g.nrows == 0) return;
+            if (start && first[(size_t)w]) flags |= kMgGateStart;
+            if (!start) flags |= kMgGateInner;
+            first[(size_t)w] = 0;
+            const MgPtrs<Ti> co = l + 1 < mg_levels ? mgp[l + 1] : MgPtrs<Ti>{};
+            launch_mg_line<Ti>(stream, g, bub_d, ctl_d, mgp[l], co, r, l, flags, rsteps, sp, partR,
+                               rz_out, stride);
+        };
+        for (int l = 0; l < mg_levels; ++l)
+            for (int w = 0; w < nw; ++w) {
+                const MgLaunch& q = mgl[l][w];
+                const int rz = l == 0 ? kMgRz : 0;
+                if (l > 0) line(l, w, q.even, kMgZeroNb);
+                line(l, w, q.odd, kMgRestrict);
+                // the coarsest level of some bubbles: one row is solved
+                // exactly; more rows take symmetric zebra passes (even, odd,
+                // even, odd, even) from zero
+                if (l > 0) line(l, w, q.ceven, kMgZeroNb | (q.codd.nrows ? 0 : rz));
+                if (q.codd.nrows) {
+                    line(l, w, q.codd, 0);
+                    line(l, w, q.ceven, 0);
+                    line(l, w, q.codd, rz);
+                    line(l, w, q.ceven, rz);
+                }
+            }
+        for (int l = mg_levels - 2; l >= 0; --l)
+            for (int w = 0; w < nw; ++w) {
+                const MgLaunch& q = mgl[l][w];
+                const int rz = l == 0 ? kMgRz : 0;
+                line(l, w, q.odd, kMgProlong | rz);
+                line(l, w, q.even, rz);
+            }
     }
 
     T state_gmax(double gmax) const {
@@ -302,6 +437,10 @@ struct Impl : Base {
             launch_assemble<T, Ti>(stream, rg, bub_d, time_d, tab, nxt, air(), sys(x64[0]),
                                    (double)state_gmax(bub_h[0].gamma_max), ctl_d, partR, stride,
                                    gnext, sp);
+        if (mg) {
+            mg_coarsen();
+            mg_vcycle(true, partR + 2 * stride);
+        }
     }
 
     // FP64 residual evaluation k >= 1: x64[k&1] = x64[(k-1)&1] + ecorr (k = 1:
@@ -319,6 +458,7 @@ struct Impl : Base {
             launch_resid<T, Ti>(stream, rg, bub_d, time_d, tab, nxt, ctl_d, s, gnext, xo, xn, k,
                                 partR, stride, partB, sp, method == 2 ? sh : nullptr,
                                 check_prev ? 1 : 0);
+        if (mg && !last) mg_vcycle(true, partR + 2 * stride);
     }
     // Chebyshev iterates: e_1 is z; e_k for k >= 2 rotates over e, Ap, p[0]
     // (k mod 3), so e_{k-1}, e_k and e_{k+1} are distinct.
@@ -386,9 +526,16 @@ struct Impl : Base {
         for (const LaunchGeom& rg : rgeo)
             launch_spmv<Ti>(stream, rg, bub_d, ctl_d, sys(x64[0]), p[(it + 1) & 1], p[it & 1], it, sp,
                             partR, partB, partA, stride);
+        cgstep(it);
+    }
+    // x, r update and z = M^-1 r of inner PCG iteration `it` (with the
+    // multigrid preconditioner: the line-Jacobi z is the V-cycle's first
+    // half-sweep)
+    void cgstep(int it) {
         for (const LaunchGeom& rg : rgeo)
             launch_cgstep<Ti>(stream, rg, bub_d, ctl_d, sys(x64[0]), p[it & 1], it, partA, partB,
                               stride);
+        if (mg) mg_vcycle(false, partB);
     }
 
     bool all_flags(std::vector<SolveCtl>& c, bool inner_flag) {
@@ -454,12 +601,10 @@ struct Impl : Base {
                                  c[0].rel_res, c[0].bb, done ? " converged" : "");
                 if (done) break;
                 for (int b = 0; b < nb; ++b) before[b] = c[b].iters;
-                for (const LaunchGeom& rg : rgeo)
-                    launch_cgstep<Ti>(stream, rg, bub_d, ctl_d, sys(x64[0]), p[0], 0, partA, partB,
-                                      stride);
+                cgstep(0);
                 for (int it = 1; it < max_inner; ++it) {
                     inner(it);
-                    if ((it & 3) == 3 && all_flags(c, true)) break;
+                    if ((mg || (it & 3) == 3) && all_flags(c, true)) break;
                 }
                 all_flags(c, true);
                 int mx = 0;
@@ -575,6 +720,30 @@ struct Impl : Base {
 // ------------------------------------------------------------------ setup --
 bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
 
+// Cells per thread of the row blocks of a row of nph cells: C in {2, 4, 8,
+// 16} dividing nph, so that the row's nph / C threads (at least 4, at most
+// kRowThreadsMax) each hold C consecutive cells. The first choice is 2 for
+// rows of at most 128 cells, else 4, doubled until the row fits one block;
+// a row that no doubling fits (nph / C not a whole number) takes the next
+// smaller C that divides it. 0: no C fits.
+int row_cpt(int nph) {
+    const int pref = nph <= 128 ? 2 : 4;
+    int cands[8], n = 0;
+    for (int c = pref; c <= 16; c *= 2) cands[n++] = c;
+    for (int c = pref / 2; c >= 2; c /= 2) cands[n++] = c;
+    for (int k = 0; k < n; ++k) {
+        const int c = cands[k];
+        if (nph % c == 0 && nph / c <= kRowThreadsMax && nph / c >= 4) return c;
+    }
+    return 0;
+}
+
+int next_pow2(int v) {
+    int p = 1;
+    while (p < v) p *= 2;
+    return p;
+}
+
 // Workgroup k of a launch is dispatched to XCD k mod nxcd, and each XCD has
 // its own L2. Work item l (a row or a tile, in spatial order) is therefore
 // placed at launch slot (l mod chunk) * nxcd + l / chunk with chunk =
@@ -661,6 +830,7 @@ void finish_setup(Impl<T, Ti>& c) {
     c.alloc_fields();
     upload_air_shapes(c);
     if (c.bimocq) c.alloc_bimocq();
+    if (c.mg) c.alloc_mg();
 }
 
 void setup(Base& c, int prec) {
@@ -685,7 +855,8 @@ void setup(Base& c, int prec) {
         BubbleDev& B = c.bub_h[b];
         B.nth = d.n_theta;
         B.nph = 2 * d.n_theta;
-        B.mask = B.nph - 1;
+        // power-of-two rows wrap phi by a mask; other rows by a comparison
+        B.mask = is_pow2(B.nph) ? B.nph - 1 : -1;
         B.rt0 = rt;
         B.ct0 = ct;
         B.pr0 = pr;
@@ -703,7 +874,9 @@ void setup(Base& c, int prec) {
         B.airflow = d.airflow;
         B.viscous = c.opt.viscous ? 1 : 0;
         B.re = c.der[b].re;
-        off += (long long)B.nth * B.nph;
+        // every bubble starts on a 256-byte boundary (the row kernels load
+        // C consecutive cells as one aligned vector)
+        off += ((long long)B.nth * B.nph + 63) / 64 * 64;
         rt += B.nth + 1;
         ct += B.nph;
         pr += B.nth;
@@ -733,7 +906,7 @@ void setup(Base& c, int prec) {
     std::vector<int4> tiles;
     for (int b = 0; b < nb; ++b) {
         const BubbleDev& B = c.bub_h[b];
-        const int w = std::min(256, B.nph), rpt = 256 / w;
+        const int w = std::min(256, next_pow2(B.nph)), rpt = 256 / w;
         for (int j = 0; j < B.nth; j += rpt)
             for (int i = 0; i < B.nph; i += w) tiles.push_back(make_int4(b, j, i, w));
     }
@@ -747,6 +920,8 @@ void setup(Base& c, int prec) {
     SF_HIP(hipMemcpy(tiles_d, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice));
     c.geom.tiles = tiles_d;
     c.geom.ntiles = (int)tiles.size();
+    c.geom.pow2_rows = true;
+    for (int b = 0; b < nb; ++b) c.geom.pow2_rows = c.geom.pow2_rows && c.bub_h[b].mask >= 0;
     // Row blocks, one launch per row width: C consecutive cells per thread
     // (4, or 2 for rows of at most 128 cells, doubled until the row needs at
     // most kRowThreadsMax threads), one thread per C cells; a row holds at
@@ -760,6 +935,7 @@ void setup(Base& c, int prec) {
     widths.erase(std::unique(widths.begin(), widths.end()), widths.end());
     c.rgeo.clear();
     c.rgeo_inner.clear();
+    c.rgeo_nph.clear();
     for (int nph : widths) {
         std::vector<int2> rows;
         for (int b = 0; b < nb; ++b)
@@ -768,16 +944,19 @@ void setup(Base& c, int prec) {
         rows = xcd_order(rows, make_int2(-1, 0), nxcd);
         int2* rows_d = dalloc<int2>(rows.size(), c.owned);
         SF_HIP(hipMemcpy(rows_d, rows.data(), rows.size() * sizeof(int2), hipMemcpyHostToDevice));
-        int cpt = nph <= 128 ? 2 : 4;
-        while (nph / cpt > kRowThreadsMax && cpt < 16) cpt *= 2;
-        if (nph < 4 * cpt || nph / cpt > kRowThreadsMax)
+        const int cpt = row_cpt(nph);
+        if (cpt == 0)
             throw Status{SF_ERR_UNSUPPORTED, "rows of " + std::to_string(nph) + " cells are not supported"};
         LaunchGeom g = c.geom;
         g.rows = rows_d;
         g.nrows = (int)rows.size();
         g.cpt = cpt;
         g.row_threads = std::max(64, (nph / cpt + 63) / 64 * 64);
-        g.lds_bytes = 6 * (size_t)(nph / cpt) * esz;
+        // the reduced system of R = nph / C interface unknowns: 6 R values
+        // (8 R when R is no power of two: the correction's second right-hand
+        // side)
+        auto lds_of = [&](int R) { return (size_t)(is_pow2(R) ? 6 : 8) * (size_t)R * esz; };
+        g.lds_bytes = lds_of(nph / cpt);
         if (g.lds_bytes > lds_limit) {
             throw Status{SF_ERR_UNSUPPORTED,
                          "line solve of nPhi = " + std::to_string(nph) + " needs " +
@@ -785,16 +964,17 @@ void setup(Base& c, int prec) {
                              std::to_string(lds_limit) + " (use SF_PREC_MIXED for this size)"};
         }
         c.rgeo.push_back(g);
+        c.rgeo_nph.push_back(nph);
         // The Chebyshev iterations (FP32 inner solve, rows of at least 2048
         // cells) hold 8 cells a thread: half the threads and barriers of
         // the line solve per row, and their 86 registers still fit every
         // row block of a CU at once. The reduction depth recorded at 4 cells
         // a thread is one level deeper than 8 cells a thread need.
         LaunchGeom gi = g;
-        if (!c.fp64_inner() && cpt == 4 && nph / 8 >= 256) {
+        if (!c.fp64_inner() && cpt == 4 && nph % 8 == 0 && nph / 8 >= 256) {
             gi.cpt = 8;
-            gi.row_threads = nph / 8;
-            gi.lds_bytes = 6 * (size_t)(nph / 8) * esz;
+            gi.row_threads = std::max(64, (nph / 8 + 63) / 64 * 64);
+            gi.lds_bytes = lds_of(nph / 8);
         }
         c.rgeo_inner.push_back(gi);
     }
@@ -813,6 +993,11 @@ void setup(Base& c, int prec) {
     // solver schedule; the inner method: PCG for an FP64 inner solve (the
     // plain path), Chebyshev for FP32 unless asked otherwise
     c.method = c.opt.inner_method > 0 ? c.opt.inner_method : (c.fp64_inner() ? 1 : 2);
+    if (c.method == 3) {
+        // PCG with the multigrid preconditioner
+        c.method = 1;
+        c.mg = true;
+    }
     if (c.method != 1 && c.method != 2) throw Status{SF_ERR_INVALID, "unknown inner method"};
     c.sp.tol = c.opt.tol > 0 ? c.opt.tol : 1e-10;
     c.sp.absolute = c.opt.tol_absolute;
@@ -1028,11 +1213,13 @@ int validate_create(const sf_options* opt, int32_t nbubbles, const sf_bubble_des
     for (int b = 0; b < nbubbles; ++b) {
         const sf_bubble_desc& d = bubbles[b];
         // a row of 2 n_theta cells is one block of at most kRowThreadsMax
-        // threads holding at most 16 cells each
-        if (d.n_theta < 4 || !is_pow2(d.n_theta) || 2 * d.n_theta > 16 * kRowThreadsMax)
+        // threads holding 2, 4, 8 or 16 cells each
+        if (d.n_theta < 4 || d.n_theta > 8 * kRowThreadsMax || row_cpt(2 * d.n_theta) == 0)
             return fail(nullptr, SF_ERR_UNSUPPORTED,
-                        "n_theta must be a power of two from 4 to " + std::to_string(8 * kRowThreadsMax) +
-                            " (bubble " + std::to_string(b) + ")");
+                        "n_theta must be from 4 to " + std::to_string(8 * kRowThreadsMax) +
+                            ", with its row of 2 n_theta cells splitting into at most " +
+                            std::to_string(kRowThreadsMax) +
+                            " equal parts of 2, 4, 8 or 16 cells (bubble " + std::to_string(b) + ")");
         if (!(d.radius > 0) || !(d.H > 0) || !(d.U > 0) || !(d.Gamma_m > 0) || !(d.T > 0) ||
             !(d.dt > 0) || !(d.M_scale > 0))
             return fail(nullptr, SF_ERR_INVALID, "non-positive physical parameter");

@@ -15,7 +15,7 @@
 namespace sfk {
 
 struct BubbleDev {
-    int nth, nph, mask;          // nph = 2 nth, mask = nph - 1 (nph power of two)
+    int nth, nph, mask;          // nph = 2 nth; mask = nph - 1 when nph is a power of two, else -1
     int rt0, ct0, pr0;           // row-table, column-table and partial-sum bases
     long long off;               // first cell of this bubble in every cell field
     double h, invh, dt;          // grid spacing, 1/h, non-dimensional step
@@ -226,29 +226,48 @@ __device__ __forceinline__ bool validate_coord(T& x, T& y, int nth, int nph) {
     return ret;
 }
 
+// A phi column x in [-nph, 2 nph) wrapped into the row: one AND for a row
+// whose length is a power of two (mask = nph - 1); otherwise (mask = -1) a
+// conditional add or subtract, clamped into the row so that no column,
+// however malformed, addresses memory outside it.
+__device__ __forceinline__ int wrap_col(int x, int nph, int mask) {
+    if (mask >= 0) return x & mask;
+    const int r = x < 0 ? x + nph : (x >= nph ? x - nph : x);
+    return min(max(r, 0), nph - 1);
+}
+
 // Cell (j, i) of thread threadIdx.x in a packed tile {bubble, first row,
-// first column, width}; tile widths are powers of two.
+// first column, width}; tile widths are powers of two (a row shorter than
+// its tile leaves the tile's last columns idle).
 __device__ __forceinline__ void tile_cell(const int4& tl, int& j, int& i) {
     const int sh = __builtin_ctz((unsigned)tl.w);
     j = tl.y + ((int)threadIdx.x >> sh);
     i = tl.z + ((int)threadIdx.x & (tl.w - 1));
 }
 
-template <typename T>
+// A field of one bubble. P2 = true: the row length is known to be a power
+// of two (mask = nph - 1), so the phi wrap is one AND; otherwise wrap_col
+// takes the row's mask at run time.
+template <typename T, bool P2 = false>
 struct Field {
     const T* __restrict__ p;   // bubble base
     int nph, mask, maxrow;
+    __device__ __forceinline__ int col(int c) const {
+        if constexpr (P2) return c & mask;
+        else return wrap_col(c, nph, mask);
+    }
+    __device__ __forceinline__ bool pow2() const { return P2 || mask >= 0; }
     // rows are clamped into the field so that no position, however
     // malformed, addresses memory outside it
-    __device__ __forceinline__ T at(int row, int col) const {
+    __device__ __forceinline__ T at(int row, int c) const {
         row = min(max(row, 0), maxrow);
-        return p[(unsigned)(row * nph + (col & mask))];
+        return p[(unsigned)(row * nph + col(c))];
     }
     template <bool NANS>
     __device__ __forceinline__ T belt(int row, int lo, T a) const {
         row = min(max(row, 0), maxrow);
         const T* r = p + (unsigned)(row * nph);
-        return lerp_s<T, NANS>(r[lo & mask], r[(lo + 1) & mask], a);
+        return lerp_s<T, NANS>(r[col(lo)], r[col(lo + 1)], a);
     }
     __device__ __forceinline__ T belt(int row, int lo, T a) const { return belt<true>(row, lo, a); }
 };
@@ -277,8 +296,9 @@ __device__ __forceinline__ float bload(__amdgpu_buffer_rsrc_t r, int off, int so
 }
 
 // x, y: sampler-relative index coordinates, x in [0, rows - 1). The phi
-// wrap acts on the integer column (nph is a power of two), so the weight
-// y - floor(y) keeps every bit of y, also for y < 0.
+// wrap acts on the integer column (the fast path serves rows whose length
+// is a power of two), so the weight y - floor(y) keeps every bit of y, also
+// for y < 0.
 __device__ __forceinline__ FastIdx fast_index(float x, float y, int nph, int mask) {
     const float fx = floorf(x), fy = floorf(y);
     FastIdx q;
@@ -320,10 +340,10 @@ __device__ __forceinline__ void split_index(T x, T y, int nph, int& tI, int& pI,
 }
 
 // sampleCentered (bubble.cu:320-376)
-template <typename T, bool NANS = true>
-__device__ T sample_centered(const Field<T>& f, int nth, T rx, T ry) {
+template <typename T, bool NANS = true, bool P2 = false>
+__device__ T sample_centered(const Field<T, P2>& f, int nth, T rx, T ry) {
     if constexpr (FastPath<T, NANS>::on) {
-        if (rx >= 0.5f && rx < (float)nth - 0.5f)
+        if (rx >= 0.5f && rx < (float)nth - 0.5f && f.pow2())
             return fast_bilerp(field_rsrc(f.p, nth, f.nph), fast_index(rx - 0.5f, ry, f.nph, f.mask),
                                f.nph * 4);
     }
@@ -331,13 +351,13 @@ __device__ T sample_centered(const Field<T>& f, int nth, T rx, T ry) {
         // FP64 fields without NaNs, away from the polar rows: the bilinear
         // interpolation of rows floor(x) and floor(x) + 1 that the general
         // branch below reduces to, the phi wrap on the integer column
-        if (rx >= 0.5 && rx < (double)nth - 0.5) {
+        if (rx >= 0.5 && rx < (double)nth - 0.5 && f.pow2()) {
             const T x = rx - T(0.5), fx = tfloor(x), fy = tfloor(ry);
             const T aT = x - fx, aP = ry - fy;
             const int pI = (int)fy;
             const T* r0 = f.p + (size_t)(int)fx * f.nph;
             const T* r1 = r0 + f.nph;
-            const int c0 = pI & f.mask, c1 = (pI + 1) & f.mask;
+            const int c0 = f.col(pI), c1 = f.col(pI + 1);
             const T lower = (T(1) - aP) * r0[c0] + aP * r0[c1];
             const T higher = (T(1) - aP) * r1[c0] + aP * r1[c1];
             return (T(1) - aT) * lower + aT * higher;
@@ -366,11 +386,11 @@ __device__ T sample_centered(const Field<T>& f, int nth, T rx, T ry) {
 
 // sampleCentered of two fields at one point (the eta and Gamma samples of
 // advectionAllCentered, bubble.cu:957-960) sharing the index computation.
-template <typename T, bool NANS = true>
-__device__ void sample_centered2(const Field<T>& f, const Field<T>& g, int nth, T rx, T ry, T& of,
+template <typename T, bool NANS = true, bool P2 = false>
+__device__ void sample_centered2(const Field<T, P2>& f, const Field<T, P2>& g, int nth, T rx, T ry, T& of,
                                  T& og) {
     if constexpr (FastPath<T, NANS>::on) {
-        if (rx >= 0.5f && rx < (float)nth - 0.5f) {
+        if (rx >= 0.5f && rx < (float)nth - 0.5f && f.pow2()) {
             const FastIdx q = fast_index(rx - 0.5f, ry, f.nph, f.mask);
             of = fast_bilerp(field_rsrc(f.p, nth, f.nph), q, f.nph * 4);
             og = fast_bilerp(field_rsrc(g.p, nth, g.nph), q, g.nph * 4);
@@ -395,10 +415,10 @@ __device__ void sample_centered2(const Field<T>& f, const Field<T>& g, int nth, 
 }
 
 // sampleVPhi (bubble.cu:178-240): values taken across the pole flip sign.
-template <typename T, bool NANS = true>
-__device__ T sample_vphi(const Field<T>& f, int nth, T rx, T ry) {
+template <typename T, bool NANS = true, bool P2 = false>
+__device__ T sample_vphi(const Field<T, P2>& f, int nth, T rx, T ry) {
     if constexpr (FastPath<T, NANS>::on) {
-        if (rx >= 0.5f && rx < (float)nth - 0.5f)
+        if (rx >= 0.5f && rx < (float)nth - 0.5f && f.pow2())
             return fast_bilerp(field_rsrc(f.p, nth, f.nph),
                                fast_index(rx - 0.5f, ry + 0.5f, f.nph, f.mask), f.nph * 4);
     }
@@ -429,10 +449,10 @@ __device__ T sample_vphi(const Field<T>& f, int nth, T rx, T ry) {
 // interpolate between the first/last row and its phi+pi mirror with half
 // weight. A sample exactly at theta = pi (which the reference reads past
 // the array for) takes the south-cap branch.
-template <typename T, bool NANS = true>
-__device__ T sample_vtheta(const Field<T>& f, int nth, T rx, T ry) {
+template <typename T, bool NANS = true, bool P2 = false>
+__device__ T sample_vtheta(const Field<T, P2>& f, int nth, T rx, T ry) {
     if constexpr (FastPath<T, NANS>::on) {
-        if (rx >= 1.0f && rx < (float)nth - 1.0f)
+        if (rx >= 1.0f && rx < (float)nth - 1.0f && f.pow2())
             return fast_bilerp(field_rsrc(f.p, nth - 1, f.nph),
                                fast_index(rx - 1.0f, ry, f.nph, f.mask), f.nph * 4);
     }
@@ -480,9 +500,9 @@ struct Circle {
     T gx, gy, sT, cT, sP, cP;   // start point: index coordinates and trig
 };
 
-template <typename T>
+template <typename T, bool P2 = false>
 struct VelCtx {
-    Field<T> uth, uph;
+    Field<T, P2> uth, uph;
     int nth;
     T invh, dt;
 };
@@ -543,8 +563,8 @@ __device__ __forceinline__ void tangent_vec(T x, T y, T z, T rho, T ir, T vT, T 
     o2 = -vT * rho;
 }
 
-template <typename T, bool NANS = true>
-__device__ __forceinline__ bool circle_mid(const VelCtx<T>& c, T gx, T gy, T sinT, T cosT, T sinP,
+template <typename T, bool NANS = true, bool P2 = false>
+__device__ __forceinline__ bool circle_mid(const VelCtx<T, P2>& c, T gx, T gy, T sinT, T cosT, T sinP,
                                            T cosP, T guT, T guP, Circle<T>& o) {
     o.gx = gx; o.gy = gy; o.sT = sinT; o.cT = cosT; o.sP = sinP; o.cP = cosP;
     T u0 = guT * (cosT * cosP) - guP * sinP;
@@ -584,8 +604,8 @@ __device__ __forceinline__ bool circle_mid(const VelCtx<T>& c, T gx, T gy, T sin
 
 // Departure point of the full step: returns its index-space coordinates and
 // the Cartesian point (bubble.cu:498-505, 734-743).
-template <typename T>
-__device__ __forceinline__ void circle_depart(const VelCtx<T>& c, const Circle<T>& o, T& s, T& co,
+template <typename T, bool P2>
+__device__ __forceinline__ void circle_depart(const VelCtx<T, P2>& c, const Circle<T>& o, T& s, T& co,
                                               T& px, T& py, T& pz, T& ti, T& pi_, T& rho, T& ir) {
     sincos_step(-o.unorm * c.dt, &s, &co);
     px = o.u0 * s + o.w0 * co;
@@ -596,8 +616,8 @@ __device__ __forceinline__ void circle_depart(const VelCtx<T>& c, const Circle<T
 
 // Velocity at the departure point transported back along the circle into
 // the start frame (bubble.cu:745-766, 881-902).
-template <typename T, bool NANS = true>
-__device__ __forceinline__ void transport_back(const VelCtx<T>& c, const Circle<T>& o, T& r0, T& r1,
+template <typename T, bool NANS = true, bool P2 = false>
+__device__ __forceinline__ void transport_back(const VelCtx<T, P2>& c, const Circle<T>& o, T& r0, T& r1,
                                                T& r2) {
     T s, co, px, py, pz, ti, pi_, rho, ir;
     circle_depart(c, o, s, co, px, py, pz, ti, pi_, rho, ir);
@@ -725,13 +745,14 @@ __device__ __forceinline__ U3 slerp_u(const U3& f, const U3& t, double alpha) {
 // sampleMapping (bubble.cu:539-603) on a unit-vector map: the corner
 // selection of the reference, three spherical interpolations.
 __device__ U3 sample_map_u(const UMap& M, size_t off, int nth, int nph, double rx, double ry) {
-    if (rx >= 0.5 && rx < (double)nth - 0.5) {
+    const int m = (nph & (nph - 1)) == 0 ? nph - 1 : -1;
+    if (rx >= 0.5 && rx < (double)nth - 0.5 && m >= 0) {
         // away from the polar rows: the corners of rows floor(x), floor(x) + 1
         // that the general branch below selects there, the phi wrap on the
         // integer column
         const double x = rx - 0.5, fx = floor(x), fy = floor(ry);
         const double aT = x - fx, aP = ry - fy;
-        const int m = nph - 1, pI = (int)fy;
+        const int pI = (int)fy;
         const size_t r0 = off + (size_t)(int)fx * nph, r1 = r0 + nph;
         const size_t c0 = (size_t)(pI & m), c1 = (size_t)((pI + 1) & m);
         auto at = [&](size_t c) { return U3{M.x[c], M.y[c], M.z[c]}; };
@@ -740,13 +761,12 @@ __device__ U3 sample_map_u(const UMap& M, size_t off, int nth, int nph, double r
     }
     double x = rx - 0.5, y = ry;
     const bool fl = validate_coord(x, y, nth, nph);
-    const int m = nph - 1;
     int pI = (int)floor(y), tI = (int)floor(x);
     const double aP = y - pI, aT = x - tI;
     if ((unsigned)pI >= (unsigned)nph) pI = pI < 0 ? 0 : nph - 1;
     auto at = [&](int row, int col) {
         row = min(max(row, 0), nth - 1);
-        const size_t c = off + (size_t)row * nph + (col & m);
+        const size_t c = off + (size_t)row * nph + wrap_col(col, nph, m);
         return U3{M.x[c], M.y[c], M.z[c]};
     };
     U3 ll{0, 0, 1}, lr{0, 0, 1}, hl{0, 0, 1}, hr{0, 0, 1};
@@ -993,6 +1013,77 @@ __device__ __forceinline__ int pcr_periodic(Ti* sa, Ti* sc, Ti* sd, Ti* ta, Ti* 
             nd[i] = (d[i] - ei * d[j]) * pre_rcp(Ti(1) - ei * ej);
         }
     }
+    __syncthreads();
+    *x = nd;
+    return done;
+}
+
+// The periodic row of a length n that is not a power of two (n >= 2, one
+// row per thread: threads t < n hold row t), by the rank-one correction of
+// the periodic system into a non-periodic one (Sherman-Morrison; the form of
+// Numerical Recipes' `cyclic`): with the corner couplings alpha = a_0 and
+// beta = c_{n-1} and gamma = -1, T = T' + u v^T where T' drops the corners
+// and has the diagonals 1 - gamma = 2 (row 0) and 1 - alpha beta / gamma =
+// 1 + alpha beta (row n-1), u = (gamma, 0, ..., 0, beta), v = (1, 0, ..., 0,
+// alpha / gamma). T' y = d and T' z = u are reduced together by
+// non-periodic cyclic reduction (a coupling past an end of the row is zero,
+// so ceil(log2 n) reductions decouple every row, exactly), and x = y -
+// (v . y) / (1 + v . z) z. The arrays (sa, sc, sd, su) and (ta, tc, td, tu)
+// alternate as in pcr_periodic; on exit *x points to the solution. Returns
+// the number of reductions.
+template <typename Ti>
+__device__ __forceinline__ int pcr_general(Ti* sa, Ti* sc, Ti* sd, Ti* su, Ti* ta, Ti* tc, Ti* td,
+                                           Ti* tu, int n, Ti** x) {
+    const int i = threadIdx.x;
+    const bool act = i < n;
+    // the corner couplings, read by every thread before rows 0 and n-1 change
+    const Ti alpha = sa[0], beta = sc[n - 1];
+    __syncthreads();
+    if (act) {
+        Ti u = Ti(0);
+        if (i == 0) {
+            sa[0] = Ti(0);
+            sc[0] = sc[0] * Ti(0.5);
+            sd[0] = sd[0] * Ti(0.5);
+            u = Ti(-0.5);
+        }
+        if (i == n - 1) {
+            const Ti inv = Ti(1) / (Ti(1) + alpha * beta);
+            sa[i] = sa[i] * inv;
+            sc[i] = Ti(0);
+            sd[i] = sd[i] * inv;
+            u = beta * inv;
+        }
+        su[i] = u;
+    }
+    __syncthreads();
+    Ti *a = sa, *c = sc, *d = sd, *w = su, *na = ta, *nc = tc, *nd = td, *nw = tu;
+    int done = 0;
+    for (int s = 1; s < n; s <<= 1) {
+        if (act) {
+            const int im = i - s, ip = i + s;
+            const Ti ai = a[i], ci = c[i];
+            Ti am = Ti(0), cm = Ti(0), dm = Ti(0), wm = Ti(0);
+            Ti ap = Ti(0), cp = Ti(0), dp = Ti(0), wp = Ti(0);
+            if (im >= 0) { am = a[im]; cm = c[im]; dm = d[im]; wm = w[im]; }
+            if (ip < n) { ap = a[ip]; cp = c[ip]; dp = d[ip]; wp = w[ip]; }
+            const Ti iD = Ti(1) / (Ti(1) - ai * cm - ci * ap);
+            na[i] = -ai * am * iD;
+            nc[i] = -ci * cp * iD;
+            nd[i] = (d[i] - ai * dm - ci * dp) * iD;
+            nw[i] = (w[i] - ai * wm - ci * wp) * iD;
+        }
+        ++done;
+        Ti* t;
+        t = a; a = na; na = t;
+        t = c; c = nc; nc = t;
+        t = d; d = nd; nd = t;
+        t = w; w = nw; nw = t;
+        __syncthreads();
+    }
+    // d holds y, w holds z
+    const Ti f = (d[0] - alpha * d[n - 1]) / (Ti(1) + w[0] - alpha * w[n - 1]);
+    if (act) nd[i] = d[i] - f * w[i];
     __syncthreads();
     *x = nd;
     return done;

@@ -595,7 +595,7 @@ int sf_write_image_exr(const char* path, int32_t width, int32_t height, const fl
 
 int sf_texture_fields(const char* thickness_exr, const char* air_exr, int32_t n_theta,
                       double radius, double* eta, double* uair0, double* vair0, int32_t* loaded) {
-    if (n_theta < 4 || (n_theta & (n_theta - 1))) return SF_ERR_UNSUPPORTED;
+    if (n_theta < 4) return SF_ERR_UNSUPPORTED;
     int got = 0;
     sfh::texture_fields(thickness_exr, air_exr, n_theta, radius, eta, uair0, vair0, &got);
     if (loaded) *loaded = got;

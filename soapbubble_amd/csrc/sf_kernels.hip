@@ -30,19 +30,20 @@ constexpr double kThresh = 0.0; // bubble.cu:27
 // One cell: eta and Gamma at the centre (j+1/2, i), u_phi at the west face
 // (j+1/2, i-1/2), u_theta at the south face (j+1, i). NANS selects the
 // reference's NaN-skipping interpolation; without it the fields hold no NaN.
-template <typename T, bool NANS>
+// P2: every bubble of the launch has rows whose length is a power of two.
+template <typename T, bool NANS, bool P2>
 __device__ __forceinline__ void advect_cell(const BubbleDev& B, const Tables& tab,
                                             const StatePtrs<T>& in, const StatePtrs<T>& out, int j,
                                             int i) {
     const size_t c = (size_t)j * B.nph + i;
-    VelCtx<T> v;
-    v.uth = Field<T>{in.uth + B.off, B.nph, B.mask, B.nth - 2};
-    v.uph = Field<T>{in.uph + B.off, B.nph, B.mask, B.nth - 1};
+    VelCtx<T, P2> v;
+    v.uth = Field<T, P2>{in.uth + B.off, B.nph, B.mask, B.nth - 2};
+    v.uph = Field<T, P2>{in.uph + B.off, B.nph, B.mask, B.nth - 1};
     v.nth = B.nth;
     v.invh = (T)B.invh;
     v.dt = (T)B.dt;
-    const Field<T> feta{in.eta + B.off, B.nph, B.mask, B.nth - 1};
-    const Field<T> fgam{in.gam + B.off, B.nph, B.mask, B.nth - 1};
+    const Field<T, P2> feta{in.eta + B.off, B.nph, B.mask, B.nth - 1};
+    const Field<T, P2> fgam{in.gam + B.off, B.nph, B.mask, B.nth - 1};
 
     const T sinTc = TabSel<T>::sTc(tab)[B.rt0 + j], cosTc = TabSel<T>::cTc(tab)[B.rt0 + j];
     const T sinPc = TabSel<T>::sPc(tab)[B.ct0 + i], cosPc = TabSel<T>::cPc(tab)[B.ct0 + i];
@@ -122,7 +123,7 @@ __device__ __forceinline__ void advect_cell(const BubbleDev& B, const Tables& ta
     }
 }
 
-template <typename T>
+template <typename T, bool P2>
 __global__ __launch_bounds__(256) void k_advect(const int4* __restrict__ tiles,
                                                 const BubbleDev* __restrict__ bub,
                                                 BubbleTime* __restrict__ btime, Tables tab,
@@ -146,9 +147,9 @@ __global__ __launch_bounds__(256) void k_advect(const int4* __restrict__ tiles,
     tile_cell(tl, j, i);
     if (j >= B.nth || i >= B.nph) return;
     if (btime[tl.x].nans)
-        advect_cell<T, true>(B, tab, in, out, j, i);
+        advect_cell<T, true, P2>(B, tab, in, out, j, i);
     else
-        advect_cell<T, false>(B, tab, in, out, j, i);
+        advect_cell<T, false, P2>(B, tab, in, out, j, i);
 }
 
 // ---------------------------------------------------------- air flow ----
@@ -202,7 +203,7 @@ __global__ __launch_bounds__(256) void k_visc_div(const int4* __restrict__ tiles
     if (j >= B.nth || i >= B.nph) return;
     const int nph = B.nph, m = B.mask;
     const size_t row = (size_t)B.off + (size_t)j * nph;
-    const double uW = (double)st.uph[row + i], uE = (double)st.uph[row + ((i + 1) & m)];
+    const double uW = (double)st.uph[row + i], uE = (double)st.uph[row + wrap_col(i + 1, nph, m)];
     const double vN = j != 0 ? (double)st.uth[row - nph + i] : 0.0;
     const double vS = j != B.nth - 1 ? (double)st.uth[row + i] : 0.0;
     const double sinN = tab.sinFace[B.rt0 + j], sinS = tab.sinFace[B.rt0 + j + 1];
@@ -234,9 +235,9 @@ __global__ __launch_bounds__(256) void k_visc_force(const int4* __restrict__ til
     auto V = [&](int jj, int ii) { return (double)st.uth[base + (size_t)jj * nph + ii]; };
     auto E = [&](int jj, int ii) { return (double)st.eta[base + (size_t)jj * nph + ii]; };
     auto D = [&](int jj, int ii) { return dv[base + (size_t)jj * nph + ii]; };
-    const int iW = (i - 1) & m, iE = (i + 1) & m;
+    const int iW = wrap_col(i - 1, nph, m), iE = wrap_col(i + 1, nph, m);
     if (j < nth - 1) {
-        const int opp = (i + hn) & m;
+        const int opp = wrap_col(i + hn, nph, m);
         const double u0 = U(j, i), u1 = U(j, iE), u2 = U(j + 1, i), u3 = U(j + 1, iE);
         const double divN = D(j, i), divS = D(j + 1, i);
         const double etaN = E(j, i), etaS = E(j + 1, i);
@@ -279,7 +280,7 @@ __global__ __launch_bounds__(256) void k_visc_force(const int4* __restrict__ til
         fv[base + (size_t)j * nph + i] = out;
     }
     {
-        const int oW = (iW + hn) & m, oC = (i + hn) & m;
+        const int oW = wrap_col(iW + hn, nph, m), oC = wrap_col(i + hn, nph, m);
         const double divW = D(j, iW), divE = D(j, i);
         const double etaW = E(j, iW), etaE = E(j, i);
         double out;
@@ -353,6 +354,21 @@ __device__ __forceinline__ void ld_row(const X* __restrict__ p, X (&v)[C]) {
 #pragma unroll
     for (int k = 0; k < C; ++k) v[k] = q[k];
 }
+// C consecutive cells of a row from column `start` on, wrapping past the
+// row's end: the phi + pi cells across a pole. One aligned vector when the
+// block stays inside the row on a multiple of C (always for power-of-two
+// rows, where n_theta is a multiple of C), else cell by cell.
+template <int C, typename X>
+__device__ __forceinline__ void ld_row_wrap(const X* __restrict__ rowp, int start, int nph, int m,
+                                            X (&v)[C]) {
+    if (m >= 0 || (start % C == 0 && start + C <= nph)) {
+        ld_row<C>(rowp + start, v);
+        return;
+    }
+#pragma unroll
+    for (int k = 0; k < C; ++k) v[k] = rowp[wrap_col(start + k, nph, m)];
+}
+
 template <int C, typename X>
 __device__ __forceinline__ void st_row(X* __restrict__ p, const X (&v)[C]) {
     X* q = (X*)__builtin_assume_aligned(p, sizeof(X) * C);
@@ -375,14 +391,14 @@ __device__ __forceinline__ void air_blend(const float* __restrict__ a0,
     float a[C], sh[C], am[C], sm[C];
     ld_row<C>(a0 + row + i0, a);
     ld_row<C>(shape + row + i0, sh);
-    const size_t mb = row + (size_t)(nph - i0 - C), cm = row + ((nph - i0) & m);
+    const size_t mb = row + (size_t)(nph - i0 - C), cm = row + wrap_col(nph - i0, nph, m);
     ld_row<C>(a0 + mb, am);
     ld_row<C>(shape + mb, sm);
     auto val = [&](float x, float y) {
         const float v = fmaf(y, kt, x);
         return __builtin_isnan(v) ? 0.0f : v;
     };
-    const size_t ce = row + ((i0 + C) & m);
+    const size_t ce = row + wrap_col(i0 + C, nph, m);
 #pragma unroll
     for (int k = 0; k < N; ++k) {
         const float own = k < C ? val(a[k], sh[k]) : val(a0[ce], shape[ce]);
@@ -418,17 +434,18 @@ __device__ __forceinline__ void row_system(const BubbleDev& B, double t, const T
     // neighbours across the poles are the phi + pi cells of the same row
     // (bubble.cu:1384-1401); their face weights vanish with sin(theta_face)
     const size_t rowN = hasN ? row - nph : row, rowS = hasS ? row + nph : row;
-    const int iNS = hasN ? i0 : (i0 + nth) & m;
-    const int iSS = hasS ? i0 : (i0 + nth) & m;
+    const int iNS = hasN ? i0 : wrap_col(i0 + nth, nph, m);
+    const int iSS = hasS ? i0 : wrap_col(i0 + nth, nph, m);
     T ev[C], eNv[C], eSv[C], gv[C], gNv[C], gSv[C];
     ld_row<C>(s.eta + row + i0, ev);
     ld_row<C>(s.gam + row + i0, gv);
-    ld_row<C>(s.eta + rowN + (hasN ? i0 : iNS), eNv);
-    ld_row<C>(s.gam + rowN + (hasN ? i0 : iNS), gNv);
-    ld_row<C>(s.eta + rowS + (hasS ? i0 : iSS), eSv);
-    ld_row<C>(s.gam + rowS + (hasS ? i0 : iSS), gSv);
-    const double eW = (double)s.eta[row + ((i0 - 1) & m)], eE = (double)s.eta[row + ((i0 + C) & m)];
-    double gW = (double)s.gam[row + ((i0 - 1) & m)], gE = (double)s.gam[row + ((i0 + C) & m)];
+    ld_row_wrap<C>(s.eta + rowN, iNS, nph, m, eNv);
+    ld_row_wrap<C>(s.gam + rowN, iNS, nph, m, gNv);
+    ld_row_wrap<C>(s.eta + rowS, iSS, nph, m, eSv);
+    ld_row_wrap<C>(s.gam + rowS, iSS, nph, m, gSv);
+    const int iWc = wrap_col(i0 - 1, nph, m), iEc = wrap_col(i0 + C, nph, m);
+    const double eW = (double)s.eta[row + iWc], eE = (double)s.eta[row + iEc];
+    double gW = (double)s.gam[row + iWc], gE = (double)s.gam[row + iEc];
 
     const double dt = B.dt, invDt = 1.0 / dt, Cr = B.Cr;
     const double CrDt = Cr * dt, MDt = B.M * dt, s2 = B.invh * B.invh;
@@ -458,14 +475,14 @@ __device__ __forceinline__ void row_system(const BubbleDev& B, double t, const T
         ld_row<C>(guess + row + i0, gc);
 #pragma unroll
         for (int k = 0; k < C; ++k) gxv[k] = gv[k] + gc[k];
-        ld_row<C>(guess + rowN + (hasN ? i0 : iNS), gc);
+        ld_row_wrap<C>(guess + rowN, iNS, nph, m, gc);
 #pragma unroll
         for (int k = 0; k < C; ++k) gNv[k] = gNv[k] + gc[k];
-        ld_row<C>(guess + rowS + (hasS ? i0 : iSS), gc);
+        ld_row_wrap<C>(guess + rowS, iSS, nph, m, gc);
 #pragma unroll
         for (int k = 0; k < C; ++k) gSv[k] = gSv[k] + gc[k];
-        gW = (double)(T)(gW + (double)guess[row + ((i0 - 1) & m)]);
-        gE = (double)(T)(gE + (double)guess[row + ((i0 + C) & m)]);
+        gW = (double)(T)(gW + (double)guess[row + iWc]);
+        gE = (double)(T)(gE + (double)guess[row + iEc]);
     }
     // Every face term of concentrationLinearSystemKernel has the factor
     // 1 / (eta_f + Cr dt): the weight s2 sin M dt / (eta_f + Cr dt), the
@@ -497,7 +514,7 @@ __device__ __forceinline__ void row_system(const BubbleDev& B, double t, const T
         ld_row<C>(s.uph + row + i0, u);
 #pragma unroll
         for (int k = 0; k < C; ++k) { uv[k] = (double)u[k]; vNv[k] = vSv[k] = 0.0; }
-        uv[C] = (double)s.uph[row + ((i0 + C) & m)];
+        uv[C] = (double)s.uph[row + iEc];
         if (hasN) {
             ld_row<C>(s.uth + row - nph + i0, vn);
 #pragma unroll
@@ -521,7 +538,7 @@ __device__ __forceinline__ void row_system(const BubbleDev& B, double t, const T
             } else {
 #pragma unroll
                 for (int k = 0; k <= C; ++k) {
-                    const int i = (i0 + k) & m, opp = (nph - i) & m;
+                    const int i = wrap_col(i0 + k, nph, m), opp = wrap_col(nph - i, nph, m);
                     uAirF[k] = 0.5 * (air_at(air.ua0, air.au, row + i, kt) * bl +
                                       air_at(air.ua0, air.au, row + opp, kt) * (1 - bl));
                 }
@@ -563,7 +580,7 @@ __device__ __forceinline__ void row_system(const BubbleDev& B, double t, const T
         if (B.airflow) {
             double vAirN = (double)vAirNv[k], vAirS = (double)vAirSv[k];
             if constexpr (sizeof(T) != 4) {
-                const int opp = (nph - i) & m;
+                const int opp = wrap_col(nph - i, nph, m);
                 if (hasN) {
                     const size_t rn = row - nph;
                     vAirN = 0.5 * (air_at(air.va0, air.av, rn + i, kt) * bl -
@@ -582,7 +599,7 @@ __device__ __forceinline__ void row_system(const BubbleDev& B, double t, const T
             // `gravity` with `VISCOUS` (bubble.cu:1458-1464): the viscous
             // forces in place of the gravity term
             const size_t cc = row + i;
-            const double fuW = air.fu[cc], fuE = air.fu[row + ((i + 1) & m)];
+            const double fuW = air.fu[cc], fuE = air.fu[row + wrap_col(i + 1, nph, m)];
             const double fvN = hasN ? air.fv[cc - nph] : 0.0;
             const double fvS = hasS ? air.fv[cc] : 0.0;
             div += B.invh * dt *
@@ -604,15 +621,17 @@ __device__ __forceinline__ void row_system(const BubbleDev& B, double t, const T
 }
 
 // Periodic phi-line solve of one row, normalised to unit diagonal
-// (x_i + a_i x_{i-1} + c_i x_{i+1} = d_i), n a power of two, thread t owning
+// (x_i + a_i x_{i-1} + c_i x_{i+1} = d_i), C dividing n, thread t owning
 // cells [C t, C t + C) in registers (threads t >= n / C hold nothing but
 // take part in the barriers). Partition method: each thread expresses its
 // interior cells k < C-1 as x_k = al_k + be_k X_{t-1} + ga_k X_t, where X_t is
 // its last cell, by a Thomas sweep with three right-hand sides; substituting
 // the neighbours' interior cells into the last-cell equations leaves a
 // periodic tridiagonal system in the R = n / C unknowns X_t, solved by
-// parallel cyclic reduction in LDS (steps as pcr_periodic); back-substitution
-// recovers the interior cells. On exit d holds x. `lds` holds 6 R values.
+// parallel cyclic reduction in LDS (steps as pcr_periodic) when R is a power
+// of two, else by pcr_general; back-substitution recovers the interior
+// cells. On exit d holds x. `lds` holds 6 R values (8 R when R is not a
+// power of two).
 template <typename Ti, int C>
 __device__ __forceinline__ int line_solve(Ti (&a)[C], Ti (&c)[C], Ti (&d)[C], int n, int steps,
                                           Ti* lds, float* redf) {
@@ -660,7 +679,7 @@ __device__ __forceinline__ int line_solve(Ti (&a)[C], Ti (&c)[C], Ti (&d)[C], in
         constexpr int L = C - 1;
         Ti A, Bd, Cc, D;
         if (C > 1) {
-            const int tn = (tid + 1) & (R - 1);
+            const int tn = tid + 1 == R ? 0 : tid + 1;
             const Ti a0n = xa[tn], b0n = xb[tn], g0n = xg[tn];
             A = a[L] * be[L > 0 ? L - 1 : 0];
             Bd = Ti(1) + a[L] * ga[L > 0 ? L - 1 : 0] + c[L] * b0n;
@@ -681,9 +700,11 @@ __device__ __forceinline__ int line_solve(Ti (&a)[C], Ti (&c)[C], Ti (&d)[C], in
     // the interface unknowns' arrays xa, xb, xg are free from here on: the
     // second buffer of the reduction
     Ti* xs;
-    const int done = pcr_periodic<Ti, 1>(ra, rc, rd, xa, xb, xg, R, steps, redf, &xs);
+    const int done = (R & (R - 1)) == 0
+                         ? pcr_periodic<Ti, 1>(ra, rc, rd, xa, xb, xg, R, steps, redf, &xs)
+                         : pcr_general<Ti>(ra, rc, rd, xg + R, xa, xb, xg, xg + 2 * R, R, &xs);
     if (act) {
-        const Ti Xt = xs[tid], Xp = xs[(tid - 1) & (R - 1)];
+        const Ti Xt = xs[tid], Xp = xs[tid == 0 ? R - 1 : tid - 1];
 #pragma unroll
         for (int k = 0; k < C - 1; ++k) d[k] = al[k] + be[k] * Xp + ga[k] * Xt;
         d[C - 1] = Xt;
@@ -808,7 +829,7 @@ struct RowOp {
                                          int nph, bool hasN, bool hasS) {
         ld_row<C>(sys.diag + row + i0, dg);
         ld_row<C>(sys.wW + row + i0, ww);
-        wwE = sys.wW[row + ((i0 + C) & m)];
+        wwE = sys.wW[row + wrap_col(i0 + C, nph, m)];
 #pragma unroll
         for (int k = 0; k < C; ++k) wn[k] = wsv[k] = Ti(0);
         if (hasN) ld_row<C>(sys.wN + row + i0, wn);
@@ -842,8 +863,8 @@ __device__ __forceinline__ void ld_stencil(const Ti* __restrict__ v, size_t row,
                                            int nph, bool hasN, bool hasS, Ti (&c)[C], Ti& w, Ti& e,
                                            Ti (&n)[C], Ti (&s)[C]) {
     ld_row<C>(v + row + i0, c);
-    w = v[row + ((i0 - 1) & m)];
-    e = v[row + ((i0 + C) & m)];
+    w = v[row + wrap_col(i0 - 1, nph, m)];
+    e = v[row + wrap_col(i0 + C, nph, m)];
 #pragma unroll
     for (int k = 0; k < C; ++k) n[k] = s[k] = Ti(0);
     if (hasN) ld_row<C>(v + row - nph + i0, n);
@@ -1128,6 +1149,180 @@ __global__ void k_cgstep(const int2* __restrict__ rows, const BubbleDev* __restr
     }
 }
 
+// ------------------------------------------ multigrid preconditioner ----
+// z = B r for the inner PCG: one V(1,1) cycle of geometric multigrid built on
+// the phi lines. Level l + 1 merges the row pairs (2J, 2J + 1) of level l
+// (semi-coarsening in theta; phi keeps every column, so the phi-line
+// relaxation stays exact for the csc(theta) anisotropy at the poles). Its
+// operator is the sum of the two rows' equations with the theta couplings
+// halved (the re-discretisation at twice the spacing: a face weight
+// s2 sin(theta_f) K_f of spacing h becomes s2 sin(theta_f) K_f / 2 for 2h);
+// restriction sums the two rows' residuals, prolongation copies a coarse
+// value to both rows (each the transpose of the other, so the cycle is a
+// symmetric preconditioner). Smoothing: zebra phi-line Gauss-Seidel, even
+// then odd rows before the coarse correction and odd then even after it.
+// From a zero start the even rows' line solve has no theta terms, which
+// makes it the line-Jacobi solve the PCG kernels already produce at level
+// 0, and leaves residuals only on the even rows, r(2J) = wN(2J) x(2J - 1) +
+// wS(2J) x(2J + 1): the odd rows' launch writes those terms into the coarse
+// level's right-hand side (U, D). After the coarse correction an odd row's
+// new value depends on its even neighbours only (the line solve replaces
+// the old value), so the prolongation is folded into the neighbours it
+// reads. The coarsest level is one row (solved exactly) or, for a row
+// count that cannot be halved, symmetric zebra passes.
+
+// Coarse operator of level l + 1 (row J) from level l (rows 2J, 2J + 1).
+template <typename Ti>
+__global__ __launch_bounds__(256) void k_mg_coarsen(const int2* __restrict__ rows,
+                                                    const BubbleDev* __restrict__ bub,
+                                                    MgPtrs<Ti> F, MgPtrs<Ti> Cn) {
+    const int2 rw = rows[blockIdx.x];
+    if (rw.x < 0) return;
+    const BubbleDev B = bub[rw.x];
+    const MgLevelDev lf = F.lv[rw.x], lc = Cn.lv[rw.x];
+    const int J = rw.y, nph = B.nph;
+    const size_t f0 = (size_t)lf.off + (size_t)(2 * J) * nph, f1 = f0 + nph;
+    const size_t c0 = (size_t)lc.off + (size_t)J * nph;
+    const bool south = 2 * J + 2 < lf.n;
+    for (int i = threadIdx.x; i < nph; i += blockDim.x) {
+        const double wN0 = (double)F.wN[f0 + i], wN1 = (double)F.wN[f1 + i];
+        const double wN2 = south ? (double)F.wN[f1 + nph + i] : 0.0;
+        Cn.wW[c0 + i] = (Ti)((double)F.wW[f0 + i] + (double)F.wW[f1 + i]);
+        Cn.wN[c0 + i] = (Ti)(0.5 * wN0);
+        Cn.dg[c0 + i] =
+            (Ti)((double)F.dg[f0 + i] + (double)F.dg[f1 + i] - 2.0 * wN1 - 0.5 * (wN0 + wN2));
+    }
+}
+
+// One half-sweep of zebra line relaxation on the rows of one parity of
+// level `level` (the launch's row list), with the options of `flags`:
+// kMgZeroNb (the first half-sweep from a zero start: no theta terms),
+// kMgProlong (the neighbours carry the coarse level's correction),
+// kMgRestrict (write this odd row's residual terms into the coarse
+// right-hand side), kMgRz (level 0: this row's r.z partial into rz_out).
+// Gates: every launch returns for a converged bubble; kMgGateInner also
+// for a finished inner solve; kMgGateStart (the first launch of the cycle at
+// the start of a refinement) ends the solve when the FP64 residual in partR
+// already meets the tolerance. The right-hand side is r0 at level 0 and U +
+// D above it.
+template <typename Ti, int C>
+__global__ void k_mg_line(const int2* __restrict__ rows, const BubbleDev* __restrict__ bub,
+                          SolveCtl* __restrict__ ctl, MgPtrs<Ti> L, MgPtrs<Ti> Cn,
+                          const Ti* __restrict__ r0, int level, int flags,
+                          const int* __restrict__ rsteps, SolveParams sp,
+                          const double* __restrict__ partR, double* __restrict__ rz_out,
+                          long stride) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    __shared__ double red[16 * 4];
+    __shared__ float redf[16];
+    const int2 rw = rows[blockIdx.x];
+    if (rw.x < 0) return;
+    SolveCtl* Cb = ctl + rw.x;
+    if (flag_load(&Cb->outer_done)) return;
+    if ((flags & kMgGateInner) && Cb->inner_done) return;
+    const BubbleDev B = bub[rw.x];
+    if (flags & kMgGateStart) {
+        double sm[2];
+        sum_partials<2>(partR, stride, B.pr0, B.nth, sm, red);
+        if (residual_norm(sm[0], sm[1], sp.absolute) <= sp.tol) {
+            if (rw.y == 0 && threadIdx.x == 0) flag_store(&Cb->outer_done, 1);
+            return;
+        }
+    }
+    const MgLevelDev lv = L.lv[rw.x];
+    const int j = rw.y, nph = B.nph, m = B.mask, n = lv.n;
+    const size_t row = (size_t)lv.off + (size_t)j * nph;
+    const bool act = (int)threadIdx.x < nph / C;
+    const int i0 = C * threadIdx.x;
+    Ti a[C], cc[C], d[C], rv[C];
+#pragma unroll
+    for (int q = 0; q < C; ++q) a[q] = cc[q] = d[q] = rv[q] = Ti(0);
+    if (act) {
+        if (level == 0) {
+            ld_row<C>(r0 + row + i0, rv);
+        } else {
+            Ti u[C], dd[C];
+            ld_row<C>(L.U + row + i0, u);
+            ld_row<C>(L.D + row + i0, dd);
+#pragma unroll
+            for (int q = 0; q < C; ++q) rv[q] = u[q] + dd[q];
+        }
+        Ti rhs[C];
+#pragma unroll
+        for (int q = 0; q < C; ++q) rhs[q] = rv[q];
+        if (!(flags & kMgZeroNb)) {
+            const size_t crow = Cn.lv ? (size_t)Cn.lv[rw.x].off : 0;
+            if (j > 0) {
+                Ti xn[C], w[C];
+                ld_row<C>(L.x + row - nph + i0, xn);
+                if (flags & kMgProlong) {
+                    Ti e[C];
+                    ld_row<C>(Cn.x + crow + (size_t)((j - 1) / 2) * nph + i0, e);
+#pragma unroll
+                    for (int q = 0; q < C; ++q) xn[q] += e[q];
+                }
+                ld_row<C>(L.wN + row + i0, w);
+#pragma unroll
+                for (int q = 0; q < C; ++q) rhs[q] += w[q] * xn[q];
+            }
+            if (j < n - 1) {
+                Ti xs[C], w[C];
+                ld_row<C>(L.x + row + nph + i0, xs);
+                if (flags & kMgProlong) {
+                    Ti e[C];
+                    ld_row<C>(Cn.x + crow + (size_t)((j + 1) / 2) * nph + i0, e);
+#pragma unroll
+                    for (int q = 0; q < C; ++q) xs[q] += e[q];
+                }
+                ld_row<C>(L.wN + row + nph + i0, w);
+#pragma unroll
+                for (int q = 0; q < C; ++q) rhs[q] += w[q] * xs[q];
+            }
+        }
+        Ti dg[C], ww[C];
+        ld_row<C>(L.dg + row + i0, dg);
+        ld_row<C>(L.wW + row + i0, ww);
+        const Ti wwE = L.wW[row + wrap_col(i0 + C, nph, m)];
+#pragma unroll
+        for (int q = 0; q < C; ++q) {
+            const Ti inv = pre_rcp(dg[q]);
+            a[q] = -ww[q] * inv;
+            cc[q] = -(q == C - 1 ? wwE : ww[q + 1]) * inv;
+            d[q] = rhs[q] * inv;
+        }
+    }
+    const int steps = level == 0 ? rsteps[B.pr0 + j] : -1;
+    line_solve<Ti, C>(a, cc, d, nph, steps, reinterpret_cast<Ti*>(lds_raw), redf);
+    double acc[1] = {0.0};
+    if (act) {
+        st_row<C>(L.x + row + i0, d);
+        if (flags & kMgRestrict) {
+            // odd row j = 2J + 1: wS(2J) x(2J + 1) into coarse row J, wN(2J + 2)
+            // x(2J + 1) into coarse row J + 1
+            const size_t cb = (size_t)Cn.lv[rw.x].off + (size_t)(j / 2) * nph + i0;
+            Ti w[C], o[C];
+            ld_row<C>(L.wN + row + i0, w);
+#pragma unroll
+            for (int q = 0; q < C; ++q) o[q] = w[q] * d[q];
+            st_row<C>(Cn.U + cb, o);
+            if (j + 1 < n) {
+                ld_row<C>(L.wN + row + nph + i0, w);
+#pragma unroll
+                for (int q = 0; q < C; ++q) o[q] = w[q] * d[q];
+                st_row<C>(Cn.D + cb + nph, o);
+            }
+        }
+        if (flags & kMgRz) {
+#pragma unroll
+            for (int q = 0; q < C; ++q) acc[0] += (double)rv[q] * (double)d[q];
+        }
+    }
+    if (flags & kMgRz) {
+        block_sum<1>(acc, red);
+        if (threadIdx.x == 0) rz_out[B.pr0 + j] = acc[0];
+    }
+}
+
 // ------------------------------------------- inner Chebyshev iteration ----
 // Chebyshev acceleration of the phi-line Jacobi iteration for A e = r_in
 // (Saad, Iterative Methods for Sparse Linear Systems, Alg. 12.1), with the
@@ -1282,7 +1477,7 @@ __device__ __forceinline__ bool update_cell(const BubbleDev& B, int ghist,
     // the viscous force, the t/12 mirror blend of the air and no separate
     // gravity term
     const bool visc = air.fu != nullptr;
-    const int opp = (nph - i) & m;
+    const int opp = wrap_col(nph - i, nph, m);
 
     // divergenceKernel_fromGamma (bubble.cu:1276-1295)
     const double ga = in.ga;
@@ -1437,7 +1632,7 @@ __global__ __launch_bounds__(128) void k_update(const int4* __restrict__ tiles,
     const bool south = j < nth - 1;
     const double* __restrict__ x = ctl[tl.x].xbuf ? x1 : x0;
     const size_t row = (size_t)B.off + (size_t)j * nph;
-    const size_t c = row + i, cW = row + ((i - 1) & m);
+    const size_t c = row + i, cW = row + wrap_col(i - 1, nph, m);
     // the reference's divisions by dt as products with 1 / dt (an ulp)
     const double invDt = 1.0 / B.dt;
     const double tnow = btime[tl.x].t;
@@ -1798,7 +1993,7 @@ __device__ __forceinline__ void local_bounds(const double* __restrict__ a, int n
                 jj = jj < 0 ? 0 : nth - 1;
                 ii += nth;
             }
-            const float v = (float)a[(size_t)jj * nph + (ii & (nph - 1))];
+            const float v = (float)a[(size_t)jj * nph + wrap_col(ii, nph, (nph & (nph - 1)) ? -1 : nph - 1)];
             lo = fminf(lo, v);
             hi = fmaxf(hi, v);
         }
@@ -1866,8 +2061,12 @@ __global__ __launch_bounds__(256) void k_bm_finish(const int4* __restrict__ tile
 template <typename T>
 void launch_advect(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, BubbleTime* bt,
                    const Tables& tab, StatePtrs<T> in, StatePtrs<T> out) {
-    hipLaunchKernelGGL(k_advect<T>, dim3(g.ntiles), dim3(256), 0, s, g.tiles, bub, bt, tab, in,
-                       out);
+    if (g.pow2_rows)
+        hipLaunchKernelGGL((k_advect<T, true>), dim3(g.ntiles), dim3(256), 0, s, g.tiles, bub, bt,
+                           tab, in, out);
+    else
+        hipLaunchKernelGGL((k_advect<T, false>), dim3(g.ntiles), dim3(256), 0, s, g.tiles, bub, bt,
+                           tab, in, out);
 }
 
 #define SF_CASES(CALL)                                 \
@@ -1963,6 +2162,25 @@ template void launch_viscous<float>(hipStream_t, const LaunchGeom&, const Bubble
 template void launch_viscous<double>(hipStream_t, const LaunchGeom&, const BubbleDev*,
                                      const Tables&, StatePtrs<double>, double*, double*, double*);
 
+template <typename Ti>
+void launch_mg_coarsen(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, MgPtrs<Ti> fine,
+                       MgPtrs<Ti> coarse) {
+    hipLaunchKernelGGL(k_mg_coarsen<Ti>, dim3(g.nrows), dim3(256), 0, s, g.rows, bub, fine, coarse);
+}
+
+template <typename Ti>
+void launch_mg_line(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, SolveCtl* ctl,
+                    MgPtrs<Ti> lvl, MgPtrs<Ti> coarse, const Ti* r0, int level, int flags,
+                    const int* rsteps, const SolveParams& sp, const double* partR,
+                    double* rz_out, long stride) {
+#define SF_CALL(C)                                                                                 \
+    hipLaunchKernelGGL((k_mg_line<Ti, C>), dim3(g.nrows), dim3(g.row_threads), g.lds_bytes, s,     \
+                       g.rows, bub, ctl, lvl, coarse, r0, level, flags, rsteps, sp, partR, rz_out, \
+                       stride)
+    SF_CASES(SF_CALL)
+#undef SF_CALL
+}
+
 void launch_finalize(hipStream_t s, int nb, const BubbleDev* bub, SolveCtl* ctl, SolveHist* hist,
                      const SolveParams& sp, const double* partR, long stride) {
     hipLaunchKernelGGL(k_finalize, dim3(nb), dim3(256), 0, s, bub, ctl, hist, sp, partR, stride);
@@ -2035,6 +2253,14 @@ SF_INST_SYS(double, double)
                                   double*, long);
 SF_INST_TI(float)
 SF_INST_TI(double)
+#define SF_INST_MG(Ti)                                                                            \
+    template void launch_mg_coarsen<Ti>(hipStream_t, const LaunchGeom&, const BubbleDev*,         \
+                                        MgPtrs<Ti>, MgPtrs<Ti>);                                  \
+    template void launch_mg_line<Ti>(hipStream_t, const LaunchGeom&, const BubbleDev*, SolveCtl*, \
+                                     MgPtrs<Ti>, MgPtrs<Ti>, const Ti*, int, int, const int*,     \
+                                     const SolveParams&, const double*, double*, long);
+SF_INST_MG(float)
+SF_INST_MG(double)
 template void launch_update<float>(hipStream_t, const LaunchGeom&, const BubbleDev*, BubbleTime*,
                                    SolveCtl*, const Tables&, StatePtrs<float>, AirPtrs<float>,
                                    const double*, const double*, StatePtrs<float>, int,

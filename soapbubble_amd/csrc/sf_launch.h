@@ -68,6 +68,34 @@ struct SolveParams {
     int pad_;
 };
 
+// One level of the multigrid preconditioner, per bubble: its row count (0:
+// the bubble has no such level) and the first cell of its rows in the
+// level's packed arrays.
+struct MgLevelDev {
+    int n;
+    int pad_;
+    long long off;
+};
+
+// The arrays of one multigrid level, packed over the bubbles: operator
+// (north and west face weights, diagonal), the iterate, and the two terms of
+// the right-hand side restricted from the level below (coarse row J: U the
+// term of its own odd row 2J + 1, D the term of the odd row 2J - 1 above).
+template <typename Ti>
+struct MgPtrs {
+    Ti* wN;
+    Ti* wW;
+    Ti* dg;
+    Ti* x;
+    Ti* U;
+    Ti* D;
+    const MgLevelDev* lv;
+};
+
+// Options of a multigrid half-sweep (k_mg_line).
+constexpr int kMgZeroNb = 1, kMgProlong = 2, kMgRestrict = 4, kMgRz = 8, kMgGateInner = 16,
+              kMgGateStart = 32;
+
 // Threads of a row block at most: one per C consecutive cells of the widest
 // row, C in {2, 4, 8, 16}.
 constexpr int kRowThreadsMax = 1024;
@@ -80,6 +108,7 @@ struct LaunchGeom {
     int row_threads;     // threads of a row block
     int cpt;             // consecutive cells per thread of a row block (2, 4, 8 or 16)
     size_t lds_bytes;    // dynamic LDS of a row block (the reduced line system)
+    bool pow2_rows;      // every bubble's row length is a power of two
 };
 
 template <typename T>
@@ -111,6 +140,17 @@ void launch_cheb(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, Solve
 template <typename T>
 void launch_viscous(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, const Tables& tab,
                     StatePtrs<T> st, double* div, double* fu, double* fv);
+// multigrid preconditioner: the coarse operator of a level's rows from the
+// level below; one half-sweep of zebra phi-line relaxation (flags as
+// k_mg_line)
+template <typename Ti>
+void launch_mg_coarsen(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, MgPtrs<Ti> fine,
+                       MgPtrs<Ti> coarse);
+template <typename Ti>
+void launch_mg_line(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, SolveCtl* ctl,
+                    MgPtrs<Ti> lvl, MgPtrs<Ti> coarse, const Ti* r0, int level, int flags,
+                    const int* rsteps, const SolveParams& sp, const double* partR,
+                    double* rz_out, long stride);
 void launch_finalize(hipStream_t s, int nb, const BubbleDev* bub, SolveCtl* ctl, SolveHist* hist,
                      const SolveParams& sp, const double* partR, long stride);
 void launch_sched_stats(hipStream_t s, SchedStat* d, SchedStat* h, int nb);

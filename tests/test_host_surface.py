@@ -138,8 +138,8 @@ def test_cli_malformed_config(tmp_path):
 
 def test_cli_rejects_grid_before_gpu(tmp_path):
     _amg(tmp_path, tol="1e-09", conv="RELATIVE_INI", iters=37)
-    r = _run(_config(tmp_path, n_theta=48))
-    assert r.returncode == 3 and "power of two" in r.stderr
+    r = _run(_config(tmp_path, n_theta=2049))
+    assert r.returncode == 3 and "n_theta must be from 4" in r.stderr
     # the outer solver's settings, not the preconditioner's max_iters = 1
     assert "tolerance 1e-09 RELATIVE, max_iters 37" in r.stdout
     r = _run(_config(tmp_path), "--precision", "single")

@@ -49,12 +49,15 @@ def test_missing_library_fails_loudly(tmp_path):
         mod._lib = saved
 
 
-@pytest.mark.parametrize("n_theta,code", [(12, 3), (2, 3), (0, 3)])
+@pytest.mark.parametrize("n_theta,code", [(3, 3), (2, 3), (0, 3), (2049, 3)])
 def test_create_rejects_bad_grid(n_theta, code):
+    """n_theta from 4 to 8192 whose row of 2 n_theta cells splits into at
+    most 1024 equal parts of 2, 4, 8 or 16 cells (2049: rows of 4098 cells,
+    2049 parts of 2, not divisible by 4)."""
     with pytest.raises(sb.SoapFilmError) as e:
         sb.SoapFilm(sb.bubble(n_theta, 0.01))
     assert e.value.code == code
-    assert "power of two" in str(e.value)
+    assert "n_theta must be from 4 to 8192" in str(e.value)
 
 
 def test_create_rejects_bad_parameters():
@@ -150,11 +153,12 @@ def test_synth_noise_bounded():
     assert np.array_equal(a["gamma"], b["gamma"])
 
 
-@pytest.mark.parametrize("n_theta", [2, 12, 16384])
+@pytest.mark.parametrize("n_theta", [2, 3, 16384, 2049])
 def test_create_rejects_unsupported_sizes(n_theta):
-    """n_theta is a power of two from 4 (rows of 8 cells) to 8192 (rows of
-    16384 cells, one block of 1024 threads with 16 cells each); the check
-    runs before any device work."""
+    """n_theta from 4 (rows of 8 cells) to 8192 (rows of 16384 cells, one
+    block of 1024 threads with 16 cells each), with rows that split into at
+    most 1024 equal parts of 2, 4, 8 or 16 cells; the check runs before any
+    device work."""
     lib = sb.load_library()
     opt = sb.Options(sb.PRECISIONS["mixed"], 0, 0, 0, 1e-10, 0, 0, 0.0, 1, 0, 0, 1, 0.5, 0.0, 0, 0)
     arr = (sb.BubbleDesc * 1)(sb.bubble(n_theta, 0.01))
@@ -183,7 +187,7 @@ def test_create_rejects_bad_options(kw, msg):
     (512, {}, 3, "at most 262144 cells"),
     (64, {"precision": "fp64"}, 3, "SF_PREC_MIXED"),
     (64, {"scalar_advection": "bimocq"}, 3, "plain scalar advection"),
-    (12, {}, 3, "power of two"),
+    (3, {}, 3, "n_theta must be from 4"),
 ])
 def test_step_noalloc_rejects_before_device_work(n_theta, kw, code, msg):
     """The entry point without allocation checks its sizes and options before
