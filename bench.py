@@ -237,9 +237,11 @@ def stiff_leg(warmup, steps, err_steps, with_oracle):
     timePerStep 0.005 s (a non-dimensional step of 0.1), where the
     line-Jacobi preconditioned operator has a spectral bound of 0.991; mixed
     precision with the geometric-multigrid preconditioned inner PCG (graph
-    replay) and with the default Chebyshev inner solve (whose schedule is too
-    long to capture: host-driven steps), ms per step of both, and the
-    multigrid path's error against the oracle after err_steps steps."""
+    replay), with the Chebyshev inner solve (whose schedule is too long to
+    capture: host-driven steps) and with the automatic choice (Chebyshev,
+    switching to multigrid after the first solve shows the schedule too
+    long), ms per step of each, and the multigrid path's error against the
+    oracle after err_steps steps."""
     import soapbubble_amd as sb
 
     n, dt, radius = 256, 0.005, 0.05
@@ -249,7 +251,7 @@ def stiff_leg(warmup, steps, err_steps, with_oracle):
           "eta": f["eta"], "gamma": f["gamma"]}
     out = {"workload": f"one bubble {n}x{nph}, radius {radius} m, dt {dt} s (config.txt), mixed",
            "steps": steps, "warmup": warmup}
-    for name, method in (("multigrid", "mg"), ("chebyshev", "auto")):
+    for name, method in (("multigrid", "mg"), ("chebyshev", "chebyshev"), ("automatic", "auto")):
         ctx = sb.SoapFilm(sb.bubble(n, dt, radius=radius), "mixed", use_graph=True,
                           inner_method=method)
         ctx.load_state(0, st)

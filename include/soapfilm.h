@@ -118,6 +118,13 @@ typedef struct {
                              viscous forces enter the assembly (with
                              gravity) and the velocity updates
                              (bubble.cu:1217-1269, 1458-1464, 1511-2038)  */
+    int32_t halt_on_failure; /* 1: a bubble whose solve misses its tolerance
+                             takes no further step (its kernels return at
+                             once; the reference sets `broken` and its frame
+                             loop stops, bubble.cu:1500-1507, 2669), so a
+                             run of several steps, captured or not, ends
+                             where the reference's would; 0: keep stepping,
+                             counting the failure                         */
 } sf_options;
 
 typedef struct {
@@ -133,7 +140,8 @@ typedef struct {
     double time;          /* air-flow time, s                                    */
     double distortion;    /* BIMOCQ: map distortion after the last step, cells   */
     int32_t reinits;      /* BIMOCQ: map reinitialisations so far                */
-    int32_t pad_;
+    int32_t no_film;      /* 1 if no cell has eta > 0 (the broken test of
+                             write_thickness_img, solver.cu:568-575)          */
 } sf_bubble_stats;
 
 typedef struct {

@@ -63,7 +63,8 @@ class Options(ctypes.Structure):
                 ("advance_time", ctypes.c_int32), ("inner_method", ctypes.c_int32),
                 ("scalar_advection", ctypes.c_int32), ("multisample", ctypes.c_int32),
                 ("blend", ctypes.c_double), ("reinit_threshold", ctypes.c_double),
-                ("initial_guess", ctypes.c_int32), ("viscous", ctypes.c_int32)]
+                ("initial_guess", ctypes.c_int32), ("viscous", ctypes.c_int32),
+                ("halt_on_failure", ctypes.c_int32)]
 
 
 class BubbleStats(ctypes.Structure):
@@ -73,7 +74,7 @@ class BubbleStats(ctypes.Structure):
                 ("volume", ctypes.c_double), ("broken", ctypes.c_int32),
                 ("steps", ctypes.c_int32), ("time", ctypes.c_double),
                 ("distortion", ctypes.c_double), ("reinits", ctypes.c_int32),
-                ("pad_", ctypes.c_int32)]
+                ("no_film", ctypes.c_int32)]
 
 
 class Timing(ctypes.Structure):
@@ -237,7 +238,8 @@ def options(precision: str = "fp64", *, device: int = 0, use_graph: bool = False
             max_outer: int = 0, max_inner: int = 0, inner_rho: float = 0.0,
             inner_method: str = "auto", scalar_advection: str = "plain", blend: float = 0.5,
             multisample: bool = True, reinit_threshold: float = 0.0,
-            initial_guess: str = "auto", viscous: bool = False) -> Options:
+            initial_guess: str = "auto", viscous: bool = False,
+            halt_on_failure: bool = False) -> Options:
     """The sf_options struct of include/soapfilm.h."""
     return Options(PRECISIONS[precision], device, int(use_graph), int(tol_absolute), tol,
                    max_outer, max_inner, inner_rho, int(advance_time),
@@ -245,7 +247,7 @@ def options(precision: str = "fp64", *, device: int = 0, use_graph: bool = False
                    {"plain": 0, "bimocq": 1}[scalar_advection], int(multisample), blend,
                    reinit_threshold,
                    {"auto": 0, "gamma_star": 1, "extrapolated": 2}[initial_guess],
-                   int(viscous))
+                   int(viscous), int(halt_on_failure))
 
 
 def step_noalloc(bubble: BubbleDesc, opt: Options, t_seconds: float, nsteps: int,
@@ -275,7 +277,7 @@ class SoapFilm:
                  max_inner: int = 0, inner_rho: float = 0.0, inner_method: str = "auto",
                  scalar_advection: str = "plain", blend: float = 0.5, multisample: bool = True,
                  reinit_threshold: float = 0.0, initial_guess: str = "auto",
-                 viscous: bool = False):
+                 viscous: bool = False, halt_on_failure: bool = False):
         self._lib = load_library()
         if isinstance(bubbles, BubbleDesc):
             bubbles = [bubbles]
@@ -286,7 +288,7 @@ class SoapFilm:
                       max_inner=max_inner, inner_rho=inner_rho, inner_method=inner_method,
                       scalar_advection=scalar_advection, blend=blend, multisample=multisample,
                       reinit_threshold=reinit_threshold, initial_guess=initial_guess,
-                      viscous=viscous)
+                      viscous=viscous, halt_on_failure=halt_on_failure)
         h = ctypes.c_void_p()
         rc = self._lib.sf_create(ctypes.byref(opt), len(self.bubbles), arr, ctypes.byref(h))
         if rc:

@@ -200,7 +200,10 @@ Solver::Solver(size_t nPhi, size_t nTheta, double radius, double dt, double H, i
     sf_options o{};
     o.precision = precision;
     o.device = device;
-    o.use_graph = 0;
+    // steps replay as captured graphs (the FP32 inner solves; FP64 runs
+    // host-driven), stopping on the device at a failed solve
+    o.use_graph = 1;
+    o.halt_on_failure = 1;
     o.tol = st.tolerance;
     o.tol_absolute = st.absolute ? 1 : 0;
     o.max_inner = st.max_iters;
@@ -245,23 +248,23 @@ void Solver::initWithConst(sf_field field, double val) {
     check(sf_upload_field(ctx_, 0, field, h.data()));
 }
 
-void Solver::stepForward() {
-    check(sf_step(ctx_, 1));
-    sf_bubble_stats s;
-    check(sf_get_stats(ctx_, 0, &s));
-    if (!s.converged) broken_ = true;   // AlgebraicMultiGridCG (bubble.cu:1500-1507)
+void Solver::stepForward() { stepForward(1); }
+
+void Solver::stepForward(int n) {
+    if (n > 0) check(sf_step(ctx_, n));
 }
 
 void Solver::stepForward(double dt) {
     check(sf_set_dt(ctx_, 0, dt));
-    stepForward();
+    stepForward(1);
     check(sf_set_dt(ctx_, 0, dt_));
 }
 
 bool Solver::isBroken() {
+    if (broken_) return true;
     sf_bubble_stats s;
     check(sf_get_stats(ctx_, 0, &s));
-    return broken_ || s.broken;
+    return s.failed_solves > 0;
 }
 
 void Solver::setTime(double t) { check(sf_set_time(ctx_, 0, t)); }
@@ -270,6 +273,10 @@ void Solver::write_thickness_img(const std::string& s, int frame) {
     char name[32];
     std::snprintf(name, sizeof name, "frame%04d.exr", frame);
     check(sf_write_thickness_exr(ctx_, 0, (s + name).c_str()));
+    // broken unless some cell has eta > 0 (solver.cu:568-575)
+    sf_bubble_stats st;
+    check(sf_get_stats(ctx_, 0, &st));
+    broken_ = st.no_film != 0;
 }
 
 void Solver::copyToCPU(sf_field field, std::vector<double>& out) {
@@ -312,10 +319,15 @@ int Bubble::run() {
     int i = 1;
     for (; i < frames_; ++i) {
         solver.setTime(T);
-        while ((T + dt_) <= i * DT_ && !solver.isBroken()) {
-            solver.stepForward();
+        // the frame's whole steps, counted as the reference advances T; a
+        // failed solve stops the bubble on the device, so the steps after it
+        // do nothing, as the reference's loop stops at isBroken()
+        int n = 0;
+        while ((T + dt_) <= i * DT_) {
             T += dt_;
+            ++n;
         }
+        if (n > 0 && !solver.isBroken()) solver.stepForward(n);
         if ((T + eps) < i * DT_ && !solver.isBroken()) solver.stepForward(i * DT_ - T);
         if (solver.isBroken()) {
             std::cerr << "Film is broken." << std::endl;

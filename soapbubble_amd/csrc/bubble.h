@@ -67,6 +67,14 @@ public:
     void initWithConst(sf_field field, double val);       // initializer.cu:22-31
     void stepForward();                                   // solver.cu:260-292
     void stepForward(double dt);                          // solver.cu:252-257, dt in s
+    // n steps of stepForward() as one call: a solve that misses its
+    // tolerance stops the bubble on the device (its later steps are no-ops),
+    // so this equals the reference's loop of stepForward() guarded by
+    // isBroken() (bubble.cu:2669), and the steps replay as captured graphs
+    void stepForward(int n);
+    // the broken flag of write_thickness_img (no cell with eta > 0 at the
+    // last frame written, solver.cu:568-575) or a solve that missed its
+    // tolerance (AlgebraicMultiGridCG, bubble.cu:1500-1507)
     bool isBroken();                                      // solver.cu:296-298
     void setBroken(bool broken) { broken_ = broken; }
     void setTime(double t);                               // currentTimeGlobal (s)

@@ -90,6 +90,11 @@ X* dalloc(size_t n, std::vector<void*>& owned) {
 template <typename X>
 std::vector<X> xcd_order(const std::vector<X>& work, X pad, int nxcd);
 
+// A schedule longer than this many inner iterations in a refinement (a solve
+// far from converging, NaN input, or a stiff operator for Chebyshev) runs
+// host-driven instead of as a graph of hundreds of mostly skipped launches.
+constexpr int kCaptureMaxInner = 48;
+
 // Everything that does not depend on the precision.
 struct Base {
     sf_options opt{};
@@ -180,6 +185,11 @@ struct Base {
     // one inner Chebyshev launch (iteration `it` of refinement `kr`)
     virtual void inner_launch(int it, int kr) = 0;
     virtual bool fp64_inner() const = 0;
+    // switches the inner solve to PCG with the multigrid preconditioner
+    virtual void enable_mg() = 0;
+    // the automatic inner method (Chebyshev) may switch to multigrid when
+    // a refinement needs more inner iterations than a captured step holds
+    bool auto_method() const { return opt.inner_method == 0 && !fp64_inner(); }
     // drops the stored solve corrections of bubble b (all when b < 0)
     void reset_guess(int b) {
         host_hist = 0;
@@ -220,6 +230,11 @@ struct Impl : Base {
         return SysPtrs<Ti>{b64, x_init, wN, wW, dg, e, r, z, Ap, rsteps, acc};
     }
     bool fp64_inner() const override { return sizeof(Ti) == 8; }
+    void enable_mg() override {
+        if (mgp.empty()) alloc_mg();
+        mg = true;
+        method = 1;
+    }
 
     void alloc_fields() {
         const size_t n = (size_t)ncells;
@@ -482,6 +497,22 @@ struct Impl : Base {
                                 sizeof(double) * kAccBubble, w, nb, hipMemcpyDeviceToHost, stream));
         SF_HIP(hipStreamSynchronize(stream));
     }
+    // the largest sub-accumulator of value v of bubble b (a maximum slot)
+    double slot_max(int b, int v) const {
+        double m = 0.0;
+        for (int q = 0; q < kAccSub; ++q) m = std::max(m, slot_h[((size_t)b * kAccVals + v) * kAccSub + q]);
+        return m;
+    }
+    // Chebyshev on the line-Jacobi interval [1 - mu, 1 + mu] reduces the
+    // residual by about mu / (1 + sqrt(1 - mu^2)) an iteration: above mu =
+    // 0.96 the FP32 inner solve's 1e-6 takes more than the captured
+    // schedule holds (kCaptureMaxInner). The bound of the assembled system.
+    bool stiff_for_chebyshev() {
+        fetch_slot(acc_slot_res(0));
+        for (int b = 0; b < nb; ++b)
+            if (slot_max(b, 4) > 0.96) return true;
+        return false;
+    }
     double slot_at(int b, int v) const {
         double s = 0.0;
         for (int q = 0; q < kAccSub; ++q) s += slot_h[((size_t)b * kAccVals + v) * kAccSub + q];
@@ -508,19 +539,33 @@ struct Impl : Base {
     // The inner solve of a refinement is done for a bubble when its inner
     // residual meets the final tolerance (times inner_frac) or has fallen by
     // the factor inner_rho from the refinement's start.
-    bool cheb_done(const std::vector<Start>& st, std::vector<double>& pb) {
+    // ... or, for a bubble whose line-Jacobi bound mu is at most 0.9, when
+    // it stagnates at the FP32 rounding floor (a target the FP32 iteration
+    // cannot reach, such as an absolute tolerance far below the rounding of
+    // |b|): |r|^2 has fallen by less than a factor 2 over the last eight
+    // iterations, where Chebyshev with mu <= 0.9 reduces it by more than
+    // 1e4 (the FP64 refinement continues from there; closer to mu = 1 the
+    // Chebyshev residual is not monotone enough for this test). `hist`
+    // holds each bubble's |r|^2 of the iterations so far, `mu` the bounds.
+    bool cheb_done(const std::vector<Start>& st, std::vector<double>& pb,
+                   std::vector<std::vector<double>>& hist, const std::vector<double>& mu) {
         SF_HIP(hipMemcpyAsync(pb.data(), partB + stride, sizeof(double) * stride,
                               hipMemcpyDeviceToHost, stream));
         SF_HIP(hipStreamSynchronize(stream));
+        bool all = true;
         for (int b = 0; b < nb; ++b) {
             if (st[b].done) continue;
             double rr = 0.0;
             for (int j = 0; j < bub_h[b].nth; ++j) rr += pb[bub_h[b].pr0 + j];
+            hist[(size_t)b].push_back(rr);
+            const std::vector<double>& h = hist[(size_t)b];
             const double tgt = sp.inner_frac * sp.tol;
-            if (!(rr <= tgt * tgt * (sp.absolute ? 1.0 : st[b].bb) || rr <= sp.inner_rho2 * st[b].rin))
-                return false;
+            const bool stalled = mu[(size_t)b] <= 0.9 && h.size() > 8 && !(rr < 0.5 * h[h.size() - 9]);
+            if (!(rr <= tgt * tgt * (sp.absolute ? 1.0 : st[b].bb) || rr <= sp.inner_rho2 * st[b].rin ||
+                  stalled))
+                all = false;
         }
-        return true;
+        return all;
     }
     void inner(int it) {
         for (const LaunchGeom& rg : rgeo)
@@ -554,12 +599,24 @@ struct Impl : Base {
     // the calibrated schedule, every kernel launched, converged bubbles
     // skip the work (graph capture).
     void solve(bool host_driven, bool finalize) override {
+        if (host_driven && method == 2 && auto_method() && stiff_for_chebyshev()) {
+            // the automatic inner method continues with multigrid: its
+            // coarse levels and first cycle for this solve, then the PCG
+            // branch below
+            if (trace_solve) std::fprintf(stderr, "[sf solve] stiff operator: multigrid from here on\n");
+            enable_mg();
+            mg_coarsen();
+            mg_vcycle(true, partR + 2 * stride);
+        }
         std::vector<int> used;
         if (method == 2) {
             // Chebyshev: kernel `it` measures the residual of e_it and
             // produces e_{it+1}; a refinement of n kernels hands e_{n+1} on
             if (host_driven) {
                 std::vector<double> pb(stride);
+                std::vector<double> mu((size_t)nb);
+                fetch_slot(acc_slot_res(0));
+                for (int b = 0; b < nb; ++b) mu[(size_t)b] = slot_max(b, 4);
                 for (int k = 0; k < max_outer; ++k) {
                     const std::vector<Start> st = refinement_start(k);
                     bool done = true;
@@ -570,14 +627,16 @@ struct Impl : Base {
                                      done ? " converged" : "");
                     if (done) break;
                     int n = max_inner;
+                    std::vector<std::vector<double>> rhist((size_t)nb);
                     for (int it = 1; it <= max_inner; ++it) {
                         cheb(it, k);
-                        if (cheb_done(st, pb)) { n = it; break; }
+                        if (cheb_done(st, pb, rhist, mu)) { n = it; break; }
                     }
                     used.push_back(n);
                     if (trace_solve) std::fprintf(stderr, "[sf solve]   chebyshev iterations %d\n", n);
                     resid(k + 1, cheb_e(n + 1), nullptr);
                 }
+
             } else {
                 // a refinement of no inner iteration (only ever the last one:
                 // its correction z is read by its residual launch, which then
@@ -664,12 +723,14 @@ struct Impl : Base {
         SF_HIP(hipMemcpyAsync(dst + B.off, tmp.data(), N * sizeof(T), hipMemcpyHostToDevice, stream));
         if (field == SF_FIELD_ETA || field == SF_FIELD_GAMMA) bimocq_dirty = true;
         if (field <= SF_FIELD_GAMMA) reset_guess(b);
-        if (nan && field <= SF_FIELD_GAMMA) {
-            // the advection samples this bubble with NaN-skipping interpolation
+        if (field <= SF_FIELD_GAMMA) {
+            // a new state steps again after a failed solve; the advection
+            // samples a state with NaNs by NaN-skipping interpolation
             BubbleTime bt;
             SF_HIP(hipMemcpyAsync(&bt, time_d + b, sizeof(bt), hipMemcpyDeviceToHost, stream));
             SF_HIP(hipStreamSynchronize(stream));
-            bt.nans = 1;
+            bt.stop &= 1;
+            if (nan) bt.nans = 1;
             SF_HIP(hipMemcpyAsync(time_d + b, &bt, sizeof(bt), hipMemcpyHostToDevice, stream));
         }
         SF_HIP(hipStreamSynchronize(stream));
@@ -881,7 +942,8 @@ void setup(Base& c, int prec) {
         ct += B.nph;
         pr += B.nth;
         nph_max = std::max(nph_max, B.nph);
-        c.time_h[b] = BubbleTime{0.0, d.dt * 1.0, c.opt.advance_time, 0, 0, 0, 0, 0};
+        c.time_h[b] = BubbleTime{0.0, d.dt * 1.0, c.opt.advance_time, 0, 0, 0, 0,
+                                 c.opt.halt_on_failure ? 1 : 0};
     }
     c.ncells = off;
     c.nrow_total = pr;
@@ -1126,10 +1188,9 @@ constexpr int kAdaptChunk = 4;
 // Steps captured in one graph launch (a whole chunk: measured 1, 2 and 4
 // steps per launch, 4 the fastest).
 constexpr int kGraphSteps = 4;
-// A schedule longer than this many inner iterations in a refinement (a solve
-// far from converging, NaN input) runs host-driven instead of as a graph of
-// hundreds of mostly skipped launches; the next sf_step calibrates again.
-constexpr int kGraphMaxInner = 48;
+// The cap on a captured schedule (kCaptureMaxInner); the next sf_step after
+// a longer one calibrates again.
+constexpr int kGraphMaxInner = kCaptureMaxInner;
 
 void drop_calibration(Base& c) {
     c.calibrated = false;
@@ -1192,6 +1253,7 @@ void fill_stats(Base& c, int b, sf_bubble_stats* out) {
     out->residual = sc.rel_res;
     out->volume = vol * B.h * B.h;
     out->broken = (pos == 0.0) || (c.steps_done > 0 && !sc.converged);
+    out->no_film = pos == 0.0 ? 1 : 0;
     out->steps = c.steps_done;
     out->time = bt.t + (bt.pending ? bt.dt_s : 0.0);
     out->failed_solves = hs.fails;
@@ -1233,6 +1295,8 @@ int validate_create(const sf_options* opt, int32_t nbubbles, const sf_bubble_des
         return fail(nullptr, SF_ERR_INVALID, "scalar_advection must be 0 (plain) or 1 (BIMOCQ)");
     if (opt->max_outer > kAccRefine) return fail(nullptr, SF_ERR_INVALID, "max_outer is at most 16");
     if (opt->viscous < 0 || opt->viscous > 1) return fail(nullptr, SF_ERR_INVALID, "viscous must be 0 or 1");
+    if (opt->halt_on_failure < 0 || opt->halt_on_failure > 1)
+        return fail(nullptr, SF_ERR_INVALID, "halt_on_failure must be 0 or 1");
     return SF_OK;
 }
 
@@ -1426,7 +1490,9 @@ int sf_set_dt(sf_ctx* ctx, int32_t b, double dt_seconds) {
             }
             // timeStepGlobal = dt U / radius with the member U = 1 (bubble.cu:2675)
             const double dtn = dt_seconds * 1.0 / c.der[k].radius;
-            if (dtn != c.bub_h[k].dt) {
+            // a change of more than a relative 1e-9 (not the rounding of a
+            // frame's last step that lands on the frame's end)
+            if (std::fabs(dtn - c.bub_h[k].dt) > 1e-9 * c.bub_h[k].dt) {
                 // the stored solve corrections and the inner schedule belong
                 // to the old step
                 c.time_h[k].ghist = c.time_h[k].gpend = 0;
@@ -1453,7 +1519,8 @@ int sf_step(sf_ctx* ctx, int32_t nsteps) {
         // solve after a reset; steps before that run host-driven
         const int warm = c.guess ? 2 : 0;
         int k = 0;
-        if (want_graph && !c.calibrated) {
+        for (int pass = 0; pass < 2 && want_graph && !c.calibrated; ++pass) {
+            const bool was_mg = c.mg;
             // calibration: one host-driven step with the guess in effect sets
             // the first schedule
             for (; k < nsteps && c.calib_steps < 1; ++k) {
@@ -1475,6 +1542,9 @@ int sf_step(sf_ctx* ctx, int32_t nsteps) {
                     c.sched.push_back(c.opt.max_inner > 0 ? c.opt.max_inner
                                                           : std::max(c.method == 2 ? n - 1 : n, 1));
                 if (c.sched.empty()) c.sched.push_back(1);
+                // a solve that misses its tolerance stops the bubble when
+                // halt_on_failure is set: one more refinement of margin
+                if (c.opt.halt_on_failure) c.sched.push_back(1);
                 c.sched.push_back(1);
                 if (c.opt.max_outer > 0) c.sched.resize((size_t)c.opt.max_outer, 1);
                 if (c.sched.size() > (size_t)kAccRefine) c.sched.resize(kAccRefine);
@@ -1485,7 +1555,12 @@ int sf_step(sf_ctx* ctx, int32_t nsteps) {
                 // residual launch): one skipped launch per step fewer
                 if (c.method == 2 && c.sched.size() >= 2) c.sched.back() = 0;
                 c.calibrated = true;
-                if (c.sched[0] > kGraphMaxInner) drop_calibration(c);
+                if (c.sched[0] > kGraphMaxInner) {
+                    drop_calibration(c);
+                    // the calibration step switched to multigrid: calibrate
+                    // that schedule
+                    if (c.mg && !was_mg) continue;
+                }
                 if (c.trace_solve) {
                     std::fprintf(stderr, "[sf solve] graph schedule:");
                     for (int n : c.sched) std::fprintf(stderr, " %d", n);

@@ -40,8 +40,17 @@ struct BubbleTime {
     // advection to apply
     int ghist;
     int gpend;
-    int pad_;
+    // stop on a failed solve (the reference's `broken`, bubble.cu:1500-1507,
+    // after which its frame loop takes no further step): bit 0 the option,
+    // bit 1 set by the update kernel when this bubble's solve missed its
+    // tolerance, bit 2 set by the next step's assembly. Kernels before the
+    // assembly skip a bubble with bits 0 and 1, those after it one with bit
+    // 2, so each launch reads a bit that an earlier launch wrote.
+    int stop;
 };
+
+__device__ __forceinline__ bool stopped_early(const BubbleTime& t) { return (t.stop & 3) == 3; }
+__device__ __forceinline__ bool stopped(const BubbleTime& t) { return (t.stop & 4) != 0; }
 
 // Per-bubble control block of the iterative solve. Every field is written by
 // one kernel (the block of a row launch that arrives last, see arrive_last)

@@ -129,7 +129,7 @@ __global__ __launch_bounds__(256) void k_advect(const int4* __restrict__ tiles,
                                                 BubbleTime* __restrict__ btime, Tables tab,
                                                 StatePtrs<T> in, StatePtrs<T> out) {
     const int4 tl = tiles[blockIdx.x];
-    if (tl.x < 0) return;
+    if (tl.x < 0 || stopped_early(btime[tl.x])) return;
     const BubbleDev B = bub[tl.x];
     // the previous step's time advance lands here, before any kernel of this
     // step reads the time
@@ -722,7 +722,7 @@ __device__ __forceinline__ int line_solve(Ti (&a)[C], Ti (&c)[C], Ti (&d)[C], in
 // to the state as the reference does.
 template <typename T, typename Ti, int C, bool VISC>
 __global__ void k_assemble(const int2* __restrict__ rows, const BubbleDev* __restrict__ bub,
-                           const BubbleTime* __restrict__ btime, Tables tab, StatePtrs<T> s,
+                           BubbleTime* __restrict__ btime, Tables tab, StatePtrs<T> s,
                            AirPtrs<T> air, SysPtrs<Ti> sys, T gmax_state,
                            SolveCtl* __restrict__ ctl, double* __restrict__ part, long stride,
                            const T* __restrict__ guess, SolveParams sp) {
@@ -731,6 +731,15 @@ __global__ void k_assemble(const int2* __restrict__ rows, const BubbleDev* __res
     __shared__ float redf[16];
     const int2 rw = rows[blockIdx.x];
     if (rw.x < 0) return;
+    if (stopped_early(btime[rw.x])) {
+        // a bubble stopped by a failed solve: its solve kernels return at
+        // once, its verdict stays the failed one
+        if (rw.y == 0 && threadIdx.x == 0) {
+            flag_store(&ctl[rw.x].outer_done, 1);
+            btime[rw.x].stop |= 4;
+        }
+        return;
+    }
     const BubbleDev B = bub[rw.x];
     const int j = rw.y, nph = B.nph;
     const bool act = (int)threadIdx.x < nph / C;
@@ -1361,7 +1370,9 @@ __global__ void k_cheb(const int2* __restrict__ rows, const BubbleDev* __restric
         return;
     }
     if (leader) Cb->iters += 1;
-    const double delta = fmax(mu * 1.02 + 1e-3, 1e-4);
+    // the interval [1 - delta, 1 + delta] widened a little, and kept clear of
+    // zero (a lower end at or below zero makes the iteration diverge)
+    const double delta = fmin(fmax(mu * 1.02 + 1e-3, 1e-4), 0.9999);
     const double sigma = 1.0 / delta;
     double rho_prev = delta, rho = delta;      // rho_0 = 1 / sigma
     for (int q = 1; q <= it; ++q) {
@@ -1613,12 +1624,14 @@ __global__ __launch_bounds__(128) void k_update(const int4* __restrict__ tiles,
                                                 double* __restrict__ acc) {
     __shared__ double red[16 * 4];
     const int4 tl = tiles[blockIdx.x];
-    if (tl.x < 0) return;
+    if (tl.x < 0 || stopped(btime[tl.x])) return;
     const BubbleDev B = bub[tl.x];
     if (tl.y == 0 && tl.z == 0) {
         // first tile of the bubble: solve verdict, and the time advance that
         // the next step's advection applies
         if (fin) finalize_block(B, ctl + tl.x, hist + tl.x, sp, partR, stride, red);
+        if (fin && threadIdx.x == 0 && (btime[tl.x].stop & 1) && !ctl[tl.x].converged)
+            btime[tl.x].stop |= 2;
         if (threadIdx.x == 0 && btime[tl.x].advance) btime[tl.x].pending = 1;
         if (threadIdx.x == 0 && gdel) btime[tl.x].gpend = 1;
         // the next assembly adds its sums into a cleared slot
@@ -1813,7 +1826,7 @@ __global__ __launch_bounds__(256) void k_bm_maps(const int4* __restrict__ tiles,
                                                  const BubbleTime* __restrict__ btime, Tables tab,
                                                  StatePtrs<T> cur, BimocqPtrs P) {
     const int4 tl = tiles[blockIdx.x];
-    if (tl.x < 0) return;
+    if (tl.x < 0 || stopped_early(btime[tl.x])) return;
     const BubbleDev B = bub[tl.x];
     int j, i;
     tile_cell(tl, j, i);
@@ -1884,7 +1897,7 @@ __global__ __launch_bounds__(256) void k_bm_accum(const int4* __restrict__ tiles
                                                   BimocqPtrs P) {
     __shared__ float red[8];
     const int4 tl = tiles[blockIdx.x];
-    if (tl.x < 0) return;
+    if (tl.x < 0 || stopped(P.bt[tl.x])) return;
     const BubbleDev B = bub[tl.x];
     int j, i;
     tile_cell(tl, j, i);
@@ -1943,6 +1956,10 @@ __global__ __launch_bounds__(256) void k_bm_accum(const int4* __restrict__ tiles
 __global__ void k_bm_decide(int nb, BimocqPtrs P) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nb) return;
+    if (stopped(P.bt[b])) {
+        P.flag[b] = 0;
+        return;
+    }
     const float d = __uint_as_float(P.dist_bits[b]);
     P.dist_bits[b] = 0u;
     P.dist_last[b] = (double)d;
@@ -2079,7 +2096,7 @@ void launch_advect(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, Bub
 
 template <typename T, typename Ti>
 void launch_assemble(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub,
-                     const BubbleTime* bt, const Tables& tab, StatePtrs<T> st, AirPtrs<T> air,
+                     BubbleTime* bt, const Tables& tab, StatePtrs<T> st, AirPtrs<T> air,
                      SysPtrs<Ti> sys, double gmax_state, SolveCtl* ctl, double* part, long stride,
                      const T* guess, const SolveParams& sp) {
 #define SF_CALL(C)                                                                                 \
@@ -2230,7 +2247,7 @@ template void launch_advect<double>(hipStream_t, const LaunchGeom&, const Bubble
                                     const Tables&, StatePtrs<double>, StatePtrs<double>);
 #define SF_INST_SYS(T, Ti)                                                                        \
     template void launch_assemble<T, Ti>(hipStream_t, const LaunchGeom&, const BubbleDev*,        \
-                                         const BubbleTime*, const Tables&, StatePtrs<T>,          \
+                                         BubbleTime*, const Tables&, StatePtrs<T>,                \
                                          AirPtrs<T>, SysPtrs<Ti>, double, SolveCtl*, double*,     \
                                          long, const T*, const SolveParams&);                     \
     template void launch_resid<T, Ti>(hipStream_t, const LaunchGeom&, const BubbleDev*,           \

@@ -116,7 +116,7 @@ void launch_advect(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, Bub
                    const Tables& tab, StatePtrs<T> in, StatePtrs<T> out);
 template <typename T, typename Ti>
 void launch_assemble(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub,
-                     const BubbleTime* bt, const Tables& tab, StatePtrs<T> st, AirPtrs<T> air,
+                     BubbleTime* bt, const Tables& tab, StatePtrs<T> st, AirPtrs<T> air,
                      SysPtrs<Ti> sys, double gmax_state, SolveCtl* ctl, double* part, long stride,
                      const T* guess, const SolveParams& sp);
 template <typename T, typename Ti>

@@ -82,3 +82,27 @@ def test_short_hierarchies_ensemble(oracle):
         e = max(rel_l2(got[k][f], ref[f]) for f in FIELDS)
         _say(f"  bubble {k} ({n}x{2 * n}): worst field rel L2 {e:.2e}")
         assert e < 5e-5, (k, n, e)
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_automatic_switch_in_the_stiff_regime(oracle, graph):
+    """The automatic inner method starts with Chebyshev; the first solve at
+    the stiff config needs far more iterations than a captured step holds,
+    so the context continues with multigrid (later solves: a handful of
+    V-cycles), eager or under graph replay, and still matches the oracle."""
+    n, dt, radius, steps = 256, 0.005, 0.05, 8
+    _say(f"automatic inner method, stiff {n}x{2 * n}, graph={graph}")
+    st, ua, va = make_case(n, 23, l_max=8, u_scale=0.3)
+    with sb.SoapFilm(sb.bubble(n, dt, radius=radius), "mixed", use_graph=graph) as ctx:
+        load(ctx, 0, st, ua, va)
+        ctx.step(steps)
+        ctx.synchronize()
+        got = ctx.state(0)
+        s = ctx.stats(0)
+    _say(f"  last solve: {s['iterations']} inner iterations, {s['outer']} refinements")
+    assert s["failed_solves"] == 0 and s["max_residual"] <= 1e-10, s
+    assert s["iterations"] <= 12, s
+    ref = copy_state(st)
+    oracle.step(oracle.params(n, dt, radius=radius), ref, ua, va, steps)
+    for k in FIELDS:
+        assert rel_l2(got[k], ref[k]) < 5e-5, k
