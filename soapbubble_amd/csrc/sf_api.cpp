@@ -1013,6 +1013,7 @@ void setup(Base& c, int prec) {
         g.rows = rows_d;
         g.nrows = (int)rows.size();
         g.cpt = cpt;
+        g.pow2_rows = is_pow2(nph);
         g.row_threads = std::max(64, (nph / cpt + 63) / 64 * 64);
         // the reduced system of R = nph / C interface unknowns: 6 R values
         // (8 R when R is no power of two: the correction's second right-hand

@@ -354,14 +354,22 @@ __device__ __forceinline__ void ld_row(const X* __restrict__ p, X (&v)[C]) {
 #pragma unroll
     for (int k = 0; k < C; ++k) v[k] = q[k];
 }
+// A phi column wrapped into the row: for rows known at compile time to be
+// of power-of-two length one AND (P2), else wrap_col.
+template <bool P2>
+__device__ __forceinline__ int wrap_t(int x, int nph, int m) {
+    if constexpr (P2) return x & m;
+    else return wrap_col(x, nph, m);
+}
+
 // C consecutive cells of a row from column `start` on, wrapping past the
 // row's end: the phi + pi cells across a pole. One aligned vector when the
 // block stays inside the row on a multiple of C (always for power-of-two
 // rows, where n_theta is a multiple of C), else cell by cell.
-template <int C, typename X>
+template <int C, bool P2, typename X>
 __device__ __forceinline__ void ld_row_wrap(const X* __restrict__ rowp, int start, int nph, int m,
                                             X (&v)[C]) {
-    if (m >= 0 || (start % C == 0 && start + C <= nph)) {
+    if (P2 || (start % C == 0 && start + C <= nph)) {
         ld_row<C>(rowp + start, v);
         return;
     }
@@ -384,21 +392,21 @@ __device__ __forceinline__ void st_row(X* __restrict__ p, const X (&v)[C]) {
 // sides come as row vectors; column (nph - i0) mod nph is the one extra.
 // FP32 fields only; the blend in FP32 too (the reference's float
 // arithmetic), which keeps the 3 C + 1 results in single registers.
-template <int C, int N, bool MINUS>
+template <int C, int N, bool MINUS, bool P2>
 __device__ __forceinline__ void air_blend(const float* __restrict__ a0,
                                           const float* __restrict__ shape, size_t row, int i0,
                                           int nph, int m, float kt, float bl, float (&out)[N]) {
     float a[C], sh[C], am[C], sm[C];
     ld_row<C>(a0 + row + i0, a);
     ld_row<C>(shape + row + i0, sh);
-    const size_t mb = row + (size_t)(nph - i0 - C), cm = row + wrap_col(nph - i0, nph, m);
+    const size_t mb = row + (size_t)(nph - i0 - C), cm = row + wrap_t<P2>(nph - i0, nph, m);
     ld_row<C>(a0 + mb, am);
     ld_row<C>(shape + mb, sm);
     auto val = [&](float x, float y) {
         const float v = fmaf(y, kt, x);
         return __builtin_isnan(v) ? 0.0f : v;
     };
-    const size_t ce = row + wrap_col(i0 + C, nph, m);
+    const size_t ce = row + wrap_t<P2>(i0 + C, nph, m);
 #pragma unroll
     for (int k = 0; k < N; ++k) {
         const float own = k < C ? val(a[k], sh[k]) : val(a0[ce], shape[ce]);
@@ -422,7 +430,7 @@ struct RowSys {
 
 // With `guess` (assembly only) the initial iterate is Gamma* + guess instead
 // of Gamma* on cells that are not broken.
-template <typename T, int C, bool RHS, bool VISC = false>
+template <typename T, int C, bool RHS, bool VISC, bool P2>
 __device__ __forceinline__ void row_system(const BubbleDev& B, double t, const Tables& tab,
                                            const StatePtrs<T>& s, const AirPtrs<T>& air, int j,
                                            int i0, RowSys<C>& q,
@@ -434,16 +442,16 @@ __device__ __forceinline__ void row_system(const BubbleDev& B, double t, const T
     // neighbours across the poles are the phi + pi cells of the same row
     // (bubble.cu:1384-1401); their face weights vanish with sin(theta_face)
     const size_t rowN = hasN ? row - nph : row, rowS = hasS ? row + nph : row;
-    const int iNS = hasN ? i0 : wrap_col(i0 + nth, nph, m);
-    const int iSS = hasS ? i0 : wrap_col(i0 + nth, nph, m);
+    const int iNS = hasN ? i0 : wrap_t<P2>(i0 + nth, nph, m);
+    const int iSS = hasS ? i0 : wrap_t<P2>(i0 + nth, nph, m);
     T ev[C], eNv[C], eSv[C], gv[C], gNv[C], gSv[C];
     ld_row<C>(s.eta + row + i0, ev);
     ld_row<C>(s.gam + row + i0, gv);
-    ld_row_wrap<C>(s.eta + rowN, iNS, nph, m, eNv);
-    ld_row_wrap<C>(s.gam + rowN, iNS, nph, m, gNv);
-    ld_row_wrap<C>(s.eta + rowS, iSS, nph, m, eSv);
-    ld_row_wrap<C>(s.gam + rowS, iSS, nph, m, gSv);
-    const int iWc = wrap_col(i0 - 1, nph, m), iEc = wrap_col(i0 + C, nph, m);
+    ld_row_wrap<C, P2>(s.eta + rowN, iNS, nph, m, eNv);
+    ld_row_wrap<C, P2>(s.gam + rowN, iNS, nph, m, gNv);
+    ld_row_wrap<C, P2>(s.eta + rowS, iSS, nph, m, eSv);
+    ld_row_wrap<C, P2>(s.gam + rowS, iSS, nph, m, gSv);
+    const int iWc = wrap_t<P2>(i0 - 1, nph, m), iEc = wrap_t<P2>(i0 + C, nph, m);
     const double eW = (double)s.eta[row + iWc], eE = (double)s.eta[row + iEc];
     double gW = (double)s.gam[row + iWc], gE = (double)s.gam[row + iEc];
 
@@ -475,10 +483,10 @@ __device__ __forceinline__ void row_system(const BubbleDev& B, double t, const T
         ld_row<C>(guess + row + i0, gc);
 #pragma unroll
         for (int k = 0; k < C; ++k) gxv[k] = gv[k] + gc[k];
-        ld_row_wrap<C>(guess + rowN, iNS, nph, m, gc);
+        ld_row_wrap<C, P2>(guess + rowN, iNS, nph, m, gc);
 #pragma unroll
         for (int k = 0; k < C; ++k) gNv[k] = gNv[k] + gc[k];
-        ld_row_wrap<C>(guess + rowS, iSS, nph, m, gc);
+        ld_row_wrap<C, P2>(guess + rowS, iSS, nph, m, gc);
 #pragma unroll
         for (int k = 0; k < C; ++k) gSv[k] = gSv[k] + gc[k];
         gW = (double)(T)(gW + (double)guess[row + iWc]);
@@ -531,14 +539,14 @@ __device__ __forceinline__ void row_system(const BubbleDev& B, double t, const T
             // FP64 fields (twice the registers) per column.
             if constexpr (sizeof(T) == 4) {
                 const float ktf = (float)kt, blf = (float)bl;
-                air_blend<C, C + 1, false>(air.ua0, air.au, row, i0, nph, m, ktf, blf, uAirF);
+                air_blend<C, C + 1, false, P2>(air.ua0, air.au, row, i0, nph, m, ktf, blf, uAirF);
                 if (hasN)
-                    air_blend<C, C, true>(air.va0, air.av, row - nph, i0, nph, m, ktf, blf, vAirNv);
-                if (hasS) air_blend<C, C, true>(air.va0, air.av, row, i0, nph, m, ktf, blf, vAirSv);
+                    air_blend<C, C, true, P2>(air.va0, air.av, row - nph, i0, nph, m, ktf, blf, vAirNv);
+                if (hasS) air_blend<C, C, true, P2>(air.va0, air.av, row, i0, nph, m, ktf, blf, vAirSv);
             } else {
 #pragma unroll
                 for (int k = 0; k <= C; ++k) {
-                    const int i = wrap_col(i0 + k, nph, m), opp = wrap_col(nph - i, nph, m);
+                    const int i = wrap_t<P2>(i0 + k, nph, m), opp = wrap_t<P2>(nph - i, nph, m);
                     uAirF[k] = 0.5 * (air_at(air.ua0, air.au, row + i, kt) * bl +
                                       air_at(air.ua0, air.au, row + opp, kt) * (1 - bl));
                 }
@@ -580,7 +588,7 @@ __device__ __forceinline__ void row_system(const BubbleDev& B, double t, const T
         if (B.airflow) {
             double vAirN = (double)vAirNv[k], vAirS = (double)vAirSv[k];
             if constexpr (sizeof(T) != 4) {
-                const int opp = wrap_col(nph - i, nph, m);
+                const int opp = wrap_t<P2>(nph - i, nph, m);
                 if (hasN) {
                     const size_t rn = row - nph;
                     vAirN = 0.5 * (air_at(air.va0, air.av, rn + i, kt) * bl -
@@ -599,7 +607,7 @@ __device__ __forceinline__ void row_system(const BubbleDev& B, double t, const T
             // `gravity` with `VISCOUS` (bubble.cu:1458-1464): the viscous
             // forces in place of the gravity term
             const size_t cc = row + i;
-            const double fuW = air.fu[cc], fuE = air.fu[row + wrap_col(i + 1, nph, m)];
+            const double fuW = air.fu[cc], fuE = air.fu[row + wrap_t<P2>(i + 1, nph, m)];
             const double fvN = hasN ? air.fv[cc - nph] : 0.0;
             const double fvS = hasS ? air.fv[cc] : 0.0;
             div += B.invh * dt *
@@ -720,7 +728,7 @@ __device__ __forceinline__ int line_solve(Ti (&a)[C], Ti (&c)[C], Ti (&d)[C], in
 // matrix. Partials as k_resid. The block of row 0 resets the bubble's
 // solver control block; the broken-cell override of Gamma* is written back
 // to the state as the reference does.
-template <typename T, typename Ti, int C, bool VISC>
+template <typename T, typename Ti, int C, bool VISC, bool P2>
 __global__ void k_assemble(const int2* __restrict__ rows, const BubbleDev* __restrict__ bub,
                            BubbleTime* __restrict__ btime, Tables tab, StatePtrs<T> s,
                            AirPtrs<T> air, SysPtrs<Ti> sys, T gmax_state,
@@ -755,7 +763,7 @@ __global__ void k_assemble(const int2* __restrict__ rows, const BubbleDev* __res
     if (act) {
         RowSys<C> q;
         // the extrapolated correction once a previous solve stored one
-        row_system<T, C, true, VISC>(B, t, tab, s, air, j, i0, q,
+        row_system<T, C, true, VISC, P2>(B, t, tab, s, air, j, i0, q,
                                      btime[rw.x].ghist > 0 ? guess : nullptr);
         Ti wn[C], ww[C], dg[C];
 #pragma unroll
@@ -896,7 +904,7 @@ __device__ __forceinline__ void ld_stencil(const Ti* __restrict__ v, size_t row,
 // FIRST (k = 1): x_0 is formed again here, bitwise as the assembly formed it
 // (Gamma* plus the stored guess, gamma_max on broken cells), instead of being
 // stored by the assembly and read back.
-template <typename T, typename Ti, int C, bool FIRST>
+template <typename T, typename Ti, int C, bool FIRST, bool P2>
 __global__ void k_resid(const int2* __restrict__ rows, const BubbleDev* __restrict__ bub,
                         const BubbleTime* __restrict__ btime, Tables tab, StatePtrs<T> s,
                         SolveCtl* __restrict__ ctl, SysPtrs<Ti> sys, const T* __restrict__ guess,
@@ -933,7 +941,7 @@ __global__ void k_resid(const int2* __restrict__ rows, const BubbleDev* __restri
         double xc[C], xN[C], xS[C], xW, xE;
         RowSys<C> w;
         if constexpr (FIRST) {
-            row_system<T, C, false>(B, 0.0, tab, s, AirPtrs<T>{}, j, i0, w,
+            row_system<T, C, false, false, P2>(B, 0.0, tab, s, AirPtrs<T>{}, j, i0, w,
                                     btime[rw.x].ghist > 0 ? guess : nullptr);
 #pragma unroll
             for (int q = 0; q < C; ++q) {
@@ -945,7 +953,7 @@ __global__ void k_resid(const int2* __restrict__ rows, const BubbleDev* __restri
             xE = w.x0E;
         } else {
             ld_stencil<double, C>(xo, row, i0, m, nph, hasN, hasS, xc, xW, xE, xN, xS);
-            row_system<T, C, false>(B, 0.0, tab, s, AirPtrs<T>{}, j, i0, w);
+            row_system<T, C, false, false, P2>(B, 0.0, tab, s, AirPtrs<T>{}, j, i0, w);
         }
         Ti ec[C], eN[C], eS[C], eW, eE;
         ld_stencil<Ti, C>(ecorr, row, i0, m, nph, hasN, hasS, ec, eW, eE, eN, eS);
@@ -2099,19 +2107,21 @@ void launch_assemble(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub,
                      BubbleTime* bt, const Tables& tab, StatePtrs<T> st, AirPtrs<T> air,
                      SysPtrs<Ti> sys, double gmax_state, SolveCtl* ctl, double* part, long stride,
                      const T* guess, const SolveParams& sp) {
+#define SF_ASM(C, V, P)                                                                            \
+    hipLaunchKernelGGL((k_assemble<T, Ti, C, V, P>), dim3(g.nrows), dim3(g.row_threads),           \
+                       g.lds_bytes, s, g.rows, bub, bt, tab, st, air, sys, (T)gmax_state, ctl,     \
+                       part, stride, guess, sp)
 #define SF_CALL(C)                                                                                 \
     do {                                                                                           \
-        if (air.fu)                                                                                \
-            hipLaunchKernelGGL((k_assemble<T, Ti, C, true>), dim3(g.nrows), dim3(g.row_threads),   \
-                               g.lds_bytes, s, g.rows, bub, bt, tab, st, air, sys, (T)gmax_state,  \
-                               ctl, part, stride, guess, sp);                                      \
-        else                                                                                       \
-            hipLaunchKernelGGL((k_assemble<T, Ti, C, false>), dim3(g.nrows), dim3(g.row_threads),  \
-                               g.lds_bytes, s, g.rows, bub, bt, tab, st, air, sys, (T)gmax_state,  \
-                               ctl, part, stride, guess, sp);                                      \
+        if (air.fu) {                                                                              \
+            if (g.pow2_rows) SF_ASM(C, true, true); else SF_ASM(C, true, false);                   \
+        } else {                                                                                   \
+            if (g.pow2_rows) SF_ASM(C, false, true); else SF_ASM(C, false, false);                 \
+        }                                                                                          \
     } while (0)
     SF_CASES(SF_CALL)
 #undef SF_CALL
+#undef SF_ASM
 }
 
 template <typename T, typename Ti>
@@ -2119,19 +2129,21 @@ void launch_resid(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, cons
                   const Tables& tab, StatePtrs<T> st, SolveCtl* ctl, SysPtrs<Ti> sys,
                   const T* guess, const double* xo, double* xn, int k, double* part, long stride,
                   const double* partB, const SolveParams& sp, SchedStat* sh, int check_prev) {
+#define SF_RES(C, F, P)                                                                           \
+    hipLaunchKernelGGL((k_resid<T, Ti, C, F, P>), dim3(g.nrows), dim3(g.row_threads), g.lds_bytes, \
+                       s, g.rows, bub, bt, tab, st, ctl, sys, guess, xo, xn, k, part, stride,     \
+                       partB, sp, sh, check_prev)
 #define SF_CALL(C)                                                                                \
     do {                                                                                          \
-        if (k == 1)                                                                               \
-            hipLaunchKernelGGL((k_resid<T, Ti, C, true>), dim3(g.nrows), dim3(g.row_threads),     \
-                               g.lds_bytes, s, g.rows, bub, bt, tab, st, ctl, sys, guess, xo, xn, \
-                               k, part, stride, partB, sp, sh, check_prev);                       \
-        else                                                                                      \
-            hipLaunchKernelGGL((k_resid<T, Ti, C, false>), dim3(g.nrows), dim3(g.row_threads),    \
-                               g.lds_bytes, s, g.rows, bub, bt, tab, st, ctl, sys, guess, xo, xn, \
-                               k, part, stride, partB, sp, sh, check_prev);                       \
+        if (k == 1) {                                                                             \
+            if (g.pow2_rows) SF_RES(C, true, true); else SF_RES(C, true, false);                  \
+        } else {                                                                                  \
+            if (g.pow2_rows) SF_RES(C, false, true); else SF_RES(C, false, false);                \
+        }                                                                                         \
     } while (0)
     SF_CASES(SF_CALL)
 #undef SF_CALL
+#undef SF_RES
 }
 
 template <typename Ti>
