@@ -125,6 +125,15 @@ typedef struct {
                              run of several steps, captured or not, ends
                              where the reference's would; 0: keep stepping,
                              counting the failure                         */
+    int32_t autotune;     /* 1: at the start of the first sf_step call of a
+                             captured context after 16 steps (the flow has
+                             developed), time the advection's cell tiles (256, 128 and 64
+                             columns of 256 threads) and the Chebyshev launch
+                             at C, 2C and 4C cells a thread per row width (C
+                             the assembly's) on the context's own data, and
+                             keep the fastest (sf_get_tuning reports them);
+                             0: the fixed choices (256 columns; 8 cells a
+                             thread for rows of >= 2048 cells, else C)     */
 } sf_options;
 
 typedef struct {
@@ -143,6 +152,18 @@ typedef struct {
     int32_t no_film;      /* 1 if no cell has eta > 0 (the broken test of
                              write_thickness_img, solver.cu:568-575)          */
 } sf_bubble_stats;
+
+/* The autotuner's choices (sf_options.autotune). */
+typedef struct {
+    int32_t tuned;             /* 1 once the autotuner has run                    */
+    int32_t advect_tile_width; /* columns of an advection tile (256 threads)      */
+    int32_t nwidths;           /* row widths of the context (at most 8 reported)  */
+    int32_t pad_;
+    int32_t row_width[8];      /* nPhi of each row width                          */
+    int32_t inner_cells[8];    /* cells a thread of its Chebyshev launch          */
+    double advect_ms[3];       /* advection at tile widths 256, 128, 64 (0: not timed) */
+    double inner_ms[8][3];     /* Chebyshev launch at C, 2C, 4C cells a thread    */
+} sf_tuning;
 
 typedef struct {
     double advect_ms, assemble_ms, solve_ms, update_ms, step_ms;
@@ -226,6 +247,9 @@ int sf_time_advection(sf_ctx* ctx, int32_t nlaunch, double* avg_ms);
 int sf_time_inner(sf_ctx* ctx, int32_t nlaunch, double* avg_ms);
 /* Solver statistics and Solver::isBroken (solver.cu:296-298, 568-574). */
 int sf_get_stats(sf_ctx* ctx, int32_t bubble, sf_bubble_stats* out);
+/* The autotuner's choices and timings (a measurement aid with no
+ * counterpart in the reference). */
+int sf_get_tuning(sf_ctx* ctx, sf_tuning* out);
 /* Device time per stage, accumulated since creation or the last reset. */
 int sf_get_timing(sf_ctx* ctx, sf_timing* out);
 int sf_reset_timing(sf_ctx* ctx);

@@ -38,7 +38,7 @@ EXPORTS = (
     "sf_set_dt", "sf_step", "sf_advect", "sf_assemble", "sf_solve", "sf_update",
     "sf_synchronize", "sf_time_advection", "sf_time_inner", "sf_get_stats", "sf_get_timing", "sf_reset_timing", "sf_last_error",
     "sf_derive_params", "sf_synth_fields", "sf_texture_fields", "sf_real_ylm",
-    "sf_write_thickness_exr", "sf_write_image_exr", "sf_step_noalloc",
+    "sf_write_thickness_exr", "sf_write_image_exr", "sf_step_noalloc", "sf_get_tuning",
 )
 
 
@@ -64,7 +64,7 @@ class Options(ctypes.Structure):
                 ("scalar_advection", ctypes.c_int32), ("multisample", ctypes.c_int32),
                 ("blend", ctypes.c_double), ("reinit_threshold", ctypes.c_double),
                 ("initial_guess", ctypes.c_int32), ("viscous", ctypes.c_int32),
-                ("halt_on_failure", ctypes.c_int32)]
+                ("halt_on_failure", ctypes.c_int32), ("autotune", ctypes.c_int32)]
 
 
 class BubbleStats(ctypes.Structure):
@@ -75,6 +75,13 @@ class BubbleStats(ctypes.Structure):
                 ("steps", ctypes.c_int32), ("time", ctypes.c_double),
                 ("distortion", ctypes.c_double), ("reinits", ctypes.c_int32),
                 ("no_film", ctypes.c_int32)]
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [("tuned", ctypes.c_int32), ("advect_tile_width", ctypes.c_int32),
+                ("nwidths", ctypes.c_int32), ("pad_", ctypes.c_int32),
+                ("row_width", ctypes.c_int32 * 8), ("inner_cells", ctypes.c_int32 * 8),
+                ("advect_ms", ctypes.c_double * 3), ("inner_ms", (ctypes.c_double * 3) * 8)]
 
 
 class Timing(ctypes.Structure):
@@ -117,6 +124,7 @@ def load_library(path: str = LIB_PATH):
         "sf_time_inner": (ctypes.c_int, [vp, ctypes.c_int32, P(ctypes.c_double)]),
         "sf_get_stats": (ctypes.c_int, [vp, ctypes.c_int32, P(BubbleStats)]),
         "sf_get_timing": (ctypes.c_int, [vp, P(Timing)]),
+        "sf_get_tuning": (ctypes.c_int, [vp, P(Tuning)]),
         "sf_reset_timing": (ctypes.c_int, [vp]),
         "sf_last_error": (ctypes.c_char_p, [vp]),
         "sf_derive_params": (ctypes.c_int, [P(BubbleDesc), P(Derived)]),
@@ -239,7 +247,7 @@ def options(precision: str = "fp64", *, device: int = 0, use_graph: bool = False
             inner_method: str = "auto", scalar_advection: str = "plain", blend: float = 0.5,
             multisample: bool = True, reinit_threshold: float = 0.0,
             initial_guess: str = "auto", viscous: bool = False,
-            halt_on_failure: bool = False) -> Options:
+            halt_on_failure: bool = False, autotune: bool = False) -> Options:
     """The sf_options struct of include/soapfilm.h."""
     return Options(PRECISIONS[precision], device, int(use_graph), int(tol_absolute), tol,
                    max_outer, max_inner, inner_rho, int(advance_time),
@@ -247,7 +255,7 @@ def options(precision: str = "fp64", *, device: int = 0, use_graph: bool = False
                    {"plain": 0, "bimocq": 1}[scalar_advection], int(multisample), blend,
                    reinit_threshold,
                    {"auto": 0, "gamma_star": 1, "extrapolated": 2}[initial_guess],
-                   int(viscous), int(halt_on_failure))
+                   int(viscous), int(halt_on_failure), int(autotune))
 
 
 def step_noalloc(bubble: BubbleDesc, opt: Options, t_seconds: float, nsteps: int,
@@ -277,7 +285,7 @@ class SoapFilm:
                  max_inner: int = 0, inner_rho: float = 0.0, inner_method: str = "auto",
                  scalar_advection: str = "plain", blend: float = 0.5, multisample: bool = True,
                  reinit_threshold: float = 0.0, initial_guess: str = "auto",
-                 viscous: bool = False, halt_on_failure: bool = False):
+                 viscous: bool = False, halt_on_failure: bool = False, autotune: bool = False):
         self._lib = load_library()
         if isinstance(bubbles, BubbleDesc):
             bubbles = [bubbles]
@@ -288,7 +296,7 @@ class SoapFilm:
                       max_inner=max_inner, inner_rho=inner_rho, inner_method=inner_method,
                       scalar_advection=scalar_advection, blend=blend, multisample=multisample,
                       reinit_threshold=reinit_threshold, initial_guess=initial_guess,
-                      viscous=viscous, halt_on_failure=halt_on_failure)
+                      viscous=viscous, halt_on_failure=halt_on_failure, autotune=autotune)
         h = ctypes.c_void_p()
         rc = self._lib.sf_create(ctypes.byref(opt), len(self.bubbles), arr, ctypes.byref(h))
         if rc:
@@ -372,6 +380,16 @@ class SoapFilm:
         t = Timing()
         self._check(self._lib.sf_get_timing(self._h, ctypes.byref(t)))
         return {n: getattr(t, n) for n, _ in Timing._fields_ if n != "pad_"}
+
+    def tuning(self) -> dict:
+        """The autotuner's choices (sf_get_tuning)."""
+        t = Tuning()
+        self._check(self._lib.sf_get_tuning(self._h, ctypes.byref(t)))
+        n = t.nwidths
+        return {"tuned": bool(t.tuned), "advect_tile_width": t.advect_tile_width,
+                "advect_ms": list(t.advect_ms),
+                "rows": [{"n_phi": t.row_width[w], "inner_cells_per_thread": t.inner_cells[w],
+                          "inner_ms": list(t.inner_ms[w])} for w in range(n)]}
 
     def reset_timing(self):
         self._check(self._lib.sf_reset_timing(self._h))

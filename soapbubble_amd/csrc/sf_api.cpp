@@ -125,6 +125,17 @@ struct Base {
     std::vector<LaunchGeom> rgeo; // row blocks, one geometry per row width
     std::vector<LaunchGeom> rgeo_inner;   // the same rows for the Chebyshev launches
     std::vector<int> rgeo_nph;            // the row width of each entry of rgeo
+    // autotuning: the candidate cell-tile lists (widths 256, 128, 64 columns;
+    // each row shorter than a width keeps its own power-of-two width) and the
+    // choices made
+    struct TileSet {
+        int width;
+        const int4* tiles;
+        int ntiles;
+    };
+    std::vector<TileSet> tile_cands;
+    bool tuned = false;
+    sf_tuning tuning{};
     double* partR = nullptr;
     double* partA = nullptr;
     double* partB = nullptr;
@@ -184,6 +195,10 @@ struct Base {
     virtual void row_stats() = 0;
     // one inner Chebyshev launch (iteration `it` of refinement `kr`)
     virtual void inner_launch(int it, int kr) = 0;
+    // average time of n launches of the advection / of inner iteration 2 of
+    // refinement 0 with geometry g (call between the assembly and the solve)
+    virtual double time_advect_geom(const LaunchGeom& g, int n) = 0;
+    virtual double time_cheb_geom(const LaunchGeom& g, int n) = 0;
     virtual bool fp64_inner() const = 0;
     // switches the inner solve to PCG with the multigrid preconditioner
     virtual void enable_mg() = 0;
@@ -488,6 +503,30 @@ struct Impl : Base {
                             cheb_e(it + 1), it, kr, sp, partB, stride);
     }
     void inner_launch(int it, int kr) override { cheb(it, kr); }
+    double time_advect_geom(const LaunchGeom& g, int n) override {
+        launch_advect<T>(stream, g, bub_d, time_d, tab, cur, nxt);
+        SF_HIP(hipEventRecord(ev[0], stream));
+        for (int k = 0; k < n; ++k) launch_advect<T>(stream, g, bub_d, time_d, tab, cur, nxt);
+        SF_HIP(hipEventRecord(ev[5], stream));
+        SF_HIP(hipEventSynchronize(ev[5]));
+        float ms = 0.f;
+        SF_HIP(hipEventElapsedTime(&ms, ev[0], ev[5]));
+        return (double)ms / n;
+    }
+    double time_cheb_geom(const LaunchGeom& g, int n) override {
+        auto one = [&] {
+            launch_cheb<Ti>(stream, g, bub_d, ctl_d, sys(x64[0]), cheb_e(2), cheb_e(1), cheb_e(3), 2,
+                            0, sp, partB, stride);
+        };
+        one();
+        SF_HIP(hipEventRecord(ev[0], stream));
+        for (int k = 0; k < n; ++k) one();
+        SF_HIP(hipEventRecord(ev[5], stream));
+        SF_HIP(hipEventSynchronize(ev[5]));
+        float ms = 0.f;
+        SF_HIP(hipEventElapsedTime(&ms, ev[0], ev[5]));
+        return (double)ms / n;
+    }
     // one accumulator slot of every bubble, copied to the host
     std::vector<double> slot_h;
     void fetch_slot(int slot) {
@@ -964,24 +1003,33 @@ void setup(Base& c, int prec) {
     }
     build_tables(c, c.owned);
 
-    // cell tiles of 256 threads: one row chunk of 256, or several short rows
-    std::vector<int4> tiles;
-    for (int b = 0; b < nb; ++b) {
-        const BubbleDev& B = c.bub_h[b];
-        const int w = std::min(256, next_pow2(B.nph)), rpt = 256 / w;
-        for (int j = 0; j < B.nth; j += rpt)
-            for (int i = 0; i < B.nph; i += w) tiles.push_back(make_int4(b, j, i, w));
-    }
     int nxcd = 1;
     if (hipDeviceGetAttribute(&nxcd, hipDeviceAttributeNumberOfXccs, c.device) != hipSuccess || nxcd < 1)
         nxcd = 1;
     if (const char* e = std::getenv("SF_XCD_ORDER")) nxcd = std::max(1, std::atoi(e));
     c.nxcd = nxcd;
-    tiles = xcd_order(tiles, make_int4(-1, 0, 0, 1), nxcd);
-    int4* tiles_d = dalloc<int4>(tiles.size(), c.owned);
-    SF_HIP(hipMemcpy(tiles_d, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice));
-    c.geom.tiles = tiles_d;
-    c.geom.ntiles = (int)tiles.size();
+    // cell tiles of 256 threads: a row chunk of `width` columns times
+    // 256 / width rows (a shorter row: its power-of-two width, several rows);
+    // width 256 by default, 128 and 64 as the autotuner's candidates
+    auto build_tiles = [&](int width) {
+        std::vector<int4> tiles;
+        for (int b = 0; b < nb; ++b) {
+            const BubbleDev& B = c.bub_h[b];
+            const int w = std::min(width, next_pow2(B.nph)), rpt = 256 / w;
+            for (int j = 0; j < B.nth; j += rpt)
+                for (int i = 0; i < B.nph; i += w) tiles.push_back(make_int4(b, j, i, w));
+        }
+        tiles = xcd_order(tiles, make_int4(-1, 0, 0, 1), nxcd);
+        int4* tiles_d = dalloc<int4>(tiles.size(), c.owned);
+        SF_HIP(hipMemcpy(tiles_d, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice));
+        return Base::TileSet{width, tiles_d, (int)tiles.size()};
+    };
+    c.tile_cands.clear();
+    c.tile_cands.push_back(build_tiles(256));
+    if (c.opt.autotune && !c.noalloc)
+        for (int w : {128, 64}) c.tile_cands.push_back(build_tiles(w));
+    c.geom.tiles = c.tile_cands[0].tiles;
+    c.geom.ntiles = c.tile_cands[0].ntiles;
     c.geom.pow2_rows = true;
     for (int b = 0; b < nb; ++b) c.geom.pow2_rows = c.geom.pow2_rows && c.bub_h[b].mask >= 0;
     // Row blocks, one launch per row width: C consecutive cells per thread
@@ -1151,6 +1199,82 @@ void eager_step(Base& c, bool host_driven) {
     c.host_hist += 1;
 }
 
+// Autotuner (sf_options.autotune), at the start of the first sf_step call
+// of a captured context after kTuneAfterSteps steps (the advection's cost
+// depends on the flow: a film started from rest has barely moved before):
+// times the advection with each candidate cell-tile list and, per
+// row width, the Chebyshev launch at C, 2C and 4C cells a thread (C the
+// assembly's, so the reduction depth it records for the line solve stays
+// deep enough), on the context's own state, and keeps the fastest of each.
+// The launches repeat the step's advection and the second inner iteration
+// of the first refinement (whose output the solve rewrites before reading
+// it); the control blocks are restored, and the step that follows computes
+// everything again.
+constexpr int kTuneAfterSteps = 16;
+
+void autotune(Base& c) {
+    const int n = 20;
+    sf_tuning& t = c.tuning;
+    std::memset(&t, 0, sizeof(t));
+    double best = 1e300;
+    size_t bi = 0;
+    for (size_t k = 0; k < c.tile_cands.size() && k < 3; ++k) {
+        LaunchGeom g = c.geom;
+        g.tiles = c.tile_cands[k].tiles;
+        g.ntiles = c.tile_cands[k].ntiles;
+        (void)c.time_advect_geom(g, 3);
+        const double ms = c.time_advect_geom(g, n);
+        t.advect_ms[k] = ms;
+        if (ms < best) { best = ms; bi = k; }
+    }
+    c.geom.tiles = c.tile_cands[bi].tiles;
+    c.geom.ntiles = c.tile_cands[bi].ntiles;
+    t.advect_tile_width = c.tile_cands[bi].width;
+    t.nwidths = (int)std::min<size_t>(c.rgeo.size(), 8);
+    for (int w = 0; w < t.nwidths; ++w) {
+        t.row_width[w] = c.rgeo_nph[(size_t)w];
+        t.inner_cells[w] = c.rgeo_inner[(size_t)w].cpt;
+    }
+    if (c.method == 2 && !c.fp64_inner()) {
+        c.advect();
+        c.assemble();
+        std::vector<SolveCtl> keep(c.nb);
+        SF_HIP(hipMemcpy(keep.data(), c.ctl_d, c.nb * sizeof(SolveCtl), hipMemcpyDeviceToHost));
+        for (int w = 0; w < t.nwidths; ++w) {
+            const int nph = c.rgeo_nph[(size_t)w];
+            const LaunchGeom& g0 = c.rgeo[(size_t)w];
+            double bt = 1e300;
+            LaunchGeom bg = c.rgeo_inner[(size_t)w];
+            for (int q = 0; q < 3; ++q) {
+                const int C = g0.cpt << q;
+                const int R = nph / C;
+                if (C > 16 || nph % C != 0 || R < 4 || R > kRowThreadsMax) continue;
+                LaunchGeom gi = g0;
+                gi.cpt = C;
+                gi.row_threads = std::max(64, (R + 63) / 64 * 64);
+                gi.lds_bytes = (size_t)(is_pow2(R) ? 6 : 8) * (size_t)R * sizeof(float);
+                (void)c.time_cheb_geom(gi, 5);
+                const double ms = c.time_cheb_geom(gi, n);
+                t.inner_ms[w][q] = ms;
+                if (ms < bt) { bt = ms; bg = gi; }
+            }
+            c.rgeo_inner[(size_t)w] = bg;
+            t.inner_cells[w] = bg.cpt;
+        }
+        SF_HIP(hipMemcpy(c.ctl_d, keep.data(), c.nb * sizeof(SolveCtl), hipMemcpyHostToDevice));
+    }
+    t.tuned = 1;
+    c.tuned = true;
+    if (c.trace_solve) {
+        std::fprintf(stderr, "[sf tune] advection tiles %d columns (%.4f %.4f %.4f ms)\n",
+                     t.advect_tile_width, t.advect_ms[0], t.advect_ms[1], t.advect_ms[2]);
+        for (int w = 0; w < t.nwidths; ++w)
+            std::fprintf(stderr, "[sf tune] rows of %d cells: Chebyshev %d cells a thread (%.4f %.4f %.4f ms)\n",
+                         t.row_width[w], t.inner_cells[w], t.inner_ms[w][0], t.inner_ms[w][1],
+                         t.inner_ms[w][2]);
+    }
+}
+
 // The executable graph of `nsteps` steps for the current schedule, followed
 // (slot >= 0) by the hand-off of the chunk's schedule statistics to pinned
 // slot `slot`; captured on the capture stream (replays already queued on the
@@ -1298,6 +1422,8 @@ int validate_create(const sf_options* opt, int32_t nbubbles, const sf_bubble_des
     if (opt->viscous < 0 || opt->viscous > 1) return fail(nullptr, SF_ERR_INVALID, "viscous must be 0 or 1");
     if (opt->halt_on_failure < 0 || opt->halt_on_failure > 1)
         return fail(nullptr, SF_ERR_INVALID, "halt_on_failure must be 0 or 1");
+    if (opt->autotune < 0 || opt->autotune > 1)
+        return fail(nullptr, SF_ERR_INVALID, "autotune must be 0 or 1");
     return SF_OK;
 }
 
@@ -1520,6 +1646,13 @@ int sf_step(sf_ctx* ctx, int32_t nsteps) {
         // solve after a reset; steps before that run host-driven
         const int warm = c.guess ? 2 : 0;
         int k = 0;
+        if (want_graph && c.opt.autotune && !c.tuned && c.steps_done >= kTuneAfterSteps) {
+            autotune(c);
+            // captured graphs hold the old launch geometry
+            for (auto& g : c.graphs) (void)hipGraphExecDestroy(g.second);
+            c.graphs.clear();
+            drop_calibration(c);
+        }
         for (int pass = 0; pass < 2 && want_graph && !c.calibrated; ++pass) {
             const bool was_mg = c.mg;
             // calibration: one host-driven step with the guess in effect sets
@@ -1749,6 +1882,12 @@ int sf_get_stats(sf_ctx* ctx, int32_t b, sf_bubble_stats* out) {
         fill_stats(*ctx->impl, b, out);
         return (int)SF_OK;
     });
+}
+
+int sf_get_tuning(sf_ctx* ctx, sf_tuning* out) {
+    if (!ctx || !ctx->impl || !out) return fail(ctx, SF_ERR_INVALID, "null argument");
+    *out = ctx->impl->tuning;
+    return SF_OK;
 }
 
 int sf_get_timing(sf_ctx* ctx, sf_timing* out) {
