@@ -166,11 +166,16 @@ def ensemble_specs(groups, l_max, seed0):
     return specs
 
 
-def ensemble_ctx(specs):
+# the packed ensembles of uneven resolution: (count, n_theta, dt) groups
+C3_GROUPS = [(32, 64, 0.005), (32, 128, 0.004), (32, 256, 0.002)]
+C4_GROUPS = [(4, 2048, 0.0005), (64, 128, 0.004)]
+
+
+def ensemble_ctx(specs, use_graph=True):
     import soapbubble_amd as sb
 
     ctx = sb.SoapFilm([sb.bubble(q["n"], q["dt"], M_scale=q["M_scale"]) for q in specs], "mixed",
-                      use_graph=True)
+                      use_graph=use_graph)
     for k, q in enumerate(specs):
         n, f = q["n"], q["fields"]
         nph = 2 * n
@@ -320,10 +325,13 @@ def main():
     ap.add_argument("--no-frames", action="store_true")
     ap.add_argument("--no-stiff", action="store_true")
     ap.add_argument("--bimocq-steps", type=int, default=10)
-    ap.add_argument("--profile", choices=("graph", "eager", "inner"), default=None,
+    ap.add_argument("--profile", choices=("graph", "eager", "inner", "c4", "c4-eager"),
+                    default=None,
                     help="run only the headline steps (for rocprofv3), graph or eager launches; "
                          "inner: after `warmup` eager steps, one assembly and `steps` + 1 "
-                         "back-to-back inner (Chebyshev) launches")
+                         "back-to-back inner (Chebyshev) launches; c4, c4-eager: the steps of "
+                         "the 4 x 2048x4096 + 64 x 128x256 ensemble (35.7M cells, past the "
+                         "256 MB last-level cache), graph or eager launches")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="run only a device-to-device copy of MIB MiB three times (torch), the "
                          "reference kernel that scales the profiler's HBM byte counters")
@@ -353,6 +361,16 @@ def main():
         ctx.close()
         print(json.dumps({"profile": "inner", "launches": args.steps + 1, "warmup": args.warmup,
                           "inner_launch_ms": ms}))
+        return
+    if args.profile in ("c4", "c4-eager"):
+        specs = ensemble_specs(C4_GROUPS, 128, 4000)
+        ctx = ensemble_ctx(specs, args.profile == "c4")
+        el = timed_run(ctx, args.warmup, args.steps)
+        fails = sum(ctx.stats(k)["failed_solves"] for k in range(len(specs)))
+        ctx.close()
+        print(json.dumps({"profile": args.profile, "steps": args.steps, "warmup": args.warmup,
+                          "cells": sum(2 * q["n"] * q["n"] for q in specs),
+                          "ms_per_step": el * 1e3 / args.steps, "failed_solves": fails}))
         return
     if args.profile:
         ctx = new_ctx("mixed", n, dt, st0, ua, va, args.profile == "graph")
@@ -516,14 +534,12 @@ def main():
     if not args.no_ensembles:
         ensembles = [
             ensemble("32 x 64x128 (dt 0.005) + 32 x 128x256 (0.004) + 32 x 256x512 (0.002)",
-                     [(32, 64, 0.005), (32, 128, 0.004), (32, 256, 0.002)], 200, args.warmup,
-                     16, 3000),
+                     C3_GROUPS, 200, args.warmup, 16, 3000),
             ensemble("4 x 2048x4096 (dt 0.0005) + 64 x 128x256 (0.004)",
-                     [(4, 2048, 0.0005), (64, 128, 0.004)], 50, args.warmup, 128, 4000,
+                     C4_GROUPS, 50, args.warmup, 128, 4000,
                      err_steps=0 if args.no_cpu else args.c4_err_steps),
         ]
 
-    log("frame loop")
     # the stiff regime of the reference's config.txt: multigrid and Chebyshev
     stiff = None
     if not args.no_stiff:
@@ -534,6 +550,7 @@ def main():
     # default scalar advection)
     frames = None
     if not args.no_frames:
+        log("frame loop")
         frames = [frame_loop(n, "plain"), frame_loop(n, "bimocq")]
 
     # HBM bytes per launch measured by rocprofv3 for this workload

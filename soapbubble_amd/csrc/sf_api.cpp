@@ -187,6 +187,10 @@ struct Base {
         if (stream && !noalloc) (void)hipStreamDestroy(stream);
     }
     virtual void advect() = 0;
+    // applies what uploads left pending (BIMOCQ's maps and initial fields
+    // restart from an uploaded eta or Gamma) as eager launches, before a
+    // step replays or captures graphs that do not hold them
+    virtual void apply_uploads() = 0;
     virtual void assemble() = 0;
     virtual void solve(bool host_driven, bool finalize) = 0;
     virtual void update(bool finalize) = 0;
@@ -443,12 +447,18 @@ struct Impl : Base {
     // Solver::advection (bubble.cu:1171-1214); with BIMOCQ the map updates
     // of stepForward (solver.cu:265-268) come first and eta is replaced by
     // its BIMOCQ value.
+    void apply_uploads() override {
+        if (!bimocq || !bimocq_dirty) return;
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        SF_HIP(hipStreamIsCapturing(stream, &cs));
+        if (cs != hipStreamCaptureStatusNone)
+            throw Status{SF_ERR_ORDER, "BIMOCQ restart pending during graph capture"};
+        launch_bimocq_init<T>(stream, geom, bub_d, cur, bm);
+        bimocq_dirty = false;
+    }
     void advect() override {
         if (bimocq) {
-            if (bimocq_dirty) {
-                launch_bimocq_init<T>(stream, geom, bub_d, cur, bm);
-                bimocq_dirty = false;
-            }
+            apply_uploads();
             launch_bimocq_maps<T>(stream, geom, bub_d, time_d, tab, cur, bm);
         }
         launch_advect<T>(stream, geom, bub_d, time_d, tab, cur, nxt);
@@ -1642,6 +1652,8 @@ int sf_step(sf_ctx* ctx, int32_t nsteps) {
         Base& c = *ctx->impl;
         if (c.stage != 0) throw Status{SF_ERR_ORDER, "a stage-level step is in progress"};
         const bool want_graph = c.opt.use_graph && !c.fp64_inner();
+        // captured steps hold no BIMOCQ restart: an upload's takes effect here
+        c.apply_uploads();
         // the extrapolated initial guess is in full effect from the third
         // solve after a reset; steps before that run host-driven
         const int warm = c.guess ? 2 : 0;

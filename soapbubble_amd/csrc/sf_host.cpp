@@ -419,21 +419,27 @@ bool write_thickness_exr(const std::string& path, int nth, int nph, const double
     return write_image_exr(path, nph, nth, gray.data(), gray.data(), gray.data());
 }
 
+// Every read is checked against the file (and an attribute's reads against
+// the attribute's own size): a truncated or corrupted file returns false,
+// never a read outside the buffer or a loop that does not advance.
 bool read_exr_channel(const std::string& path, const std::string& channel, int* width,
                       int* height, std::vector<float>* data) {
     std::ifstream f(path, std::ios::binary);
     if (!f) return false;
     std::string buf((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+    const size_t n = buf.size();
     size_t p = 0;
     auto rd32 = [&](int32_t& v) {
-        if (p + 4 > buf.size()) return false;
+        if (p > n || n - p < 4) return false;
         std::memcpy(&v, buf.data() + p, 4);
         p += 4;
         return true;
     };
-    auto rdstr = [&](std::string& s) {
+    // a NUL-terminated string that ends before `lim`
+    auto rdstr = [&](std::string& s, size_t lim) {
+        if (p >= lim) return false;
         size_t e = buf.find('\0', p);
-        if (e == std::string::npos) return false;
+        if (e == std::string::npos || e >= lim) return false;
         s = buf.substr(p, e - p);
         p = e + 1;
         return true;
@@ -444,66 +450,88 @@ bool read_exr_channel(const std::string& path, const std::string& channel, int* 
     struct Chan { std::string name; int type; };
     std::vector<Chan> chans;
     int comp = -1, x0 = 0, y0 = 0, x1 = -1, y1 = -1;
+    bool have_window = false;
     for (;;) {
         std::string name, type;
-        if (!rdstr(name)) return false;
+        if (!rdstr(name, n)) return false;
         if (name.empty()) break;
         int32_t size;
-        if (!rdstr(type) || !rd32(size) || p + size > buf.size()) return false;
-        const size_t end = p + size;
+        if (!rdstr(type, n) || !rd32(size) || size < 0 || (size_t)size > n - p) return false;
+        const size_t end = p + (size_t)size;
         if (name == "channels") {
+            // name, pixel type (0 uint, 1 half, 2 float), pLinear, reserved,
+            // xSampling, ySampling; a NUL ends the list
             while (p < end && buf[p] != '\0') {
                 Chan c;
-                if (!rdstr(c.name)) return false;
-                int32_t t;
+                if (!rdstr(c.name, end) || end - p < 16) return false;
+                int32_t t, xs, ys;
                 std::memcpy(&t, buf.data() + p, 4);
+                std::memcpy(&xs, buf.data() + p + 8, 4);
+                std::memcpy(&ys, buf.data() + p + 12, 4);
+                if (t < 0 || t > 2 || xs != 1 || ys != 1) return false;
                 c.type = t;
                 p += 16;
                 chans.push_back(c);
             }
         } else if (name == "compression") {
+            if (size < 1) return false;
             comp = (unsigned char)buf[p];
         } else if (name == "dataWindow") {
+            if (size < 16) return false;
             std::memcpy(&x0, buf.data() + p, 4);
             std::memcpy(&y0, buf.data() + p + 4, 4);
             std::memcpy(&x1, buf.data() + p + 8, 4);
             std::memcpy(&y1, buf.data() + p + 12, 4);
+            have_window = true;
         }
         p = end;
     }
-    if (comp != 0 || chans.empty()) return false;
-    const int w = x1 - x0 + 1, h = y1 - y0 + 1;
-    if (w <= 0 || h <= 0) return false;
+    if (comp != 0 || chans.empty() || !have_window) return false;
+    const int64_t w64 = (int64_t)x1 - x0 + 1, h64 = (int64_t)y1 - y0 + 1;
+    if (w64 <= 0 || h64 <= 0) return false;
     std::sort(chans.begin(), chans.end(), [](const Chan& a, const Chan& b) { return a.name < b.name; });
     int ci = -1;
-    size_t chan_off = 0, line_bytes = 0;
+    uint64_t chan_off = 0, pix_bytes = 0;
     for (size_t k = 0; k < chans.size(); ++k) {
-        const size_t bpp = chans[k].type == 1 ? 2 : 4;
-        if (chans[k].name == channel) { ci = (int)k; chan_off = line_bytes; }
-        line_bytes += bpp * (size_t)w;
+        if (chans[k].name == channel) { ci = (int)k; chan_off = pix_bytes; }
+        pix_bytes += chans[k].type == 1 ? 2 : 4;
     }
     if (ci < 0) return false;
-    std::vector<uint64_t> offs(h);
-    if (p + 8ull * h > buf.size()) return false;
-    std::memcpy(offs.data(), buf.data() + p, 8ull * h);
+    // every scanline holds its 8-byte offset, its 8-byte header and
+    // pix_bytes per pixel: the image must fit in the file
+    if ((uint64_t)w64 > n / pix_bytes || (uint64_t)h64 > n / 16 ||
+        (uint64_t)h64 * (16 + (uint64_t)w64 * pix_bytes) > n)
+        return false;
+    const int w = (int)w64, h = (int)h64;
+    const size_t line_bytes = (size_t)pix_bytes * (size_t)w;
+    chan_off *= (uint64_t)w;
+    std::vector<uint64_t> offs((size_t)h);
+    if (p > n || (n - p) / 8 < (size_t)h) return false;
+    std::memcpy(offs.data(), buf.data() + p, 8 * (size_t)h);
     data->assign((size_t)w * h, 0.0f);
     for (int k = 0; k < h; ++k) {
-        size_t q = offs[k];
+        const uint64_t q0 = offs[(size_t)k];
         int32_t y, sz;
-        if (q + 8 > buf.size()) return false;
+        if (n < 8 || q0 > n - 8) return false;
+        size_t q = (size_t)q0;
         std::memcpy(&y, buf.data() + q, 4);
         std::memcpy(&sz, buf.data() + q + 4, 4);
         q += 8;
-        if ((size_t)sz != line_bytes || q + sz > buf.size() || y < y0 || y > y1) return false;
+        if (sz < 0 || (size_t)sz != line_bytes || line_bytes > n - q || y < y0 || y > y1)
+            return false;
         const char* src = buf.data() + q + chan_off;
-        float* dst = data->data() + (size_t)(y - y0) * w;
+        float* dst = data->data() + (size_t)((int64_t)y - y0) * w;
         for (int i = 0; i < w; ++i) {
             if (chans[ci].type == 1) {
                 uint16_t hv;
                 std::memcpy(&hv, src + 2 * i, 2);
                 dst[i] = half_to_float(hv);
-            } else {
+            } else if (chans[ci].type == 2) {
                 std::memcpy(&dst[i], src + 4 * i, 4);
+            } else {
+                uint32_t u;
+                std::memcpy(&u, src + 4 * i, 4);
+                dst[i] = (float)u;
             }
         }
     }

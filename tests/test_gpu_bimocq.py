@@ -134,3 +134,35 @@ def test_bimocq_smallest_grid_fp64(oracle, threshold):
         e = rel_l2(got[k], ref[k])
         _say(f"  {k}: rel L2 {e:.3e}")
         assert e < 1e-9, (k, e)
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_bimocq_eta_upload_restarts_maps(oracle, use_graph):
+    """Uploading eta between steps restarts BIMOCQ from the uploaded field
+    (identity maps, initial fields = the upload), also where the steps
+    replay captured graphs: with the previous solve's Gamma* as the initial
+    guess the second sf_step call replays graphs from its first step."""
+    _say(f"BIMOCQ mixed 32x64, 6 steps, eta upload, 6 steps, graph {use_graph}")
+    n, dt, steps = 32, 0.005, 6
+    st, ua, va = make_case(n, 31, u_scale=0.8)
+    eta2 = sb.synth_fields(n, 32, 4, 6, 0.0, 1.0)["eta"]
+    p = oracle.params(n, dt)
+    ref = copy_state(st)
+    bm = oracle.BimocqState(p, ref["eta"], ref["gamma"], threshold=0.01)
+    oracle.bimocq_step(p, ref, ua, va, bm, steps)
+    ref["eta"] = eta2.copy()
+    bm = oracle.BimocqState(p, ref["eta"], ref["gamma"], threshold=0.01)
+    oracle.bimocq_step(p, ref, ua, va, bm, steps)
+    with sb.SoapFilm(sb.bubble(n, dt), "mixed", use_graph=use_graph, initial_guess="gamma_star",
+                     scalar_advection="bimocq", reinit_threshold=0.01) as ctx:
+        load(ctx, 0, st, ua, va)
+        ctx.step(steps)
+        ctx.upload(0, "eta", eta2)
+        ctx.step(steps)
+        got = ctx.state(0)
+        s = ctx.stats(0)
+    assert s["failed_solves"] == 0
+    for k in ("u_theta", "u_phi", "eta", "gamma"):
+        e = rel_l2(got[k], ref[k])
+        _say(f"  {k}: rel L2 {e:.3e}")
+        assert e < 1e-4, (k, e)
