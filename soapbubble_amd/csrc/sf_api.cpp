@@ -682,7 +682,12 @@ struct Impl : Base {
                         if (cheb_done(st, pb, rhist, mu)) { n = it; break; }
                     }
                     used.push_back(n);
-                    if (trace_solve) std::fprintf(stderr, "[sf solve]   chebyshev iterations %d\n", n);
+                    if (trace_solve) {
+                        std::fprintf(stderr, "[sf solve]   chebyshev iterations %d, |r|/|b|:", n);
+                        for (double rr : rhist[0])
+                            std::fprintf(stderr, " %.2e", std::sqrt(rr / (st[0].bb > 0 ? st[0].bb : 1.0)));
+                        std::fprintf(stderr, "\n");
+                    }
                     resid(k + 1, cheb_e(n + 1), nullptr);
                 }
 

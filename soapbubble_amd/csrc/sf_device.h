@@ -330,6 +330,38 @@ __device__ __forceinline__ float fast_bilerp(__amdgpu_buffer_rsrc_t r, const Fas
     return fmaf(q.aT, hi - lo, lo);
 }
 
+// A traced position in index coordinates, x = ix + fx, y = iy + fy. FP32
+// traces keep the start point's integer index apart from the fraction,
+// which holds the start point's half index and the increment turned along
+// the circle: the interpolation weights then carry the increment's
+// precision instead of the ulp of the absolute index (6e-5 of a cell at
+// index 1000). FP64 traces hold the absolute position in fx, fy (ix = iy =
+// 0).
+template <typename T>
+struct TPos {
+    int ix, iy;
+    T fx, fy;
+    __device__ __forceinline__ T x() const { return (T)ix + fx; }
+    __device__ __forceinline__ T y() const { return (T)iy + fy; }
+};
+
+// fast_index of the point p + (ox, oy) (ox, oy half-integers, added to the
+// fraction exactly): weights from the fraction, the row and column from the
+// integer parts. True when the lower row lies in [0, rows - 2], the case
+// the fast path serves (rows: the field's row count).
+__device__ __forceinline__ bool fast_index_rel(const TPos<float>& p, float ox, float oy, int rows,
+                                               int nph, int mask, FastIdx& q) {
+    const float rx = p.fx + ox, ry = p.fy + oy;
+    const float fx = floorf(rx), fy = floorf(ry);
+    q.aT = rx - fx;
+    q.aP = ry - fy;
+    const int tI = p.ix + (int)fx, pI = p.iy + (int)fy;
+    const int row = tI * nph;
+    q.off = (row + (pI & mask)) * 4;
+    q.off1 = (row + ((pI + 1) & mask)) * 4;
+    return (unsigned)tI < (unsigned)(rows - 1);
+}
+
 template <typename T, bool NANS>
 struct FastPath { static constexpr bool on = false; };
 template <>
@@ -349,9 +381,9 @@ __device__ __forceinline__ void split_index(T x, T y, int nph, int& tI, int& pI,
 }
 
 // sampleCentered (bubble.cu:320-376)
-template <typename T, bool NANS = true, bool P2 = false>
+template <typename T, bool NANS = true, bool P2 = false, bool FAST = true>
 __device__ T sample_centered(const Field<T, P2>& f, int nth, T rx, T ry) {
-    if constexpr (FastPath<T, NANS>::on) {
+    if constexpr (FAST && FastPath<T, NANS>::on) {
         if (rx >= 0.5f && rx < (float)nth - 0.5f && f.pow2())
             return fast_bilerp(field_rsrc(f.p, nth, f.nph), fast_index(rx - 0.5f, ry, f.nph, f.mask),
                                f.nph * 4);
@@ -395,10 +427,10 @@ __device__ T sample_centered(const Field<T, P2>& f, int nth, T rx, T ry) {
 
 // sampleCentered of two fields at one point (the eta and Gamma samples of
 // advectionAllCentered, bubble.cu:957-960) sharing the index computation.
-template <typename T, bool NANS = true, bool P2 = false>
+template <typename T, bool NANS = true, bool P2 = false, bool FAST = true>
 __device__ void sample_centered2(const Field<T, P2>& f, const Field<T, P2>& g, int nth, T rx, T ry, T& of,
                                  T& og) {
-    if constexpr (FastPath<T, NANS>::on) {
+    if constexpr (FAST && FastPath<T, NANS>::on) {
         if (rx >= 0.5f && rx < (float)nth - 0.5f && f.pow2()) {
             const FastIdx q = fast_index(rx - 0.5f, ry, f.nph, f.mask);
             of = fast_bilerp(field_rsrc(f.p, nth, f.nph), q, f.nph * 4);
@@ -424,9 +456,9 @@ __device__ void sample_centered2(const Field<T, P2>& f, const Field<T, P2>& g, i
 }
 
 // sampleVPhi (bubble.cu:178-240): values taken across the pole flip sign.
-template <typename T, bool NANS = true, bool P2 = false>
+template <typename T, bool NANS = true, bool P2 = false, bool FAST = true>
 __device__ T sample_vphi(const Field<T, P2>& f, int nth, T rx, T ry) {
-    if constexpr (FastPath<T, NANS>::on) {
+    if constexpr (FAST && FastPath<T, NANS>::on) {
         if (rx >= 0.5f && rx < (float)nth - 0.5f && f.pow2())
             return fast_bilerp(field_rsrc(f.p, nth, f.nph),
                                fast_index(rx - 0.5f, ry + 0.5f, f.nph, f.mask), f.nph * 4);
@@ -458,9 +490,9 @@ __device__ T sample_vphi(const Field<T, P2>& f, int nth, T rx, T ry) {
 // interpolate between the first/last row and its phi+pi mirror with half
 // weight. A sample exactly at theta = pi (which the reference reads past
 // the array for) takes the south-cap branch.
-template <typename T, bool NANS = true, bool P2 = false>
+template <typename T, bool NANS = true, bool P2 = false, bool FAST = true>
 __device__ T sample_vtheta(const Field<T, P2>& f, int nth, T rx, T ry) {
-    if constexpr (FastPath<T, NANS>::on) {
+    if constexpr (FAST && FastPath<T, NANS>::on) {
         if (rx >= 1.0f && rx < (float)nth - 1.0f && f.pow2())
             return fast_bilerp(field_rsrc(f.p, nth - 1, f.nph),
                                fast_index(rx - 1.0f, ry, f.nph, f.mask), f.nph * 4);
@@ -495,6 +527,40 @@ __device__ T sample_vtheta(const Field<T, P2>& f, int nth, T rx, T ry) {
     return fl ? -v : v;
 }
 
+// The samplers at a traced position: the FP32 interior fast path from the
+// split position, else the general sampler at the absolute one.
+template <typename T, bool NANS = true, bool P2 = false>
+__device__ __forceinline__ T sample_vtheta(const Field<T, P2>& f, int nth, const TPos<T>& p) {
+    if constexpr (FastPath<T, NANS>::on) {
+        FastIdx q;
+        if (f.pow2() && fast_index_rel(p, -1.0f, 0.0f, nth - 1, f.nph, f.mask, q))
+            return fast_bilerp(field_rsrc(f.p, nth - 1, f.nph), q, f.nph * 4);
+    }
+    return sample_vtheta<T, NANS, P2, false>(f, nth, p.x(), p.y());
+}
+template <typename T, bool NANS = true, bool P2 = false>
+__device__ __forceinline__ T sample_vphi(const Field<T, P2>& f, int nth, const TPos<T>& p) {
+    if constexpr (FastPath<T, NANS>::on) {
+        FastIdx q;
+        if (f.pow2() && fast_index_rel(p, -0.5f, 0.5f, nth, f.nph, f.mask, q))
+            return fast_bilerp(field_rsrc(f.p, nth, f.nph), q, f.nph * 4);
+    }
+    return sample_vphi<T, NANS, P2, false>(f, nth, p.x(), p.y());
+}
+template <typename T, bool NANS = true, bool P2 = false>
+__device__ __forceinline__ void sample_centered2(const Field<T, P2>& f, const Field<T, P2>& g, int nth,
+                                                 const TPos<T>& p, T& of, T& og) {
+    if constexpr (FastPath<T, NANS>::on) {
+        FastIdx q;
+        if (f.pow2() && fast_index_rel(p, -0.5f, 0.0f, nth, f.nph, f.mask, q)) {
+            of = fast_bilerp(field_rsrc(f.p, nth, f.nph), q, f.nph * 4);
+            og = fast_bilerp(field_rsrc(g.p, nth, g.nph), q, g.nph * 4);
+            return;
+        }
+    }
+    sample_centered2<T, NANS, P2, false>(f, g, nth, p.x(), p.y(), of, og);
+}
+
 // ------------------------------------------------------ great circle ----
 // The start frame (w, u_, v_) and speed after the transported midpoint
 // velocity, i.e. the first two stages shared by traceGreatCircle
@@ -507,6 +573,7 @@ template <typename T>
 struct Circle {
     T w0, w1, w2, u0, u1, u2, v0, v1, v2, unorm;
     T gx, gy, sT, cT, sP, cP;   // start point: index coordinates and trig
+    TPos<T> start;              // start point, split as the traced positions
 };
 
 template <typename T, bool P2 = false>
@@ -540,26 +607,46 @@ __device__ __forceinline__ float small_angle(float n, float d, float k_inv) {
 // sin phi0; scale rho): the same angles, with the grid-exact start index
 // carrying the magnitude so FP32 keeps the small increment's precision.
 template <typename T>
-__device__ __forceinline__ void point_index(const Circle<T>& o, T x, T y, T z, T invh, T& ti,
-                                            T& pi_, T& rho, T& ir);
+__device__ __forceinline__ void point_index(const Circle<T>& o, T x, T y, T z, T invh, TPos<T>& p,
+                                            T& rho, T& ir);
 template <>
 __device__ __forceinline__ void point_index(const Circle<double>& o, double x, double y, double z,
-                                            double invh, double& ti, double& pi_, double& rho,
-                                            double& ir) {
+                                            double invh, TPos<double>& p, double& rho, double& ir) {
     rho = sqrt(x * x + y * y);
     ir = 1.0 / rho;
-    ti = atan2(rho, z) * invh;
-    pi_ = atan2(y, x) * invh;
+    p.ix = p.iy = 0;
+    p.fx = atan2(rho, z) * invh;
+    p.fy = atan2(y, x) * invh;
 }
 template <>
 __device__ __forceinline__ void point_index(const Circle<float>& o, float x, float y, float z,
-                                            float invh, float& ti, float& pi_, float& rho,
-                                            float& ir) {
+                                            float invh, TPos<float>& p, float& rho, float& ir) {
     const float r2 = x * x + y * y;
     ir = __builtin_amdgcn_rsqf(r2);
     rho = r2 > 0.0f ? r2 * ir : 0.0f;
-    ti = o.gx + small_angle(rho * o.cT - z * o.sT, z * o.cT + rho * o.sT, 1.0f) * invh;
-    pi_ = o.gy + small_angle(y * o.cP - x * o.sP, x * o.cP + y * o.sP, ir) * invh;
+    p.ix = o.start.ix;
+    p.iy = o.start.iy;
+    p.fx = o.start.fx + small_angle(rho * o.cT - z * o.sT, z * o.cT + rho * o.sT, 1.0f) * invh;
+    p.fy = o.start.fy + small_angle(y * o.cP - x * o.sP, x * o.cP + y * o.sP, ir) * invh;
+}
+
+// The start point (index coordinates gx, gy: multiples of 1/2) in the split
+// form of the traced positions.
+template <typename T>
+__device__ __forceinline__ TPos<T> start_pos(T gx, T gy) {
+    TPos<T> p;
+    if constexpr (sizeof(T) == 4) {
+        const T fx = tfloor(gx), fy = tfloor(gy);
+        p.ix = (int)fx;
+        p.iy = (int)fy;
+        p.fx = gx - fx;
+        p.fy = gy - fy;
+    } else {
+        p.ix = p.iy = 0;
+        p.fx = gx;
+        p.fy = gy;
+    }
+    return p;
 }
 
 template <typename T>
@@ -576,6 +663,7 @@ template <typename T, bool NANS = true, bool P2 = false>
 __device__ __forceinline__ bool circle_mid(const VelCtx<T, P2>& c, T gx, T gy, T sinT, T cosT, T sinP,
                                            T cosP, T guT, T guP, Circle<T>& o) {
     o.gx = gx; o.gy = gy; o.sT = sinT; o.cT = cosT; o.sP = sinP; o.cP = cosP;
+    o.start = start_pos(gx, gy);
     T u0 = guT * (cosT * cosP) - guP * sinP;
     T u1 = guT * (cosT * sinP) + guP * cosP;
     T u2 = -guT * sinT;
@@ -587,10 +675,11 @@ __device__ __forceinline__ bool circle_mid(const VelCtx<T, P2>& c, T gx, T gy, T
     T s, co;
     sincos_step(T(-0.5) * o.unorm * c.dt, &s, &co);
     T mx = o.u0 * s + o.w0 * co, my = o.u1 * s + o.w1 * co, mz = o.u2 * s + o.w2 * co;
-    T mti, mpi, rho, ir;
-    point_index(o, mx, my, mz, c.invh, mti, mpi, rho, ir);
-    T muT = sample_vtheta<T, NANS>(c.uth, c.nth, mti, mpi);
-    T muP = sample_vphi<T, NANS>(c.uph, c.nth, mti, mpi);
+    TPos<T> mp;
+    T rho, ir;
+    point_index(o, mx, my, mz, c.invh, mp, rho, ir);
+    T muT = sample_vtheta<T, NANS>(c.uth, c.nth, mp);
+    T muP = sample_vphi<T, NANS>(c.uph, c.nth, mp);
     T m0, m1, m2;
     tangent_vec(mx, my, mz, rho, ir, muT, muP, m0, m1, m2);
     T uc0 = o.u0 * co - o.w0 * s, uc1 = o.u1 * co - o.w1 * s, uc2 = o.u2 * co - o.w2 * s;
@@ -615,12 +704,12 @@ __device__ __forceinline__ bool circle_mid(const VelCtx<T, P2>& c, T gx, T gy, T
 // the Cartesian point (bubble.cu:498-505, 734-743).
 template <typename T, bool P2>
 __device__ __forceinline__ void circle_depart(const VelCtx<T, P2>& c, const Circle<T>& o, T& s, T& co,
-                                              T& px, T& py, T& pz, T& ti, T& pi_, T& rho, T& ir) {
+                                              T& px, T& py, T& pz, TPos<T>& pos, T& rho, T& ir) {
     sincos_step(-o.unorm * c.dt, &s, &co);
     px = o.u0 * s + o.w0 * co;
     py = o.u1 * s + o.w1 * co;
     pz = o.u2 * s + o.w2 * co;
-    point_index(o, px, py, pz, c.invh, ti, pi_, rho, ir);
+    point_index(o, px, py, pz, c.invh, pos, rho, ir);
 }
 
 // Velocity at the departure point transported back along the circle into
@@ -628,10 +717,11 @@ __device__ __forceinline__ void circle_depart(const VelCtx<T, P2>& c, const Circ
 template <typename T, bool NANS = true, bool P2 = false>
 __device__ __forceinline__ void transport_back(const VelCtx<T, P2>& c, const Circle<T>& o, T& r0, T& r1,
                                                T& r2) {
-    T s, co, px, py, pz, ti, pi_, rho, ir;
-    circle_depart(c, o, s, co, px, py, pz, ti, pi_, rho, ir);
-    T puT = sample_vtheta<T, NANS>(c.uth, c.nth, ti, pi_);
-    T puP = sample_vphi<T, NANS>(c.uph, c.nth, ti, pi_);
+    T s, co, px, py, pz, rho, ir;
+    TPos<T> pp;
+    circle_depart(c, o, s, co, px, py, pz, pp, rho, ir);
+    T puT = sample_vtheta<T, NANS>(c.uth, c.nth, pp);
+    T puP = sample_vphi<T, NANS>(c.uph, c.nth, pp);
     T p0, p1, p2;
     tangent_vec(px, py, pz, rho, ir, puT, puP, p0, p1, p2);
     T uc0 = o.u0 * co - o.w0 * s, uc1 = o.u1 * co - o.w1 * s, uc2 = o.u2 * co - o.w2 * s;

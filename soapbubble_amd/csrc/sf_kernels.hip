@@ -73,13 +73,15 @@ __device__ __forceinline__ void advect_cell(const BubbleDev& B, const Tables& ta
             guP = sample_vphi<T, NANS>(v.uph, B.nth, gx, gy);
         }
         Circle<T> o;
-        T tx = gx, ty = gy;
+        TPos<T> tp;
         if (circle_mid<T, NANS>(v, gx, gy, sinTc, cosTc, sinPc, cosPc, guT, guP, o)) {
             T s, co, px, py, pz, rho, ir;
-            circle_depart(v, o, s, co, px, py, pz, tx, ty, rho, ir);
+            circle_depart(v, o, s, co, px, py, pz, tp, rho, ir);
+        } else {
+            tp = o.start;
         }
         T en, gn;
-        sample_centered2<T, NANS>(feta, fgam, B.nth, tx, ty, en, gn);
+        sample_centered2<T, NANS>(feta, fgam, B.nth, tp, en, gn);
         out.eta[B.off + c] = en;
         out.gam[B.off + c] = gn;
     }
@@ -1788,19 +1790,24 @@ __global__ __launch_bounds__(256) void k_bm_init(const int4* __restrict__ tiles,
 // traceGreatCircle from the cell centre (j + 1/2, i) over dt (either sign)
 template <typename T, bool NANS>
 __device__ __forceinline__ void trace_centre(const VelCtx<T>& v, const Tables& tab,
-                                             const BubbleDev& B, int j, int i, T& tx, T& ty) {
+                                             const BubbleDev& B, int j, int i, double& tx,
+                                             double& ty) {
     const T gx = (T)j + T(0.5), gy = (T)i;
     const T sinTc = TabSel<T>::sTc(tab)[B.rt0 + j], cosTc = TabSel<T>::cTc(tab)[B.rt0 + j];
     const T sinPc = TabSel<T>::sPc(tab)[B.ct0 + i], cosPc = TabSel<T>::cPc(tab)[B.ct0 + i];
     const T guT = sample_vtheta<T, NANS>(v.uth, B.nth, gx, gy);
     const T guP = sample_vphi<T, NANS>(v.uph, B.nth, gx, gy);
     Circle<T> o;
-    tx = gx;
-    ty = gy;
+    TPos<T> tp;
     if (circle_mid<T, NANS>(v, gx, gy, sinTc, cosTc, sinPc, cosPc, guT, guP, o)) {
         T s, co, px, py, pz, rho, ir;
-        circle_depart(v, o, s, co, px, py, pz, tx, ty, rho, ir);
+        circle_depart(v, o, s, co, px, py, pz, tp, rho, ir);
+    } else {
+        tp = o.start;
     }
+    // the split position summed in FP64 (exact)
+    tx = (double)tp.ix + (double)tp.fx;
+    ty = (double)tp.iy + (double)tp.fy;
 }
 
 // updateForward / updateBackward (bubble.cu:1049-1118 with
@@ -1818,9 +1825,9 @@ __device__ __forceinline__ void bm_maps_cell(const BubbleDev& B, const Tables& t
     const size_t o = (size_t)B.off, c = o + (size_t)j * B.nph + i;
     for (int dir = 0; dir < 2; ++dir) {
         v.dt = dir == 0 ? (T)(-B.dt) : (T)B.dt;
-        T tx, ty;
+        double tx, ty;
         trace_centre<T, NANS>(v, tab, B, j, i, tx, ty);
-        const U3 s = sample_map_u(dir == 0 ? P.fwd : P.bwd, o, B.nth, B.nph, (double)tx, (double)ty);
+        const U3 s = sample_map_u(dir == 0 ? P.fwd : P.bwd, o, B.nth, B.nph, tx, ty);
         const UMap& out = dir == 0 ? P.nfwd : P.nbwd;
         out.x[c] = s.x;
         out.y[c] = s.y;
