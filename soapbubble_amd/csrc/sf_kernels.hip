@@ -1343,6 +1343,10 @@ __global__ void k_mg_line(const int2* __restrict__ rows, const BubbleDev* __rest
 }
 
 // ------------------------------------------- inner Chebyshev iteration ----
+// Line-Jacobi bound up to which the Chebyshev recurrence starts after one
+// plain line-Jacobi step (see k_cheb).
+constexpr double kChebLeadMu = 0.5;
+
 // Chebyshev acceleration of the phi-line Jacobi iteration for A e = r_in
 // (Saad, Iterative Methods for Sparse Linear Systems, Alg. 12.1), with the
 // spectrum of M^-1 A inside [1 - mu, 1 + mu] (mu from the assembly, widened
@@ -1384,12 +1388,23 @@ __global__ void k_cheb(const int2* __restrict__ rows, const BubbleDev* __restric
     // zero (a lower end at or below zero makes the iteration diverge)
     const double delta = fmin(fmax(mu * 1.02 + 1e-3, 1e-4), 0.9999);
     const double sigma = 1.0 / delta;
+    // Jacobi lead: on a weakly coupled system (mu <= kChebLeadMu) the
+    // recurrence restarts after the first iterate e_1 = z_0 (a plain line-
+    // Jacobi step), so the residual polynomial after k line solves is
+    // nu T_{k-1}(nu / delta) / T_{k-1}(1 / delta) (nu the eigenvalues of the
+    // line-Jacobi iteration matrix, in [-delta, delta]) instead of
+    // T_k(nu / delta) / T_k(1 / delta): a zero at nu = 0 for every k, where
+    // the line solves leave most of the residual of such a system (even-
+    // degree Chebyshev polynomials are +-1 / T_k(1 / delta) there).
+    const int kk = it - (mu <= kChebLeadMu ? 1 : 0);
     double rho_prev = delta, rho = delta;      // rho_0 = 1 / sigma
-    for (int q = 1; q <= it; ++q) {
+    for (int q = 1; q <= kk; ++q) {
         rho_prev = rho;
         rho = 1.0 / (2.0 * sigma - rho_prev);
     }
-    const Ti cd = (Ti)(rho * rho_prev), cz = (Ti)(2.0 * rho / delta);
+    // the restart's first step: e_{k+1} = e_k + z_k
+    const Ti cd = kk == 0 ? Ti(0) : (Ti)(rho * rho_prev),
+             cz = kk == 0 ? Ti(1) : (Ti)(2.0 * rho / delta);
     const int steps = sys.rsteps[B.pr0 + j];
     const size_t row = (size_t)B.off + (size_t)j * nph;
     const bool hasN = j > 0, hasS = j < nth - 1;
