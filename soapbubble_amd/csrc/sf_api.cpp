@@ -118,8 +118,15 @@ struct Base {
     hipEvent_t cev[2] = {};                // end of a chunk of replays
     int chunk_sched[2] = {0, 0};           // first-refinement iterations each chunk ran with
     hipStream_t cstream = nullptr;         // capture stream (graphs are built while replays run)
-    // executable graphs by (schedule, steps per launch, statistics slot)
-    std::map<std::vector<int>, hipGraphExec_t> graphs;
+    // executable graphs by (schedule, steps per launch, statistics slot),
+    // with the tick of their last use (at most graph_cache_max kept)
+    struct GraphEntry {
+        hipGraphExec_t exec;
+        unsigned long long used;
+    };
+    std::map<std::vector<int>, GraphEntry> graphs;
+    unsigned long long graph_tick = 0;
+    size_t graph_cache_max = 16;
     Tables tab{};
     LaunchGeom geom{};            // cell tiles
     std::vector<LaunchGeom> rgeo; // row blocks, one geometry per row width
@@ -176,7 +183,7 @@ struct Base {
     bool noalloc = false;
     virtual ~Base() {
         if (stream) (void)hipStreamSynchronize(stream);
-        for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
+        for (auto& g : graphs) (void)hipGraphExecDestroy(g.second.exec);
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
         for (auto& e : cev)
@@ -1128,6 +1135,8 @@ void setup(Base& c, int prec) {
     c.sp.tol = c.opt.tol > 0 ? c.opt.tol : 1e-10;
     c.sp.absolute = c.opt.tol_absolute;
     c.sp.inner_frac = 0.5;
+    // test hook: a smaller executable-graph cache (1 evicts on every capture)
+    if (const char* e = std::getenv("SF_GRAPH_CACHE_MAX")) c.graph_cache_max = (size_t)std::max(1, std::atoi(e));
     if (c.opt.scalar_advection < 0 || c.opt.scalar_advection > 1)
         throw Status{SF_ERR_INVALID, "scalar_advection must be 0 (plain) or 1 (BIMOCQ)"};
     c.bimocq = c.opt.scalar_advection == 1;
@@ -1294,13 +1303,30 @@ void autotune(Base& c) {
 // (slot >= 0) by the hand-off of the chunk's schedule statistics to pinned
 // slot `slot`; captured on the capture stream (replays already queued on the
 // context's stream keep running) and kept for reuse. One graph launch per
-// chunk: a launch costs about as much as a short kernel.
+// chunk: a launch costs about as much as a short kernel. The cache holds at
+// most graph_cache_max executables (the least recently used goes first):
+// unbounded, a schedule that drifts over many lengths kept three
+// executables of four steps per length, e.g. about 260 000 instantiated
+// kernel nodes for lengths up to 200 inner launches.
 hipGraphExec_t step_graph(Base& c, int nsteps, int slot) {
     std::vector<int> key = c.sched;
     key.push_back(nsteps);
     key.push_back(slot);
     auto it = c.graphs.find(key);
-    if (it != c.graphs.end()) return it->second;
+    if (it != c.graphs.end()) {
+        it->second.used = ++c.graph_tick;
+        return it->second.exec;
+    }
+    if (c.graphs.size() >= c.graph_cache_max) {
+        // the least recently used executable goes, once no replay of it can
+        // still be queued
+        SF_HIP(hipStreamSynchronize(c.stream));
+        auto lru = c.graphs.begin();
+        for (auto q = c.graphs.begin(); q != c.graphs.end(); ++q)
+            if (q->second.used < lru->second.used) lru = q;
+        (void)hipGraphExecDestroy(lru->second.exec);
+        c.graphs.erase(lru);
+    }
     hipGraph_t g;
     std::swap(c.stream, c.cstream);
     hipError_t e = hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal);
@@ -1320,7 +1346,7 @@ hipGraphExec_t step_graph(Base& c, int nsteps, int slot) {
     e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     SF_HIP(e);
-    c.graphs[key] = x;
+    c.graphs[key] = Base::GraphEntry{x, ++c.graph_tick};
     return x;
 }
 
@@ -1666,7 +1692,7 @@ int sf_step(sf_ctx* ctx, int32_t nsteps) {
         if (want_graph && c.opt.autotune && !c.tuned && c.steps_done >= kTuneAfterSteps) {
             autotune(c);
             // captured graphs hold the old launch geometry
-            for (auto& g : c.graphs) (void)hipGraphExecDestroy(g.second);
+            for (auto& g : c.graphs) (void)hipGraphExecDestroy(g.second.exec);
             c.graphs.clear();
             drop_calibration(c);
         }

@@ -254,3 +254,33 @@ def test_inner_timing_checks_its_place():
         assert e.value.code == 1
         ctx.solve()
         ctx.update()
+
+
+def test_graph_cache_eviction_changes_nothing(monkeypatch):
+    """A one-executable graph cache (the test hook SF_GRAPH_CACHE_MAX=1:
+    every capture evicts the previous executable after the stream drains)
+    against the default cache, through schedule changes and step-size
+    changes: the same kernels run in the same order, so the states agree
+    bitwise."""
+    n, dt1, dt2 = 64, 0.004, 0.008
+    _say(f"graph cache eviction {n}x{2 * n}")
+    st, ua, va = make_case(n, 23, l_max=6, u_scale=0.5)
+    out = []
+    for cache in (None, "1"):
+        if cache is None:
+            monkeypatch.delenv("SF_GRAPH_CACHE_MAX", raising=False)
+        else:
+            monkeypatch.setenv("SF_GRAPH_CACHE_MAX", cache)
+        with sb.SoapFilm(sb.bubble(n, dt1), "mixed", use_graph=True) as ctx:
+            load(ctx, 0, st, ua, va)
+            ctx.step(20)
+            ctx.set_dt(dt2)
+            ctx.step(10)
+            ctx.set_dt(dt1)
+            ctx.step(10)
+            ctx.synchronize()
+            s = ctx.stats(0)
+            assert s["failed_solves"] == 0 and s["max_residual"] <= 1e-10, cache
+            out.append(ctx.state(0))
+    for k in FIELDS:
+        assert np.array_equal(out[0][k], out[1][k]), k
