@@ -284,6 +284,19 @@ double sf_real_ylm(int32_t l, int32_t m, double theta, double phi);
 /* write_thickness_img (solver.cu:541-593): eta * H * 2e5 as a HALF RGB EXR
  * (channels B, G, R, uncompressed). */
 int sf_write_thickness_exr(sf_ctx* ctx, int32_t bubble, const char* path);
+/* The pixels of write_thickness_img's frame (solver.cu:560-582) as binary16
+ * values: eta H 2e5 rounded to binary32, then to binary16 as the reference's
+ * EXR writer rounds (nearest, ties away from zero), converted on the device;
+ * host holds n_theta x nPhi values (pixel (x = i, y = j) of the nPhi x
+ * n_theta image at host[j nPhi + i]). With sf_write_half_exr it writes the
+ * same file as sf_write_thickness_exr, and lets a caller write the file on
+ * another thread while the next steps run (the reference writes it
+ * synchronously). */
+int sf_thickness_half(sf_ctx* ctx, int32_t bubble, uint16_t* host);
+/* sf_write_image_exr of a gray image given as its binary16 values (row-major,
+ * width x height; host only): the bytes sf_write_image_exr writes when its
+ * three planes are the gray plane whose halves these are. */
+int sf_write_half_exr(const char* path, int32_t width, int32_t height, const uint16_t* gray);
 /* Solver::write_image (solver.cu:338-385), host only: width x height float
  * planes r, g, b (row-major) written as an uncompressed scanline OpenEXR with
  * HALF channels B, G, R, byte for byte as the reference's tinyexr call

@@ -39,6 +39,7 @@ EXPORTS = (
     "sf_synchronize", "sf_time_advection", "sf_time_inner", "sf_get_stats", "sf_get_timing", "sf_reset_timing", "sf_last_error",
     "sf_derive_params", "sf_synth_fields", "sf_texture_fields", "sf_real_ylm",
     "sf_write_thickness_exr", "sf_write_image_exr", "sf_step_noalloc", "sf_get_tuning",
+    "sf_thickness_half", "sf_write_half_exr",
 )
 
 
@@ -136,6 +137,9 @@ def load_library(path: str = LIB_PATH):
         "sf_real_ylm": (ctypes.c_double, [ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
                                           ctypes.c_double]),
         "sf_write_thickness_exr": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.c_char_p]),
+        "sf_thickness_half": (ctypes.c_int, [vp, ctypes.c_int32, P(ctypes.c_uint16)]),
+        "sf_write_half_exr": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, ctypes.c_int32,
+                                             P(ctypes.c_uint16)]),
         "sf_write_image_exr": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, ctypes.c_int32,
                                               P(ctypes.c_float), P(ctypes.c_float),
                                               P(ctypes.c_float)]),
@@ -222,6 +226,17 @@ def write_image_exr(path: str, r, g, b) -> None:
     h, w = planes[0].shape
     fp = [x.ctypes.data_as(ctypes.POINTER(ctypes.c_float)) for x in planes]
     rc = lib.sf_write_image_exr(path.encode(), w, h, *fp)
+    if rc:
+        raise SoapFilmError(rc, f"cannot write {path}")
+
+
+def write_half_exr(path: str, gray) -> None:
+    """write_image_exr of a gray image given as its binary16 values (a
+    (height, width) uint16 array; host only)."""
+    lib = load_library()
+    g = np.ascontiguousarray(gray, dtype=np.uint16)
+    h, w = g.shape
+    rc = lib.sf_write_half_exr(path.encode(), w, h, g.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)))
     if rc:
         raise SoapFilmError(rc, f"cannot write {path}")
 
@@ -396,6 +411,14 @@ class SoapFilm:
 
     def write_thickness_exr(self, b: int, path: str):
         self._check(self._lib.sf_write_thickness_exr(self._h, b, path.encode()))
+
+    def thickness_half(self, b: int) -> np.ndarray:
+        """The frame's pixels (eta H 2e5) as binary16 values, (n_theta, n_phi)."""
+        n = self.bubbles[b].n_theta
+        out = np.empty((n, 2 * n), dtype=np.uint16)
+        self._check(self._lib.sf_thickness_half(self._h, b,
+                                                out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16))))
+        return out
 
     def load_state(self, b: int, state: dict) -> None:
         for k, v in state.items():

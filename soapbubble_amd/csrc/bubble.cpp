@@ -217,7 +217,23 @@ Solver::Solver(size_t nPhi, size_t nTheta, double radius, double dt, double H, i
 }
 
 Solver::~Solver() {
+    for (Writer& w : writers_)
+        if (w.thread.joinable()) w.thread.join();
     if (ctx_) sf_destroy(ctx_);
+}
+
+void Solver::finish(Writer& w) {
+    if (w.thread.joinable()) w.thread.join();
+    if (!w.error.empty()) {
+        const std::string m = w.error;
+        w.error.clear();
+        throw Error(SF_ERR_INVALID, m);
+    }
+}
+
+void Solver::finish_writes() {
+    // in the order the frames were handed over
+    for (int k = 0; k < kWriters; ++k) finish(writers_[(next_writer_ + k) % kWriters]);
 }
 
 void Solver::initThicknessfromPic(const std::string& path) {
@@ -272,7 +288,16 @@ void Solver::setTime(double t) { check(sf_set_time(ctx_, 0, t)); }
 void Solver::write_thickness_img(const std::string& s, int frame) {
     char name[32];
     std::snprintf(name, sizeof name, "frame%04d.exr", frame);
-    check(sf_write_thickness_exr(ctx_, 0, (s + name).c_str()));
+    Writer& w = writers_[next_writer_];
+    next_writer_ = (next_writer_ + 1) % kWriters;
+    finish(w);
+    w.frame.resize(nPhi_ * nTheta_);
+    check(sf_thickness_half(ctx_, 0, w.frame.data()));
+    const std::string path = s + name;
+    w.thread = std::thread([&w, path, this] {
+        if (sf_write_half_exr(path.c_str(), (int32_t)nPhi_, (int32_t)nTheta_, w.frame.data()) != SF_OK)
+            w.error = "cannot write " + path;
+    });
     // broken unless some cell has eta > 0 (solver.cu:568-575)
     sf_bubble_stats st;
     check(sf_get_stats(ctx_, 0, &st));
@@ -337,6 +362,7 @@ int Bubble::run() {
         if (verbose) std::cout << "Frame " << i << " is ready" << std::endl;
         solver.write_thickness_img(outputDir_, i);
     }
+    solver.finish_writes();
     return i;
 }
 

@@ -9,6 +9,7 @@
 
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/soapfilm.h"
@@ -78,7 +79,14 @@ public:
     bool isBroken();                                      // solver.cu:296-298
     void setBroken(bool broken) { broken_ = broken; }
     void setTime(double t);                               // currentTimeGlobal (s)
-    void write_thickness_img(const std::string& s, int frame);   // solver.cu:541-593
+    // write_thickness_img (solver.cu:541-593): the frame's pixels are
+    // converted on the device, and the file is written on a thread of its
+    // own while the caller goes on stepping (at most kWriters files in
+    // flight: the oldest write is finished, and its failure reported,
+    // before its buffer takes a new frame); finish_writes() and the
+    // destructor finish them all
+    void write_thickness_img(const std::string& s, int frame);
+    void finish_writes();
     void copyToCPU(sf_field field, std::vector<double>& out);    // solver.cu:243-249
     sf_ctx* context() { return ctx_; }
     size_t nPhi() const { return nPhi_; }
@@ -90,6 +98,15 @@ private:
     size_t nPhi_, nTheta_;
     double radius_, dt_;
     bool broken_ = false;
+    static constexpr int kWriters = 2;
+    struct Writer {
+        std::vector<uint16_t> frame;   // the frame being written
+        std::thread thread;
+        std::string error;
+    };
+    Writer writers_[kWriters];
+    int next_writer_ = 0;
+    void finish(Writer& w);
 };
 
 class Bubble {

@@ -82,7 +82,12 @@ X* dalloc(size_t n, std::vector<void*>& owned) {
     hipError_t e = hipMalloc(&p, n * sizeof(X));
     if (e != hipSuccess) throw Status{SF_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e)};
     owned.push_back(p);
+    // the clear goes to the null stream, which does not order against the
+    // context's non-blocking stream: it is finished before the array is
+    // handed out (arrays allocated lazily are written by the context's
+    // kernels right after)
     SF_HIP(hipMemset(p, 0, n * sizeof(X)));
+    SF_HIP(hipDeviceSynchronize());
     return static_cast<X*>(p);
 }
 
@@ -203,6 +208,8 @@ struct Base {
     virtual void update(bool finalize) = 0;
     virtual void upload(int b, int field, const double* h) = 0;
     virtual void download(int b, int field, double* h) = 0;
+    // the frame output's binary16 gray plane of bubble b (eta H 2e5)
+    virtual void thickness_half(int b, uint16_t* h) = 0;
     virtual void row_stats() = 0;
     // one inner Chebyshev launch (iteration `it` of refinement `kr`)
     virtual void inner_launch(int it, int kr) = 0;
@@ -805,6 +812,21 @@ struct Impl : Base {
         for (size_t k = 0; k < n; ++k) h[k] = (double)tmp[k];
     }
 
+    unsigned short* half_d = nullptr;   // frame output plane (allocated on first use)
+    size_t half_cap = 0;
+    void thickness_half(int b, uint16_t* h) override {
+        const BubbleDev& B = bub_h[b];
+        const size_t N = (size_t)B.nth * B.nph;
+        if (half_cap < N) {
+            size_t most = 0;
+            for (int k = 0; k < nb; ++k) most = std::max(most, (size_t)bub_h[k].nth * bub_h[k].nph);
+            half_d = dalloc<unsigned short>(most, owned);
+            half_cap = most;
+        }
+        launch_thickness_half<T>(stream, cur.eta + B.off, (long)N, desc[b].H, half_d);
+        SF_HIP(hipMemcpyAsync(h, half_d, N * sizeof(uint16_t), hipMemcpyDeviceToHost, stream));
+        SF_HIP(hipStreamSynchronize(stream));
+    }
     void download(int b, int field, double* h) override {
         const BubbleDev& B = bub_h[b];
         const size_t N = (size_t)B.nth * B.nph, Nt = N - B.nph, o = (size_t)B.off;
@@ -1627,15 +1649,10 @@ int sf_set_time(sf_ctx* ctx, int32_t b, double t) {
     return guarded(ctx, [&] {
         if (!ctx || !ctx->impl) throw Status{SF_ERR_INVALID, "null context"};
         Base& c = *ctx->impl;
-        SF_HIP(hipStreamSynchronize(c.stream));
-        SF_HIP(hipMemcpy(c.time_h.data(), c.time_d, c.nb * sizeof(BubbleTime), hipMemcpyDeviceToHost));
-        for (int k = 0; k < c.nb; ++k)
-            if (b < 0 || b == k) {
-                c.time_h[k].t = t;
-                c.time_h[k].pending = 0;
-            }
         if (b >= c.nb) throw Status{SF_ERR_INVALID, "bubble index out of range"};
-        SF_HIP(hipMemcpy(c.time_d, c.time_h.data(), c.nb * sizeof(BubbleTime), hipMemcpyHostToDevice));
+        // in stream order, after the steps already queued (no host wait)
+        launch_set_time(c.stream, c.time_d, c.nb, b < 0 ? -1 : b, t);
+        SF_HIP(hipGetLastError());
         return (int)SF_OK;
     });
 }
@@ -1948,6 +1965,17 @@ int sf_reset_timing(sf_ctx* ctx) {
 const char* sf_last_error(const sf_ctx* ctx) {
     if (ctx && ctx->impl) return ctx->impl->err.c_str();
     return g_last_error.c_str();
+}
+
+int sf_thickness_half(sf_ctx* ctx, int32_t b, uint16_t* host) {
+    return guarded(ctx, [&] {
+        check_index(ctx, b);
+        if (!host) throw Status{SF_ERR_INVALID, "null buffer"};
+        Base& c = *ctx->impl;
+        if (c.stage != 0) throw Status{SF_ERR_ORDER, "a stage-level step is in progress"};
+        c.thickness_half(b, host);
+        return (int)SF_OK;
+    });
 }
 
 int sf_write_thickness_exr(sf_ctx* ctx, int32_t b, const char* path) {

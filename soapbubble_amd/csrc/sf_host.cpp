@@ -9,6 +9,12 @@
 #include <fstream>
 #include <thread>
 
+#include <cerrno>
+#include <climits>
+#include <fcntl.h>
+#include <sys/uio.h>
+#include <unistd.h>
+
 namespace sfh {
 
 namespace {
@@ -329,9 +335,12 @@ void attr(std::string& s, const char* name, const char* type, const std::string&
 
 }  // namespace
 
-bool write_image_exr(const std::string& path, int w, int hgt, const float* r, const float* g,
-                     const float* b) {
-    // write_image (solver.cu:338-385): channels B, G, R, HALF, uncompressed
+namespace {
+
+// The header of write_image's file (solver.cu:338-385): channels B, G, R,
+// HALF, uncompressed, w x hgt, the attributes in the reference writer's
+// order, then the end-of-header byte.
+std::string exr_header(int w, int hgt) {
     std::string hdr;
     const uint32_t magic = 20000630u;
     hdr.append(reinterpret_cast<const char*>(&magic), 4);
@@ -363,6 +372,15 @@ bool write_image_exr(const std::string& path, int w, int hgt, const float* r, co
     putf(fw, (float)w);   // the image width, as the reference's writer stores it
     attr(hdr, "screenWindowWidth", "float", fw);
     hdr += '\0';
+    return hdr;
+}
+
+}  // namespace
+
+bool write_image_exr(const std::string& path, int w, int hgt, const float* r, const float* g,
+                     const float* b) {
+    // write_image (solver.cu:338-385): channels B, G, R, HALF, uncompressed
+    const std::string hdr = exr_header(w, hgt);
     // offset table, then per scanline: y, byte count, the B, G, R halves
     const uint64_t block = 8 + (uint64_t)w * 3 * 2;
     const size_t head = hdr.size() + 8ull * hgt;
@@ -409,6 +427,61 @@ bool write_image_exr(const std::string& path, int w, int hgt, const float* r, co
     if (!f) return false;
     f.write(out.data(), (std::streamsize)out.size());
     return (bool)f;
+}
+
+// The file write_image_exr writes for a gray image whose halves are
+// `gray` (row-major, w x hgt): the header and offset table, then each
+// scanline's y and byte count and its row three times (B, G, R), gathered
+// straight from `gray` by writev.
+bool write_half_exr(const std::string& path, int w, int hgt, const uint16_t* gray) {
+    std::string head = exr_header(w, hgt);
+    const uint64_t block = 8 + (uint64_t)w * 3 * 2;
+    const uint64_t start = head.size() + 8ull * hgt;
+    for (int y = 0; y < hgt; ++y) {
+        const uint64_t pos = start + block * (uint64_t)y;
+        head.append(reinterpret_cast<const char*>(&pos), 8);
+    }
+    std::vector<int32_t> lh(2 * (size_t)hgt);
+    for (int y = 0; y < hgt; ++y) {
+        lh[2 * (size_t)y] = y;
+        lh[2 * (size_t)y + 1] = w * 3 * 2;
+    }
+    const int fd = ::open(path.c_str(), O_WRONLY | O_CREAT | O_TRUNC, 0644);
+    if (fd < 0) return false;
+    std::vector<iovec> iov;
+    iov.reserve(1 + 4 * (size_t)hgt);
+    iov.push_back(iovec{head.data(), head.size()});
+    for (int y = 0; y < hgt; ++y) {
+        void* row = const_cast<uint16_t*>(gray + (size_t)y * w);
+        iov.push_back(iovec{&lh[2 * (size_t)y], 8});
+        for (int k = 0; k < 3; ++k) iov.push_back(iovec{row, 2 * (size_t)w});
+    }
+    // writev in batches of at most IOV_MAX, resuming after a short write
+    bool ok = true;
+    size_t at = 0;
+    while (ok && at < iov.size()) {
+        const int cnt = (int)std::min<size_t>(iov.size() - at, IOV_MAX);
+        const ssize_t got = ::writev(fd, iov.data() + at, cnt);
+        if (got < 0) {
+            if (errno == EINTR) continue;
+            ok = false;
+            break;
+        }
+        size_t left = (size_t)got;
+        while (left > 0 && at < iov.size()) {
+            if (left >= iov[at].iov_len) {
+                left -= iov[at].iov_len;
+                ++at;
+            } else {
+                iov[at].iov_base = static_cast<char*>(iov[at].iov_base) + left;
+                iov[at].iov_len -= left;
+                left = 0;
+            }
+        }
+        if (got == 0 && cnt > 0) ok = false;
+    }
+    if (::close(fd) != 0) ok = false;
+    return ok;
 }
 
 bool write_thickness_exr(const std::string& path, int nth, int nph, const double* eta, double H) {
@@ -614,6 +687,11 @@ void texture_fields(const char* thickness_path, const char* air_path, int nth, d
 }  // namespace sfh
 
 extern "C" {
+
+int sf_write_half_exr(const char* path, int32_t width, int32_t height, const uint16_t* gray) {
+    if (!path || !gray || width <= 0 || height <= 0) return SF_ERR_INVALID;
+    return sfh::write_half_exr(path, width, height, gray) ? SF_OK : SF_ERR_INVALID;
+}
 
 int sf_write_image_exr(const char* path, int32_t width, int32_t height, const float* r,
                        const float* g, const float* b) {

@@ -29,6 +29,8 @@ void synth_fields(int nth, uint64_t seed, int lmax, int lair, double noise, doub
 // HALF-float RGB OpenEXR, scanline, uncompressed, channels B,G,R
 // (write_image, solver.cu:338-385); value eta * H * 2e5 in every channel.
 bool write_thickness_exr(const std::string& path, int nth, int nph, const double* eta, double H);
+// write_image_exr of a gray image given as its binary16 values
+bool write_half_exr(const std::string& path, int w, int h, const uint16_t* gray);
 // write_image (solver.cu:338-385): planes r, g, b of w x h floats as HALF
 // channels B, G, R.
 bool write_image_exr(const std::string& path, int w, int h, const float* r, const float* g,

@@ -1723,6 +1723,58 @@ __global__ __launch_bounds__(128) void k_update(const int4* __restrict__ tiles,
     if (nan_out) btime[tl.x].nans = 1;
 }
 
+// currentTimeGlobal set by the host (bubble.cu:2662): the air-flow time of
+// bubble b (all when b < 0), with no pending advance, in stream order.
+__global__ void k_set_time(BubbleTime* __restrict__ bt, int nb, int b, double t) {
+    for (int k = threadIdx.x; k < nb; k += blockDim.x)
+        if (b < 0 || b == k) {
+            bt[k].t = t;
+            bt[k].pending = 0;
+        }
+}
+
+void launch_set_time(hipStream_t s, BubbleTime* bt, int nb, int b, double t) {
+    hipLaunchKernelGGL(k_set_time, dim3(1), dim3(64), 0, s, bt, nb, b, t);
+}
+
+// ------------------------------------------------------ frame output ----
+// write_thickness_img (solver.cu:560-582) up to the file: the gray value
+// eta H 2e5 of every cell, rounded to binary32 as the host writer rounds it
+// (the products in FP64, then one rounding), then to binary16 as the
+// reference's EXR writer converts (tinyexr float_to_half_full: truncate the
+// significand, add one unit when the highest dropped bit is set; binary32
+// subnormals to signed zero, overflow to infinity, NaN to a quiet NaN).
+// Cell c of the bubble is pixel (x = i, y = j) of the nPhi x nTheta image.
+__device__ __forceinline__ unsigned short half_of(float f) {
+    const unsigned int x = __float_as_uint(f);
+    const unsigned int sign = (x >> 16) & 0x8000u;
+    const int e = (int)((x >> 23) & 0xFFu);
+    const unsigned int mant = x & 0x7FFFFFu;
+    if (e == 0) return (unsigned short)sign;
+    if (e == 255) return (unsigned short)(sign | 0x7C00u | (mant ? 0x200u : 0u));
+    const int ne = e - 127 + 15;
+    if (ne >= 31) return (unsigned short)(sign | 0x7C00u);
+    if (ne <= 0) {
+        const int shift = 14 - ne;
+        if (shift > 24) return (unsigned short)sign;
+        const unsigned int m = mant | 0x800000u;
+        unsigned int r = m >> shift;
+        if ((m >> (shift - 1)) & 1u) ++r;
+        return (unsigned short)(sign | r);
+    }
+    unsigned int r = ((unsigned int)ne << 10) | (mant >> 13);
+    if (mant & 0x1000u) ++r;
+    return (unsigned short)(sign | r);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_thickness_half(const T* __restrict__ eta, long n,
+                                                        double H, unsigned short* __restrict__ out) {
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    out[k] = half_of((float)((double)eta[k] * H * 2e5));
+}
+
 // Marks a bubble whose state fields hold a NaN (the sticky flag that selects
 // the NaN-skipping samplers), for state that reached the device without
 // passing through the host upload's scan.
@@ -2262,6 +2314,13 @@ void launch_update(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, Bub
 }
 
 template <typename T>
+void launch_thickness_half(hipStream_t s, const T* eta, long n, double H, unsigned short* out) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_thickness_half<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, eta,
+                       n, H, out);
+}
+
+template <typename T>
 void launch_nan_scan(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, BubbleTime* bt,
                      StatePtrs<T> st) {
     hipLaunchKernelGGL(k_nan_scan<T>, dim3(g.ntiles), dim3(256), 0, s, g.tiles, bub, bt, st);
@@ -2322,6 +2381,9 @@ template void launch_update<double>(hipStream_t, const LaunchGeom&, const Bubble
                                     const double*, const double*, StatePtrs<double>, int,
                                     SolveHist*, const SolveParams&, const double*, long, double*,
                                     double*, double*, double*, double*);
+template void launch_thickness_half<float>(hipStream_t, const float*, long, double, unsigned short*);
+template void launch_thickness_half<double>(hipStream_t, const double*, long, double,
+                                            unsigned short*);
 template void launch_nan_scan<float>(hipStream_t, const LaunchGeom&, const BubbleDev*,
                                      BubbleTime*, StatePtrs<float>);
 template void launch_nan_scan<double>(hipStream_t, const LaunchGeom&, const BubbleDev*,

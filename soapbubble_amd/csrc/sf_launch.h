@@ -160,6 +160,11 @@ void launch_update(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, Bub
                    const double* x0, const double* x1, StatePtrs<T> out, int fin, SolveHist* hist,
                    const SolveParams& sp, const double* partR, long stride, double* chg_e,
                    double* chg_g, T* gdel, T* gnext, double* acc);
+// the air-flow time of bubble b (all when b < 0), no pending advance
+void launch_set_time(hipStream_t s, BubbleTime* bt, int nb, int b, double t);
+// the frame output's half-precision gray plane of n cells (k_thickness_half)
+template <typename T>
+void launch_thickness_half(hipStream_t s, const T* eta, long n, double H, unsigned short* out);
 // BIMOCQ stages (packed over all bubbles)
 template <typename T>
 void launch_bimocq_init(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub,

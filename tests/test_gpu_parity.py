@@ -189,6 +189,31 @@ def test_thickness_exr_roundtrip(tmp_path):
         assert np.array_equal(img[c].astype(np.float16).view(np.uint16), want), c
 
 
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_thickness_half_matches_the_file(tmp_path, prec):
+    """The device conversion of the frame's pixels (sf_thickness_half) holds
+    the halves the synchronous writer puts in every channel, and written by
+    sf_write_half_exr it gives the same file, byte for byte; on a second
+    bubble of another size in the same context."""
+    from exr_reader import read_exr
+    sizes = [32, 48]
+    with sb.SoapFilm([sb.bubble(n, 0.005) for n in sizes], prec) as ctx:
+        for k, n in enumerate(sizes):
+            st, ua, va = make_case(n, 31 + k)
+            load(ctx, k, st, ua, va)
+        ctx.step(2)
+        for k, n in enumerate(sizes):
+            p1, p2 = str(tmp_path / f"a{k}.exr"), str(tmp_path / f"b{k}.exr")
+            ctx.write_thickness_exr(k, p1)
+            half = ctx.thickness_half(k)
+            assert half.shape == (n, 2 * n)
+            _, chans, img = read_exr(p1)
+            for c in chans:
+                assert np.array_equal(img[c].astype(np.float16).view(np.uint16), half), (k, c)
+            sb.write_half_exr(p2, half)
+            assert open(p1, "rb").read() == open(p2, "rb").read(), k
+
+
 @pytest.mark.parametrize("prec,tol", [("fp64", 1e-9), ("mixed", 5e-5)])
 def test_packed_ensemble(oracle, prec, tol):
     """Bubbles of different sizes, steps, air and Marangoni numbers in one

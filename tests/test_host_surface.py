@@ -2,6 +2,7 @@
 initAirflowfromPic, initializer.cu:50-135) through sf_texture_fields, and
 the soapBubble driver's argument, config.txt and AMG-config handling up to
 the point where it needs a GPU."""
+import ctypes
 import math
 import os
 import subprocess
@@ -238,3 +239,26 @@ def test_write_image_large_scanlines(tmp_path, gray):
     assert np.array_equal(offs, hdr_end + 8 * h + np.arange(h) * (8 + 6 * w))
     ys = [int(np.frombuffer(data, "<i4", 1, int(o))[0]) for o in offs[::97]]
     assert ys == list(range(0, h, 97))
+
+
+@pytest.mark.parametrize("h,w", [(3, 8), (1201, 1024)])
+def test_write_half_matches_float_writer(tmp_path, h, w):
+    """sf_write_half_exr writes the bytes sf_write_image_exr writes for the
+    gray plane whose halves it is given (1201 rows: more scanlines than one
+    writev batch holds)."""
+    rng = np.random.default_rng(5)
+    plane = rng.uniform(-3, 3, (h, w)).astype(np.float32)
+    plane.ravel()[:4] = [1 + 2 ** -11, 70000.0, 1e-40, 0.0]
+    p1, p2 = str(tmp_path / "f.exr"), str(tmp_path / "h.exr")
+    sb.write_image_exr(p1, plane, plane, plane)
+    sb.write_half_exr(p2, _half_away(plane.ravel()).reshape(h, w))
+    assert open(p1, "rb").read() == open(p2, "rb").read()
+
+
+def test_write_half_rejects_bad_arguments(tmp_path):
+    lib = sb.load_library()
+    g = np.zeros((2, 2), np.uint16)
+    ptr = g.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16))
+    assert lib.sf_write_half_exr(str(tmp_path / "x.exr").encode(), 0, 2, ptr) == 1
+    assert lib.sf_write_half_exr(str(tmp_path / "x.exr").encode(), 2, 2, None) == 1
+    assert lib.sf_write_half_exr(str(tmp_path / "no" / "dir" / "x.exr").encode(), 2, 2, ptr) == 1
