@@ -134,9 +134,17 @@ struct Base {
     size_t graph_cache_max = 16;
     Tables tab{};
     LaunchGeom geom{};            // cell tiles
-    std::vector<LaunchGeom> rgeo; // row blocks, one geometry per row width
-    std::vector<LaunchGeom> rgeo_inner;   // the same rows for the Chebyshev launches
+    std::vector<LaunchGeom> rgeo; // row blocks, one geometry per row width (multigrid)
     std::vector<int> rgeo_nph;            // the row width of each entry of rgeo
+    // the row stages' launches: one per group of row widths that share a
+    // cell count a thread (every width, in practice), narrow rows packed
+    // several to a block; pgeo_inner the same rows for the Chebyshev
+    // launches, pgeo_width the widest row of each group
+    std::vector<LaunchGeom> pgeo, pgeo_inner;
+    std::vector<int> pgeo_width;
+    std::vector<std::vector<int>> pgeo_widths;   // the row widths of each group
+    LaunchGeom sgeo{};            // one row a block, every row (row statistics)
+    size_t lds_limit = 0;         // dynamic LDS a block may take
     // autotuning: the candidate cell-tile lists (widths 256, 128, 64 columns;
     // each row shorter than a width keeps its own power-of-two width) and the
     // choices made
@@ -487,7 +495,7 @@ struct Impl : Base {
         if (vfu) launch_viscous<T>(stream, geom, bub_d, tab, nxt, vdiv, vfu, vfv);
         // the broken-cell value of Gamma* is the state-precision gamma_max,
         // one value for a context whose bubbles share Gamma_m and T
-        for (const LaunchGeom& rg : rgeo)
+        for (const LaunchGeom& rg : pgeo)
             launch_assemble<T, Ti>(stream, rg, bub_d, time_d, tab, nxt, air(), sys(x64[0]),
                                    (double)state_gmax(bub_h[0].gamma_max), ctl_d, partR, stride,
                                    gnext, sp);
@@ -508,7 +516,7 @@ struct Impl : Base {
         SysPtrs<Ti> s = sys(x64[0]);
         s.e = ecorr;
         if (last) s.z = nullptr;
-        for (const LaunchGeom& rg : rgeo)
+        for (const LaunchGeom& rg : pgeo)
             launch_resid<T, Ti>(stream, rg, bub_d, time_d, tab, nxt, ctl_d, s, gnext, xo, xn, k,
                                 partR, stride, partB, sp, method == 2 ? sh : nullptr,
                                 check_prev ? 1 : 0);
@@ -522,7 +530,7 @@ struct Impl : Base {
         return bufs[k % 3];
     }
     void cheb(int it, int kr) {
-        for (const LaunchGeom& rg : rgeo_inner)
+        for (const LaunchGeom& rg : pgeo_inner)
             launch_cheb<Ti>(stream, rg, bub_d, ctl_d, sys(x64[0]), cheb_e(it), cheb_e(it - 1),
                             cheb_e(it + 1), it, kr, sp, partB, stride);
     }
@@ -631,7 +639,7 @@ struct Impl : Base {
         return all;
     }
     void inner(int it) {
-        for (const LaunchGeom& rg : rgeo)
+        for (const LaunchGeom& rg : pgeo)
             launch_spmv<Ti>(stream, rg, bub_d, ctl_d, sys(x64[0]), p[(it + 1) & 1], p[it & 1], it, sp,
                             partR, partB, partA, stride);
         cgstep(it);
@@ -640,7 +648,7 @@ struct Impl : Base {
     // multigrid preconditioner: the line-Jacobi z is the V-cycle's first
     // half-sweep)
     void cgstep(int it) {
-        for (const LaunchGeom& rg : rgeo)
+        for (const LaunchGeom& rg : pgeo)
             launch_cgstep<Ti>(stream, rg, bub_d, ctl_d, sys(x64[0]), p[it & 1], it, partA, partB,
                               stride);
         if (mg) mg_vcycle(false, partB);
@@ -719,7 +727,7 @@ struct Impl : Base {
             std::vector<SolveCtl> c(nb);
             std::vector<int> before(nb);
             for (int k = 0; k < max_outer; ++k) {
-                for (const LaunchGeom& rg : rgeo)
+                for (const LaunchGeom& rg : pgeo)
                     launch_spmv<Ti>(stream, rg, bub_d, ctl_d, sys(x64[0]), p[1], p[0], 0, sp, partR,
                                     partB, partA, stride);
                 const bool done = all_flags(c, false);
@@ -857,7 +865,7 @@ struct Impl : Base {
     }
 
     void row_stats() override {
-        for (const LaunchGeom& rg : rgeo) launch_row_stats<T>(stream, rg, bub_d, tab, cur.eta, partS, stride);
+        launch_row_stats<T>(stream, sgeo, bub_d, tab, cur.eta, partS, stride);
     }
 };
 
@@ -977,6 +985,49 @@ void finish_setup(Impl<T, Ti>& c) {
     if (c.mg) c.alloc_mg();
 }
 
+// The row blocks of one launch over the rows of the widths `ws` (C cells a
+// thread): a block has the threads of the widest row (whole waves); a row of
+// R = nph / C threads rounded up to w whole waves shares a block with
+// blockDim / w - 1 further rows of its bubble when two or more fit (the
+// kernels' row_seg), and the block's LDS holds each of its rows' reduced
+// line system (6 R values, 8 R when R is no power of two). Items are
+// (bubble, first row), in the XCD-aware order.
+LaunchGeom pack_rows(Base& c, const std::vector<int>& ws, int C, size_t esz, size_t lds_limit) {
+    int T = 64;
+    for (int w : ws) T = std::max(T, (w / C + 63) / 64 * 64);
+    size_t lds = 0;
+    bool packed = false;
+    std::vector<int2> items;
+    for (int b = 0; b < c.nb; ++b) {
+        const int nph = c.bub_h[b].nph;
+        if (std::find(ws.begin(), ws.end(), nph) == ws.end()) continue;
+        const int R = nph / C, w = (R + 63) / 64 * 64;
+        const int k = 2 * w > T ? 1 : T / w;
+        packed = packed || k > 1;
+        lds = std::max(lds, (size_t)k * (size_t)(is_pow2(R) ? 6 : 8) * (size_t)R * esz);
+        for (int j = 0; j < c.bub_h[b].nth; j += k) items.push_back(make_int2(b, j));
+    }
+    if (lds > lds_limit)
+        throw Status{SF_ERR_UNSUPPORTED, "line solve of rows of " + std::to_string(ws.back()) +
+                                             " cells needs " + std::to_string(lds) +
+                                             " B of LDS per block, device allows " +
+                                             std::to_string(lds_limit) +
+                                             " (use SF_PREC_MIXED for this size)"};
+    items = xcd_order(items, make_int2(-1, 0), c.nxcd);
+    int2* rows_d = dalloc<int2>(items.size(), c.owned);
+    SF_HIP(hipMemcpy(rows_d, items.data(), items.size() * sizeof(int2), hipMemcpyHostToDevice));
+    LaunchGeom g = c.geom;
+    g.rows = rows_d;
+    g.nrows = (int)items.size();
+    g.cpt = C;
+    g.row_threads = T;
+    g.lds_bytes = lds;
+    g.packed = packed;
+    g.pow2_rows = true;
+    for (int w : ws) g.pow2_rows = g.pow2_rows && is_pow2(w);
+    return g;
+}
+
 void setup(Base& c, int prec) {
     const int nb = c.nb;
     SF_HIP(hipSetDevice(c.device));
@@ -1083,12 +1134,12 @@ void setup(Base& c, int prec) {
     // width keeps a narrow row from occupying a wide row's block.
     const size_t esz = c.fp64_inner() ? 8 : 4;
     const size_t lds_limit = (size_t)prop.sharedMemPerBlock - 1024;
+    c.lds_limit = lds_limit;
     std::vector<int> widths;
     for (int b = 0; b < nb; ++b) widths.push_back(c.bub_h[b].nph);
     std::sort(widths.begin(), widths.end());
     widths.erase(std::unique(widths.begin(), widths.end()), widths.end());
     c.rgeo.clear();
-    c.rgeo_inner.clear();
     c.rgeo_nph.clear();
     for (int nph : widths) {
         std::vector<int2> rows;
@@ -1120,18 +1171,71 @@ void setup(Base& c, int prec) {
         }
         c.rgeo.push_back(g);
         c.rgeo_nph.push_back(nph);
-        // The Chebyshev iterations (FP32 inner solve, rows of at least 2048
-        // cells) hold 8 cells a thread: half the threads and barriers of
-        // the line solve per row, and their 86 registers still fit every
-        // row block of a CU at once. The reduction depth recorded at 4 cells
-        // a thread is one level deeper than 8 cells a thread need.
-        LaunchGeom gi = g;
-        if (!c.fp64_inner() && cpt == 4 && nph % 8 == 0 && nph / 8 >= 256) {
-            gi.cpt = 8;
-            gi.row_threads = std::max(64, (nph / 8 + 63) / 64 * 64);
-            gi.lds_bytes = lds_of(nph / 8);
+    }
+    // The row stages' launches: widths grouped (in increasing order) while
+    // one cell count a thread C suits all of them (C divides the row, 4 to
+    // 1024 threads a row), C the largest any of them prefers; in each
+    // group's launch a block has the threads of the widest row, and a
+    // narrower row takes its thread count rounded up to whole waves, as
+    // many rows of one bubble to a block as fit (row_seg). The Chebyshev
+    // launches take 8 cells a thread where the assembly takes 4 and the
+    // widest row has at least 2048 cells.
+    auto fits = [](int nph, int C) { return nph % C == 0 && nph / C >= 4 && nph / C <= kRowThreadsMax; };
+    c.pgeo.clear();
+    c.pgeo_inner.clear();
+    c.pgeo_width.clear();
+    c.pgeo_widths.clear();
+    std::vector<std::vector<int>> groups;
+    std::vector<int> gcpt;
+    for (int nph : widths) {
+        if (!groups.empty()) {
+            // a shared C above 4 only where every member takes it anyway (8
+            // or more cells a thread spill the assembly's registers), and
+            // widths within a factor 4 of the group's narrowest: a launch
+            // whose widest rows fill whole blocks runs them on the packed
+            // code, which holds more registers (measured on 68 bubbles of
+            // 4096 and 256 cells a row in one launch: 7 % slower than two)
+            const int C = std::max(gcpt.back(), row_cpt(nph));
+            bool ok = fits(nph, C) && (C <= 4 || row_cpt(nph) == C) && nph <= 4 * groups.back().front();
+            for (int w : groups.back()) ok = ok && fits(w, C) && (C <= 4 || row_cpt(w) == C);
+            if (ok) {
+                groups.back().push_back(nph);
+                gcpt.back() = C;
+                continue;
+            }
         }
-        c.rgeo_inner.push_back(gi);
+        groups.push_back({nph});
+        gcpt.push_back(row_cpt(nph));
+    }
+    for (size_t gi = 0; gi < groups.size(); ++gi) {
+        const std::vector<int>& ws = groups[gi];
+        const int C = gcpt[gi];
+        const int wmax = ws.back();
+        int Ci = C;
+        if (!c.fp64_inner() && C == 4 && wmax / 8 >= 256) {
+            bool ok8 = true;
+            for (int w : ws) ok8 = ok8 && fits(w, 8);
+            if (ok8) Ci = 8;
+        }
+        c.pgeo.push_back(pack_rows(c, ws, C, esz, lds_limit));
+        c.pgeo_inner.push_back(pack_rows(c, ws, Ci, esz, lds_limit));
+        c.pgeo_width.push_back(wmax);
+        c.pgeo_widths.push_back(ws);
+    }
+    {
+        // row statistics: one row a block, all rows, 256 threads striding
+        std::vector<int2> rows;
+        for (int b = 0; b < nb; ++b)
+            for (int j = 0; j < c.bub_h[b].nth; ++j) rows.push_back(make_int2(b, j));
+        rows = xcd_order(rows, make_int2(-1, 0), nxcd);
+        int2* rows_d = dalloc<int2>(rows.size(), c.owned);
+        SF_HIP(hipMemcpy(rows_d, rows.data(), rows.size() * sizeof(int2), hipMemcpyHostToDevice));
+        c.sgeo = c.geom;
+        c.sgeo.rows = rows_d;
+        c.sgeo.nrows = (int)rows.size();
+        c.sgeo.row_threads = 256;
+        c.sgeo.cpt = 1;
+        c.sgeo.lds_bytes = 0;
     }
 
     c.partR = dalloc<double>(5 * (size_t)c.stride, c.owned);
@@ -1276,10 +1380,10 @@ void autotune(Base& c) {
     c.geom.tiles = c.tile_cands[bi].tiles;
     c.geom.ntiles = c.tile_cands[bi].ntiles;
     t.advect_tile_width = c.tile_cands[bi].width;
-    t.nwidths = (int)std::min<size_t>(c.rgeo.size(), 8);
+    t.nwidths = (int)std::min<size_t>(c.pgeo_inner.size(), 8);
     for (int w = 0; w < t.nwidths; ++w) {
-        t.row_width[w] = c.rgeo_nph[(size_t)w];
-        t.inner_cells[w] = c.rgeo_inner[(size_t)w].cpt;
+        t.row_width[w] = c.pgeo_width[(size_t)w];
+        t.inner_cells[w] = c.pgeo_inner[(size_t)w].cpt;
     }
     if (c.method == 2 && !c.fp64_inner()) {
         c.advect();
@@ -1287,24 +1391,27 @@ void autotune(Base& c) {
         std::vector<SolveCtl> keep(c.nb);
         SF_HIP(hipMemcpy(keep.data(), c.ctl_d, c.nb * sizeof(SolveCtl), hipMemcpyDeviceToHost));
         for (int w = 0; w < t.nwidths; ++w) {
-            const int nph = c.rgeo_nph[(size_t)w];
-            const LaunchGeom& g0 = c.rgeo[(size_t)w];
+            const std::vector<int>& ws = c.pgeo_widths[(size_t)w];
+            const int C0 = c.pgeo[(size_t)w].cpt;
             double bt = 1e300;
-            LaunchGeom bg = c.rgeo_inner[(size_t)w];
+            LaunchGeom bg = c.pgeo_inner[(size_t)w];
             for (int q = 0; q < 3; ++q) {
-                const int C = g0.cpt << q;
-                const int R = nph / C;
-                if (C > 16 || nph % C != 0 || R < 4 || R > kRowThreadsMax) continue;
-                LaunchGeom gi = g0;
-                gi.cpt = C;
-                gi.row_threads = std::max(64, (R + 63) / 64 * 64);
-                gi.lds_bytes = (size_t)(is_pow2(R) ? 6 : 8) * (size_t)R * sizeof(float);
+                const int C = C0 << q;
+                bool ok = C <= 16;
+                for (int nph : ws) ok = ok && nph % C == 0 && nph / C >= 4 && nph / C <= kRowThreadsMax;
+                if (!ok) continue;
+                LaunchGeom gi;
+                try {
+                    gi = pack_rows(c, ws, C, sizeof(float), c.lds_limit);
+                } catch (const Status&) {
+                    continue;   // more LDS than a block has
+                }
                 (void)c.time_cheb_geom(gi, 5);
                 const double ms = c.time_cheb_geom(gi, n);
                 t.inner_ms[w][q] = ms;
                 if (ms < bt) { bt = ms; bg = gi; }
             }
-            c.rgeo_inner[(size_t)w] = bg;
+            c.pgeo_inner[(size_t)w] = bg;
             t.inner_cells[w] = bg.cpt;
         }
         SF_HIP(hipMemcpy(c.ctl_d, keep.data(), c.nb * sizeof(SolveCtl), hipMemcpyHostToDevice));

@@ -937,6 +937,92 @@ __device__ __forceinline__ double block_max(double v, double* red) {
     return m;
 }
 
+// The rows of a row block. A row of R threads (C cells each) takes R rounded
+// up to whole waves; a block of at least twice that holds blockDim.x / w
+// rows j0, j0 + 1, ... (packed: rows of several widths share one launch),
+// one segment of w threads each; otherwise its one row has the whole block.
+// seg: this thread's row (j0 + seg), lt: its place in the row, nt: threads a
+// segment, nseg: segments of the block.
+struct RowSeg {
+    int seg, lt, nt, nseg;
+};
+// PACKED = false: the launch has one row a block (compiled to constants).
+template <bool PACKED = true>
+__device__ __forceinline__ RowSeg row_seg(int R) {
+    const int w = (R + 63) & ~63;
+    RowSeg s;
+    if (!PACKED || 2 * w > (int)blockDim.x) {
+        s.nt = blockDim.x;
+        s.seg = 0;
+        s.lt = threadIdx.x;
+        s.nseg = 1;
+    } else {
+        s.nt = w;
+        s.seg = (int)threadIdx.x / w;
+        s.lt = (int)threadIdx.x - s.seg * w;
+        s.nseg = (int)blockDim.x / w;
+    }
+    return s;
+}
+
+// Deterministic sums of NV values over each segment's threads (the block
+// sum when the block holds one row); every thread receives its segment's.
+// Segments are whole waves. `red` must hold 16 * NV doubles of LDS.
+template <int NV>
+__device__ __forceinline__ void seg_sum(double (&v)[NV], double* red, const RowSeg& s) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
+    __syncthreads();
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) red[k * 16 + wid] = v[k];
+    }
+    __syncthreads();
+    const int w0 = (s.seg * s.nt) >> 6, nw = (s.nt + 63) >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        double t = 0.0;
+        for (int w = 0; w < nw; ++w) t += red[k * 16 + w0 + w];
+        v[k] = t;
+    }
+    __syncthreads();
+}
+// The segment's maximum of a non-negative value.
+__device__ __forceinline__ double seg_max(double v, double* red, const RowSeg& s) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_down(v, o, 64));
+    __syncthreads();
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    const int w0 = (s.seg * s.nt) >> 6, nw = (s.nt + 63) >> 6;
+    double m = 0.0;
+    for (int w = 0; w < nw; ++w) m = fmax(m, red[w0 + w]);
+    __syncthreads();
+    return m;
+}
+// The line-solve depth of a block's rows: the recorded depth of its one row,
+// or the largest over a packed block's rows (every row of a block runs the
+// same reduction levels; a deeper reduction than recorded only continues
+// an already decoupled one). `red` must hold 16 doubles of LDS.
+template <bool PACKED = true>
+__device__ __forceinline__ int block_steps(const int* __restrict__ rsteps, int base, int j0,
+                                           int nrows, const RowSeg& s, double* red) {
+    if (!PACKED || s.nseg == 1) return rsteps[base + j0];
+    int mx = -1;
+    if (threadIdx.x < 64)
+        for (int r = threadIdx.x; r < min(s.nseg, nrows); r += 64) mx = max(mx, rsteps[base + j0 + r]);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) mx = max(mx, __shfl_xor(mx, o, 64));
+    __syncthreads();
+    if (threadIdx.x == 0) red[0] = (double)mx;
+    __syncthreads();
+    const int r = (int)red[0];
+    __syncthreads();
+    return r;
+}
+
 // Maximum of the per-row values [base, base+n) of one slot.
 __device__ __forceinline__ double max_partials(const double* __restrict__ part, int base, int n,
                                                double* red) {
@@ -1055,8 +1141,8 @@ __device__ __forceinline__ float wave_max(float v) {
 // thread.
 template <typename Ti, int CPT>
 __device__ __forceinline__ int pcr_periodic(Ti* sa, Ti* sc, Ti* sd, Ti* ta, Ti* tc, Ti* td, int n,
-                                            int steps, float* red, Ti** x) {
-    const int tid = threadIdx.x, nt = blockDim.x, m = n - 1;
+                                            int steps, float* red, Ti** x, int tid, int nt) {
+    const int m = n - 1;
     const int full = 31 - __clz(n) - 1;     // log2(n) - 1
     Ti *a = sa, *c = sc, *d = sd, *na = ta, *nc = tc, *nd = td;
     int done = 0;
@@ -1085,11 +1171,12 @@ __device__ __forceinline__ int pcr_periodic(Ti* sa, Ti* sc, Ti* sd, Ti* ta, Ti* 
         t = c; c = nc; nc = t;
         t = d; d = nd; nd = t;
         if (steps < 0) {
+            // block-wide (every row of a packed block takes the same depth)
             mx = wave_max(mx);
-            if ((tid & 63) == 0) red[tid >> 6] = mx;
+            if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
             __syncthreads();
             float bm = 0.0f;
-            for (int w = 0; w < (nt + 63) >> 6; ++w) bm = fmaxf(bm, red[w]);
+            for (int w = 0; w < ((int)blockDim.x + 63) >> 6; ++w) bm = fmaxf(bm, red[w]);
             __syncthreads();
             if (bm < (float)PcrTol<Ti>::v) {   // d already holds x
                 *x = d;
@@ -1132,8 +1219,7 @@ __device__ __forceinline__ int pcr_periodic(Ti* sa, Ti* sc, Ti* sd, Ti* ta, Ti* 
 // the number of reductions.
 template <typename Ti>
 __device__ __forceinline__ int pcr_general(Ti* sa, Ti* sc, Ti* sd, Ti* su, Ti* ta, Ti* tc, Ti* td,
-                                           Ti* tu, int n, Ti** x) {
-    const int i = threadIdx.x;
+                                           Ti* tu, int n, Ti** x, int i) {
     const bool act = i < n;
     // the corner couplings, read by every thread before rows 0 and n-1 change
     const Ti alpha = sa[0], beta = sc[n - 1];

@@ -644,10 +644,14 @@ __device__ __forceinline__ void row_system(const BubbleDev& B, double t, const T
 // power of two).
 template <typename Ti, int C>
 __device__ __forceinline__ int line_solve(Ti (&a)[C], Ti (&c)[C], Ti (&d)[C], int n, int steps,
-                                          Ti* lds, float* redf) {
-    const int tid = threadIdx.x, R = n / C;
+                                          Ti* lds, float* redf, const RowSeg& sg) {
+    const int R = n / C;
+    // threads past the block's last segment take no part (an index past the
+    // row keeps them out of the LDS)
+    const int tid = sg.seg < sg.nseg ? sg.lt : R;
     const bool act = tid < R;
-    Ti* ra = lds;
+    // this row's share of the block's LDS
+    Ti* ra = lds + (size_t)(sg.seg < sg.nseg ? sg.seg : 0) * ((R & (R - 1)) == 0 ? 6 : 8) * R;
     Ti* rc = ra + R;
     Ti* rd = rc + R;
     Ti* xa = rd + R;
@@ -711,8 +715,8 @@ __device__ __forceinline__ int line_solve(Ti (&a)[C], Ti (&c)[C], Ti (&d)[C], in
     // second buffer of the reduction
     Ti* xs;
     const int done = (R & (R - 1)) == 0
-                         ? pcr_periodic<Ti, 1>(ra, rc, rd, xa, xb, xg, R, steps, redf, &xs)
-                         : pcr_general<Ti>(ra, rc, rd, xg + R, xa, xb, xg, xg + 2 * R, R, &xs);
+                         ? pcr_periodic<Ti, 1>(ra, rc, rd, xa, xb, xg, R, steps, redf, &xs, tid, sg.nt)
+                         : pcr_general<Ti>(ra, rc, rd, xg + R, xa, xb, xg, xg + 2 * R, R, &xs, tid);
     if (act) {
         const Ti Xt = xs[tid], Xp = xs[tid == 0 ? R - 1 : tid - 1];
 #pragma unroll
@@ -730,7 +734,7 @@ __device__ __forceinline__ int line_solve(Ti (&a)[C], Ti (&c)[C], Ti (&d)[C], in
 // matrix. Partials as k_resid. The block of row 0 resets the bubble's
 // solver control block; the broken-cell override of Gamma* is written back
 // to the state as the reference does.
-template <typename T, typename Ti, int C, bool VISC, bool P2>
+template <typename T, typename Ti, int C, bool VISC, bool P2, bool PK>
 __global__ void k_assemble(const int2* __restrict__ rows, const BubbleDev* __restrict__ bub,
                            BubbleTime* __restrict__ btime, Tables tab, StatePtrs<T> s,
                            AirPtrs<T> air, SysPtrs<Ti> sys, T gmax_state,
@@ -751,9 +755,13 @@ __global__ void k_assemble(const int2* __restrict__ rows, const BubbleDev* __res
         return;
     }
     const BubbleDev B = bub[rw.x];
-    const int j = rw.y, nph = B.nph;
-    const bool act = (int)threadIdx.x < nph / C;
-    const int i0 = C * threadIdx.x;
+    const int nph = B.nph;
+    // this thread's row of the block (several rows of a narrow width share one)
+    const RowSeg sg = row_seg<PK>(nph / C);
+    const int j = rw.y + sg.seg;
+    const bool valid = !PK || (sg.seg < sg.nseg && j < B.nth);
+    const bool act = valid && sg.lt < nph / C;
+    const int i0 = C * sg.lt;
     const double t = btime[rw.x].t;
     const size_t row = (size_t)B.off + (size_t)j * nph;
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
@@ -810,21 +818,22 @@ __global__ void k_assemble(const int2* __restrict__ rows, const BubbleDev* __res
             if (is_broken((double)s.gam[c], (double)s.eta[c], B.gamma_max)) s.gam[c] = gmax_state;
         }
     }
-    const int steps = line_solve<Ti, C>(a, cc, d, nph, -1, reinterpret_cast<Ti*>(lds_raw), redf);
+    const int steps = line_solve<Ti, C>(a, cc, d, nph, -1, reinterpret_cast<Ti*>(lds_raw), redf, sg);
     if (act) {
         st_row<C>(sys.z + row + i0, d);
 #pragma unroll
         for (int k = 0; k < C; ++k) acc[2] += (double)rin[k] * (double)d[k];
     }
-    block_sum<4>(acc, red);
-    mu = block_max((double)muf, red);
-    if (threadIdx.x == 0) {
+    seg_sum<4>(acc, red, sg);
+    mu = seg_max((double)muf, red, sg);
+    if (!valid) return;
+    if (sg.lt == 0) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) part[q * stride + B.pr0 + j] = acc[q];
         part[4 * stride + B.pr0 + j] = mu;
         sys.rsteps[B.pr0 + j] = steps;
     }
-    if (threadIdx.x == 0) {
+    if (sg.lt == 0) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) acc_add(sys.acc, rw.x, acc_slot_res(0), q, j, acc[q]);
         acc_put_max(sys.acc, rw.x, acc_slot_res(0), 4, j, mu);
@@ -836,7 +845,7 @@ __global__ void k_assemble(const int2* __restrict__ rows, const BubbleDev* __res
         }
     }
     // the first refinement's sums start from zero
-    if (j == 0 && threadIdx.x < 64) acc_zero(sys.acc, rw.x, acc_slot_res(1));
+    if (j == 0 && sg.lt < 64) acc_zero(sys.acc, rw.x, acc_slot_res(1));
 }
 
 // The five-point operator of the inner solve applied to v on this
@@ -906,7 +915,7 @@ __device__ __forceinline__ void ld_stencil(const Ti* __restrict__ v, size_t row,
 // FIRST (k = 1): x_0 is formed again here, bitwise as the assembly formed it
 // (Gamma* plus the stored guess, gamma_max on broken cells), instead of being
 // stored by the assembly and read back.
-template <typename T, typename Ti, int C, bool FIRST, bool P2>
+template <typename T, typename Ti, int C, bool FIRST, bool P2, bool PK>
 __global__ void k_resid(const int2* __restrict__ rows, const BubbleDev* __restrict__ bub,
                         const BubbleTime* __restrict__ btime, Tables tab, StatePtrs<T> s,
                         SolveCtl* __restrict__ ctl, SysPtrs<Ti> sys, const T* __restrict__ guess,
@@ -929,10 +938,13 @@ __global__ void k_resid(const int2* __restrict__ rows, const BubbleDev* __restri
     const bool next = sys.z != nullptr;
     const Ti* __restrict__ ecorr = sys.e;
     const BubbleDev B = bub[rw.x];
-    const int j = rw.y, nph = B.nph, m = B.mask, nth = B.nth;
-    const bool act = (int)threadIdx.x < nph / C;
-    const int i0 = C * threadIdx.x;
-    const int steps = sys.rsteps[B.pr0 + j];
+    const int nph = B.nph, m = B.mask, nth = B.nth;
+    const RowSeg sg = row_seg<PK>(nph / C);
+    const int j = rw.y + sg.seg;
+    const bool valid = !PK || (sg.seg < sg.nseg && j < nth);
+    const bool act = valid && sg.lt < nph / C;
+    const int i0 = C * sg.lt;
+    const int steps = next ? block_steps<PK>(sys.rsteps, B.pr0, rw.y, nth - rw.y, sg, red) : 0;
     const size_t row = (size_t)B.off + (size_t)j * nph;
     const bool hasN = j > 0, hasS = j < nth - 1;
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
@@ -991,19 +1003,20 @@ __global__ void k_resid(const int2* __restrict__ rows, const BubbleDev* __restri
         }
     }
     if (next) {
-        line_solve<Ti, C>(a, cc, d, nph, steps, reinterpret_cast<Ti*>(lds_raw), redf);
+        line_solve<Ti, C>(a, cc, d, nph, steps, reinterpret_cast<Ti*>(lds_raw), redf, sg);
         if (act) {
             st_row<C>(sys.z + row + i0, d);
 #pragma unroll
             for (int q = 0; q < C; ++q) acc[2] += (double)rin[q] * (double)d[q];
         }
     }
-    block_sum<4>(acc, red);
-    if (threadIdx.x == 0) {
+    seg_sum<4>(acc, red, sg);
+    if (!valid) return;
+    if (sg.lt == 0) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) part[q * stride + B.pr0 + j] = acc[q];
     }
-    if (threadIdx.x == 0) {
+    if (sg.lt == 0) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) acc_add(sys.acc, rw.x, acc_slot_res(k), q, j, acc[q]);
         if (j == 0) {
@@ -1012,18 +1025,18 @@ __global__ void k_resid(const int2* __restrict__ rows, const BubbleDev* __restri
         }
     }
     // the next refinement's sums start from zero
-    if (j == 0 && threadIdx.x < 64 && k < kAccRefine) acc_zero(sys.acc, rw.x, acc_slot_res(k + 1));
-    if (sh && j == 0 && threadIdx.x < 64) {
+    if (j == 0 && sg.lt < 64 && k < kAccRefine) acc_zero(sys.acc, rw.x, acc_slot_res(k + 1));
+    if (sh && j == 0 && sg.lt < 64) {
         if (k == 1) {
             double rr = 0.0;
-            for (int r = threadIdx.x; r < nth; r += 64) rr += partB[stride + B.pr0 + r];
+            for (int r = sg.lt; r < nth; r += 64) rr += partB[stride + B.pr0 + r];
 #pragma unroll
             for (int o = 32; o > 0; o >>= 1) rr += __shfl_xor(rr, o, 64);
             const double tgt = sp.inner_frac * sp.tol;
             const double bb = acc_sum(sys.acc, rw.x, acc_slot_res(0), 1);
             const double ratio = rr / (tgt * tgt * (sp.absolute ? 1.0 : bb));
-            if (threadIdx.x == 0 && !(ratio <= sh[rw.x].tail_ratio)) sh[rw.x].tail_ratio = ratio;
-        } else if (threadIdx.x == 0) {
+            if (sg.lt == 0 && !(ratio <= sh[rw.x].tail_ratio)) sh[rw.x].tail_ratio = ratio;
+        } else if (sg.lt == 0) {
             sh[rw.x].extra += 1;
         }
     }
@@ -1033,7 +1046,7 @@ __global__ void k_resid(const int2* __restrict__ rows, const BubbleDev* __restri
 // p = z + beta p_old (beta from the r.z partials of the previous step),
 // Ap, partial p.Ap. Iteration 0 of a refinement also decides whether the
 // FP64 residual already meets the tolerance.
-template <typename Ti, int C>
+template <typename Ti, int C, bool PK>
 __global__ void k_spmv(const int2* __restrict__ rows, const BubbleDev* __restrict__ bub,
                        SolveCtl* __restrict__ ctl, SysPtrs<Ti> sys, const Ti* __restrict__ pold,
                        Ti* __restrict__ pnew, int it, SolveParams sp, const double* __restrict__ partR,
@@ -1044,11 +1057,15 @@ __global__ void k_spmv(const int2* __restrict__ rows, const BubbleDev* __restric
     SolveCtl* Cb = ctl + rw.x;
     if (Cb->outer_done || (it > 0 && Cb->inner_done)) return;
     const BubbleDev B = bub[rw.x];
-    const int j = rw.y, nph = B.nph, m = B.mask, nth = B.nth;
-    const bool leader = (j == 0 && threadIdx.x == 0);
+    const int nph = B.nph, m = B.mask, nth = B.nth;
+    const RowSeg sg = row_seg<PK>(nph / C);
+    const int j = rw.y + sg.seg;
+    const bool valid = !PK || (sg.seg < sg.nseg && j < nth);
+    // the block-level decisions below are the same in every block
+    const bool leader = (rw.y == 0 && threadIdx.x == 0);
     const bool hasN = j > 0, hasS = j < nth - 1;
-    const bool act = (int)threadIdx.x < nph / C;
-    const int i0 = C * threadIdx.x;
+    const bool act = valid && sg.lt < nph / C;
+    const int i0 = C * sg.lt;
     const size_t row = (size_t)B.off + (size_t)j * nph;
     double beta = 0.0;
     if (it == 0) {
@@ -1106,12 +1123,12 @@ __global__ void k_spmv(const int2* __restrict__ rows, const BubbleDev* __restric
 #pragma unroll
         for (int q = 0; q < C; ++q) acc[0] += (double)zc[q] * (double)ap[q];
     }
-    block_sum<1>(acc, red);
-    if (threadIdx.x == 0) partA[B.pr0 + j] = acc[0];
+    seg_sum<1>(acc, red, sg);
+    if (valid && sg.lt == 0) partA[B.pr0 + j] = acc[0];
 }
 
 // -------------------------------------------- inner PCG: update + M^-1 ----
-template <typename Ti, int C>
+template <typename Ti, int C, bool PK>
 __global__ void k_cgstep(const int2* __restrict__ rows, const BubbleDev* __restrict__ bub,
                          SolveCtl* __restrict__ ctl, SysPtrs<Ti> sys, const Ti* __restrict__ p,
                          int it, const double* __restrict__ partA, double* __restrict__ partB,
@@ -1124,10 +1141,13 @@ __global__ void k_cgstep(const int2* __restrict__ rows, const BubbleDev* __restr
     SolveCtl* Cb = ctl + rw.x;
     if (Cb->outer_done || Cb->inner_done) return;
     const BubbleDev B = bub[rw.x];
-    const int j = rw.y, nph = B.nph, m = B.mask, nth = B.nth;
-    const bool act = (int)threadIdx.x < nph / C;
-    const int i0 = C * threadIdx.x;
-    const int steps = sys.rsteps[B.pr0 + j];
+    const int nph = B.nph, m = B.mask, nth = B.nth;
+    const RowSeg sg = row_seg<PK>(nph / C);
+    const int j = rw.y + sg.seg;
+    const bool valid = !PK || (sg.seg < sg.nseg && j < nth);
+    const bool act = valid && sg.lt < nph / C;
+    const int i0 = C * sg.lt;
+    const int steps = block_steps<PK>(sys.rsteps, B.pr0, rw.y, nth - rw.y, sg, red);
     const size_t row = (size_t)B.off + (size_t)j * nph;
     double sm[1];
     sum_partials<1>(partA, stride, B.pr0, nth, sm, red);
@@ -1154,14 +1174,14 @@ __global__ void k_cgstep(const int2* __restrict__ rows, const BubbleDev* __restr
         op.load(sys, row, i0, m, nph, j > 0, j < nth - 1);
         op.line(rv, a, cc, d);
     }
-    line_solve<Ti, C>(a, cc, d, nph, steps, reinterpret_cast<Ti*>(lds_raw), redf);
+    line_solve<Ti, C>(a, cc, d, nph, steps, reinterpret_cast<Ti*>(lds_raw), redf, sg);
     if (act) {
         st_row<C>(sys.z + row + i0, d);
 #pragma unroll
         for (int q = 0; q < C; ++q) acc[0] += (double)rv[q] * (double)d[q];
     }
-    block_sum<2>(acc, red);
-    if (threadIdx.x == 0) {
+    seg_sum<2>(acc, red, sg);
+    if (valid && sg.lt == 0) {
         partB[B.pr0 + j] = acc[0];
         partB[stride + B.pr0 + j] = acc[1];
         if (j == 0) Cb->iters += 1;
@@ -1251,6 +1271,8 @@ __global__ void k_mg_line(const int2* __restrict__ rows, const BubbleDev* __rest
     const MgLevelDev lv = L.lv[rw.x];
     const int j = rw.y, nph = B.nph, m = B.mask, n = lv.n;
     const size_t row = (size_t)lv.off + (size_t)j * nph;
+    // one row a block (the half-sweeps' row lists are per row width)
+    const RowSeg sg = row_seg<false>(nph / C);
     const bool act = (int)threadIdx.x < nph / C;
     const int i0 = C * threadIdx.x;
     Ti a[C], cc[C], d[C], rv[C];
@@ -1311,7 +1333,7 @@ __global__ void k_mg_line(const int2* __restrict__ rows, const BubbleDev* __rest
         }
     }
     const int steps = level == 0 ? rsteps[B.pr0 + j] : -1;
-    line_solve<Ti, C>(a, cc, d, nph, steps, reinterpret_cast<Ti*>(lds_raw), redf);
+    line_solve<Ti, C>(a, cc, d, nph, steps, reinterpret_cast<Ti*>(lds_raw), redf, sg);
     double acc[1] = {0.0};
     if (act) {
         st_row<C>(L.x + row + i0, d);
@@ -1358,7 +1380,7 @@ constexpr double kChebLeadMu = 0.5;
 // Launch 1 of refinement `kr` ends the solve (outer_done) when the FP64
 // residual at the refinement's start met the tolerance; later launches of
 // the step then return at once. Partial |r_k|^2 goes to partB slot 1.
-template <typename Ti, int C>
+template <typename Ti, int C, bool PK>
 __global__ void k_cheb(const int2* __restrict__ rows, const BubbleDev* __restrict__ bub,
                        SolveCtl* __restrict__ ctl, SysPtrs<Ti> sys, const Ti* __restrict__ ek,
                        const Ti* __restrict__ ekm1, Ti* __restrict__ enext, int it, int kr,
@@ -1377,8 +1399,11 @@ __global__ void k_cheb(const int2* __restrict__ rows, const BubbleDev* __restric
         it == 1 && residual_norm(acc_sum(acc0, b, acc_slot_res(kr), 0),
                                  acc_sum(acc0, b, acc_slot_res(kr), 1), sp.absolute) <= sp.tol;
     const BubbleDev B = bub[b];
-    const int j = rw.y, nph = B.nph, m = B.mask, nth = B.nth;
-    const bool leader = (j == 0 && threadIdx.x == 0);
+    const int nph = B.nph, m = B.mask, nth = B.nth;
+    const RowSeg sg = row_seg<PK>(nph / C);
+    const int j = rw.y + sg.seg;
+    const bool valid = !PK || (sg.seg < sg.nseg && j < nth);
+    const bool leader = (rw.y == 0 && threadIdx.x == 0);
     if (od || converged) {
         if (leader && converged) flag_store(&Cb->outer_done, 1);
         return;
@@ -1405,11 +1430,11 @@ __global__ void k_cheb(const int2* __restrict__ rows, const BubbleDev* __restric
     // the restart's first step: e_{k+1} = e_k + z_k
     const Ti cd = kk == 0 ? Ti(0) : (Ti)(rho * rho_prev),
              cz = kk == 0 ? Ti(1) : (Ti)(2.0 * rho / delta);
-    const int steps = sys.rsteps[B.pr0 + j];
+    const int steps = block_steps<PK>(sys.rsteps, B.pr0, rw.y, nth - rw.y, sg, red);
     const size_t row = (size_t)B.off + (size_t)j * nph;
     const bool hasN = j > 0, hasS = j < nth - 1;
-    const bool act = (int)threadIdx.x < nph / C;
-    const int i0 = C * threadIdx.x;
+    const bool act = valid && sg.lt < nph / C;
+    const int i0 = C * sg.lt;
     // e_1 = z_0 and e_0 = 0, so d_0 = e_1 - e_0 = z_0
     const Ti* __restrict__ e = it == 1 ? sys.z : ek;
     Ti ev[C], dvv[C], a[C], cc[C], d[C];
@@ -1442,7 +1467,7 @@ __global__ void k_cheb(const int2* __restrict__ rows, const BubbleDev* __restric
         }
         op.line(r, a, cc, d);
     }
-    line_solve<Ti, C>(a, cc, d, nph, steps, reinterpret_cast<Ti*>(lds_raw), redf);
+    line_solve<Ti, C>(a, cc, d, nph, steps, reinterpret_cast<Ti*>(lds_raw), redf, sg);
     if (act) {
         Ti en[C];
 #pragma unroll
@@ -1450,8 +1475,8 @@ __global__ void k_cheb(const int2* __restrict__ rows, const BubbleDev* __restric
         st_row<C>(enext + row + i0, en);
     }
     double a1[1] = {acc};
-    block_sum<1>(a1, red);
-    if (threadIdx.x == 0) partB[stride + B.pr0 + j] = a1[0];
+    seg_sum<1>(a1, red, sg);
+    if (valid && sg.lt == 0) partB[stride + B.pr0 + j] = a1[0];
 }
 
 // ----------------------------------------------------------- finalize ----
@@ -2181,10 +2206,14 @@ void launch_assemble(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub,
                      BubbleTime* bt, const Tables& tab, StatePtrs<T> st, AirPtrs<T> air,
                      SysPtrs<Ti> sys, double gmax_state, SolveCtl* ctl, double* part, long stride,
                      const T* guess, const SolveParams& sp) {
-#define SF_ASM(C, V, P)                                                                            \
-    hipLaunchKernelGGL((k_assemble<T, Ti, C, V, P>), dim3(g.nrows), dim3(g.row_threads),           \
+#define SF_ASM1(C, V, P, K)                                                                        \
+    hipLaunchKernelGGL((k_assemble<T, Ti, C, V, P, K>), dim3(g.nrows), dim3(g.row_threads),        \
                        g.lds_bytes, s, g.rows, bub, bt, tab, st, air, sys, (T)gmax_state, ctl,     \
                        part, stride, guess, sp)
+#define SF_ASM(C, V, P)                                                                            \
+    do {                                                                                           \
+        if (g.packed) SF_ASM1(C, V, P, true); else SF_ASM1(C, V, P, false);                        \
+    } while (0)
 #define SF_CALL(C)                                                                                 \
     do {                                                                                           \
         if (air.fu) {                                                                              \
@@ -2196,6 +2225,7 @@ void launch_assemble(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub,
     SF_CASES(SF_CALL)
 #undef SF_CALL
 #undef SF_ASM
+#undef SF_ASM1
 }
 
 template <typename T, typename Ti>
@@ -2203,10 +2233,14 @@ void launch_resid(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, cons
                   const Tables& tab, StatePtrs<T> st, SolveCtl* ctl, SysPtrs<Ti> sys,
                   const T* guess, const double* xo, double* xn, int k, double* part, long stride,
                   const double* partB, const SolveParams& sp, SchedStat* sh, int check_prev) {
+#define SF_RES1(C, F, P, K)                                                                      \
+    hipLaunchKernelGGL((k_resid<T, Ti, C, F, P, K>), dim3(g.nrows), dim3(g.row_threads),          \
+                       g.lds_bytes, s, g.rows, bub, bt, tab, st, ctl, sys, guess, xo, xn, k, part, \
+                       stride, partB, sp, sh, check_prev)
 #define SF_RES(C, F, P)                                                                           \
-    hipLaunchKernelGGL((k_resid<T, Ti, C, F, P>), dim3(g.nrows), dim3(g.row_threads), g.lds_bytes, \
-                       s, g.rows, bub, bt, tab, st, ctl, sys, guess, xo, xn, k, part, stride,     \
-                       partB, sp, sh, check_prev)
+    do {                                                                                          \
+        if (g.packed) SF_RES1(C, F, P, true); else SF_RES1(C, F, P, false);                       \
+    } while (0)
 #define SF_CALL(C)                                                                                \
     do {                                                                                          \
         if (k == 1) {                                                                             \
@@ -2218,39 +2252,55 @@ void launch_resid(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, cons
     SF_CASES(SF_CALL)
 #undef SF_CALL
 #undef SF_RES
+#undef SF_RES1
 }
 
 template <typename Ti>
 void launch_spmv(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, SolveCtl* ctl,
                  SysPtrs<Ti> sys, const Ti* pold, Ti* pnew, int it, const SolveParams& sp,
                  const double* partR, const double* partB, double* partA, long stride) {
-#define SF_CALL(C)                                                                        \
-    hipLaunchKernelGGL((k_spmv<Ti, C>), dim3(g.nrows), dim3(g.row_threads), 0, s, g.rows, \
+#define SF_CALL1(C, K)                                                                       \
+    hipLaunchKernelGGL((k_spmv<Ti, C, K>), dim3(g.nrows), dim3(g.row_threads), 0, s, g.rows, \
                        bub, ctl, sys, pold, pnew, it, sp, partR, partB, partA, stride)
+#define SF_CALL(C)                                                                           \
+    do {                                                                                     \
+        if (g.packed) SF_CALL1(C, true); else SF_CALL1(C, false);                            \
+    } while (0)
     SF_CASES(SF_CALL)
 #undef SF_CALL
+#undef SF_CALL1
 }
 
 template <typename Ti>
 void launch_cgstep(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, SolveCtl* ctl,
                    SysPtrs<Ti> sys, const Ti* p, int it, const double* partA, double* partB,
                    long stride) {
-#define SF_CALL(C)                                                                               \
-    hipLaunchKernelGGL((k_cgstep<Ti, C>), dim3(g.nrows), dim3(g.row_threads), g.lds_bytes, s,   \
+#define SF_CALL1(C, K)                                                                            \
+    hipLaunchKernelGGL((k_cgstep<Ti, C, K>), dim3(g.nrows), dim3(g.row_threads), g.lds_bytes, s, \
                        g.rows, bub, ctl, sys, p, it, partA, partB, stride)
+#define SF_CALL(C)                                                                                \
+    do {                                                                                          \
+        if (g.packed) SF_CALL1(C, true); else SF_CALL1(C, false);                                 \
+    } while (0)
     SF_CASES(SF_CALL)
 #undef SF_CALL
+#undef SF_CALL1
 }
 
 template <typename Ti>
 void launch_cheb(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, SolveCtl* ctl,
                  SysPtrs<Ti> sys, const Ti* ek, const Ti* ekm1, Ti* enext, int it, int kr,
                  const SolveParams& sp, double* partB, long stride) {
-#define SF_CALL(C)                                                                             \
-    hipLaunchKernelGGL((k_cheb<Ti, C>), dim3(g.nrows), dim3(g.row_threads), g.lds_bytes, s,   \
+#define SF_CALL1(C, K)                                                                          \
+    hipLaunchKernelGGL((k_cheb<Ti, C, K>), dim3(g.nrows), dim3(g.row_threads), g.lds_bytes, s, \
                        g.rows, bub, ctl, sys, ek, ekm1, enext, it, kr, sp, partB, stride)
+#define SF_CALL(C)                                                                              \
+    do {                                                                                        \
+        if (g.packed) SF_CALL1(C, true); else SF_CALL1(C, false);                               \
+    } while (0)
     SF_CASES(SF_CALL)
 #undef SF_CALL
+#undef SF_CALL1
 }
 
 template <typename T>

@@ -109,6 +109,7 @@ struct LaunchGeom {
     int cpt;             // consecutive cells per thread of a row block (2, 4, 8 or 16)
     size_t lds_bytes;    // dynamic LDS of a row block (the reduced line system)
     bool pow2_rows;      // every bubble's row length is a power of two
+    bool packed;         // some block holds several rows (RowSeg)
 };
 
 template <typename T>
