@@ -649,14 +649,22 @@ __device__ __forceinline__ TPos<T> start_pos(T gx, T gy) {
     return p;
 }
 
+// The components along uc and along vc = q x uc of the tangent vector with
+// (e_theta, e_phi) components (vT, vP) at the unit point q = (x, y, z)
+// (rho = sin theta, ir = 1 / rho), uc a unit tangent at q. (e_theta,
+// e_phi, q) is right-handed, so vc = (e_theta . uc) e_phi - (e_phi . uc)
+// e_theta and the two components are a plane rotation of (vT, vP): the
+// reference's dot products with the Cartesian vector (bubble.cu:476-487,
+// 754-765) without forming it. At a pole (rho = 0) the frame of phi = 0.
 template <typename T>
-__device__ __forceinline__ void tangent_vec(T x, T y, T z, T rho, T ir, T vT, T vP, T& o0, T& o1,
-                                            T& o2) {
+__device__ __forceinline__ void circle_components(T x, T y, T z, T rho, T ir, T vT, T vP, T uc0,
+                                                  T uc1, T uc2, T& along, T& across) {
     T cp = T(1), sp = T(0);
     if (rho > T(0)) { cp = x * ir; sp = y * ir; }
-    o0 = vT * z * cp - vP * sp;
-    o1 = vT * z * sp + vP * cp;
-    o2 = -vT * rho;
+    const T ct = z * (cp * uc0 + sp * uc1) - rho * uc2;   // e_theta . uc
+    const T cf = cp * uc1 - sp * uc0;                       // e_phi . uc
+    along = vT * ct + vP * cf;
+    across = vP * ct - vT * cf;
 }
 
 template <typename T, bool NANS = true, bool P2 = false>
@@ -680,12 +688,10 @@ __device__ __forceinline__ bool circle_mid(const VelCtx<T, P2>& c, T gx, T gy, T
     point_index(o, mx, my, mz, c.invh, mp, rho, ir);
     T muT = sample_vtheta<T, NANS>(c.uth, c.nth, mp);
     T muP = sample_vphi<T, NANS>(c.uph, c.nth, mp);
-    T m0, m1, m2;
-    tangent_vec(mx, my, mz, rho, ir, muT, muP, m0, m1, m2);
+    // the midpoint velocity along the circle (mguP) and across it (mguT)
     T uc0 = o.u0 * co - o.w0 * s, uc1 = o.u1 * co - o.w1 * s, uc2 = o.u2 * co - o.w2 * s;
-    T vc0 = my * uc2 - mz * uc1, vc1 = mz * uc0 - mx * uc2, vc2 = mx * uc1 - my * uc0;
-    T mguT = m0 * vc0 + m1 * vc1 + m2 * vc2;
-    T mguP = m0 * uc0 + m1 * uc1 + m2 * uc2;
+    T mguP, mguT;
+    circle_components(mx, my, mz, rho, ir, muT, muP, uc0, uc1, uc2, mguP, mguT);
     // v_ of the start frame = w x u_
     T v0 = o.w1 * o.u2 - o.w2 * o.u1, v1 = o.w2 * o.u0 - o.w0 * o.u2, v2 = o.w0 * o.u1 - o.w1 * o.u0;
     u0 = mguP * o.u0 + mguT * v0;
@@ -722,12 +728,9 @@ __device__ __forceinline__ void transport_back(const VelCtx<T, P2>& c, const Cir
     circle_depart(c, o, s, co, px, py, pz, pp, rho, ir);
     T puT = sample_vtheta<T, NANS>(c.uth, c.nth, pp);
     T puP = sample_vphi<T, NANS>(c.uph, c.nth, pp);
-    T p0, p1, p2;
-    tangent_vec(px, py, pz, rho, ir, puT, puP, p0, p1, p2);
     T uc0 = o.u0 * co - o.w0 * s, uc1 = o.u1 * co - o.w1 * s, uc2 = o.u2 * co - o.w2 * s;
-    T vc0 = py * uc2 - pz * uc1, vc1 = pz * uc0 - px * uc2, vc2 = px * uc1 - py * uc0;
-    T a = p0 * vc0 + p1 * vc1 + p2 * vc2;
-    T b = p0 * uc0 + p1 * uc1 + p2 * uc2;
+    T b, a;
+    circle_components(px, py, pz, rho, ir, puT, puP, uc0, uc1, uc2, b, a);
     r0 = b * o.u0 + a * o.v0;
     r1 = b * o.u1 + a * o.v1;
     r2 = b * o.u2 + a * o.v2;
