@@ -242,6 +242,18 @@ struct Base {
             if (b < 0 || b == k) t[k].ghist = t[k].gpend = 0;
         SF_HIP(hipMemcpy(time_d, t.data(), nb * sizeof(BubbleTime), hipMemcpyHostToDevice));
     }
+    // the control blocks copied on the context's stream, after the launches
+    // queued there (a copy on the null stream is not ordered with them: the
+    // stream is non-blocking)
+    void ctl_to_host(std::vector<SolveCtl>& h) {
+        h.resize((size_t)nb);
+        SF_HIP(hipMemcpyAsync(h.data(), ctl_d, nb * sizeof(SolveCtl), hipMemcpyDeviceToHost, stream));
+        SF_HIP(hipStreamSynchronize(stream));
+    }
+    void ctl_to_device(const std::vector<SolveCtl>& h) {
+        SF_HIP(hipMemcpyAsync(ctl_d, h.data(), nb * sizeof(SolveCtl), hipMemcpyHostToDevice, stream));
+        SF_HIP(hipStreamSynchronize(stream));
+    }
 };
 
 template <typename T, typename Ti>
@@ -1388,8 +1400,8 @@ void autotune(Base& c) {
     if (c.method == 2 && !c.fp64_inner()) {
         c.advect();
         c.assemble();
-        std::vector<SolveCtl> keep(c.nb);
-        SF_HIP(hipMemcpy(keep.data(), c.ctl_d, c.nb * sizeof(SolveCtl), hipMemcpyDeviceToHost));
+        std::vector<SolveCtl> keep;
+        c.ctl_to_host(keep);
         for (int w = 0; w < t.nwidths; ++w) {
             const std::vector<int>& ws = c.pgeo_widths[(size_t)w];
             const int C0 = c.pgeo[(size_t)w].cpt;
@@ -1414,7 +1426,7 @@ void autotune(Base& c) {
             c.pgeo_inner[(size_t)w] = bg;
             t.inner_cells[w] = bg.cpt;
         }
-        SF_HIP(hipMemcpy(c.ctl_d, keep.data(), c.nb * sizeof(SolveCtl), hipMemcpyHostToDevice));
+        c.ctl_to_device(keep);
     }
     t.tuned = 1;
     c.tuned = true;
@@ -1471,6 +1483,12 @@ hipGraphExec_t step_graph(Base& c, int nsteps, int slot) {
     }
     std::swap(c.stream, c.cstream);
     SF_HIP(e);
+    if (c.trace_solve) {
+        size_t nodes = 0;
+        (void)hipGraphGetNodes(g, nullptr, &nodes);
+        std::fprintf(stderr, "[sf solve] captured %d steps, schedule %d: %zu graph nodes\n", nsteps,
+                     c.sched.empty() ? 0 : c.sched[0], nodes);
+    }
     hipGraphExec_t x = nullptr;
     e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
@@ -1967,8 +1985,8 @@ int sf_time_inner(sf_ctx* ctx, int32_t nlaunch, double* avg_ms) {
         // its buffer holds): they write e_3, which the solve writes before
         // it reads it, so the solve that follows is unchanged; the control
         // blocks (their iteration counts) are restored
-        std::vector<SolveCtl> keep(c.nb);
-        SF_HIP(hipMemcpy(keep.data(), c.ctl_d, c.nb * sizeof(SolveCtl), hipMemcpyDeviceToHost));
+        std::vector<SolveCtl> keep;
+        c.ctl_to_host(keep);
         c.inner_launch(2, 0);
         SF_HIP(hipEventRecord(c.ev[0], c.stream));
         for (int k = 0; k < nlaunch; ++k) c.inner_launch(2, 0);
@@ -1976,7 +1994,7 @@ int sf_time_inner(sf_ctx* ctx, int32_t nlaunch, double* avg_ms) {
         SF_HIP(hipEventSynchronize(c.ev[5]));
         float ms = 0.f;
         SF_HIP(hipEventElapsedTime(&ms, c.ev[0], c.ev[5]));
-        SF_HIP(hipMemcpy(c.ctl_d, keep.data(), c.nb * sizeof(SolveCtl), hipMemcpyHostToDevice));
+        c.ctl_to_device(keep);
         *avg_ms = (double)ms / nlaunch;
         return (int)SF_OK;
     });

@@ -200,22 +200,28 @@ def test_set_dt_under_graph_replay(oracle):
         assert e < 5e-5, k
 
 
-@pytest.mark.parametrize("n,prec", [(64, "mixed"), (64, "fp64"), (1024, "mixed")])
-def test_inner_timing_leaves_the_step_unchanged(n, prec):
+@pytest.mark.parametrize("n,prec,graph", [(64, "mixed", False), (64, "fp64", False),
+                                          (1024, "mixed", False), (1024, "mixed", True)])
+def test_inner_timing_leaves_the_step_unchanged(n, prec, graph):
     """sf_time_inner between the assembly and the solve (the measurement of
     the inner launch) changes nothing the step computes: the state after the
     step is bitwise the one of the same step without it, and so are the
     solve statistics. The solve's convergence tests read sums accumulated by
     atomics in no fixed order, but no test lies within rounding of its
-    bound here, so the decisions and hence every value are the same."""
+    bound here, so the decisions and hence every value are the same. With
+    graph replays before it (a captured context after its host-driven
+    steps), the timing follows the assembly with no host call in between
+    that waits for the stream: its save and restore of the control blocks
+    must be ordered with the queued assembly."""
     dt = 0.004 if n == 64 else 0.001
-    _say(f"inner timing {prec} {n}x{2 * n}")
+    _say(f"inner timing {prec} {n}x{2 * n} graph={graph}")
     st, ua, va = make_case(n, 23, l_max=8, u_scale=0.5)
     out = []
     for timed in (False, True):
-        with sb.SoapFilm(sb.bubble(n, dt), prec, inner_method="chebyshev") as ctx:
+        with sb.SoapFilm(sb.bubble(n, dt), prec, inner_method="chebyshev",
+                         use_graph=graph) as ctx:
             load(ctx, 0, st, ua, va)
-            ctx.step(2)
+            ctx.step(10 if graph else 2)
             ctx.advect()
             ctx.assemble()
             if timed:
