@@ -244,6 +244,7 @@ void Solver::initThicknessfromPic(const std::string& path) {
                             nullptr, &got));
     if (!(got & 1)) std::cout << "No thickness image provided, initialize with eta = 0.5" << std::endl;
     check(sf_upload_field(ctx_, 0, SF_FIELD_ETA, eta.data()));
+    stats_fresh_ = false;
 }
 
 void Solver::initAirflowfromPic(const std::string& path) {
@@ -255,6 +256,7 @@ void Solver::initAirflowfromPic(const std::string& path) {
     if (!(got & 2)) std::cout << "No airflow image provided, initialize with zero" << std::endl;
     check(sf_upload_field(ctx_, 0, SF_FIELD_UAIR0, u.data()));
     check(sf_upload_field(ctx_, 0, SF_FIELD_VAIR0, v.data()));
+    stats_fresh_ = false;
 }
 
 void Solver::initWithConst(sf_field field, double val) {
@@ -262,12 +264,15 @@ void Solver::initWithConst(sf_field field, double val) {
                                                                            : nTheta_ * nPhi_;
     std::vector<double> h(n, val);
     check(sf_upload_field(ctx_, 0, field, h.data()));
+    stats_fresh_ = false;
 }
 
 void Solver::stepForward() { stepForward(1); }
 
 void Solver::stepForward(int n) {
-    if (n > 0) check(sf_step(ctx_, n));
+    if (n <= 0) return;
+    stats_fresh_ = false;
+    check(sf_step(ctx_, n));
 }
 
 void Solver::stepForward(double dt) {
@@ -276,11 +281,17 @@ void Solver::stepForward(double dt) {
     check(sf_set_dt(ctx_, 0, dt_));
 }
 
+const sf_bubble_stats& Solver::stats() {
+    if (!stats_fresh_) {
+        check(sf_get_stats(ctx_, 0, &stats_));
+        stats_fresh_ = true;
+    }
+    return stats_;
+}
+
 bool Solver::isBroken() {
     if (broken_) return true;
-    sf_bubble_stats s;
-    check(sf_get_stats(ctx_, 0, &s));
-    return s.failed_solves > 0;
+    return stats().failed_solves > 0;
 }
 
 void Solver::setTime(double t) { check(sf_set_time(ctx_, 0, t)); }
@@ -299,9 +310,7 @@ void Solver::write_thickness_img(const std::string& s, int frame) {
             w.error = "cannot write " + path;
     });
     // broken unless some cell has eta > 0 (solver.cu:568-575)
-    sf_bubble_stats st;
-    check(sf_get_stats(ctx_, 0, &st));
-    broken_ = st.no_film != 0;
+    broken_ = stats().no_film != 0;
 }
 
 void Solver::copyToCPU(sf_field field, std::vector<double>& out) {

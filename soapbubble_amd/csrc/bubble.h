@@ -98,6 +98,11 @@ private:
     size_t nPhi_, nTheta_;
     double radius_, dt_;
     bool broken_ = false;
+    // the last statistics read back, valid until the next step or upload
+    // (the broken checks of a frame share one read)
+    sf_bubble_stats stats_{};
+    bool stats_fresh_ = false;
+    const sf_bubble_stats& stats();
     static constexpr int kWriters = 2;
     struct Writer {
         std::vector<uint16_t> frame;   // the frame being written

@@ -422,12 +422,23 @@ __device__ __forceinline__ void air_blend(const float* __restrict__ a0,
 // evaluated once and shared by its two cells; each expression is the one of
 // concentrationLinearSystemKernel (bubble.cu:1330-1472), so a face weight
 // has the same value whichever cell computes it.
-template <int C>
+template <int C, typename T>
 struct RowSys {
     double wN[C], wS[C], wF[C + 1];   // effective weights; wF[k] is the face west of cell k
     double diag[C], rhs[C];
-    double x0[C], x0N[C], x0S[C], x0W, x0E;
     bool brk[C];
+    // the initial iterate on the cells and their neighbours: a value of the
+    // state precision, or gamma_max on a broken cell; kept as the state's
+    // value and the broken flags (an FP32 state then takes half the
+    // registers of FP64 copies)
+    T x0v[C], x0Nv[C], x0Sv[C], x0Wv, x0Ev;
+    bool bN[C], bS[C], bW, bE;
+    double gmax;
+    __device__ __forceinline__ double x0(int k) const { return brk[k] ? gmax : (double)x0v[k]; }
+    __device__ __forceinline__ double x0N(int k) const { return bN[k] ? gmax : (double)x0Nv[k]; }
+    __device__ __forceinline__ double x0S(int k) const { return bS[k] ? gmax : (double)x0Sv[k]; }
+    __device__ __forceinline__ double x0W() const { return bW ? gmax : (double)x0Wv; }
+    __device__ __forceinline__ double x0E() const { return bE ? gmax : (double)x0Ev; }
 };
 
 // With `guess` (assembly only) the initial iterate is Gamma* + guess instead
@@ -435,7 +446,7 @@ struct RowSys {
 template <typename T, int C, bool RHS, bool VISC, bool P2>
 __device__ __forceinline__ void row_system(const BubbleDev& B, double t, const Tables& tab,
                                            const StatePtrs<T>& s, const AirPtrs<T>& air, int j,
-                                           int i0, RowSys<C>& q,
+                                           int i0, RowSys<C, T>& q,
                                            const T* __restrict__ guess = nullptr) {
     const int nph = B.nph, m = B.mask, nth = B.nth;
     const size_t row = (size_t)B.off + (size_t)j * nph;
@@ -513,7 +524,7 @@ __device__ __forceinline__ void row_system(const BubbleDev& B, double t, const T
         const bool br = k == C ? bE : q.brk[k];
         q.wF[k] = (bl || br) ? 0.0 : wFr[k];
     }
-    double uv[C + 1], vNv[C], vSv[C];
+    T uv[C + 1], vNv[C], vSv[C];            // the film velocities (state precision)
     T uAirF[C + 1], vAirNv[C], vAirSv[C];   // the blended air (state precision)
 #pragma unroll
     for (int k = 0; k < C; ++k) vAirNv[k] = vAirSv[k] = T(0);
@@ -523,17 +534,17 @@ __device__ __forceinline__ void row_system(const BubbleDev& B, double t, const T
         T u[C], vn[C], vs[C];
         ld_row<C>(s.uph + row + i0, u);
 #pragma unroll
-        for (int k = 0; k < C; ++k) { uv[k] = (double)u[k]; vNv[k] = vSv[k] = 0.0; }
-        uv[C] = (double)s.uph[row + iEc];
+        for (int k = 0; k < C; ++k) { uv[k] = u[k]; vNv[k] = vSv[k] = T(0); }
+        uv[C] = s.uph[row + iEc];
         if (hasN) {
             ld_row<C>(s.uth + row - nph + i0, vn);
 #pragma unroll
-            for (int k = 0; k < C; ++k) vNv[k] = (double)vn[k];
+            for (int k = 0; k < C; ++k) vNv[k] = vn[k];
         }
         if (hasS) {
             ld_row<C>(s.uth + row + i0, vs);
 #pragma unroll
-            for (int k = 0; k < C; ++k) vSv[k] = (double)vs[k];
+            for (int k = 0; k < C; ++k) vSv[k] = vs[k];
         }
         if (B.airflow) {
             // per phi face, shared by the two cells of the face; v at the
@@ -568,9 +579,11 @@ __device__ __forceinline__ void row_system(const BubbleDev& B, double t, const T
         const bool brk = q.brk[k];
         q.wN[k] = (brk || bN[k]) ? 0.0 : wN;
         q.wS[k] = (brk || bS[k]) ? 0.0 : wS;
-        q.x0[k] = brk ? gmax : (double)gxv[k];
-        q.x0N[k] = bN[k] ? gmax : (double)gNv[k];
-        q.x0S[k] = bS[k] ? gmax : (double)gSv[k];
+        q.x0v[k] = gxv[k];
+        q.x0Nv[k] = gNv[k];
+        q.x0Sv[k] = gSv[k];
+        q.bN[k] = bN[k];
+        q.bS[k] = bS[k];
         q.rhs[k] = 0.0;
         if (brk) {
             q.diag[k] = 1.0;
@@ -580,7 +593,7 @@ __device__ __forceinline__ void row_system(const BubbleDev& B, double t, const T
         q.diag[k] = qdiv(sinTheta * invDt, g[k]) - (((-wN) + (-wW)) + (-wE) + (-wS));
         if (!RHS) continue;
         const int i = i0 + k;
-        double uW = uv[k], uE = uv[k + 1], vN = vNv[k], vS = vSv[k];
+        double uW = (double)uv[k], uE = (double)uv[k + 1], vN = (double)vNv[k], vS = (double)vSv[k];
         if (__builtin_isnan(vN)) vN = 0.0;
         if (__builtin_isnan(vS)) vS = 0.0;
         if (__builtin_isnan(uW)) uW = 0.0;
@@ -626,8 +639,11 @@ __device__ __forceinline__ void row_system(const BubbleDev& B, double t, const T
         if (k == C - 1 ? bE : q.brk[k + 1]) rr += wE * gmax;
         q.rhs[k] = rr;
     }
-    q.x0W = bW ? gmax : gW;
-    q.x0E = bE ? gmax : gE;
+    q.x0Wv = (T)gW;
+    q.x0Ev = (T)gE;
+    q.bW = bW;
+    q.bE = bE;
+    q.gmax = gmax;
 }
 
 // Periodic phi-line solve of one row, normalised to unit diagonal
@@ -771,7 +787,7 @@ __global__ void k_assemble(const int2* __restrict__ rows, const BubbleDev* __res
 #pragma unroll
     for (int k = 0; k < C; ++k) a[k] = cc[k] = d[k] = rin[k] = Ti(0);
     if (act) {
-        RowSys<C> q;
+        RowSys<C, T> q;
         // the extrapolated correction once a previous solve stored one
         row_system<T, C, true, VISC, P2>(B, t, tab, s, air, j, i0, q,
                                      btime[rw.x].ghist > 0 ? guess : nullptr);
@@ -779,9 +795,9 @@ __global__ void k_assemble(const int2* __restrict__ rows, const BubbleDev* __res
 #pragma unroll
         for (int k = 0; k < C; ++k) {
             const double wW = q.wF[k], wE = q.wF[k + 1];
-            const double xl = k == 0 ? q.x0W : q.x0[k - 1];
-            const double xr = k == C - 1 ? q.x0E : q.x0[k + 1];
-            const double ax = q.diag[k] * q.x0[k] - q.wN[k] * q.x0N[k] - q.wS[k] * q.x0S[k] -
+            const double xl = k == 0 ? q.x0W() : q.x0(k - 1);
+            const double xr = k == C - 1 ? q.x0E() : q.x0(k + 1);
+            const double ax = q.diag[k] * q.x0(k) - q.wN[k] * q.x0N(k) - q.wS[k] * q.x0S(k) -
                               wW * xl - wE * xr;
             const double r0 = q.rhs[k] - ax;
             // x^T N x <= sum (wN + wS) x_i^2 and x^T M x >= sum (diag - wW - wE)
@@ -953,18 +969,18 @@ __global__ void k_resid(const int2* __restrict__ rows, const BubbleDev* __restri
     for (int q = 0; q < C; ++q) a[q] = cc[q] = d[q] = rin[q] = Ti(0);
     if (act) {
         double xc[C], xN[C], xS[C], xW, xE;
-        RowSys<C> w;
+        RowSys<C, T> w;
         if constexpr (FIRST) {
             row_system<T, C, false, false, P2>(B, 0.0, tab, s, AirPtrs<T>{}, j, i0, w,
                                     btime[rw.x].ghist > 0 ? guess : nullptr);
 #pragma unroll
             for (int q = 0; q < C; ++q) {
-                xc[q] = w.x0[q];
-                xN[q] = w.x0N[q];
-                xS[q] = w.x0S[q];
+                xc[q] = w.x0(q);
+                xN[q] = w.x0N(q);
+                xS[q] = w.x0S(q);
             }
-            xW = w.x0W;
-            xE = w.x0E;
+            xW = w.x0W();
+            xE = w.x0E();
         } else {
             ld_stencil<double, C>(xo, row, i0, m, nph, hasN, hasS, xc, xW, xE, xN, xS);
             row_system<T, C, false, false, P2>(B, 0.0, tab, s, AirPtrs<T>{}, j, i0, w);
