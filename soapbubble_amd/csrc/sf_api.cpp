@@ -519,9 +519,12 @@ struct Impl : Base {
 
     // FP64 residual evaluation k >= 1: x64[k&1] = x64[(k-1)&1] + ecorr (k = 1:
     // x_0 formed again from Gamma* and the guess, as the assembly formed it); sh
-    // (captured steps) collects the schedule statistics. `last`: no refinement
-    // follows (no line solve of the new residual); `check_prev`: refinement
-    // k - 1 ran no inner iteration, so this launch tests its start residual.
+    // (captured steps) collects the schedule statistics. `last`: no line
+    // solve of the new residual (no refinement follows, or only one whose
+    // residual launch forms its correction itself); `check_prev`: refinement
+    // k - 1 ran no inner iteration, so this launch tests its start residual;
+    // ecorr = nullptr: the correction is M^-1 r of the stored residual,
+    // formed by this launch.
     void resid(int k, Ti* ecorr, SchedStat* sh, bool last = false, bool check_prev = false) {
         const double* xo = x64[(k + 1) & 1];
         double* xn = x64[k & 1];
@@ -726,13 +729,18 @@ struct Impl : Base {
                 }
 
             } else {
-                // a refinement of no inner iteration (only ever the last one:
-                // its correction z is read by its residual launch, which then
-                // writes no z of its own) takes e_1 = z of the residual before
+                // a refinement of no inner iteration (only ever the last one)
+                // takes z = M^-1 r of the residual before as its correction;
+                // its residual launch forms that z itself (on its rows and
+                // one row either side), so the residual launch before it
+                // solves no line system: the spare refinement usually finds
+                // the tolerance met and returns at once
                 for (size_t k = 0; k < sched.size(); ++k) {
                     for (int it = 1; it <= sched[k]; ++it) cheb(it, (int)k);
-                    resid((int)k + 1, cheb_e(sched[k] + 1), sstat_d, k + 1 == sched.size(),
-                          sched[k] == 0 && k > 0);
+                    const bool zp = sched[k] == 0 && k > 0;
+                    const bool zp_next = k + 2 == sched.size() && sched[k + 1] == 0;
+                    resid((int)k + 1, zp ? nullptr : cheb_e(sched[k] + 1), sstat_d,
+                          k + 1 == sched.size() || zp_next, zp);
                 }
             }
         } else if (host_driven) {

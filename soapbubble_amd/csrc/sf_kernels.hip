@@ -915,6 +915,37 @@ __device__ __forceinline__ void ld_stencil(const Ti* __restrict__ v, size_t row,
     if (hasS) ld_row<C>(v + row + nph + i0, s);
 }
 
+// z = M^-1 r on row jr of this thread's segment (zeros where the row does
+// not exist or the thread holds no cells), from the stored FP32 residual,
+// diagonal and west-face weights: the line system the residual launch that
+// stored r formed from the same roundings. Every thread of the block calls
+// it (the line solve synchronises the block).
+template <typename Ti, int C>
+__device__ __forceinline__ void z_of_stored_residual(const SysPtrs<Ti>& sys, const BubbleDev& B,
+                                                     int jr, int i0, bool act, int steps,
+                                                     Ti* lds, float* redf, const RowSeg& sg,
+                                                     Ti (&z)[C]) {
+    Ti a[C], c[C];
+#pragma unroll
+    for (int q = 0; q < C; ++q) a[q] = c[q] = z[q] = Ti(0);
+    if (act && jr >= 0 && jr < B.nth) {
+        const size_t row = (size_t)B.off + (size_t)jr * B.nph;
+        Ti rr[C], dg[C], ww[C];
+        ld_row<C>(sys.r + row + i0, rr);
+        ld_row<C>(sys.diag + row + i0, dg);
+        ld_row<C>(sys.wW + row + i0, ww);
+        const Ti wwE = sys.wW[row + wrap_col(i0 + C, B.nph, B.mask)];
+#pragma unroll
+        for (int q = 0; q < C; ++q) {
+            const Ti inv = pre_rcp(dg[q]);
+            a[q] = -ww[q] * inv;
+            c[q] = -(q + 1 < C ? ww[q + 1] : wwE) * inv;
+            z[q] = rr[q] * inv;
+        }
+    }
+    (void)line_solve<Ti, C>(a, c, z, B.nph, steps, lds, redf, sg);
+}
+
 // ----------------------------------------------------- FP64 residual ----
 // x_new = x_old + e (the inner correction of refinement k-1), r = b - A x_new
 // in FP64 with the operator recomputed from eta and Gamma* (bitwise the
@@ -930,8 +961,13 @@ __device__ __forceinline__ void ld_stencil(const Ti* __restrict__ v, size_t row,
 // here: a start residual (slot k - 1) that met the tolerance ends the solve.
 // FIRST (k = 1): x_0 is formed again here, bitwise as the assembly formed it
 // (Gamma* plus the stored guess, gamma_max on broken cells), instead of being
-// stored by the assembly and read back.
-template <typename T, typename Ti, int C, bool FIRST, bool P2, bool PK>
+// stored by the assembly and read back. ZP (the last refinement, with no
+// inner iteration): its correction e = M^-1 r of the residual before is
+// formed here, on the block's rows and one row either side, so that the
+// residual launch before needs no line solve when no further refinement
+// runs (the usual case: this launch then returns at once); r is not stored
+// (neighbouring blocks read the stored one).
+template <typename T, typename Ti, int C, bool FIRST, bool P2, bool PK, bool ZP>
 __global__ void k_resid(const int2* __restrict__ rows, const BubbleDev* __restrict__ bub,
                         const BubbleTime* __restrict__ btime, Tables tab, StatePtrs<T> s,
                         SolveCtl* __restrict__ ctl, SysPtrs<Ti> sys, const T* __restrict__ guess,
@@ -953,6 +989,7 @@ __global__ void k_resid(const int2* __restrict__ rows, const BubbleDev* __restri
     }
     const bool next = sys.z != nullptr;
     const Ti* __restrict__ ecorr = sys.e;
+    (void)ecorr;
     const BubbleDev B = bub[rw.x];
     const int nph = B.nph, m = B.mask, nth = B.nth;
     const RowSeg sg = row_seg<PK>(nph / C);
@@ -963,6 +1000,42 @@ __global__ void k_resid(const int2* __restrict__ rows, const BubbleDev* __restri
     const int steps = next ? block_steps<PK>(sys.rsteps, B.pr0, rw.y, nth - rw.y, sg, red) : 0;
     const size_t row = (size_t)B.off + (size_t)j * nph;
     const bool hasN = j > 0, hasS = j < nth - 1;
+    Ti ec[C], eN[C], eS[C], eW = Ti(0), eE = Ti(0);
+    if constexpr (ZP) {
+        // the depth of the block's rows and the two beside them
+        const int jlo = max(rw.y - 1, 0);
+        const int jhi = min(rw.y + (PK ? sg.nseg : 1), nth - 1);
+        int zsteps = -1;
+        if (threadIdx.x < 64)
+            for (int r = jlo + (int)threadIdx.x; r <= jhi; r += 64)
+                zsteps = max(zsteps, sys.rsteps[B.pr0 + r]);
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) zsteps = max(zsteps, __shfl_xor(zsteps, o, 64));
+        if (threadIdx.x == 0) red[0] = (double)zsteps;
+        __syncthreads();
+        zsteps = (int)red[0];
+        __syncthreads();
+        Ti* zl = reinterpret_cast<Ti*>(lds_raw);
+        z_of_stored_residual<Ti, C>(sys, B, j - 1, i0, act && hasN, zsteps, zl, redf, sg, eN);
+        z_of_stored_residual<Ti, C>(sys, B, j + 1, i0, act && hasS, zsteps, zl, redf, sg, eS);
+        z_of_stored_residual<Ti, C>(sys, B, j, i0, act, zsteps, zl, redf, sg, ec);
+        // the west and east neighbours of the thread's cells on row j: the
+        // last cell of the thread before and the first of the thread after
+        // (periodic in phi); the line solve left the LDS free
+        const int R = nph / C;
+        Ti* first = zl + (size_t)(sg.seg < sg.nseg ? sg.seg : 0) * 2 * R;
+        Ti* last = first + R;
+        if (act) {
+            first[sg.lt] = ec[0];
+            last[sg.lt] = ec[C - 1];
+        }
+        __syncthreads();
+        if (act) {
+            eW = last[sg.lt == 0 ? R - 1 : sg.lt - 1];
+            eE = first[sg.lt + 1 == R ? 0 : sg.lt + 1];
+        }
+        __syncthreads();
+    }
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     Ti a[C], cc[C], d[C], rin[C];
 #pragma unroll
@@ -985,8 +1058,7 @@ __global__ void k_resid(const int2* __restrict__ rows, const BubbleDev* __restri
             ld_stencil<double, C>(xo, row, i0, m, nph, hasN, hasS, xc, xW, xE, xN, xS);
             row_system<T, C, false, false, P2>(B, 0.0, tab, s, AirPtrs<T>{}, j, i0, w);
         }
-        Ti ec[C], eN[C], eS[C], eW, eE;
-        ld_stencil<Ti, C>(ecorr, row, i0, m, nph, hasN, hasS, ec, eW, eE, eN, eS);
+        if constexpr (!ZP) ld_stencil<Ti, C>(ecorr, row, i0, m, nph, hasN, hasS, ec, eW, eE, eN, eS);
         double bv[C];
         ld_row<C>(sys.b64 + row + i0, bv);
         double xnew[C];
@@ -1007,7 +1079,7 @@ __global__ void k_resid(const int2* __restrict__ rows, const BubbleDev* __restri
             acc[3] += (double)rin[q] * (double)rin[q];
         }
         st_row<C>(xn + row + i0, xnew);
-        st_row<C>(sys.r + row + i0, rin);
+        if constexpr (!ZP) st_row<C>(sys.r + row + i0, rin);
         // the line system of the inner operator: the assembly stored exactly
         // these roundings of the same FP64 weights
 #pragma unroll
@@ -2249,20 +2321,22 @@ void launch_resid(hipStream_t s, const LaunchGeom& g, const BubbleDev* bub, cons
                   const Tables& tab, StatePtrs<T> st, SolveCtl* ctl, SysPtrs<Ti> sys,
                   const T* guess, const double* xo, double* xn, int k, double* part, long stride,
                   const double* partB, const SolveParams& sp, SchedStat* sh, int check_prev) {
-#define SF_RES1(C, F, P, K)                                                                      \
-    hipLaunchKernelGGL((k_resid<T, Ti, C, F, P, K>), dim3(g.nrows), dim3(g.row_threads),          \
+#define SF_RES1(C, F, P, K, Z)                                                                   \
+    hipLaunchKernelGGL((k_resid<T, Ti, C, F, P, K, Z>), dim3(g.nrows), dim3(g.row_threads),       \
                        g.lds_bytes, s, g.rows, bub, bt, tab, st, ctl, sys, guess, xo, xn, k, part, \
                        stride, partB, sp, sh, check_prev)
-#define SF_RES(C, F, P)                                                                           \
+#define SF_RES(C, F, P, Z)                                                                        \
     do {                                                                                          \
-        if (g.packed) SF_RES1(C, F, P, true); else SF_RES1(C, F, P, false);                       \
+        if (g.packed) SF_RES1(C, F, P, true, Z); else SF_RES1(C, F, P, false, Z);                 \
     } while (0)
 #define SF_CALL(C)                                                                                \
     do {                                                                                          \
         if (k == 1) {                                                                             \
-            if (g.pow2_rows) SF_RES(C, true, true); else SF_RES(C, true, false);                  \
+            if (g.pow2_rows) SF_RES(C, true, true, false); else SF_RES(C, true, false, false);    \
+        } else if (sys.e) {                                                                       \
+            if (g.pow2_rows) SF_RES(C, false, true, false); else SF_RES(C, false, false, false);  \
         } else {                                                                                  \
-            if (g.pow2_rows) SF_RES(C, false, true); else SF_RES(C, false, false);                \
+            if (g.pow2_rows) SF_RES(C, false, true, true); else SF_RES(C, false, false, true);    \
         }                                                                                         \
     } while (0)
     SF_CASES(SF_CALL)

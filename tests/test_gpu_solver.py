@@ -290,3 +290,77 @@ def test_graph_cache_eviction_changes_nothing(monkeypatch):
             out.append(ctx.state(0))
     for k in FIELDS:
         assert np.array_equal(out[0][k], out[1][k]), k
+
+
+def _line_solve_rows(diag, w_west, r, n, nph):
+    """z = M^-1 r for the periodic phi-line blocks M of the operator:
+    diag z_i - wW_i z_(i-1) - wW_(i+1) z_(i+1) = r_i on every theta row."""
+    import scipy.sparse as sps
+    import scipy.sparse.linalg as spl
+
+    blocks = []
+    for j in range(n):
+        d = diag[j * nph:(j + 1) * nph]
+        ww = w_west[j * nph:(j + 1) * nph]
+        we = np.roll(ww, -1)
+        m = sps.diags([d, -ww[1:], -we[:-1]], [0, -1, 1], shape=(nph, nph), format="lil")
+        m[0, nph - 1] = -ww[0]
+        m[nph - 1, 0] = -we[nph - 1]
+        blocks.append(m.tocsr())
+    return spl.spsolve(sps.block_diag(blocks, format="csc"), r)
+
+
+@pytest.mark.parametrize("n,dt", [(64, 0.004), (256, 0.002), (1024, 0.001)])
+def test_spare_refinement_forms_its_correction(n, dt):
+    """The captured step's last refinement of no inner iteration corrects
+    x_1 by z = M^-1 r_1, the phi-line solve of the FP32 residual before it,
+    which its residual launch forms itself on its rows and one row either
+    side. Two contexts take the same replayed step from the same state: one
+    refinement of one Chebyshev launch (its solution is x_1), and the same
+    plus the spare refinement (x_2). x_2 - x_1 must be M^-1 r_1, with r_1 =
+    b - A x_1 in FP64 (A assembled by the oracle from the step's advected
+    fields, b the library's), rounded to FP32, and M the phi-line blocks of
+    the stored FP32 operator; the spare must have run (three FP64 residual
+    evaluations). Rows of 128 cells pack several to a block, 512-cell rows
+    two, 2048-cell rows one."""
+    from oracle import oracle as O
+
+    nph = 2 * n
+    _say(f"spare refinement {n}x{nph}")
+    st, ua, va = make_case(n, 29, l_max=8, u_scale=0.5)
+    names = ("solution", "rhs", "diag", "w_west", "u_theta_star", "u_phi_star", "eta_star",
+             "gamma_star")
+    got = {}
+    for outer in (1, 2):
+        with sb.SoapFilm(sb.bubble(n, dt), "mixed", use_graph=True, inner_method="chebyshev",
+                         max_inner=1, max_outer=outer, initial_guess="gamma_star") as ctx:
+            load(ctx, 0, st, ua, va)
+            ctx.step(3)
+            # the same state again: the next step replays the captured graph
+            load(ctx, 0, st, ua, va)
+            ctx.step(1)
+            ctx.synchronize()
+            got[outer] = {f: ctx.download(0, f) for f in names}
+            got[outer]["outer"] = ctx.stats(0)["outer"]
+            got[outer]["residual"] = ctx.stats(0)["residual"]
+    assert got[2]["outer"] == 3, got[2]["outer"]
+    for f in names[1:]:
+        assert np.array_equal(got[1][f], got[2][f]), f
+    g = got[1]
+    x1, x2 = g["solution"], got[2]["solution"]
+    star = {k: g[k + "_star"] for k in ("u_theta", "u_phi", "eta", "gamma")}
+    p = O.params(n, dt)
+    N, Nt = O.sizes(n)
+    val, _, _ = O.assemble(p, star, np.zeros(N), np.zeros(Nt))
+    r = (g["rhs"] - O.spmv(p, val, x1)).astype(np.float32).astype(np.float64)
+    z = _line_solve_rows(g["diag"], g["w_west"], r, n, nph)
+    e = rel_l2(x2 - x1, z)
+    _say(f"  |r_1| / |b| {np.linalg.norm(r) / np.linalg.norm(g['rhs']):.2e}, "
+         f"rel L2 of x_2 - x_1 against M^-1 r_1: {e:.2e}")
+    assert np.linalg.norm(z) > 0
+    assert e < 1e-3
+    # the residual of x_2 that the launch evaluated: A x_2 takes the
+    # correction of the rows either side and of the neighbouring threads
+    res = np.linalg.norm(g["rhs"] - O.spmv(p, val, x2)) / np.linalg.norm(g["rhs"])
+    _say(f"  residual of x_2: library {got[2]['residual']:.4e}, oracle operator {res:.4e}")
+    assert abs(got[2]["residual"] - res) <= 1e-3 * res
