@@ -133,8 +133,13 @@ public:
     // start; stops when the film breaks. Writes frame%04d.exr into
     // outputDir (frame 0 is the initial state). Returns the frames completed.
     int run();
+    // wall time of the last run()'s frame loop (frames 1.. and the writes
+    // they started), s: the frames' cost without the process start,
+    // textures and context set-up
+    double loopSeconds() const { return loop_seconds_; }
 
 private:
+    double loop_seconds_ = 0.0;
     double radius_, H_, Gamma_m_, T_, dt_, DT_, blend_;
     size_t nTheta_;
     int frames_, device_;

@@ -47,6 +47,7 @@ int main(int argc, char** argv) {
         }
         const int done = bubble.run();
         std::cout << "Frames completed: " << done << std::endl;
+        std::cout << "Frame loop seconds: " << bubble.loopSeconds() << std::endl;
     } catch (const sfb::Error& e) {
         std::cerr << "soapBubble: " << e.what() << std::endl;
         return 3;
