@@ -107,3 +107,27 @@ def test_cli_broken_film_stops(tmp_path):
     assert "Film is broken." in r.stderr
     assert "Frames completed: 1" in r.stdout
     assert not (tmp_path / "out" / "frame0001.exr").exists()
+
+
+def test_cli_steps_replay_graphs(tmp_path):
+    """With mixed precision the driver's frames replay captured graphs
+    whatever the AMG config's max_iters (the cap of AmgX's PCG, 100 here):
+    the inner schedule stays automatic. The solve trace shows the captures
+    and a schedule within what a captured step holds, and the driver
+    reports its frame-loop time."""
+    n, dt = 64, 0.001 * 0.05
+    print(f"soapBubble {n}x{2 * n} mixed, 4 frames of 10 steps, solve trace", flush=True)
+    _amg(tmp_path, tol="1e-07", conv="ABSOLUTE", iters=100)
+    cfg = _config(tmp_path, n_theta=n, radius=0.05, H=1e-6, timePerStep=dt,
+                  timePerFrame=10 * dt, frameNumbers=4)
+    env = dict(os.environ, SF_SOLVE_TRACE="1")
+    r = subprocess.run([CLI, cfg, "--precision", "mixed", "--advection", "plain"],
+                       capture_output=True, text=True, timeout=120, env=env)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "Frames completed: 4" in r.stdout
+    assert "max_iters 100" in r.stdout
+    assert "captured 4 steps" in r.stderr, r.stderr[-2000:]
+    scheds = [int(l.split(":")[1].split()[0]) for l in r.stderr.splitlines()
+              if l.startswith("[sf solve] graph schedule:")]
+    assert scheds and max(scheds) <= 48, scheds
+    assert float(r.stdout.split("Frame loop seconds:")[1].split()[0]) > 0
