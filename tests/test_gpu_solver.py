@@ -310,7 +310,8 @@ def _line_solve_rows(diag, w_west, r, n, nph):
     return spl.spsolve(sps.block_diag(blocks, format="csc"), r)
 
 
-@pytest.mark.parametrize("n,dt", [(64, 0.004), (256, 0.002), (1024, 0.001)])
+@pytest.mark.parametrize("n,dt", [(64, 0.004), (256, 0.002), (1024, 0.001), (100, 0.003),
+                                  (200, 0.002)])
 def test_spare_refinement_forms_its_correction(n, dt):
     """The captured step's last refinement of no inner iteration corrects
     x_1 by z = M^-1 r_1, the phi-line solve of the FP32 residual before it,
@@ -322,7 +323,9 @@ def test_spare_refinement_forms_its_correction(n, dt):
     fields, b the library's), rounded to FP32, and M the phi-line blocks of
     the stored FP32 operator; the spare must have run (three FP64 residual
     evaluations). Rows of 128 cells pack several to a block, 512-cell rows
-    two, 2048-cell rows one."""
+    two, 2048-cell rows one; rows of 200 and 400 cells (50 and 100 line
+    unknowns) take the general line solve and the non-power-of-two phi
+    wrap."""
     from oracle import oracle as O
 
     nph = 2 * n
