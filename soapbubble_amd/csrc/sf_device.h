@@ -961,7 +961,10 @@ __device__ __forceinline__ RowSeg row_seg(int R) {
         s.nseg = 1;
     } else {
         s.nt = w;
-        s.seg = (int)threadIdx.x / w;
+        // segments are whole waves, so the segment (and the row it gives)
+        // is wave-uniform: said to the compiler, per-row values (the row's
+        // table entries, offsets) load into scalar registers
+        s.seg = __builtin_amdgcn_readfirstlane((int)threadIdx.x / w);
         s.lt = (int)threadIdx.x - s.seg * w;
         s.nseg = (int)blockDim.x / w;
     }
