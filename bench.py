@@ -321,6 +321,8 @@ def main():
                     help="steps of the FP64 oracle (CPU baseline and the headline's error)")
     ap.add_argument("--c4-err-steps", type=int, default=5,
                     help="steps of the 68-bubble ensemble compared with the oracle per bubble")
+    ap.add_argument("--c3-err-steps", type=int, default=10,
+                    help="steps of the 96-bubble ensemble compared with the oracle per bubble")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-bimocq", action="store_true")
@@ -542,7 +544,8 @@ def main():
     if not args.no_ensembles:
         ensembles = [
             ensemble("32 x 64x128 (dt 0.005) + 32 x 128x256 (0.004) + 32 x 256x512 (0.002)",
-                     C3_GROUPS, 200, args.warmup, 16, 3000),
+                     C3_GROUPS, 200, args.warmup, 16, 3000,
+                     err_steps=0 if args.no_cpu else args.c3_err_steps),
             ensemble("4 x 2048x4096 (dt 0.0005) + 64 x 128x256 (0.004)",
                      C4_GROUPS, 50, args.warmup, 128, 4000,
                      err_steps=0 if args.no_cpu else args.c4_err_steps),

@@ -78,3 +78,14 @@ def test_c4_batch_per_bubble():
     _say("c4 batch 4 x 2048 + 64 x 128")
     worst = _run(oracle, [(4, 2048, 0.0005), (64, 128, 0.004)], 3, 128, 4000)
     assert worst < 5e-5
+
+
+def test_c3_batch_per_bubble():
+    """The uneven batch itself: thirty-two bubbles each at 64x128, 128x256
+    and 256x512 (dt 0.005, 0.004, 0.002) in one packed context (one launch
+    per stage, narrow rows several to a block), 10 steps; every bubble's
+    every field against its own oracle run."""
+    from oracle import oracle
+    _say("c3 batch 32 x (64, 128, 256)")
+    worst = _run(oracle, [(32, 64, 0.005), (32, 128, 0.004), (32, 256, 0.002)], 10, 16, 3000)
+    assert worst < 5e-5
