@@ -88,11 +88,12 @@ def inner_bytes(n_theta, esz):
 
 
 def frame_loop(n_theta, advection, frames=(3, 13), steps_per_frame=10, tol="1e-07",
-               convergence="ABSOLUTE"):
+               convergence="ABSOLUTE", dt=0.001):
     """The soapBubble driver end to end (config.txt, init textures, the frame
     loop of Bubble::run with its per-step broken-film check, one EXR frame a
     frame), mixed precision, the reference's AMG settings (PCGF_F.json:
-    ABSOLUTE 1e-7, or the given tolerance). Two runs of different frame counts; the difference of
+    ABSOLUTE 1e-7, or the given tolerance), radius 0.05 m and the non-dimensional
+    step dt (timePerStep = dt x radius / U). Two runs of different frame counts; the difference of
     the frame-loop times the driver reports, per frame, leaves out the
     process start (HIP initialisation, textures, context set-up) and the
     loop's first frames (calibration, graph instantiation). Textures: smooth
@@ -119,7 +120,7 @@ def frame_loop(n_theta, advection, frames=(3, 13), steps_per_frame=10, tol="1e-0
                     '"max_iters": 1}, "solver": "PCGF", "max_iters": 100, "convergence": '
                     f'"{convergence}", "tolerance": {tol}, "norm": "L2"}}}}')
         radius = 0.05
-        dt_s = 0.001 * radius            # a non-dimensional step of 0.001
+        dt_s = dt * radius               # U = 1
         el, loop = {}, {}
         for fr in frames:
             vals = dict(radius=radius, H=1e-6, U=1, cm=0.5, Gammam=1e-6, T=298.15, Ds=0.0,
@@ -142,7 +143,7 @@ def frame_loop(n_theta, advection, frames=(3, 13), steps_per_frame=10, tol="1e-0
         a, b = frames
         per = (loop[b] - loop[a]) / (b - a)
         return {"workload": f"soapBubble {n_theta}x{2 * n_theta}, {advection} advection, "
-                            f"{steps_per_frame} steps of dt 0.001 a frame, EXR frames, "
+                            f"{steps_per_frame} steps of dt {dt:g} a frame, EXR frames, "
                             f"{convergence} {tol}",
                 "frames": list(frames), "elapsed_s": [el[a], el[b]],
                 "loop_s": [loop[a], loop[b]],
@@ -565,7 +566,12 @@ def main():
         # the reference's AMG settings, and the headline's tolerance (its
         # steps against the headline step: the frame loop's own cost)
         frames = [frame_loop(n, "plain"), frame_loop(n, "bimocq"),
-                  frame_loop(n, "plain", tol="1e-10", convergence="RELATIVE")]
+                  frame_loop(n, "plain", tol="1e-10", convergence="RELATIVE"),
+                  # the reference's config.txt itself: 256x512, timePerStep
+                  # 0.005 s at radius 0.05 (dt 0.1, the stiff regime: the
+                  # automatic inner method switches to multigrid), 4 steps a
+                  # frame, BIMOCQ (its default advection)
+                  frame_loop(256, "bimocq", steps_per_frame=4, dt=0.1)]
 
     # HBM bytes per launch measured by rocprofv3 for this workload
     traffic = adv_traffic = None
